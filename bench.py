@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Throughput of the gfx950 path tracer on the 1080p / 1M-triangle forward workload.
+
+One step = one full frame: 1920x1080 pixels x 64 samples, 8 bounces, on a 1,048,576-triangle
+fBm-displaced sphere plus ground under a 2048x1024 log-normal environment map (CDF + MIS). The
+scene is uploaded once; each timed step renders into HBM-resident AOVs (no host copies inside the
+timed region).
+
+Prints one JSON line: Msamples/s, the dominant kernel's roofline (closest-hit extend kernel:
+algorithmic bytes per launch from the reference's traversal counters over its average launch time,
+measured with HIP events on the render stream) and the CPU oracle's throughput on a bounded row band
+of the same frame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+# Algorithmic bytes of the extend kernel per unit of work (per-unit figures of the reference's
+# counters, bvh.cs:188-203): 32 B per child-box test (a 64-byte sibling pair per interior visit),
+# 40 B per triangle test (TriangleLean 36 B + item index 4 B), 48 B per ray (queue index 4 B,
+# origin + direction 24 B, hit record 20 B).
+BYTES_PER_BOX, BYTES_PER_TRI, BYTES_PER_RAY = 32, 40, 48
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(spec, seconds_target=15.0):
+    """Times the CPU oracle (the C++ restatement with the reference's 32x32-block scheduler) on a
+    centred row band of the same frame, sized to about `seconds_target` seconds."""
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    osc = O.OracleScene(spec)
+    setup = time.perf_counter() - t0
+    mid = spec.height // 2
+    probe_rows = 2
+    t0 = time.perf_counter()
+    osc.render(mid, mid + probe_rows, threads=threads)
+    dt = max(time.perf_counter() - t0, 1e-3)
+    rows = int(max(probe_rows, min(spec.height, probe_rows * seconds_target / dt)))
+    y0 = max(0, mid - rows // 2)
+    y1 = min(spec.height, y0 + rows)
+    t0 = time.perf_counter()
+    out = osc.render(y0, y1, threads=threads)
+    dt = time.perf_counter() - t0
+    samples = float(out["samples"].astype("int64").sum() - (y1 - y0) * spec.width)  # count paths traced
+    msps = samples / dt / 1e6
+    return {"value": round(msps, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"rows {y0}-{y1} of the {spec.width}x{spec.height} frame at {spec.max_samples} spp "
+                      f"({int(samples)} samples, {dt:.1f} s, scene build {setup:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0, help="image scale (1.0 = 1920x1080)")
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (pixels shard trivially across GPUs; see the "
+              "sharding notes); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from sharpray_amd import scenes
+
+    t0 = time.perf_counter()
+    spec = scenes.config_plain_gpu(scale=args.scale, spp=args.spp)
+    log(f"[bench] scene spec: {spec.triangle_count} triangles, {spec.width}x{spec.height}, "
+        f"{spec.max_samples} spp ({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    scene, view = scenes.instantiate(spec)
+    scene.build()
+    log(f"[bench] BVH build + upload {time.perf_counter() - t0:.1f} s")
+
+    from sharpray_amd import Renderer
+
+    def frame(**kw):
+        return Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
+                        spec.variance_threshold, spec.bounces, spec.indirect_clamp, **kw)
+
+    for i in range(args.warmup):
+        r = frame()
+        r.wait()
+        log(f"[bench] warmup {i} done")
+    # Timed region: K full frames back to back, host-synchronised on both sides.
+    walls, gpus, ext_ms, ext_launch = [], [], 0.0, 0
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        r = frame(kernel_timing=True)
+        r.wait()
+        walls.append(time.perf_counter() - t0)
+        gpus.append(r.gpu_seconds())
+        kt = r.kernel_times()
+        ext_ms += kt["extend"][0]
+        ext_launch += kt["extend"][1]
+        log(f"[bench] step {i}: {walls[-1] * 1e3:.1f} ms wall, {gpus[-1] * 1e3:.1f} ms GPU, "
+            f"extend {kt['extend'][0]:.1f} ms / {kt['extend'][1]} launches, shade {kt['shade'][0]:.1f} ms, "
+            f"shadow {kt['shadow'][0]:.1f} ms, other {kt['other'][0]:.1f} ms")
+    samples = spec.width * spec.height * spec.max_samples
+    ms_per_step = 1e3 * sum(walls) / len(walls)
+    msps = samples / (sum(walls) / len(walls)) / 1e6
+
+    # Traversal counters of one (untimed) frame give the algorithmic bytes of the extend kernel.
+    r = frame(counters=True)
+    r.wait()
+    c = r.counters()
+    ext_bytes = (BYTES_PER_BOX * c["BvhIntersectNode"] + BYTES_PER_TRI * c["BvhIntersectShape"] +
+                 BYTES_PER_RAY * c["SceneTrace"])
+    launches_per_frame = ext_launch / args.steps
+    avg_launch_s = ext_ms / ext_launch / 1e3
+    bytes_per_launch = ext_bytes / launches_per_frame
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "extend_hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+    log(f"[bench] counters: {c}")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU oracle baseline ...")
+        cpu = cpu_baseline(spec, args.cpu_seconds)
+        log(f"[bench] CPU: {cpu}")
+
+    line = {
+        "metric": "Msamples/s @1080p/1M tris",
+        "value": round(msps, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": spec.name, "width": spec.width, "height": spec.height, "spp": spec.max_samples,
+                   "bounces": spec.bounces, "triangles": spec.triangle_count, "env_map": "2048x1024 f32",
+                   "rng": "reference xorwow per pixel"},
+        "gpu_ms_per_step": round(1e3 * sum(gpus) / len(gpus), 3),
+        "roofline": {"kernel": "k_extend (closest hit)", "bound": "hbm", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
+                     "traffic": traffic, "avg_launch_us": round(avg_launch_s * 1e6, 2),
+                     "algorithmic_bytes_per_launch": round(bytes_per_launch)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,216 @@
+/*
+ * sharpray.h - C ABI of the MI355X (gfx950) path tracer.
+ *
+ * This is the drop-in boundary for SharpRay's per-pixel path-tracing loop. The reference keeps
+ * that loop behind in-process C# classes (Scene, Renderer, Compositor); a C# host binds the
+ * functions below through P/Invoke instead. Each declaration names the reference member it
+ * replaces (file:line in BastianBlokland/sharpray, src/).
+ *
+ * Conventions
+ *  - Every function returns an int32 status: SR_OK (0) or a negative SR_ERR_* code; the text of
+ *    the last error on the calling thread is available from sr_last_error().
+ *  - Host arrays use the reference's blittable layouts, so pinned C# arrays pass straight through:
+ *    Color/Vec3 = 3 x f32, Vec2 = 2 x f32, Vec4 = 4 x f32, Triangle = 36 x f32 (math.cs:1491-1500).
+ *  - The caller owns every host buffer; the library owns all device memory.
+ *  - One context per thread; calls on a context are serialised by the caller. GPU work runs
+ *    asynchronously on the context's HIP stream and sr_render_tick waits on it.
+ */
+#ifndef SHARPRAY_H
+#define SHARPRAY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SR_OK 0
+#define SR_ERR_INVALID -1 /* bad argument (null pointer, zero size, out-of-range id) */
+#define SR_ERR_STATE -2   /* call out of order (scene already built / not built, no render) */
+#define SR_ERR_HIP -3     /* a HIP runtime call failed */
+#define SR_ERR_LIMIT -4   /* input exceeds a documented device-layout limit */
+#define SR_ERR_BUILD -5   /* BVH build hit a split with an empty side (bvh.cs:448) */
+
+typedef struct sr_ctx sr_ctx;
+
+/* Transform (math.cs:773-786): position, unit quaternion (x, y, z, w), non-zero scale. */
+typedef struct sr_transform {
+    float pos[3];
+    float rot[4];
+    float scale[3];
+} sr_transform;
+
+/* Material (scene.cs:161-171). Texture ids come from sr_texture_create; -1 = none. */
+typedef struct sr_material {
+    float albedo[3];
+    float roughness;
+    float metallic;
+    float transparency;
+    float ior;
+    float radiance[3];
+    int32_t color_texture;
+    int32_t roughness_texture;
+    int32_t metallic_texture;
+    int32_t normal_texture;
+} sr_material;
+
+/* SkyProcedural (scene.cs:341-352). */
+typedef struct sr_sky_procedural {
+    float radiance_top[3];
+    float radiance_middle[3];
+    float radiance_bottom[3];
+    float sun_dir[3];
+    float sun_radiance[3];
+    float sun_angle;
+} sr_sky_procedural;
+
+/* Fog (scene.cs:438-443): exponential height fog. */
+typedef struct sr_fog {
+    float density;
+    float color[3];
+    float anisotropy;
+    float height_falloff;
+} sr_fog;
+
+/* View (math.cs:1582-1611): transform without scale, horizontal fov (radians), near plane. */
+typedef struct sr_view {
+    float pos[3];
+    float rot[4];
+    float fov;
+    float near_plane;
+} sr_view;
+
+/* Renderer ctor arguments (renderer.cs:40-51). block_size is kept for API parity; the GPU
+ * renders sample-major over whole bands of rows and reports progress in those blocks.
+ * rng_policy: SR_RNG_REFERENCE reproduces the reference's per-pixel stateful xorwow stream
+ * (renderer.cs:156); SR_RNG_COUNTER draws from a counter-based hash of (seed, pixel, sample,
+ * draw index) so one path's divergence cannot shift another sample's draws (no reference
+ * counterpart). tick_bands: number of row bands the frame is split into (>= 1); sr_render_tick
+ * returns after each band. */
+#define SR_RNG_REFERENCE 0
+#define SR_RNG_COUNTER 1
+typedef struct sr_render_params {
+    uint32_t width;
+    uint32_t height;
+    uint32_t block_size;
+    uint32_t min_samples;
+    uint32_t max_samples;
+    float variance_threshold;
+    uint32_t bounces;
+    float indirect_clamp;
+    uint32_t rng_policy;
+    uint32_t rng_seed;
+    uint32_t tick_bands;
+} sr_render_params;
+
+/* Compositor ctor denoise arguments (compose.cs:22-34). */
+typedef struct sr_denoise_params {
+    float radius;
+    float strength;
+    float strength_max;
+    float luminance_boost;
+    float luminance_limit;
+    float normal_limit;
+    float depth_limit;
+} sr_denoise_params;
+
+/* ---- Context ------------------------------------------------------------------------------- */
+/* device = -1 creates a host-only context: scene preparation and BVH export work, rendering
+ * calls return SR_ERR_STATE. */
+int sr_create(int32_t device, sr_ctx** out);
+int sr_destroy(sr_ctx* ctx);
+const char* sr_last_error(void);
+
+/* ---- Scene assets -------------------------------------------------------------------------- */
+/* Texture (texture.cs:17-30): linear RGB float texels, row-major w*h*3. filter 0 = bilinear,
+ * 1 = point (texture.cs:4); tiling (texture.cs:10). */
+int sr_texture_create(sr_ctx* ctx, const float* rgb, uint32_t width, uint32_t height, int32_t filter,
+                      float tiling_x, float tiling_y, int32_t* out_id);
+/* Mesh ctor (mesh.cs:18-46): triangles in the C# Triangle layout (36 floats each). Builds the
+ * binned-SAH BVH with sahCostIntersect 0.5 (mesh.cs:41) and `split_bins` bins (reference: 8). */
+int sr_mesh_create(sr_ctx* ctx, const float* triangles, uint32_t count, uint32_t split_bins, int32_t* out_id);
+
+/* ---- Scene (scene.cs:507-602) -------------------------------------------------------------- */
+/* Scene.SetSky(new SkyTexture(texture, multiplier, rotYaw)) (scene.cs:543-551, 389-404). */
+int sr_scene_set_sky_texture(sr_ctx* ctx, int32_t texture, float multiplier, float rot_yaw);
+/* Scene.SetSky(new SkyProcedural(...)) (scene.cs:341-352). */
+int sr_scene_set_sky_procedural(sr_ctx* ctx, const sr_sky_procedural* sky);
+/* Scene.SetFog (scene.cs:553-561). */
+int sr_scene_set_fog(sr_ctx* ctx, const sr_fog* fog);
+/* Scene.AddObject(new Object(name, trans, material, shape)) (scene.cs:563-571, 214-223).
+ * shape_kind 0 = mesh (mesh id), 1 = Sphere (shape_params = center xyz, radius; math.cs:1171),
+ * 2 = AABox (shape_params = min xyz, max xyz; math.cs:936). */
+#define SR_SHAPE_MESH 0
+#define SR_SHAPE_SPHERE 1
+#define SR_SHAPE_BOX 2
+int sr_scene_add_object(sr_ctx* ctx, const sr_transform* trans, const sr_material* material, int32_t shape_kind,
+                        int32_t mesh, const float* shape_params, int32_t* out_id);
+/* Scene.Build (scene.cs:517-541): scene BVH (32 bins, sahCostIntersect 10) and device upload.
+ * Idempotent like the reference. */
+int sr_scene_build(sr_ctx* ctx);
+
+/* ---- Renderer (renderer.cs:40-237) --------------------------------------------------------- */
+/* Renderer ctor (renderer.cs:40-96): builds the scene if needed, allocates the seven AOVs and
+ * starts rendering asynchronously. */
+int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* params);
+/* Renderer.Tick (renderer.cs:98-105): blocks until the next band completes, returns
+ * (completed blocks, total blocks); returns (total, total) without blocking once done. */
+int sr_render_tick(sr_ctx* ctx, uint32_t* step, uint32_t* total);
+/* Copies the AOVs of renderer.cs:14-20 to host arrays (any may be null): Radiance and
+ * RadianceFog (w*h*3), Normals (w*h*3, zero when no hit), Uv (w*h*2), Depth (w*h, +inf when no
+ * hit), Samples (w*h), Variance (w*h). Waits for the render to finish. */
+int sr_render_read(sr_ctx* ctx, float* radiance, float* radiance_fog, float* normals, float* uv, float* depth,
+                   uint32_t* samples, float* variance);
+/* Device addresses of the same seven AOVs (valid until the next sr_render_begin / sr_destroy),
+ * for callers that keep the image resident in HBM. No counterpart in the reference. */
+int sr_render_device_aovs(sr_ctx* ctx, void** radiance, void** radiance_fog, void** normals, void** uv,
+                          void** depth, void** samples, void** variance);
+
+/* ---- Compositor (compose.cs:101-204) ------------------------------------------------------- */
+/* Compositor.Filter over the last render's AOVs (joint bilateral, variance-driven). Writes w*h*3
+ * floats to out_rgb (host). */
+int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* params, float* out_rgb);
+/* Same filter on caller-provided device arrays (radiance/normals: w*h*3, depth/variance: w*h). */
+int sr_denoise_bilateral_device(sr_ctx* ctx, const sr_denoise_params* params, const float* d_radiance,
+                                const float* d_normals, const float* d_depth, const float* d_variance,
+                                uint32_t width, uint32_t height, float* d_out_rgb);
+
+/* ---- Diagnostics --------------------------------------------------------------------------- */
+/* Sample-0 primary hit of every pixel (renderer.cs:156,175; math.cs:1613-1636; scene.cs:602-625):
+ * object index, triangle index (-1 for analytic shapes / misses) and world distance (+inf). */
+int sr_trace_primary(sr_ctx* ctx, const sr_view* view, uint32_t width, uint32_t height, int32_t* out_object,
+                     int32_t* out_triangle, float* out_dist);
+/* The reference-order BVH of a mesh (mesh >= 0) or of the scene (mesh = -1): node_count nodes of
+ * 8 words (min xyz, max xyz as f32, child and shapeCount as i32; bvh.cs:26-31) and the item
+ * permutation (bvh.cs:53). Pass null arrays to query node_count / item_count only. */
+int sr_bvh_export(sr_ctx* ctx, int32_t mesh, float* nodes, int32_t* items, uint32_t* node_count,
+                  uint32_t* item_count);
+/* Render-path counters in Counters.Type order subset (counters.cs:24-79):
+ * [0] Pixel [1] Sample [2] SampleBounce [3] SampleHit [4] SampleMiss [5] SampleTerminate
+ * [6] SampleFogScatter [7] SampleFogEscape [8] SamplePixelConverged [9] SamplePixelMaxed
+ * [10] SceneTrace [11] SceneOcclude [12] MeshIntersect [13] BvhIntersectNode
+ * [14] BvhIntersectShape. Collected only when enabled with sr_counters_enable. */
+#define SR_COUNTER_COUNT 15
+int sr_counters_enable(sr_ctx* ctx, int32_t enable);
+int sr_counters_read(sr_ctx* ctx, int64_t* out);
+/* Kernel timing of the next renders: with timing enabled, an event is recorded on the render
+ * stream after every launch; sr_profile_read sums the spans per kernel kind of the last render:
+ * ms[4] / launches[4] for [0] extend (closest hit), [1] shade, [2] shadow, [3] raygen + accumulate.
+ * No counterpart in the reference (its TimeScope covers whole phases, counters.cs:4-20). */
+int sr_profile_enable(sr_ctx* ctx, int32_t enable);
+int sr_profile_read(sr_ctx* ctx, double* ms, uint32_t* launches);
+/* Seconds of GPU time of the last render (HIP events around the whole frame). */
+int sr_render_gpu_seconds(sr_ctx* ctx, double* out_seconds);
+
+/* ---- Host data preparation (the C# host computes these in Triangle's constructors) ---------- */
+/* Triangle ctors (math.cs:1503-1531) and Mesh.ComputeSmoothNormals (mesh.cs:99-123): pos n*9,
+ * normals n*9 or null, uv n*6 or null (null = the OBJ defaults (0,0),(1,0),(0,1); obj.cs:417-427).
+ * smooth != 0 with null normals recomputes area-weighted vertex normals. out: n*36. */
+int sr_triangles_build(const float* pos, const float* normals, const float* uv, uint32_t count, int32_t smooth,
+                       float* out_triangles);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHARPRAY_H */

@@ -1,0 +1,202 @@
+"""ctypes wrapper of the CPU oracle (oracle/sr_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by bench.py's
+cpu_baseline leg, as the checker. The product (sharpray_amd) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+COUNTERS = ["Pixel", "Sample", "SampleBounce", "SampleHit", "SampleMiss", "SampleTerminate", "SampleFogScatter",
+            "SampleFogEscape", "SamplePixelConverged", "SamplePixelMaxed", "SceneTrace", "SceneOcclude",
+            "MeshIntersect", "BvhIntersectNode", "BvhIntersectShape", "FilterSample", "DenoiseEarlyOut"]
+
+_F = ctypes.POINTER(ctypes.c_float)
+_I = ctypes.POINTER(ctypes.c_int32)
+_U = ctypes.POINTER(ctypes.c_uint32)
+_L = ctypes.POINTER(ctypes.c_int64)
+_V = ctypes.c_void_p
+
+_SIGS = {
+    "oc_real_size": (ctypes.c_int, []),
+    "oc_rng_uints": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _U]),
+    "oc_rng_floats": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
+    "oc_splitmix64": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint32]),
+    "oc_rng_sphere": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
+    "oc_tri_intersect": (None, [_F, _F, ctypes.c_uint32, _F]),
+    "oc_box_intersect": (None, [_F, _F, ctypes.c_uint32, _F]),
+    "oc_brdf_scalar": (None, [ctypes.c_int, _F, _F, ctypes.c_uint32, _F]),
+    "oc_ggx_sample": (None, [ctypes.c_float, _F, ctypes.c_uint32, _F]),
+    "oc_hg_scatter": (None, [_F, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
+    "oc_cdf1": (ctypes.c_float, [_F, ctypes.c_uint32, _F, _F, ctypes.c_uint32, _U]),
+    "oc_triangles_make": (None, [_F, _F, _F, ctypes.c_uint32, ctypes.c_int, _F]),
+    "oc_scene_new": (_V, []),
+    "oc_scene_free": (None, [_V]),
+    "oc_scene_add_texture": (ctypes.c_int, [_V, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_float,
+                                            ctypes.c_float]),
+    "oc_scene_add_mesh": (ctypes.c_int, [_V, _F, ctypes.c_uint32, ctypes.c_int]),
+    "oc_scene_add_object": (ctypes.c_int, [_V, _F, _F, _I, ctypes.c_int, ctypes.c_int, _F]),
+    "oc_scene_set_sky_texture": (None, [_V, ctypes.c_int, ctypes.c_float, ctypes.c_float]),
+    "oc_scene_set_sky_procedural": (None, [_V, _F]),
+    "oc_scene_set_fog": (None, [_V, ctypes.c_float, _F, ctypes.c_float, ctypes.c_float]),
+    "oc_scene_build": (ctypes.c_int, [_V]),
+    "oc_bvh_export": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, _F, _I]),
+    "oc_bvh_stats": (ctypes.c_float, [_V, ctypes.c_int, ctypes.c_int, _I]),
+    "oc_trace_primary": (None, [_V, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _I, _I,
+                                _F, _F]),
+    "oc_render": (ctypes.c_int, [_V, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_float, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_int, _F, _F, _F, _F, _F, _U, _F, _L]),
+    "oc_filter": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _F, _L]),
+    "oc_sky_eval": (None, [_V, _F, ctypes.c_uint32, _F, _F]),
+    "oc_sky_sample": (None, [_V, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
+    "oc_surface_eval": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, _F]),
+    "oc_surface_scatter": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
+    "oc_fog_optical_depth": (None, [_F, _F, _F, ctypes.c_uint32, _F]),
+    "oc_fog_scatter": (None, [_F, _F, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                              ctypes.c_uint32, _F]),
+}
+
+_libs = {}
+
+
+def build():
+    """Compiles both oracle libraries (float and FP64) with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib(precision="f32"):
+    if precision not in _libs:
+        path = os.path.join(HERE, f"liboracle_{precision}.so")
+        if not os.path.exists(path):
+            build()
+        h = ctypes.CDLL(path)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _libs[precision] = h
+    return _libs[precision]
+
+
+def f(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32).ctypes.data_as(_F)
+
+
+def _keep(a, dtype):
+    return None if a is None else np.ascontiguousarray(a, dtype=dtype)
+
+
+class OracleScene:
+    """A scene inside the oracle, built from a sharpray_amd.scenes.SceneSpec."""
+
+    def __init__(self, spec, precision="f32", rebuild_triangles=None):
+        self.L = lib(precision)
+        self.spec = spec
+        self.h = self.L.oc_scene_new()
+        self._refs = []
+        for t in spec.textures:
+            rgb = _keep(t, np.float32)
+            self._refs.append(rgb)
+            self.L.oc_scene_add_texture(self.h, f(rgb), rgb.shape[1], rgb.shape[0], 0, 1.0, 1.0)
+        self.mesh_ids = []
+        for m, bins in zip(spec.meshes, spec.mesh_bins):
+            m = _keep(m, np.float32)
+            self._refs.append(m)
+            mid = self.L.oc_scene_add_mesh(self.h, f(m), m.shape[0], bins)
+            if mid < 0:
+                raise RuntimeError("oracle: degenerate BVH split")
+            self.mesh_ids.append(mid)
+        for o in spec.objects:
+            tex = np.array(o.textures, dtype=np.int32)
+            params = None if o.params is None else _keep(o.params, np.float32)
+            self._refs += [tex, params]
+            self.L.oc_scene_add_object(self.h, f(o.transform), f(o.material), tex.ctypes.data_as(_I), o.kind,
+                                       o.mesh, f(params))
+        if spec.sky_texture >= 0:
+            self.L.oc_scene_set_sky_texture(self.h, spec.sky_texture, spec.sky_multiplier, spec.sky_yaw)
+        else:
+            self.L.oc_scene_set_sky_procedural(self.h, f(spec.sky_procedural))
+        if spec.fog is not None:
+            d, c, g, fo = spec.fog
+            self.L.oc_scene_set_fog(self.h, d, f(np.asarray(c, np.float32)), g, fo)
+        if self.L.oc_scene_build(self.h) != 0:
+            raise RuntimeError("oracle: scene build failed (sky required)")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.oc_scene_free(self.h)
+            self.h = None
+
+    def bvh(self, mesh=None):
+        level, mid = (0, 0) if mesh is None else (1, mesh)
+        n = self.L.oc_bvh_export(self.h, level, mid, None, None)
+        nitems = len(self.spec.objects) if mesh is None else self.spec.meshes[mesh].shape[0]
+        nodes = np.empty((n, 8), np.float32)
+        items = np.empty(nitems, np.int32)
+        self.L.oc_bvh_export(self.h, level, mid, nodes.ctypes.data_as(_F), items.ctypes.data_as(_I))
+        return nodes, items
+
+    def bvh_stats(self, mesh=None):
+        level, mid = (0, 0) if mesh is None else (1, mesh)
+        ints = np.zeros(7, np.int32)
+        sah = self.L.oc_bvh_stats(self.h, level, mid, ints.ctypes.data_as(_I))
+        keys = ["nodeCount", "depthMax", "leafCount", "leafSizeMin", "leafSizeMax", "shapeCount", "depthWeighted"]
+        d = dict(zip(keys, ints.tolist()))
+        d["sah"] = sah
+        return d
+
+    def trace_primary(self, y0=0, y1=None):
+        s = self.spec
+        y1 = s.height if y1 is None else y1
+        n = (y1 - y0) * s.width
+        obj = np.empty(n, np.int32)
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        nrm = np.empty((n, 3), np.float32)
+        self.L.oc_trace_primary(self.h, f(s.view), s.width, s.height, y0, y1, obj.ctypes.data_as(_I),
+                                prim.ctypes.data_as(_I), t.ctypes.data_as(_F), nrm.ctypes.data_as(_F))
+        shp = (y1 - y0, s.width)
+        return obj.reshape(shp), prim.reshape(shp), t.reshape(shp), nrm.reshape(shp + (3,))
+
+    def render(self, y0=0, y1=None, threads=None, counters=False):
+        s = self.spec
+        y1 = s.height if y1 is None else y1
+        threads = threads or os.cpu_count() or 1
+        n = (y1 - y0) * s.width
+        out = dict(radiance=np.empty((n, 3), np.float32), radiance_fog=np.empty((n, 3), np.float32),
+                   normals=np.empty((n, 3), np.float32), uv=np.empty((n, 2), np.float32),
+                   depth=np.empty(n, np.float32), samples=np.empty(n, np.uint32), variance=np.empty(n, np.float32))
+        ctr = np.zeros(len(COUNTERS), np.int64)
+        rc = self.L.oc_render(self.h, f(s.view), s.width, s.height, s.min_samples, s.max_samples,
+                              s.variance_threshold, s.bounces, s.indirect_clamp, y0, y1, threads,
+                              out["radiance"].ctypes.data_as(_F), out["radiance_fog"].ctypes.data_as(_F),
+                              out["normals"].ctypes.data_as(_F), out["uv"].ctypes.data_as(_F),
+                              out["depth"].ctypes.data_as(_F), out["samples"].ctypes.data_as(_U),
+                              out["variance"].ctypes.data_as(_F), ctr.ctypes.data_as(_L) if counters else None)
+        if rc != 0:
+            raise RuntimeError("oracle: render on an unbuilt scene")
+        if counters:
+            out["counters"] = dict(zip(COUNTERS, ctr.tolist()))
+        return out
+
+
+def triangles_make(pos, normals=None, uv=None, smooth=False, precision="f32"):
+    pos = _keep(pos, np.float32).reshape(-1, 9)
+    n = pos.shape[0]
+    out = np.empty((n, 36), np.float32)
+    nrm = _keep(normals, np.float32)
+    uvs = _keep(uv, np.float32)
+    lib(precision).oc_triangles_make(f(pos), f(nrm), f(uvs), n, 1 if smooth else 0, out.ctypes.data_as(_F))
+    return out
+
+
+def bilateral(params7, radiance, normals, depth, variance, width, height, threads=None):
+    out = np.empty((width * height, 3), np.float32)
+    samples = np.zeros(1, np.int64)
+    lib().oc_filter(f(params7), f(radiance), f(normals), f(depth), f(variance), width, height,
+                    threads or os.cpu_count() or 1, out.ctypes.data_as(_F), samples.ctypes.data_as(_L))
+    return out, int(samples[0])

@@ -1,0 +1,464 @@
+"""Host-side mirror of SharpRay's Scene / Renderer / Compositor API over the gfx950 C ABI.
+
+Names, argument meaning and error behaviour follow the reference's C# classes
+(BastianBlokland/sharpray src/scene.cs, renderer.cs, compose.cs, math.cs, mesh.cs, texture.cs);
+every call goes through libsharpray_hip.so (include/sharpray.h). Arrays are numpy float32 in the
+reference's blittable layouts (Color/Vec3 = 3 floats, Vec2 = 2, Triangle = 36).
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _native as N
+from ._native import SharpRayError, check, fptr, iptr, uptr
+
+__all__ = [
+    "SharpRayError", "Quat", "Transform", "View", "Texture", "Mesh", "Sphere", "AABox", "Material", "Object",
+    "SkyTexture", "SkyProcedural", "Fog", "Scene", "Renderer", "Compositor", "triangles_build", "RNG_REFERENCE",
+    "RNG_COUNTER",
+]
+
+RNG_REFERENCE = N.SR_RNG_REFERENCE
+RNG_COUNTER = N.SR_RNG_COUNTER
+
+
+def _f32(a, shape_tail=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape_tail is not None and a.shape[-len(shape_tail):] != tuple(shape_tail):
+        raise ValueError(f"expected trailing shape {shape_tail}, got {a.shape}")
+    return a
+
+
+class Quat:
+    """Quat (math.cs:564-659): x, y, z, w."""
+
+    def __init__(self, x=0.0, y=0.0, z=0.0, w=1.0):
+        self.xyzw = np.array([x, y, z, w], dtype=np.float32)
+
+    @staticmethod
+    def identity():
+        return Quat()
+
+    @staticmethod
+    def angle_axis(angle, axis):
+        """Quat.AngleAxis (math.cs:621-626), evaluated in float32."""
+        ax = np.asarray(axis, dtype=np.float32)
+        h = np.float32(angle) * np.float32(0.5)
+        s = np.float32(math.sin(float(h)))
+        return Quat(*(ax * s).tolist(), float(np.float32(math.cos(float(h)))))
+
+    def __mul__(self, o):  # Quat * Quat (math.cs:603-607)
+        a, b = self.xyzw.astype(np.float64), o.xyzw.astype(np.float64)
+        return Quat(
+            a[3] * b[0] + a[0] * b[3] + a[1] * b[2] - a[2] * b[1],
+            a[3] * b[1] + a[1] * b[3] + a[2] * b[0] - a[0] * b[2],
+            a[3] * b[2] + a[2] * b[3] + a[0] * b[1] - a[1] * b[0],
+            a[3] * b[3] - a[0] * b[0] - a[1] * b[1] - a[2] * b[2],
+        )
+
+
+class Transform:
+    """Transform (math.cs:773-840): position, rotation, scale."""
+
+    def __init__(self, pos=(0, 0, 0), rot=None, scale=(1, 1, 1)):
+        self.pos = np.asarray(pos, dtype=np.float32)
+        self.rot = rot if rot is not None else Quat.identity()
+        self.scale = np.asarray(scale, dtype=np.float32)
+        if np.any(self.scale == 0):
+            raise ValueError("Scale cannot be zero")
+
+    @staticmethod
+    def identity():
+        return Transform()
+
+    def _c(self):
+        t = N.Transform_()
+        t.pos[:] = self.pos.tolist()
+        t.rot[:] = self.rot.xyzw.tolist()
+        t.scale[:] = self.scale.tolist()
+        return t
+
+    def as_array(self):
+        return np.concatenate([self.pos, self.rot.xyzw, self.scale]).astype(np.float32)
+
+
+class View:
+    """View (math.cs:1582-1663): unscaled transform, field of view (radians), near plane."""
+
+    def __init__(self, trans=None, fov=math.radians(60.0), near=0.1):
+        self.trans = trans or Transform()
+        self.fov = float(np.float32(fov))
+        self.near = float(near)
+        if not (0.0 < self.fov < math.pi):
+            raise ValueError("Invalid fov")
+
+    def _c(self):
+        v = N.View_()
+        v.pos[:] = self.trans.pos.tolist()
+        v.rot[:] = self.trans.rot.xyzw.tolist()
+        v.fov = self.fov
+        v.near_plane = self.near
+        return v
+
+    def as_array(self):
+        return np.concatenate([self.trans.pos, self.trans.rot.xyzw, [self.fov, self.near]]).astype(np.float32)
+
+
+class Texture:
+    """Texture (texture.cs:6-179): linear RGB texels (h, w, 3), bilinear or point filter, tiling."""
+
+    BILINEAR, POINT = 0, 1
+
+    def __init__(self, rgb, filter=0, tiling=(1.0, 1.0)):
+        self.rgb = _f32(rgb)
+        if self.rgb.ndim != 3 or self.rgb.shape[2] != 3:
+            raise ValueError("texture must be (h, w, 3)")
+        self.filter = int(filter)
+        self.tiling = (float(tiling[0]), float(tiling[1]))
+
+    @property
+    def width(self):
+        return self.rgb.shape[1]
+
+    @property
+    def height(self):
+        return self.rgb.shape[0]
+
+    @staticmethod
+    def from_hdr(rgbe, max_luminance=None):
+        """Texture.FromHdr (texture.cs:154-163) of RGBE pixels (h, w, 4) uint8 (math.cs:196-202)."""
+        rgbe = np.asarray(rgbe, dtype=np.uint8)
+        e = rgbe[..., 3].astype(np.int32)
+        scale = np.ldexp(np.float32(1.0), e - 128).astype(np.float32) / np.float32(256.0)
+        rgb = rgbe[..., :3].astype(np.float32) * scale[..., None]
+        rgb[e == 0] = 0.0
+        if max_luminance is not None:
+            lum = (np.float32(0.2126) * rgb[..., 0] + np.float32(0.7152) * rgb[..., 1]) + np.float32(0.0722) * rgb[..., 2]
+            over = lum > np.float32(max_luminance)
+            rgb[over] = rgb[over] * (np.float32(max_luminance) / lum[over])[:, None]
+        return Texture(rgb)
+
+
+def triangles_build(pos, normals=None, uv=None, smooth=False):
+    """Triangle records (n, 36) from positions (n, 9), optional normals (n, 9) and uvs (n, 6): the
+    Triangle ctors (math.cs:1503-1531) plus Mesh.ComputeSmoothNormals (mesh.cs:99-123)."""
+    pos = _f32(pos).reshape(-1, 9)
+    n = pos.shape[0]
+    nrm = _f32(normals).reshape(-1, 9) if normals is not None else None
+    uvs = _f32(uv).reshape(-1, 6) if uv is not None else None
+    out = np.empty((n, 36), dtype=np.float32)
+    check(N.lib().sr_triangles_build(fptr(pos), fptr(nrm), fptr(uvs), n, 1 if smooth else 0, fptr(out)),
+          "triangles_build")
+    return out
+
+
+class Mesh:
+    """Mesh (mesh.cs:13-172) from Triangle records (n, 36); the BVH is built by the library."""
+
+    def __init__(self, triangles, split_bins=8):
+        self.triangles = _f32(triangles).reshape(-1, 36)
+        self.split_bins = int(split_bins)
+
+
+class Sphere:
+    """Sphere (math.cs:1171-1234)."""
+
+    def __init__(self, center, radius):
+        if radius <= 0:
+            raise ValueError("Radius must be positive")
+        self.params = np.array(list(center) + [radius, 0, 0], dtype=np.float32)
+
+
+class AABox:
+    """AABox as a shape (math.cs:936-1059)."""
+
+    def __init__(self, mn, mx):
+        self.params = np.array(list(mn) + list(mx), dtype=np.float32)
+
+
+class Material:
+    """Material (scene.cs:161-202)."""
+
+    def __init__(self, albedo, roughness, metallic=0.0, transparency=0.0, ior=1.5, radiance=(0.0, 0.0, 0.0),
+                 color_texture=None, roughness_texture=None, metallic_texture=None, normal_texture=None):
+        self.albedo = np.asarray(albedo if np.ndim(albedo) else (albedo,) * 3, dtype=np.float32)
+        self.roughness = float(roughness)
+        self.metallic = float(metallic)
+        self.transparency = float(transparency)
+        self.ior = float(ior)
+        self.radiance = np.asarray(radiance, dtype=np.float32)
+        self.textures = [color_texture, roughness_texture, metallic_texture, normal_texture]
+
+    def as_array(self):
+        return np.array(list(self.albedo) + [self.roughness, self.metallic, self.transparency, self.ior] +
+                        list(self.radiance), dtype=np.float32)
+
+
+class Object:
+    """Object (scene.cs:204-270): name, transform, material, shape (Mesh, Sphere or AABox)."""
+
+    def __init__(self, name, trans, material, shape):
+        self.name = name
+        self.trans = trans
+        self.material = material
+        self.shape = shape
+
+
+class SkyTexture:
+    """SkyTexture (scene.cs:379-435)."""
+
+    def __init__(self, texture, multiplier=1.0, rot_yaw=0.0):
+        self.texture = texture
+        self.multiplier = float(multiplier)
+        self.rot_yaw = float(rot_yaw)
+
+
+class SkyProcedural:
+    """SkyProcedural (scene.cs:334-377) with its SunProcedural (scene.cs:281-332)."""
+
+    def __init__(self, radiance_top, radiance_middle, radiance_bottom, sun_dir, sun_radiance, sun_angle):
+        self.values = [radiance_top, radiance_middle, radiance_bottom, sun_dir, sun_radiance]
+        self.sun_angle = float(sun_angle)
+
+    def as_array(self):
+        return np.array([c for v in self.values for c in v] + [self.sun_angle], dtype=np.float32)
+
+
+class Fog:
+    """Fog (scene.cs:438-505): exponential height fog."""
+
+    def __init__(self, density, color, anisotropy, height_falloff):
+        self.density = float(density)
+        self.color = np.asarray(color, dtype=np.float32)
+        self.anisotropy = float(anisotropy)
+        self.height_falloff = float(height_falloff)
+
+
+class Scene:
+    """Scene (scene.cs:507-881) backed by one device context."""
+
+    def __init__(self, device=0):
+        self._ctx = ctypes.c_void_p()
+        check(N.lib().sr_create(device, ctypes.byref(self._ctx)), "sr_create")
+        self._built = False
+        self._sky = None
+        self.objects = []
+        self._tex_ids = {}
+        self._mesh_ids = {}
+
+    def __del__(self):
+        ctx = getattr(self, "_ctx", None)
+        if ctx:
+            N.lib().sr_destroy(ctx)
+            self._ctx = None
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def _guard(self):
+        if self._built:
+            raise SharpRayError("Scene already built")
+
+    def texture_id(self, tex):
+        if tex is None:
+            return -1
+        key = id(tex)
+        if key not in self._tex_ids:
+            out = ctypes.c_int32()
+            check(N.lib().sr_texture_create(self._ctx, fptr(tex.rgb), tex.width, tex.height, tex.filter,
+                                            tex.tiling[0], tex.tiling[1], ctypes.byref(out)), "texture")
+            self._tex_ids[key] = (out.value, tex)
+        return self._tex_ids[key][0]
+
+    def mesh_id(self, mesh):
+        key = id(mesh)
+        if key not in self._mesh_ids:
+            out = ctypes.c_int32()
+            tris = mesh.triangles
+            check(N.lib().sr_mesh_create(self._ctx, fptr(tris), tris.shape[0], mesh.split_bins, ctypes.byref(out)),
+                  "mesh")
+            self._mesh_ids[key] = (out.value, mesh)
+        return self._mesh_ids[key][0]
+
+    def set_sky(self, sky):
+        self._guard()
+        if isinstance(sky, SkyTexture):
+            check(N.lib().sr_scene_set_sky_texture(self._ctx, self.texture_id(sky.texture), sky.multiplier,
+                                                   sky.rot_yaw), "set_sky")
+        elif isinstance(sky, SkyProcedural):
+            p = N.SkyProcedural_()
+            arr = sky.as_array()
+            for i, name in enumerate(["radiance_top", "radiance_middle", "radiance_bottom", "sun_dir", "sun_radiance"]):
+                getattr(p, name)[:] = arr[i * 3:i * 3 + 3].tolist()
+            p.sun_angle = sky.sun_angle
+            check(N.lib().sr_scene_set_sky_procedural(self._ctx, ctypes.byref(p)), "set_sky")
+        else:
+            raise TypeError("sky must be SkyTexture or SkyProcedural")
+        self._sky = sky
+
+    def set_fog(self, fog):
+        self._guard()
+        f = N.Fog_()
+        f.density = fog.density
+        f.color[:] = fog.color.tolist()
+        f.anisotropy = fog.anisotropy
+        f.height_falloff = fog.height_falloff
+        check(N.lib().sr_scene_set_fog(self._ctx, ctypes.byref(f)), "set_fog")
+        self.fog = fog
+
+    def add_object(self, obj):
+        self._guard()
+        m = obj.material
+        cm = N.Material_()
+        cm.albedo[:] = m.albedo.tolist()
+        cm.roughness, cm.metallic, cm.transparency, cm.ior = m.roughness, m.metallic, m.transparency, m.ior
+        cm.radiance[:] = m.radiance.tolist()
+        ids = [self.texture_id(t) for t in m.textures]
+        cm.color_texture, cm.roughness_texture, cm.metallic_texture, cm.normal_texture = ids
+        params = None
+        if isinstance(obj.shape, Mesh):
+            kind, mesh = N.SR_SHAPE_MESH, self.mesh_id(obj.shape)
+        elif isinstance(obj.shape, Sphere):
+            kind, mesh, params = N.SR_SHAPE_SPHERE, -1, obj.shape.params
+        elif isinstance(obj.shape, AABox):
+            kind, mesh, params = N.SR_SHAPE_BOX, -1, obj.shape.params
+        else:
+            raise TypeError("unsupported shape")
+        tr = obj.trans._c()
+        out = ctypes.c_int32()
+        check(N.lib().sr_scene_add_object(self._ctx, ctypes.byref(tr), ctypes.byref(cm), kind, mesh, fptr(params),
+                                          ctypes.byref(out)), "add_object")
+        self.objects.append(obj)
+        return out.value
+
+    def build(self):
+        """Scene.Build (scene.cs:517-541); idempotent."""
+        check(N.lib().sr_scene_build(self._ctx), "build")
+        self._built = True
+
+    def bvh(self, mesh=None):
+        """(nodes (n, 8) float32 with child/count as int32 bit patterns, items) of a mesh or the scene."""
+        if mesh is None:
+            mid = -1
+        elif isinstance(mesh, int):
+            mid = mesh
+        else:
+            mid = self.mesh_id(mesh)
+        nc, ic = ctypes.c_uint32(), ctypes.c_uint32()
+        check(N.lib().sr_bvh_export(self._ctx, mid, None, None, ctypes.byref(nc), ctypes.byref(ic)), "bvh")
+        nodes = np.empty((nc.value, 8), dtype=np.float32)
+        items = np.empty(ic.value, dtype=np.int32)
+        check(N.lib().sr_bvh_export(self._ctx, mid, fptr(nodes), iptr(items), None, None), "bvh")
+        return nodes, items
+
+    def trace_primary(self, view, width, height):
+        """Sample-0 primary hits: (object, triangle, dist) arrays of shape (height, width)."""
+        obj = np.empty(width * height, dtype=np.int32)
+        tri = np.empty(width * height, dtype=np.int32)
+        dist = np.empty(width * height, dtype=np.float32)
+        v = view._c()
+        check(N.lib().sr_trace_primary(self._ctx, ctypes.byref(v), width, height, iptr(obj), iptr(tri), fptr(dist)),
+              "trace_primary")
+        return obj.reshape(height, width), tri.reshape(height, width), dist.reshape(height, width)
+
+
+class Renderer:
+    """Renderer (renderer.cs:5-238). Construction starts the render asynchronously on the GPU."""
+
+    def __init__(self, scene, view, width, height, block_size, min_samples, max_samples, variance_threshold, bounces,
+                 indirect_clamp, rng_policy=RNG_REFERENCE, rng_seed=0, tick_bands=1, counters=False,
+                 kernel_timing=False):
+        if width <= 0 or height <= 0 or block_size <= 0:
+            raise ValueError("size must be positive")
+        if not (0 < min_samples <= max_samples):
+            raise ValueError("need 0 < minSamples <= maxSamples")
+        self.scene = scene
+        self.view = view
+        self.width, self.height = int(width), int(height)
+        p = N.RenderParams_()
+        p.width, p.height, p.block_size = self.width, self.height, int(block_size)
+        p.min_samples, p.max_samples = int(min_samples), int(max_samples)
+        p.variance_threshold = float(variance_threshold)
+        p.bounces, p.indirect_clamp = int(bounces), float(indirect_clamp)
+        p.rng_policy, p.rng_seed, p.tick_bands = int(rng_policy), int(rng_seed), int(tick_bands)
+        self.params = p
+        check(N.lib().sr_counters_enable(scene.ctx, 1 if counters else 0), "counters")
+        check(N.lib().sr_profile_enable(scene.ctx, 1 if kernel_timing else 0), "kernel timing")
+        v = view._c()
+        check(N.lib().sr_render_begin(scene.ctx, ctypes.byref(v), ctypes.byref(p)), "render_begin")
+        scene._built = True
+        self._aovs = None
+
+    def tick(self):
+        """Renderer.Tick (renderer.cs:98-105): (completed blocks, total blocks)."""
+        s, t = ctypes.c_uint32(), ctypes.c_uint32()
+        check(N.lib().sr_render_tick(self.scene.ctx, ctypes.byref(s), ctypes.byref(t)), "tick")
+        return s.value, t.value
+
+    def wait(self):
+        while True:
+            s, t = self.tick()
+            if s == t:
+                return
+
+    def _read(self):
+        if self._aovs is None:
+            n = self.width * self.height
+            a = dict(
+                radiance=np.empty((n, 3), np.float32), radiance_fog=np.empty((n, 3), np.float32),
+                normals=np.empty((n, 3), np.float32), uv=np.empty((n, 2), np.float32),
+                depth=np.empty(n, np.float32), samples=np.empty(n, np.uint32), variance=np.empty(n, np.float32))
+            check(N.lib().sr_render_read(self.scene.ctx, fptr(a["radiance"]), fptr(a["radiance_fog"]),
+                                         fptr(a["normals"]), fptr(a["uv"]), fptr(a["depth"]), uptr(a["samples"]),
+                                         fptr(a["variance"])), "render_read")
+            self._aovs = a
+        return self._aovs
+
+    radiance = property(lambda self: self._read()["radiance"])
+    radiance_fog = property(lambda self: self._read()["radiance_fog"])
+    normals = property(lambda self: self._read()["normals"])
+    uv = property(lambda self: self._read()["uv"])
+    depth = property(lambda self: self._read()["depth"])
+    samples = property(lambda self: self._read()["samples"])
+    variance = property(lambda self: self._read()["variance"])
+
+    def gpu_seconds(self):
+        out = ctypes.c_double()
+        check(N.lib().sr_render_gpu_seconds(self.scene.ctx, ctypes.byref(out)), "gpu_seconds")
+        return out.value
+
+    def kernel_times(self):
+        """{kind: (total ms, launches)} for extend, shade, shadow and other kernels of this render."""
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_uint32 * 4)()
+        check(N.lib().sr_profile_read(self.scene.ctx, ms, n), "kernel times")
+        return {k: (ms[i], n[i]) for i, k in enumerate(["extend", "shade", "shadow", "other"])}
+
+    def counters(self):
+        out = (ctypes.c_int64 * N.SR_COUNTER_COUNT)()
+        check(N.lib().sr_counters_read(self.scene.ctx, out), "counters")
+        return dict(zip(N.COUNTER_NAMES, list(out)))
+
+
+class Compositor:
+    """Compositor (compose.cs:7-267): the joint bilateral denoise of Filter (compose.cs:101-204)."""
+
+    def __init__(self, denoise_radius, denoise_strength, denoise_strength_max, denoise_luminance_boost,
+                 denoise_luminance_limit, denoise_normal_limit, denoise_depth_limit):
+        p = N.DenoiseParams_()
+        p.radius, p.strength, p.strength_max = denoise_radius, denoise_strength, denoise_strength_max
+        p.luminance_boost, p.luminance_limit = denoise_luminance_boost, denoise_luminance_limit
+        p.normal_limit, p.depth_limit = denoise_normal_limit, denoise_depth_limit
+        self.params = p
+
+    def as_array(self):
+        p = self.params
+        return np.array([p.radius, p.strength, p.strength_max, p.luminance_boost, p.luminance_limit, p.normal_limit,
+                         p.depth_limit], dtype=np.float32)
+
+    def filter(self, renderer):
+        out = np.empty((renderer.width * renderer.height, 3), np.float32)
+        check(N.lib().sr_denoise_bilateral(renderer.scene.ctx, ctypes.byref(self.params), fptr(out)), "denoise")
+        return out

@@ -1,0 +1,42 @@
+// Host-side scene preparation for the gfx950 tracer: binned-SAH BVH construction in the
+// reference's node order, Triangle construction, and tabulated CDFs for environment sampling.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "sr_math.h"
+
+namespace sr {
+
+// One BVH node exactly as bvh.cs:26-31 lays it out (AABox 24 B, Child, ShapeCount): 32 bytes.
+struct RefNode {
+    box3 bounds;
+    int32_t child;
+    int32_t count;
+};
+static_assert(sizeof(RefNode) == 32, "RefNode must match the reference's 32-byte node");
+
+struct BvhResult {
+    std::vector<RefNode> nodes;  // nodes[0..used)
+    std::vector<int32_t> items;  // item permutation (bvh.cs:53)
+    int used = 0;
+    bool degenerate = false;     // a split left one side empty
+    int max_depth = 0;
+};
+
+// Builds the BVH of bvh.cs:58-75/285-476 over `n` primitive bounds: same split candidates, costs,
+// partition and depth-first pair allocation, so nodes and items match the reference bit for bit.
+BvhResult build_bvh(const box3* bounds, int n, int bins, float cost_traverse, float cost_intersect);
+
+// Triangle ctors (math.cs:1503-1531) and Mesh.ComputeSmoothNormals (mesh.cs:99-123).
+void build_triangles(const float* pos, const float* nrm, const float* uv, uint32_t n, bool smooth, float* out36);
+
+// Bounds of a TriangleLean given as (A, B-A, C-A) (math.cs:1353-1359).
+box3 lean_bounds(const float* t9);
+
+// Cdf1 (cdf.cs:7-57): prefix sums normalised by the total; returns the total weight.
+float build_cdf1(const float* w, int n, float* cdf_out /* n+1 */);
+
+}  // namespace sr

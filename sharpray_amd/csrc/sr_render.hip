@@ -1,0 +1,1710 @@
+// gfx950 wavefront path tracer behind the sharpray.h C ABI.
+//
+// One camera path per pixel is in flight. A frame is rendered sample-major: every sample round
+// runs raygen -> (extend -> shade -> shadow) x (bounces + 1) -> accumulate over a band of pixels.
+// Paths that terminate are compacted out of the ray queue by shade (wave-aggregated atomic
+// appends), so extend and shadow only ever see live rays. The per-pixel RNG state persists in HBM
+// between rounds, which keeps the reference's sequential per-pixel xorwow stream intact
+// (renderer.cs:156-212), and the per-pixel sums are updated in sample order.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sharpray.h"
+#include "sr_device.h"
+#include "sr_host.h"
+
+using namespace sr;
+
+namespace {
+
+constexpr float kInf = __builtin_huge_valf();
+constexpr int kBlock = 256;
+constexpr uint32_t kMaxChild = (1u << 24) - 1;
+constexpr int kStack = 64;  // bvh.cs:436 caps depth at 64, so at most 64 deferred siblings
+
+// ---- Frame state --------------------------------------------------------------------------------
+struct alignas(16) PathRec {  // 128 B per path
+    vec3 o, d;
+    vec3 energy;
+    vec3 rad, radFog;
+    float scatterPdf;
+    vec3 medium;
+    uint32_t flags;  // bit0: scatterPdf valid, bit1: inside a medium
+    uint32_t rng[5];
+    uint32_t pad[6];
+};
+struct alignas(16) HitRec {  // 32 B
+    float t, u, v;
+    int32_t obj, slot;
+    int32_t pad[3];
+};
+struct alignas(16) ShadowRec {  // 80 B
+    vec3 o, d;
+    vec3 skyRad, f;
+    float w, pdf;
+    vec3 energy;
+    uint32_t path;  // bit31: add to radianceFog
+};
+struct alignas(16) PixAcc {  // 96 B
+    vec3 radSum, radFogSum;
+    float lumSum, lumSumSqr, lumSurf, lumSurfSqr;
+    vec3 normalSum, normalFallback;
+    uint32_t normalCount;
+    vec2 uv;
+    uint32_t hasUv;
+    float depthSum;
+    uint32_t depthCount;
+    uint32_t sampleIndex;
+    uint32_t active;
+};
+
+struct ViewD {
+    vec3 pos;
+    quat rot;
+    float nearPlane, tanFov, aspect;
+};
+
+struct Frame {
+    PathRec* paths;
+    HitRec* hits;
+    ShadowRec* shadows;
+    PixAcc* acc;
+    uint32_t* queue[2];
+    uint32_t* shadowQueue;
+    uint32_t* counts;  // [0],[1] ray queues, [2] shadow queue
+    unsigned long long* ctr;  // SR_COUNTER_COUNT counters or null
+    float* outRad;
+    float* outRadFog;
+    float* outNormal;
+    float* outUv;
+    float* outDepth;
+    uint32_t* outSamples;
+    float* outVar;
+    uint32_t W, H, y0, rows, tilesX, nSlots;
+    uint32_t minS, maxS, bounces, seed;
+    float threshold, clamp;
+    ViewD view;
+};
+
+// Slot q of a band maps to an 8x8 pixel tile, so a wave's 64 camera rays start coherent.
+SR_DEV bool slot_pixel(const Frame& F, uint32_t q, uint32_t& x, uint32_t& y) {
+    const uint32_t tile = q >> 6, in = q & 63u;
+    x = (tile % F.tilesX) * 8u + (in & 7u);
+    y = F.y0 + (tile / F.tilesX) * 8u + (in >> 3);
+    return x < F.W && y < F.y0 + F.rows;
+}
+
+SR_DEV uint32_t lane_id() { return __lane_id(); }
+
+// Appends `pred` lanes to a queue with one atomic per wave; returns this lane's slot.
+SR_DEV uint32_t wave_append(uint32_t* count, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0) return 0;
+    const int leader = __ffsll(mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane_id()) - 1ull));
+}
+
+template <class R> SR_DEV R rng_load(const PathRec& p);
+template <> SR_DEV XorwowRng rng_load<XorwowRng>(const PathRec& p) {
+    XorwowRng r;
+    r.s0 = p.rng[0]; r.s1 = p.rng[1]; r.s2 = p.rng[2]; r.s3 = p.rng[3]; r.s4 = p.rng[4];
+    return r;
+}
+template <> SR_DEV CounterRng rng_load<CounterRng>(const PathRec& p) {
+    CounterRng r;
+    r.key0 = p.rng[0]; r.key1 = p.rng[1]; r.sample = p.rng[2]; r.draw = p.rng[3];
+    return r;
+}
+SR_DEV void rng_store(PathRec* p, const XorwowRng& r) {
+    p->rng[0] = r.s0; p->rng[1] = r.s1; p->rng[2] = r.s2; p->rng[3] = r.s3; p->rng[4] = r.s4;
+}
+SR_DEV void rng_store(PathRec* p, const CounterRng& r) {
+    p->rng[0] = r.key0; p->rng[1] = r.key1; p->rng[2] = r.sample; p->rng[3] = r.draw;
+}
+
+// ---- Traversal ----------------------------------------------------------------------------------
+struct Tally {
+    uint32_t box, shape, mesh;
+};
+
+SR_DEV uint32_t pack_node(float childBits, float countBits) {
+    return ((uint32_t)__float_as_int(countBits) << 24) | (uint32_t)__float_as_int(childBits);
+}
+
+// Closest hit in one mesh BVH (bvh.cs:165-216 + mesh.cs:51-62). Nearer child first, ties to
+// child B; prune when the entry distance is >= the best hit; accept strictly closer hits.
+template <bool COUNT>
+SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, float& bestT,
+                         float& bu, float& bv, int32_t& bslot, Tally& tl) {
+    const float4* N = S.nodes;
+    uint32_t stk[kStack];
+    float stt[kStack];
+    int sp = 0;
+    bestT = kInf;
+    bool found = false;
+    const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    float curT = 0.0f;
+    for (;;) {
+        bool next = false;
+        if (curT < bestT) {
+            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+            if (cnt == 0) {
+                if (COUNT) tl.box += 2;
+                const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)child) * 2;
+                const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+                float tA = 0.0f, tB = 0.0f;
+                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= bestT;
+                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= bestT;
+                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+                if (hA && hB) {
+                    const bool aNear = tA < tB;
+                    stk[sp] = aNear ? pB : pA;
+                    stt[sp] = aNear ? tB : tA;
+                    ++sp;
+                    cur = aNear ? pA : pB;
+                    curT = aNear ? tA : tB;
+                    next = true;
+                } else if (hA) {
+                    cur = pA; curT = tA; next = true;
+                } else if (hB) {
+                    cur = pB; curT = tB; next = true;
+                }
+            } else {
+                if (COUNT) tl.shape += cnt;
+                const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
+                for (uint32_t i = 0; i < cnt; ++i, T += 9) {
+                    float t, u, v;
+                    if (tri_hit(T, r, t, u, v) && t < bestT) {
+                        bestT = t; bu = u; bv = v;
+                        bslot = (int32_t)(child + i);
+                        found = true;
+                    }
+                }
+            }
+        }
+        if (!next) {
+            if (sp == 0) break;
+            --sp;
+            cur = stk[sp];
+            curT = stt[sp];
+        }
+    }
+    return found;
+}
+
+// Whether any primitive of a mesh is hit (the same node visits as the closest-hit traversal up
+// to its first accepted hit, which is where the existence of a hit is decided).
+SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r) {
+    const float4* N = S.nodes;
+    uint32_t stk[kStack];
+    int sp = 0;
+    const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    for (;;) {
+        bool next = false;
+        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+        if (cnt == 0) {
+            const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)child) * 2;
+            const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+            float tA = 0.0f, tB = 0.0f;
+            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
+            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
+            const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+            if (hA && hB) {
+                const bool aNear = tA < tB;
+                stk[sp++] = aNear ? pB : pA;
+                cur = aNear ? pA : pB;
+                next = true;
+            } else if (hA) {
+                cur = pA; next = true;
+            } else if (hB) {
+                cur = pB; next = true;
+            }
+        } else {
+            const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
+            for (uint32_t i = 0; i < cnt; ++i, T += 9) {
+                float t, u, v;
+                if (tri_hit(T, r, t, u, v)) return true;
+            }
+        }
+        if (!next) {
+            if (sp == 0) return false;
+            cur = stk[--sp];
+        }
+    }
+}
+
+// Object.Intersect (scene.cs:227 -> math.cs:67-79): distance in world units only; the hit
+// record keeps (u, v, slot) so shading can rebuild the full ShapeHit.
+template <bool COUNT>
+SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r, float& dist, float& u, float& v,
+                           int32_t& slot, Tally& tl) {
+    float scale;
+    const ray3 lr = to_object(ob, r, scale);
+    float t = 0.0f;
+    slot = -1;
+    u = v = 0.0f;
+    bool hit;
+    if (ob.kind == 0) {
+        if (COUNT) tl.mesh += 1;
+        hit = ob.nodeBase >= 0 && mesh_closest<COUNT>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, tl);
+    } else {
+        ShapeHitD h;
+        hit = ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
+        t = h.dist;
+    }
+    if (!hit) return false;
+    dist = t / scale;
+    return true;
+}
+
+SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r) {
+    float scale;
+    const ray3 lr = to_object(ob, r, scale);
+    if (ob.kind == 0) return ob.nodeBase >= 0 && mesh_any(S, ob.nodeBase, ob.triBase, lr);
+    ShapeHitD h;
+    return ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
+}
+
+// Scene BVH over objects (costI 10, 32 bins; scene.cs:528-533), two-level closest hit.
+template <bool COUNT>
+SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& tl) {
+    if (S.sceneNodes == 0) return false;
+    const float4* N = S.nodes;
+    uint32_t stk[kStack];
+    float stt[kStack];
+    int sp = 0;
+    float best = kInf;
+    bool found = false;
+    const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    float curT = 0.0f;
+    for (;;) {
+        bool next = false;
+        if (curT < best) {
+            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+            if (cnt == 0) {
+                if (COUNT) tl.box += 2;
+                const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)child) * 2;
+                const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+                float tA = 0.0f, tB = 0.0f;
+                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= best;
+                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= best;
+                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+                if (hA && hB) {
+                    const bool aNear = tA < tB;
+                    stk[sp] = aNear ? pB : pA;
+                    stt[sp] = aNear ? tB : tA;
+                    ++sp;
+                    cur = aNear ? pA : pB;
+                    curT = aNear ? tA : tB;
+                    next = true;
+                } else if (hA) {
+                    cur = pA; curT = tA; next = true;
+                } else if (hB) {
+                    cur = pB; curT = tB; next = true;
+                }
+            } else {
+                if (COUNT) tl.shape += cnt;
+                for (uint32_t i = 0; i < cnt; ++i) {
+                    const int32_t oi = S.sceneItems[child + i];
+                    float d, u, v;
+                    int32_t slot;
+                    if (object_closest<COUNT>(S, S.objects[oi], r, d, u, v, slot, tl) && d < best) {
+                        best = d;
+                        out.t = d; out.u = u; out.v = v; out.obj = oi; out.slot = slot;
+                        found = true;
+                    }
+                }
+            }
+        }
+        if (!next) {
+            if (sp == 0) break;
+            --sp;
+            cur = stk[sp];
+            curT = stt[sp];
+        }
+    }
+    return found;
+}
+
+SR_DEV bool scene_any(const DevScene& S, const ray3& r) {
+    if (S.sceneNodes == 0) return false;
+    const float4* N = S.nodes;
+    uint32_t stk[kStack];
+    int sp = 0;
+    const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    for (;;) {
+        bool next = false;
+        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+        if (cnt == 0) {
+            const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)child) * 2;
+            const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+            float tA = 0.0f, tB = 0.0f;
+            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
+            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
+            const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+            if (hA && hB) {
+                const bool aNear = tA < tB;
+                stk[sp++] = aNear ? pB : pA;
+                cur = aNear ? pA : pB;
+                next = true;
+            } else if (hA) {
+                cur = pA; next = true;
+            } else if (hB) {
+                cur = pB; next = true;
+            }
+        } else {
+            for (uint32_t i = 0; i < cnt; ++i)
+                if (object_any(S, S.objects[S.sceneItems[child + i]], r)) return true;
+        }
+        if (!next) {
+            if (sp == 0) return false;
+            cur = stk[--sp];
+        }
+    }
+}
+
+// Scene.Trace (scene.cs:602-625) from a hit record: world-space ShapeHit, material sampling,
+// normal mapping and tangent re-orthogonalisation.
+SR_DEV void make_surface(const DevScene& S, const ray3& r, const HitRec& h, SurfaceD& s) {
+    const DevObject& ob = S.objects[h.obj];
+    ShapeHitD sh;
+    if (ob.kind == 0) {
+        const int32_t tri = S.slotTri[ob.triBase + h.slot];
+        inflate(S.tris + (size_t)(ob.triBase + h.slot) * 9, S.attrs + (size_t)(ob.triBase + tri) * 27, h.u, h.v, sh);
+    } else {
+        float scale;
+        const ray3 lr = to_object(ob, r, scale);
+        if (ob.kind == 1) sphere_hit(ob.shape, lr, sh, true);
+        else box_hit(ob.shape, lr, sh, true);
+    }
+    const vec3 wn = normalized(rotate(ob.rot, sh.norm / ob.scale));
+    const vec3 wng = normalized(rotate(ob.rot, sh.normGeo / ob.scale));
+    const vec3 wt = normalized(rotate(ob.rot, xyz(sh.tan)));
+    const vec2 uv = sh.uv;
+    const vec3 texC = ob.texColor >= 0 ? tex_sample(S, ob.texColor, uv) : splat3(1.0f);
+    s.albedo = texC * ob.albedo;
+    s.roughness = (ob.texRough >= 0 ? tex_sample(S, ob.texRough, uv).x : 1.0f) * ob.roughness;
+    s.metallic = (ob.texMetal >= 0 ? tex_sample(S, ob.texMetal, uv).x : 1.0f) * ob.metallic;
+    const vec4 htan = mk4(wt.x, wt.y, wt.z, sh.tan.w);
+    vec3 n = wn;
+    if (ob.texNormal >= 0) {
+        const vec3 c = tex_sample(S, ob.texNormal, uv);
+        const vec3 tn = normalized_or(mk3(c.x * 2.0f - 1.0f, -(c.y * 2.0f - 1.0f), c.z * 2.0f - 1.0f), up3());
+        n = normalized_or(from_tangent(tn, htan, wn), wn);
+    }
+    const vec3 td = normalized_or(wt - dot3(wt, n) * n, wt);
+    s.radiance = ob.radiance;
+    s.transparency = ob.transparency;
+    s.ior = ob.ior;
+    s.f0 = ob.f0;
+    s.normal = n;
+    s.normalGeo = wng;
+    s.tangent = mk4(td.x, td.y, td.z, sh.tan.w);
+    s.uv = uv;
+}
+
+template <bool COUNT>
+SR_DEV void flush_tally(const Frame& F, const Tally& tl) {
+    if (!COUNT || F.ctr == nullptr) return;
+    if (tl.box) atomicAdd(&F.ctr[13], (unsigned long long)tl.box);
+    if (tl.shape) atomicAdd(&F.ctr[14], (unsigned long long)tl.shape);
+    if (tl.mesh) atomicAdd(&F.ctr[12], (unsigned long long)tl.mesh);
+}
+SR_DEV void bump(const Frame& F, int idx, uint32_t n) {
+    if (F.ctr != nullptr && n) atomicAdd(&F.ctr[idx], (unsigned long long)n);
+}
+
+// ---- Kernels ------------------------------------------------------------------------------------
+__global__ void k_init(Frame F) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) { F.counts[0] = 0; F.counts[1] = 0; F.counts[2] = 0; }
+    if (q >= F.nSlots) return;
+    uint32_t x, y;
+    PixAcc a;
+    memset(&a, 0, sizeof(a));
+    a.active = slot_pixel(F, q, x, y) ? 1u : 0u;
+    F.acc[q] = a;
+    if (a.active) {
+        const XorwowRng r = xorwow_seed(x, y);  // new Rng(x, y), renderer.cs:156
+        rng_store(&F.paths[q], r);
+        if (F.ctr) atomicAdd(&F.ctr[0], 1ull);
+    }
+}
+
+// Camera ray of sample `s` for every active pixel: jitter (2 draws), View.Ray (math.cs:1613-1636).
+template <class R>
+__global__ void k_raygen(Frame F) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    bool live = false;
+    if (q < F.nSlots && F.acc[q].active) {
+        uint32_t x, y;
+        slot_pixel(F, q, x, y);
+        PathRec* p = &F.paths[q];
+        R rng;
+        if constexpr (__is_same(R, CounterRng)) {
+            rng.key0 = F.seed;
+            rng.key1 = y * F.W + x;
+            rng.sample = F.acc[q].sampleIndex;
+            rng.draw = 0;
+        } else {
+            rng = rng_load<R>(*p);
+        }
+        const float jx = next_float(rng);
+        const float jy = next_float(rng);
+        const vec2 sp = mk2(((float)x + jx) / (float)F.W, ((float)y + jy) / (float)F.H);
+        const float ndcX = sp.x * 2.0f - 1.0f;
+        const float ndcY = -sp.y * 2.0f + 1.0f;
+        float th, tv;
+        if (F.view.aspect >= 1.0f) { th = F.view.tanFov; tv = th / F.view.aspect; }
+        else { tv = F.view.tanFov; th = tv * F.view.aspect; }
+        const vec3 local = mk3(ndcX * th, ndcY * tv, 1.0f);
+        const vec3 origin = F.view.pos + rotate(F.view.rot, mk3(0.0f, 0.0f, F.view.nearPlane));
+        const vec3 dir = rotate(F.view.rot, normalized(local));
+        p->o = origin;
+        p->d = dir;
+        p->energy = splat3(1.0f);
+        p->rad = splat3(0.0f);
+        p->radFog = splat3(0.0f);
+        p->scatterPdf = 0.0f;
+        p->flags = 0;
+        rng_store(p, rng);
+        live = true;
+    }
+    const uint32_t slot = wave_append(&F.counts[0], live);
+    if (live) {
+        F.queue[0][slot] = q;
+        if (F.ctr) atomicAdd(&F.ctr[1], 1ull);
+    }
+}
+
+// Closest hit for every queued ray.
+template <bool COUNT>
+__global__ void __launch_bounds__(kBlock) k_extend(Frame F, DevScene S, int cur) {
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gtid == 0) { F.counts[cur ^ 1] = 0; F.counts[2] = 0; }
+    const uint32_t n = F.counts[cur];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t* Q = F.queue[cur];
+    if (COUNT && gtid == 0 && F.ctr) {
+        atomicAdd(&F.ctr[2], (unsigned long long)n);
+        atomicAdd(&F.ctr[10], (unsigned long long)n);
+    }
+    Tally tl = {0, 0, 0};
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const uint32_t q = Q[i];
+        const PathRec* p = &F.paths[q];
+        const ray3 r = mk_ray(p->o, p->d);
+        HitRec h;
+        h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
+        scene_closest<COUNT>(S, r, h, tl);
+        F.hits[q] = h;
+    }
+    flush_tally<COUNT>(F, tl);
+}
+
+// One path segment (scene.cs:641-737): fog scatter, surface hit or sky miss. Direct sky light is
+// resolved as a deferred shadow ray whose contribution the shadow kernel adds before the next
+// segment, which keeps the reference's summation order.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur, uint32_t seg) {
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = F.counts[cur];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t* Q = F.queue[cur];
+    const bool primary = seg == 0;
+    const bool last = seg == F.bounces;
+    uint32_t cHit = 0, cMiss = 0, cTerm = 0, cFog = 0, cEsc = 0, cOcc = 0;
+    // Loop until every lane of the wave is done so the wave-level appends stay convergent.
+    const uint32_t iters = (n + stride - 1) / stride;
+    for (uint32_t it = 0; it < iters; ++it) {
+        const uint32_t i = gtid + it * stride;
+        const bool valid = i < n;
+        bool cont = false, shadow = false;
+        ShadowRec sr;
+        uint32_t q = 0;
+        if (valid) {
+            q = Q[i];
+            PathRec p = F.paths[q];
+            const HitRec h = F.hits[q];
+            R rng = rng_load<R>(p);
+            const ray3 ray = mk_ray(p.o, p.d);
+            const bool hit = h.obj >= 0;
+            float sd = 0.0f;
+            if (S.fog.enabled && fog_scatter(S.fog, ray, 0.0f, hit ? h.t : kInf, rng, sd)) {
+                ++cFog;
+                p.energy = p.energy * S.fog.color;
+                const vec3 pos = at(ray, sd);
+                vec3 ld;
+                float lpdf;
+                if (sky_light_rand(S, rng, ld, lpdf) && lpdf > 0.0f) {
+                    const float fogPdf = hg_pdf(dot3(ray.d, ld), S.fog.anisotropy);
+                    sr.o = pos; sr.d = ld;
+                    sr.skyRad = sky_radiance(S, ld);
+                    sr.f = splat3(fogPdf);
+                    sr.w = power_heuristic(lpdf, fogPdf);
+                    sr.pdf = lpdf;
+                    sr.energy = p.energy;
+                    sr.path = q | (primary ? 0x80000000u : 0u);
+                    shadow = true;
+                }
+                bool term = false;
+                if (seg >= 3) {
+                    const float survive = net_min(1.0f, max_comp(p.energy));
+                    if (next_float(rng) >= survive) { term = true; ++cTerm; }
+                    else p.energy = p.energy / survive;
+                }
+                if (!term) {
+                    const vec3 nd = hg_scatter(ray.d, S.fog.anisotropy, rng);
+                    p.flags |= 1u;
+                    p.scatterPdf = hg_pdf(dot3(ray.d, nd), S.fog.anisotropy);
+                    p.o = pos;
+                    p.d = nd;
+                    cont = !last;
+                }
+            } else if (hit) {
+                ++cHit;
+                if (S.fog.enabled) ++cEsc;
+                SurfaceD s;
+                make_surface(S, ray, h, s);
+                const float dist = h.t;
+                const vec3 view = -ray.d;
+                const float bias = net_max(1e-4f, dist * 1e-4f);
+                const vec3 hitPos = at(ray, dist - bias);
+                if (p.flags & 2u) p.energy = p.energy * beer(p.medium, dist);
+                p.rad = p.rad + s.radiance * p.energy;
+                if (primary) {
+                    PixAcc* a = &F.acc[q];
+                    a->normalSum = a->normalSum + s.normal;
+                    a->normalFallback = s.normal;
+                    a->normalCount += 1;
+                    a->depthSum += dist;
+                    a->depthCount += 1;
+                    if (!a->hasUv) { a->hasUv = 1; a->uv = s.uv; }
+                }
+                // SampleSkyDirect (scene.cs:774-794): the sides are tested before the shadow ray,
+                // which only skips rays whose contribution is black either way.
+                vec3 ld;
+                float lpdf;
+                if (sky_light_rand(S, rng, ld, lpdf) && lpdf > 0.0f) {
+                    const vec3 nn = dot3(s.normal, view) >= 0.0f ? s.normal : -s.normal;
+                    const vec3 ng = dot3(s.normalGeo, view) >= 0.0f ? s.normalGeo : -s.normalGeo;
+                    if (dot3(nn, ld) > 0.0f && dot3(ng, ld) > 0.0f) {
+                        sr.o = hitPos; sr.d = ld;
+                        sr.skyRad = sky_radiance(S, ld);
+                        sr.f = opaque_eval(s, view, ld);
+                        sr.w = power_heuristic(lpdf, opaque_pdf(s, view, ld, spec_prob(s, view)));
+                        sr.pdf = lpdf;
+                        sr.energy = p.energy;
+                        sr.path = q;
+                        shadow = true;
+                    }
+                }
+                bool term = false;
+                if (seg >= 3) {
+                    const float survive = net_min(1.0f, max_comp(p.energy));
+                    if (next_float(rng) >= survive) { term = true; ++cTerm; }
+                    else p.energy = p.energy / survive;
+                }
+                if (!term) {
+                    const ScatterD sc = surface_scatter(s, ray.d, view, rng);
+                    p.energy = p.energy * sc.weight;
+                    p.energy = clamp_lum(p.energy, F.clamp);
+                    const bool enter = dot3(sc.dir, s.normalGeo) < 0.0f;
+                    if (sc.refracted) {
+                        if (enter) { p.flags |= 2u; p.medium = s.albedo; }
+                        else p.flags &= ~2u;
+                    } else if (!enter) {
+                        p.flags &= ~2u;
+                    }
+                    if (sc.hasPdf) { p.flags |= 1u; p.scatterPdf = sc.pdf; }
+                    else p.flags &= ~1u;
+                    p.o = at(ray, dist) + sc.dir * bias;
+                    p.d = sc.dir;
+                    cont = !last;
+                }
+            } else {
+                ++cMiss;
+                if (S.fog.enabled) ++cEsc;
+                float mis = 1.0f;
+                if (p.flags & 1u) {
+                    float lp = 0.0f;
+                    if (!sky_light_pdf(S, ray.d, lp)) lp = 0.0f;
+                    mis = power_heuristic(p.scatterPdf, lp);
+                }
+                p.rad = p.rad + sky_radiance(S, ray.d) * mis * p.energy;
+            }
+            rng_store(&p, rng);
+            F.paths[q] = p;
+        }
+        if (shadow) ++cOcc;
+        const uint32_t ns = wave_append(&F.counts[2], shadow);
+        if (shadow) F.shadows[ns] = sr;
+        const uint32_t nc = wave_append(&F.counts[cur ^ 1], cont);
+        if (cont) F.queue[cur ^ 1][nc] = q;
+    }
+    if (F.ctr) {
+        bump(F, 3, cHit); bump(F, 4, cMiss); bump(F, 5, cTerm); bump(F, 6, cFog); bump(F, 7, cEsc);
+        bump(F, 11, cOcc);
+    }
+}
+
+// ShadowTransmittance (scene.cs:576-600) and the deferred direct-light contribution.
+__global__ void __launch_bounds__(kBlock) k_shadow(Frame F, DevScene S) {
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = F.counts[2];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t extra = 0;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const ShadowRec sr = F.shadows[i];
+        ray3 ray = mk_ray(sr.o, sr.d);
+        vec3 tr = splat3(1.0f);
+        if (!S.anyTransparent) {
+            if (scene_any(S, ray)) tr = splat3(0.0f);
+        } else {
+            Tally tl = {0, 0, 0};
+            for (;;) {
+                HitRec h;
+                h.obj = -1;
+                if (!scene_closest<false>(S, ray, h, tl)) break;
+                const DevObject& ob = S.objects[h.obj];
+                if (ob.transparency <= 0.0f) { tr = splat3(0.0f); break; }
+                SurfaceD s;
+                make_surface(S, ray, h, s);
+                const float nDotV = fabsf(dot3(s.normalGeo, ray.d));
+                const vec3 fr = fresnel3(nDotV, base_refl(ob.f0, s.albedo, ob.metallic));
+                tr = tr * (s.albedo * ob.transparency * (splat3(1.0f) - fr));
+                ray = mk_ray(at(ray, h.t + net_max(1e-4f, h.t * 1e-4f)), ray.d);
+                ++extra;
+                if (!(max_comp(tr) > 1e-4f)) break;
+            }
+        }
+        if (max_comp(tr) > 0.0f) {
+            if (S.fog.enabled) tr = tr * expf(-fog_optical_depth(S.fog, mk_ray(sr.o, sr.d), 0.0f, kInf));
+            const vec3 c = sr.skyRad * tr * sr.f * sr.w / sr.pdf * sr.energy;
+            const uint32_t q = sr.path & 0x7fffffffu;
+            PathRec* p = &F.paths[q];
+            if (sr.path & 0x80000000u) p->radFog = p->radFog + c;
+            else p->rad = p->rad + c;
+        }
+    }
+    if (F.ctr && extra) atomicAdd(&F.ctr[11], (unsigned long long)extra);
+}
+
+// Renderer.RelativeStdErr (renderer.cs:232-237).
+SR_DEV float rel_std_err(float sum, float sumSqr, uint32_t n) {
+    const float mean = sum / (float)n;
+    const float variance = net_max(0.0f, sumSqr / (float)n - mean * mean);
+    return sqrtf(variance / (float)n) / net_max(mean, 1e-4f);
+}
+
+// Per-pixel sample accumulation, convergence test and AOV write-back (renderer.cs:180-229).
+__global__ void k_accumulate(Frame F) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) { F.counts[0] = 0; F.counts[1] = 0; F.counts[2] = 0; }
+    if (q >= F.nSlots) return;
+    PixAcc a = F.acc[q];
+    if (!a.active) return;
+    const PathRec& p = F.paths[q];
+    const vec3 rad = p.rad, radFog = p.radFog;
+    a.radSum = a.radSum + rad;
+    a.radFogSum = a.radFogSum + radFog;
+    const float lum = luminance(rad) + luminance(radFog);
+    a.lumSum += lum;
+    a.lumSumSqr += lum * lum;
+    const float lumSurf = luminance(rad);
+    a.lumSurf += lumSurf;
+    a.lumSurfSqr += lumSurf * lumSurf;
+    const uint32_t si = a.sampleIndex;
+    bool done = false;
+    uint32_t count = 0;
+    if (si >= F.minS - 1u && rel_std_err(a.lumSum, a.lumSumSqr, si + 1) < F.threshold) {
+        done = true;
+        count = si + 1;
+    } else if (si + 1 == F.maxS) {
+        done = true;
+        count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
+    }
+    if (!done) {
+        a.sampleIndex = si + 1;
+        F.acc[q] = a;
+        return;
+    }
+    a.active = 0;
+    F.acc[q] = a;
+    uint32_t x, y;
+    slot_pixel(F, q, x, y);
+    const size_t o = (size_t)y * F.W + x;
+    const float var = rel_std_err(a.lumSurf, a.lumSurfSqr, count);
+    const vec3 r = a.radSum / (float)count;
+    const vec3 rf = a.radFogSum / (float)count;
+    const vec3 nrm = a.normalCount > 0 ? normalized_or(a.normalSum, a.normalFallback) : splat3(0.0f);
+    const float depth = a.depthCount > 0 ? a.depthSum / (float)a.depthCount : kInf;
+    F.outRad[o * 3] = r.x; F.outRad[o * 3 + 1] = r.y; F.outRad[o * 3 + 2] = r.z;
+    F.outRadFog[o * 3] = rf.x; F.outRadFog[o * 3 + 1] = rf.y; F.outRadFog[o * 3 + 2] = rf.z;
+    F.outNormal[o * 3] = nrm.x; F.outNormal[o * 3 + 1] = nrm.y; F.outNormal[o * 3 + 2] = nrm.z;
+    F.outUv[o * 2] = a.hasUv ? a.uv.x : 0.0f;
+    F.outUv[o * 2 + 1] = a.hasUv ? a.uv.y : 0.0f;
+    F.outDepth[o] = depth;
+    F.outSamples[o] = count;
+    F.outVar[o] = var;
+    if (F.ctr) atomicAdd(&F.ctr[count < F.maxS ? 8 : 9], 1ull);
+}
+
+// Primary-hit diagnostic: sample 0 of every pixel of a band.
+__global__ void k_primary(Frame F, DevScene S, int32_t* obj, int32_t* tri, float* dist) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= F.nSlots) return;
+    uint32_t x, y;
+    if (!slot_pixel(F, q, x, y)) return;
+    XorwowRng rng = xorwow_seed(x, y);
+    const float jx = next_float(rng);
+    const float jy = next_float(rng);
+    const vec2 sp = mk2(((float)x + jx) / (float)F.W, ((float)y + jy) / (float)F.H);
+    const float ndcX = sp.x * 2.0f - 1.0f;
+    const float ndcY = -sp.y * 2.0f + 1.0f;
+    float th, tv;
+    if (F.view.aspect >= 1.0f) { th = F.view.tanFov; tv = th / F.view.aspect; }
+    else { tv = F.view.tanFov; th = tv * F.view.aspect; }
+    const vec3 origin = F.view.pos + rotate(F.view.rot, mk3(0.0f, 0.0f, F.view.nearPlane));
+    const vec3 dir = rotate(F.view.rot, normalized(mk3(ndcX * th, ndcY * tv, 1.0f)));
+    HitRec h;
+    h.t = kInf; h.obj = -1; h.slot = -1;
+    Tally tl = {0, 0, 0};
+    const size_t o = (size_t)y * F.W + x;
+    if (scene_closest<false>(S, mk_ray(origin, dir), h, tl)) {
+        const DevObject& ob = S.objects[h.obj];
+        obj[o] = h.obj;
+        tri[o] = ob.kind == 0 ? S.slotTri[ob.triBase + h.slot] : -1;
+        dist[o] = h.t;
+    } else {
+        obj[o] = -1; tri[o] = -1; dist[o] = kInf;
+    }
+}
+
+// Compositor.Filter (compose.cs:101-204): joint bilateral filter over the circular window.
+struct FilterParams {
+    float strength, strengthMax, lumBoost, lumLimitInv, normalLimitInv, depthLimitInv, radiusInv;
+    int32_t kr;
+};
+__global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* rad, const float* nrm, const float* dep,
+                                                 const float* var, int32_t W, int32_t H, float* out) {
+    const int32_t cx = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int32_t cy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (cx >= W || cy >= H) return;
+    const int32_t ci = cy * W + cx;
+    const vec3 cr = mk3(rad[ci * 3], rad[ci * 3 + 1], rad[ci * 3 + 2]);
+    const vec3 cn = mk3(nrm[ci * 3], nrm[ci * 3 + 1], nrm[ci * 3 + 2]);
+    const float cd = dep[ci];
+    const bool hasCN = len_sqr(cn) > 0.0f;
+    const bool hasCD = !isinf(cd);
+    const float lum = luminance(cr);
+    const float lb = 1.0f + lum * fp.lumBoost;
+    const float dw = net_min(var[ci] * lb * fp.strength, fp.strengthMax);
+    if (dw < 1e-4f) {
+        out[ci * 3] = cr.x; out[ci * 3 + 1] = cr.y; out[ci * 3 + 2] = cr.z;
+        return;
+    }
+    float wSum = 1.0f;
+    vec3 rSum = cr;
+    const int32_t kr = fp.kr;
+    for (int32_t ky = -kr; ky <= kr; ++ky) {
+        const int32_t ky2 = ky * ky;
+        const int32_t ny = cy + ky;
+        for (int32_t kx = -kr; kx <= kr; ++kx) {
+            if (kx == 0 && ky == 0) continue;
+            const float kd2 = (float)(kx * kx + ky2);
+            if (kd2 > (float)(kr * kr)) continue;
+            const int32_t nx = cx + kx;
+            if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+            const int32_t ni = ny * W + nx;
+            const vec3 nr = mk3(rad[ni * 3], rad[ni * 3 + 1], rad[ni * 3 + 2]);
+            const vec3 nn = mk3(nrm[ni * 3], nrm[ni * 3 + 1], nrm[ni * 3 + 2]);
+            const float nd = dep[ni];
+            const bool hasNN = len_sqr(nn) > 0.0f;
+            const bool hasND = !isinf(nd);
+            if (hasCN != hasNN || hasCD != hasND) continue;
+            float w = dw * expf(-kd2 * fp.radiusInv);
+            if (hasCN && w > 0.0f) w *= expf(-len_sqr(cn - nn) * fp.normalLimitInv);
+            if (hasCD && w > 0.0f) {
+                const float dd = cd - nd;
+                w *= expf(-(dd * dd) * fp.depthLimitInv);
+            }
+            if (w > 0.0f) {
+                const float ld = net_max(0.0f, luminance(nr) - lum);
+                w *= expf(-(ld * ld) * fp.lumLimitInv);
+            }
+            wSum += w;
+            rSum = rSum + nr * w;
+        }
+    }
+    const vec3 r = rSum / wSum;
+    out[ci * 3] = r.x; out[ci * 3 + 1] = r.y; out[ci * 3 + 2] = r.z;
+}
+
+}  // namespace
+
+// =================================================================================================
+// Host side
+// =================================================================================================
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(SR_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HostTexture {
+    std::vector<float> rgb;
+    uint32_t w, h;
+    int32_t filter;
+    float tilx, tily;
+};
+struct HostMesh {
+    std::vector<float> tris;  // 36 floats per Triangle
+    BvhResult bvh;
+    uint32_t bins;
+};
+struct HostObject {
+    sr_transform trans;
+    sr_material mat;
+    int32_t kind, mesh;
+    float shape[6];
+};
+
+template <class T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t n = 0;
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct sr_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<HostTexture> textures;
+    std::vector<HostMesh> meshes;
+    std::vector<HostObject> objects;
+    int skyKind = 0;
+    int32_t skyTex = -1;
+    float skyMult = 1.0f, skyYaw = 0.0f;
+    sr_sky_procedural skyProc{};
+    bool hasFog = false;
+    sr_fog fog{};
+    bool built = false;
+    BvhResult sceneBvh;
+    std::vector<int32_t> meshNodeBase, meshTriBase;
+    DevScene dev{};
+    // device scene arrays
+    DevBuf<float4> dNodes;
+    DevBuf<float> dTris, dAttrs, dTexels, dCdf;
+    DevBuf<int32_t> dSlotTri, dSceneItems;
+    DevBuf<DevObject> dObjects;
+    DevBuf<DevTexture> dTextures;
+    // frame
+    DevBuf<PathRec> dPaths;
+    DevBuf<HitRec> dHits;
+    DevBuf<ShadowRec> dShadows;
+    DevBuf<PixAcc> dAcc;
+    DevBuf<uint32_t> dQueue0, dQueue1, dShadowQueue, dCounts;
+    DevBuf<unsigned long long> dCtr;
+    DevBuf<float> dRad, dRadFog, dNormal, dUv, dDepth, dVar;
+    DevBuf<uint32_t> dSamples;
+    bool rendering = false;
+    bool countersOn = false;
+    sr_render_params params{};
+    std::vector<hipEvent_t> bandEvents;
+    std::vector<uint32_t> bandEnd;  // last row (exclusive) per band
+    size_t bandsDone = 0;
+    hipEvent_t evStart = nullptr, evEnd = nullptr;
+    int gridCUs = 256;
+    // Kernel timing: one event after every launch of a frame; kernel i spans events i-1..i.
+    bool profileOn = false;
+    std::vector<hipEvent_t> profEvents;
+    std::vector<int> profKinds;  // 0 extend, 1 shade, 2 shadow, 3 other
+    size_t profUsed = 0;
+
+    ~sr_ctx() {
+        for (auto e : bandEvents) (void)hipEventDestroy(e);
+        for (auto e : profEvents) (void)hipEventDestroy(e);
+        if (evStart) (void)hipEventDestroy(evStart);
+        if (evEnd) (void)hipEventDestroy(evEnd);
+        dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release();
+        dSlotTri.release(); dSceneItems.release(); dObjects.release(); dTextures.release();
+        dPaths.release(); dHits.release(); dShadows.release(); dAcc.release();
+        dQueue0.release(); dQueue1.release(); dShadowQueue.release(); dCounts.release(); dCtr.release();
+        dRad.release(); dRadFog.release(); dNormal.release(); dUv.release(); dDepth.release(); dVar.release();
+        dSamples.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+template <class T>
+int upload(DevBuf<T>& b, const T* src, size_t n, hipStream_t s) {
+    b.release();
+    if (n == 0) return SR_OK;
+    HIP_TRY(hipMalloc((void**)&b.ptr, n * sizeof(T)));
+    b.n = n;
+    HIP_TRY(hipMemcpyAsync(b.ptr, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    return SR_OK;
+}
+template <class T>
+int ensure(DevBuf<T>& b, size_t n) {
+    if (b.n >= n && b.ptr) return SR_OK;
+    b.release();
+    HIP_TRY(hipMalloc((void**)&b.ptr, std::max<size_t>(n, 1) * sizeof(T)));
+    b.n = n;
+    return SR_OK;
+}
+
+quat mkq(const float* r) { quat q; q.x = r[0]; q.y = r[1]; q.z = r[2]; q.w = r[3]; return q; }
+
+// Quat.AngleAxis (math.cs:621-626).
+quat angle_axis(float angle, vec3 axis) {
+    const vec3 v = axis * sinf(angle * 0.5f);
+    quat q; q.x = v.x; q.y = v.y; q.z = v.z; q.w = cosf(angle * 0.5f);
+    return q;
+}
+// Quat.Look = FromMat4(Mat4.RotateLook(forward, upRef)) (math.cs:631-659, 742-758).
+quat quat_look(vec3 fwdIn, vec3 upRef) {
+    vec3 c0 = mk3(1, 0, 0), c1 = mk3(0, 1, 0), c2 = mk3(0, 0, 1);
+    if (!(len_sqr(fwdIn) <= 1e-6f) && !(len_sqr(upRef) <= 1e-6f)) {
+        const vec3 f = normalized(fwdIn);
+        vec3 rt = cross3(upRef, f);
+        rt = len_sqr(rt) > 1e-6f ? normalized(rt) : mk3(1, 0, 0);
+        c0 = rt; c1 = cross3(f, rt); c2 = f;
+    }
+    quat q;
+    const float tr = c0.x + c1.y + c2.z;
+    if (tr > 1e-6f) {
+        const float s = sqrtf(tr + 1.0f) * 2.0f;
+        q.x = (c1.z - c2.y) / s; q.y = (c2.x - c0.z) / s; q.z = (c0.y - c1.x) / s; q.w = s * 0.25f;
+    } else if (c0.x > c1.y && c0.x > c2.z) {
+        const float s = sqrtf(1.0f + c0.x - c1.y - c2.z) * 2.0f;
+        q.x = s * 0.25f; q.y = (c1.x + c0.y) / s; q.z = (c2.x + c0.z) / s; q.w = (c1.z - c2.y) / s;
+    } else if (c1.y > c2.z) {
+        const float s = sqrtf(1.0f + c1.y - c0.x - c2.z) * 2.0f;
+        q.x = (c1.x + c0.y) / s; q.y = s * 0.25f; q.z = (c2.y + c1.z) / s; q.w = (c2.x - c0.z) / s;
+    } else {
+        const float s = sqrtf(1.0f + c2.z - c0.x - c1.y) * 2.0f;
+        q.x = (c2.x + c0.z) / s; q.y = (c2.y + c1.z) / s; q.z = s * 0.25f; q.w = (c0.y - c1.x) / s;
+    }
+    return q;
+}
+
+// Object world bounds: AABox.Transform then Box.Bounds (scene.cs:221-222; math.cs:974, 1101-1114).
+box3 object_world_bounds(const box3& lb, const sr_transform& t) {
+    const quat rq = mkq(t.rot);
+    const vec3 sc = mk3(t.scale[0], t.scale[1], t.scale[2]);
+    const vec3 center = mk3(t.pos[0], t.pos[1], t.pos[2]) + rotate(rq, box_mid(lb) * sc);
+    const vec3 size = box_ext(lb) * sc;
+    box3 local;
+    local.mn = center - size * 0.5f;
+    local.mx = center + size * 0.5f;
+    const vec3 c = box_mid(local);
+    const vec3 h = box_ext(local) * 0.5f;
+    const vec3 ax = rotate(rq, mk3(h.x, 0.0f, 0.0f));
+    const vec3 ay = rotate(rq, mk3(0.0f, h.y, 0.0f));
+    const vec3 az = rotate(rq, mk3(0.0f, 0.0f, h.z));
+    const vec3 ext = mk3(fabsf(ax.x) + fabsf(ay.x) + fabsf(az.x), fabsf(ax.y) + fabsf(ay.y) + fabsf(az.y),
+                         fabsf(ax.z) + fabsf(ay.z) + fabsf(az.z));
+    box3 r;
+    r.mn = c - (ext * 2.0f) * 0.5f;
+    r.mx = c + (ext * 2.0f) * 0.5f;
+    return r;
+}
+
+// MathF.Pow through libm (kept out of line so the compiler cannot fold pow(x, 2) into x * x).
+__attribute__((noinline)) float libm_powf(float x, float y) {
+    volatile float vy = y;
+    return powf(x, vy);
+}
+
+// Appends a BVH to the node pool: one pad node, then the nodes, starting at an even index.
+int append_bvh(std::vector<float4>& pool, const BvhResult& b, int32_t& base) {
+    if (pool.size() % 4 != 0) pool.resize(pool.size() + 2, make_float4(0, 0, 0, 0));
+    base = (int32_t)(pool.size() / 2);
+    pool.resize(pool.size() + 2, make_float4(0, 0, 0, 0));
+    for (int i = 0; i < b.used; ++i) {
+        const RefNode& nd = b.nodes[i];
+        if ((uint32_t)nd.child > kMaxChild || nd.count > 255 || nd.count < 0)
+            return fail(SR_ERR_LIMIT, "BVH node exceeds the device layout (child < 2^24, leaf size < 256)");
+        int32_t ci = nd.child, cn = nd.count;
+        pool.push_back(make_float4(nd.bounds.mn.x, nd.bounds.mn.y, nd.bounds.mn.z, nd.bounds.mx.x));
+        pool.push_back(make_float4(nd.bounds.mx.y, nd.bounds.mx.z, __builtin_bit_cast(float, ci), __builtin_bit_cast(float, cn)));
+    }
+    return SR_OK;
+}
+
+int grid_for(const sr_ctx* c, uint32_t maxItems) {
+    const uint32_t want = (maxItems + kBlock - 1) / kBlock;
+    const uint32_t cap = (uint32_t)c->gridCUs * 8u;
+    return (int)std::max(1u, std::min(want, cap));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sr_last_error(void) { return g_err.c_str(); }
+
+int sr_create(int32_t device, sr_ctx** out) {
+    if (!out) return fail(SR_ERR_INVALID, "out is null");
+    *out = nullptr;
+    if (device == -1) {  // host-only context: scene preparation and BVH export, no rendering
+        sr_ctx* c = new sr_ctx();
+        c->device = -1;
+        *out = c;
+        return SR_OK;
+    }
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(SR_ERR_INVALID, "no such HIP device");
+    HIP_TRY(hipSetDevice(device));
+    sr_ctx* c = new sr_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(SR_ERR_HIP, hipGetErrorString(e)); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->gridCUs = prop.multiProcessorCount;
+    *out = c;
+    return SR_OK;
+}
+
+int sr_destroy(sr_ctx* ctx) {
+    if (!ctx) return SR_OK;
+    if (ctx->device >= 0) {
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    }
+    delete ctx;
+    return SR_OK;
+}
+
+int sr_texture_create(sr_ctx* ctx, const float* rgb, uint32_t w, uint32_t h, int32_t filter, float tx, float ty,
+                      int32_t* out_id) {
+    if (!ctx || !rgb || !out_id || w == 0 || h == 0) return fail(SR_ERR_INVALID, "texture: bad arguments");
+    if (filter != 0 && filter != 1) return fail(SR_ERR_INVALID, "texture: filter must be 0 or 1");
+    HostTexture t;
+    t.rgb.assign(rgb, rgb + (size_t)w * h * 3);
+    t.w = w; t.h = h; t.filter = filter; t.tilx = tx; t.tily = ty;
+    ctx->textures.push_back(std::move(t));
+    *out_id = (int32_t)ctx->textures.size() - 1;
+    return SR_OK;
+}
+
+int sr_mesh_create(sr_ctx* ctx, const float* tris, uint32_t count, uint32_t bins, int32_t* out_id) {
+    if (!ctx || !out_id || (count && !tris)) return fail(SR_ERR_INVALID, "mesh: bad arguments");
+    if (bins < 2) return fail(SR_ERR_INVALID, "mesh: split_bins must be >= 2");
+    if (ctx->built) return fail(SR_ERR_STATE, "Scene already built");
+    if (count > kMaxChild) return fail(SR_ERR_LIMIT, "mesh: more than 2^24-1 triangles");
+    HostMesh m;
+    m.tris.assign(tris, tris + (size_t)count * 36);
+    m.bins = bins;
+    std::vector<box3> bb(count);
+    for (uint32_t i = 0; i < count; ++i) bb[i] = lean_bounds(tris + (size_t)i * 36);
+    m.bvh = build_bvh(bb.data(), (int)count, (int)bins, 1.0f, 0.5f);
+    if (m.bvh.degenerate) return fail(SR_ERR_BUILD, "mesh BVH: a split produced an empty child (bvh.cs:448)");
+    ctx->meshes.push_back(std::move(m));
+    *out_id = (int32_t)ctx->meshes.size() - 1;
+    return SR_OK;
+}
+
+int sr_scene_set_sky_texture(sr_ctx* ctx, int32_t tex, float mult, float yaw) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    if (ctx->built) return fail(SR_ERR_STATE, "Scene already built");
+    if (tex < 0 || tex >= (int32_t)ctx->textures.size()) return fail(SR_ERR_INVALID, "sky: bad texture id");
+    ctx->skyKind = 1; ctx->skyTex = tex; ctx->skyMult = mult; ctx->skyYaw = yaw;
+    return SR_OK;
+}
+
+int sr_scene_set_sky_procedural(sr_ctx* ctx, const sr_sky_procedural* sky) {
+    if (!ctx || !sky) return fail(SR_ERR_INVALID, "sky: null");
+    if (ctx->built) return fail(SR_ERR_STATE, "Scene already built");
+    ctx->skyKind = 2;
+    ctx->skyProc = *sky;
+    return SR_OK;
+}
+
+int sr_scene_set_fog(sr_ctx* ctx, const sr_fog* fog) {
+    if (!ctx || !fog) return fail(SR_ERR_INVALID, "fog: null");
+    if (ctx->built) return fail(SR_ERR_STATE, "Scene already built");
+    ctx->hasFog = true;
+    ctx->fog = *fog;
+    return SR_OK;
+}
+
+int sr_scene_add_object(sr_ctx* ctx, const sr_transform* trans, const sr_material* mat, int32_t kind, int32_t mesh,
+                        const float* params, int32_t* out_id) {
+    if (!ctx || !trans || !mat) return fail(SR_ERR_INVALID, "object: null argument");
+    if (ctx->built) return fail(SR_ERR_STATE, "Scene already built");
+    if (kind < 0 || kind > 2) return fail(SR_ERR_INVALID, "object: bad shape kind");
+    if (kind == 0 && (mesh < 0 || mesh >= (int32_t)ctx->meshes.size())) return fail(SR_ERR_INVALID, "object: bad mesh id");
+    if (kind != 0 && !params) return fail(SR_ERR_INVALID, "object: shape params required");
+    const int32_t texs[4] = {mat->color_texture, mat->roughness_texture, mat->metallic_texture, mat->normal_texture};
+    for (int32_t t : texs)
+        if (t < -1 || t >= (int32_t)ctx->textures.size()) return fail(SR_ERR_INVALID, "object: bad texture id");
+    HostObject o;
+    o.trans = *trans;
+    o.mat = *mat;
+    o.kind = kind;
+    o.mesh = mesh;
+    memset(o.shape, 0, sizeof(o.shape));
+    if (kind == 1) memcpy(o.shape, params, 4 * sizeof(float));
+    if (kind == 2) memcpy(o.shape, params, 6 * sizeof(float));
+    ctx->objects.push_back(o);
+    if (out_id) *out_id = (int32_t)ctx->objects.size() - 1;
+    return SR_OK;
+}
+
+int sr_scene_build(sr_ctx* ctx) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    if (ctx->built) return SR_OK;
+    if (ctx->skyKind == 0) return fail(SR_ERR_STATE, "Scene sky is required");
+    if (ctx->device >= 0) HIP_TRY(hipSetDevice(ctx->device));
+    // Scene BVH over object world bounds (scene.cs:528-533).
+    std::vector<box3> ob(ctx->objects.size());
+    for (size_t i = 0; i < ctx->objects.size(); ++i) {
+        const HostObject& o = ctx->objects[i];
+        box3 lb;
+        if (o.kind == 0) {
+            const HostMesh& m = ctx->meshes[o.mesh];
+            lb = m.bvh.used > 0 ? m.bvh.nodes[0].bounds : box_empty();
+        } else if (o.kind == 1) {
+            const vec3 c = mk3(o.shape[0], o.shape[1], o.shape[2]);
+            lb.mn = c - splat3(o.shape[3]);
+            lb.mx = c + splat3(o.shape[3]);
+        } else {
+            lb.mn = mk3(o.shape[0], o.shape[1], o.shape[2]);
+            lb.mx = mk3(o.shape[3], o.shape[4], o.shape[5]);
+        }
+        ob[i] = object_world_bounds(lb, o.trans);
+    }
+    ctx->sceneBvh = build_bvh(ob.data(), (int)ob.size(), 32, 1.0f, 10.0f);
+    if (ctx->sceneBvh.degenerate) return fail(SR_ERR_BUILD, "scene BVH: a split produced an empty child");
+
+    // Node pool, triangles in item order, attributes in triangle order.
+    std::vector<float4> nodes;
+    std::vector<float> tris, attrs;
+    std::vector<int32_t> slotTri;
+    ctx->meshNodeBase.assign(ctx->meshes.size(), 0);
+    ctx->meshTriBase.assign(ctx->meshes.size(), 0);
+    for (size_t m = 0; m < ctx->meshes.size(); ++m) {
+        const HostMesh& hm = ctx->meshes[m];
+        int rc = append_bvh(nodes, hm.bvh, ctx->meshNodeBase[m]);
+        if (rc) return rc;
+        const int32_t base = (int32_t)slotTri.size();
+        ctx->meshTriBase[m] = base;
+        const size_t n = hm.bvh.items.size();
+        for (size_t s = 0; s < n; ++s) {
+            const int32_t t = hm.bvh.items[s];
+            slotTri.push_back(t);
+            const float* src = hm.tris.data() + (size_t)t * 36;
+            tris.insert(tris.end(), src, src + 9);
+        }
+        for (size_t t = 0; t < n; ++t) {
+            const float* src = hm.tris.data() + t * 36 + 9;
+            attrs.insert(attrs.end(), src, src + 27);
+        }
+    }
+    int32_t sceneBase = 0;
+    int rc = append_bvh(nodes, ctx->sceneBvh, sceneBase);
+    if (rc) return rc;
+
+    std::vector<float> texels, cdf;
+    std::vector<DevTexture> dtex;
+    for (const HostTexture& t : ctx->textures) {
+        DevTexture d;
+        memset(&d, 0, sizeof(d));
+        d.offset = (int32_t)(texels.size() / 3);
+        d.w = (int32_t)t.w; d.h = (int32_t)t.h; d.filter = t.filter; d.tilx = t.tilx; d.tily = t.tily;
+        texels.insert(texels.end(), t.rgb.begin(), t.rgb.end());
+        dtex.push_back(d);
+    }
+
+    DevScene& S = ctx->dev;
+    memset(&S, 0, sizeof(S));
+    S.sceneBase = sceneBase;
+    S.sceneNodes = ctx->sceneBvh.used;
+    // Sky (scene.cs:389-404 / 341-352 / 290-298).
+    DevSky& K = S.sky;
+    K.kind = ctx->skyKind;
+    if (ctx->skyKind == 1) {
+        const HostTexture& t = ctx->textures[ctx->skyTex];
+        const int W = (int)t.w, H = (int)t.h;
+        K.tex = ctx->skyTex;
+        K.multiplier = ctx->skyMult;
+        K.rot = angle_axis(ctx->skyYaw, up3());
+        K.rotInv = conj(K.rot);
+        K.w = W; K.h = H;
+        K.cdfOff = 0;
+        cdf.assign((size_t)(H + 1) + (size_t)H * (W + 1), 0.0f);
+        std::vector<float> rowW(H), colW(W);
+        for (int y = 0; y < H; ++y) {
+            const float s = sinf(kPi * ((float)y + 0.5f) / (float)H);
+            for (int x = 0; x < W; ++x) {
+                const float* c = t.rgb.data() + ((size_t)y * W + x) * 3;
+                colW[x] = luminance(mk3(c[0], c[1], c[2])) * s;
+            }
+            rowW[y] = build_cdf1(colW.data(), W, cdf.data() + (H + 1) + (size_t)y * (W + 1));
+        }
+        K.total = build_cdf1(rowW.data(), H, cdf.data());
+        K.pdfScale = (float)(W * H) / (K.total * 2.0f * kPi * kPi);
+    } else {
+        const sr_sky_procedural& p = ctx->skyProc;
+        K.top = mk3(p.radiance_top[0], p.radiance_top[1], p.radiance_top[2]);
+        K.middle = mk3(p.radiance_middle[0], p.radiance_middle[1], p.radiance_middle[2]);
+        K.bottom = mk3(p.radiance_bottom[0], p.radiance_bottom[1], p.radiance_bottom[2]);
+        K.sunDir = mk3(p.sun_dir[0], p.sun_dir[1], p.sun_dir[2]);
+        K.sunRad = mk3(p.sun_radiance[0], p.sun_radiance[1], p.sun_radiance[2]);
+        K.sunCos = cosf(p.sun_angle);
+        K.sunRot = quat_look(K.sunDir, up3());
+        K.sunPdfDenom = kPi * (1.0f - K.sunCos);
+    }
+    S.fog.enabled = ctx->hasFog ? 1 : 0;
+    S.fog.density = ctx->fog.density;
+    S.fog.color = mk3(ctx->fog.color[0], ctx->fog.color[1], ctx->fog.color[2]);
+    S.fog.anisotropy = ctx->fog.anisotropy;
+    S.fog.heightFalloff = ctx->fog.height_falloff;
+
+    std::vector<DevObject> dobj(ctx->objects.size());
+    for (size_t i = 0; i < ctx->objects.size(); ++i) {
+        const HostObject& o = ctx->objects[i];
+        DevObject& d = dobj[i];
+        memset(&d, 0, sizeof(d));
+        d.rot = mkq(o.trans.rot);
+        d.pos = mk3(o.trans.pos[0], o.trans.pos[1], o.trans.pos[2]);
+        d.scale = mk3(o.trans.scale[0], o.trans.scale[1], o.trans.scale[2]);
+        d.kind = o.kind;
+        if (o.kind == 0) {
+            d.nodeBase = ctx->meshes[o.mesh].bvh.used > 0 ? ctx->meshNodeBase[o.mesh] : -1;
+            d.triBase = ctx->meshTriBase[o.mesh];
+        }
+        d.texColor = o.mat.color_texture; d.texRough = o.mat.roughness_texture;
+        d.texMetal = o.mat.metallic_texture; d.texNormal = o.mat.normal_texture;
+        d.albedo = mk3(o.mat.albedo[0], o.mat.albedo[1], o.mat.albedo[2]);
+        d.roughness = o.mat.roughness; d.metallic = o.mat.metallic;
+        d.transparency = o.mat.transparency; d.ior = o.mat.ior;
+        d.radiance = mk3(o.mat.radiance[0], o.mat.radiance[1], o.mat.radiance[2]);
+        d.f0 = libm_powf((o.mat.ior - 1.0f) / (o.mat.ior + 1.0f), 2.0f);
+        memcpy(d.shape, o.shape, sizeof(d.shape));
+        if (o.mat.transparency > 0.0f) S.anyTransparent = 1;
+    }
+
+    if (ctx->device < 0) {  // host-only context
+        ctx->built = true;
+        return SR_OK;
+    }
+    hipStream_t s = ctx->stream;
+    if ((rc = upload(ctx->dNodes, nodes.data(), nodes.size(), s))) return rc;
+    if ((rc = upload(ctx->dTris, tris.data(), tris.size(), s))) return rc;
+    if ((rc = upload(ctx->dAttrs, attrs.data(), attrs.size(), s))) return rc;
+    if ((rc = upload(ctx->dSlotTri, slotTri.data(), slotTri.size(), s))) return rc;
+    if ((rc = upload(ctx->dSceneItems, ctx->sceneBvh.items.data(), ctx->sceneBvh.items.size(), s))) return rc;
+    if ((rc = upload(ctx->dObjects, dobj.data(), dobj.size(), s))) return rc;
+    if ((rc = upload(ctx->dTexels, texels.data(), texels.size(), s))) return rc;
+    if ((rc = upload(ctx->dTextures, dtex.data(), dtex.size(), s))) return rc;
+    if ((rc = upload(ctx->dCdf, cdf.data(), cdf.size(), s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    S.nodes = ctx->dNodes.ptr;
+    S.tris = ctx->dTris.ptr;
+    S.attrs = ctx->dAttrs.ptr;
+    S.slotTri = ctx->dSlotTri.ptr;
+    S.sceneItems = ctx->dSceneItems.ptr;
+    S.objects = ctx->dObjects.ptr;
+    S.texels = ctx->dTexels.ptr;
+    S.textures = ctx->dTextures.ptr;
+    S.cdf = ctx->dCdf.ptr;
+    ctx->built = true;
+    return SR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+ViewD make_view(const sr_view* v, uint32_t W, uint32_t H) {
+    ViewD d;
+    d.pos = mk3(v->pos[0], v->pos[1], v->pos[2]);
+    d.rot = mkq(v->rot);
+    d.nearPlane = v->near_plane;
+    d.tanFov = tanf(v->fov * 0.5f);  // MathF.Tan(Fov * 0.5f) (math.cs:1623)
+    d.aspect = (float)W / (float)H;  // renderer.cs:62
+    return d;
+}
+
+Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows) {
+    Frame F;
+    memset(&F, 0, sizeof(F));
+    F.paths = c->dPaths.ptr;
+    F.hits = c->dHits.ptr;
+    F.shadows = c->dShadows.ptr;
+    F.acc = c->dAcc.ptr;
+    F.queue[0] = c->dQueue0.ptr;
+    F.queue[1] = c->dQueue1.ptr;
+    F.shadowQueue = c->dShadowQueue.ptr;
+    F.counts = c->dCounts.ptr;
+    F.ctr = c->countersOn ? c->dCtr.ptr : nullptr;
+    F.outRad = c->dRad.ptr;
+    F.outRadFog = c->dRadFog.ptr;
+    F.outNormal = c->dNormal.ptr;
+    F.outUv = c->dUv.ptr;
+    F.outDepth = c->dDepth.ptr;
+    F.outSamples = c->dSamples.ptr;
+    F.outVar = c->dVar.ptr;
+    F.W = c->params.width;
+    F.H = c->params.height;
+    F.y0 = y0;
+    F.rows = rows;
+    F.tilesX = (F.W + 7) / 8;
+    F.nSlots = F.tilesX * ((rows + 7) / 8) * 64;
+    F.minS = c->params.min_samples;
+    F.maxS = c->params.max_samples;
+    F.bounces = c->params.bounces;
+    F.seed = c->params.rng_seed;
+    F.threshold = c->params.variance_threshold;
+    F.clamp = c->params.indirect_clamp;
+    return F;
+}
+
+int alloc_frame(sr_ctx* c, uint32_t maxSlots) {
+    const size_t px = (size_t)c->params.width * c->params.height;
+    int rc;
+    if ((rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dHits, maxSlots)) || (rc = ensure(c->dShadows, maxSlots)) ||
+        (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxSlots)) || (rc = ensure(c->dQueue1, maxSlots)) ||
+        (rc = ensure(c->dShadowQueue, maxSlots)) || (rc = ensure(c->dCounts, 4)) || (rc = ensure(c->dCtr, SR_COUNTER_COUNT)) ||
+        (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) || (rc = ensure(c->dNormal, px * 3)) ||
+        (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) || (rc = ensure(c->dVar, px)) ||
+        (rc = ensure(c->dSamples, px)))
+        return rc;
+    return SR_OK;
+}
+
+void mark(sr_ctx* c, int kind) {
+    if (!c->profileOn) return;
+    if (c->profUsed == c->profEvents.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        c->profEvents.push_back(e);
+        c->profKinds.push_back(kind);
+    }
+    c->profKinds[c->profUsed] = kind;
+    (void)hipEventRecord(c->profEvents[c->profUsed++], c->stream);
+}
+
+template <class R, bool COUNT>
+void enqueue_band(sr_ctx* c, const Frame& F) {
+    hipStream_t s = c->stream;
+    const int slotBlocks = (int)((F.nSlots + kBlock - 1) / kBlock);
+    const int g = grid_for(c, F.nSlots);
+    mark(c, 3);
+    hipLaunchKernelGGL(k_init, dim3(slotBlocks), dim3(kBlock), 0, s, F);
+    mark(c, 3);
+    for (uint32_t smp = 0; smp < F.maxS; ++smp) {
+        hipLaunchKernelGGL(k_raygen<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
+        mark(c, 3);
+        int cur = 0;
+        for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
+            hipLaunchKernelGGL(k_extend<COUNT>, dim3(g), dim3(kBlock), 0, s, F, c->dev, cur);
+            mark(c, 0);
+            hipLaunchKernelGGL(k_shade<R>, dim3(g), dim3(kBlock), 0, s, F, c->dev, cur, seg);
+            mark(c, 1);
+            hipLaunchKernelGGL(k_shadow, dim3(g), dim3(kBlock), 0, s, F, c->dev);
+            mark(c, 2);
+            cur ^= 1;
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3(slotBlocks), dim3(kBlock), 0, s, F);
+        mark(c, 3);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p) {
+    if (!ctx || !view || !p) return fail(SR_ERR_INVALID, "render: null argument");
+    if (ctx->device < 0) return fail(SR_ERR_STATE, "host-only context (device -1) cannot render");
+    if (p->width == 0 || p->height == 0 || p->block_size == 0) return fail(SR_ERR_INVALID, "render: zero size");
+    if (p->min_samples == 0 || p->min_samples > p->max_samples) return fail(SR_ERR_INVALID, "render: need 0 < minSamples <= maxSamples");
+    if (p->rng_policy > 1) return fail(SR_ERR_INVALID, "render: unknown rng policy");
+    if (p->width > 32768 || p->height > 32768) return fail(SR_ERR_LIMIT, "render: image larger than 32768");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->built) {
+        int rc = sr_scene_build(ctx);
+        if (rc) return rc;
+    }
+    if (ctx->rendering) HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->params = *p;
+    if (ctx->params.tick_bands == 0) ctx->params.tick_bands = 1;
+    const uint32_t bands = std::min(ctx->params.tick_bands, (p->height + 7) / 8);
+    uint32_t rowsPer = ((p->height + bands - 1) / bands + 7) / 8 * 8;
+    const uint32_t tilesX = (p->width + 7) / 8;
+    const uint32_t maxSlots = tilesX * ((rowsPer + 7) / 8) * 64;
+    int rc = alloc_frame(ctx, maxSlots);
+    if (rc) return rc;
+    for (auto e : ctx->bandEvents) (void)hipEventDestroy(e);
+    ctx->bandEvents.clear();
+    ctx->bandEnd.clear();
+    if (!ctx->evStart) HIP_TRY(hipEventCreate(&ctx->evStart));
+    if (!ctx->evEnd) HIP_TRY(hipEventCreate(&ctx->evEnd));
+    if (ctx->countersOn) HIP_TRY(hipMemsetAsync(ctx->dCtr.ptr, 0, SR_COUNTER_COUNT * sizeof(unsigned long long), ctx->stream));
+    HIP_TRY(hipEventRecord(ctx->evStart, ctx->stream));
+    ctx->profUsed = 0;
+    const ViewD vd = make_view(view, p->width, p->height);
+    for (uint32_t y0 = 0; y0 < p->height; y0 += rowsPer) {
+        const uint32_t rows = std::min(rowsPer, p->height - y0);
+        Frame F = frame_for(ctx, y0, rows);
+        F.view = vd;
+        if (p->rng_policy == SR_RNG_COUNTER) {
+            if (ctx->countersOn) enqueue_band<CounterRng, true>(ctx, F);
+            else enqueue_band<CounterRng, false>(ctx, F);
+        } else {
+            if (ctx->countersOn) enqueue_band<XorwowRng, true>(ctx, F);
+            else enqueue_band<XorwowRng, false>(ctx, F);
+        }
+        HIP_TRY(hipGetLastError());
+        hipEvent_t ev;
+        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(ev, ctx->stream));
+        ctx->bandEvents.push_back(ev);
+        ctx->bandEnd.push_back(y0 + rows);
+    }
+    HIP_TRY(hipEventRecord(ctx->evEnd, ctx->stream));
+    ctx->bandsDone = 0;
+    ctx->rendering = true;
+    return SR_OK;
+}
+
+int sr_render_tick(sr_ctx* ctx, uint32_t* step, uint32_t* total) {
+    if (!ctx || !step || !total) return fail(SR_ERR_INVALID, "tick: null argument");
+    if (!ctx->rendering) return fail(SR_ERR_STATE, "no render in progress");
+    const sr_render_params& p = ctx->params;
+    const uint32_t bx = (p.width + p.block_size - 1) / p.block_size;
+    const uint32_t by = (p.height + p.block_size - 1) / p.block_size;
+    *total = bx * by;
+    if (ctx->bandsDone < ctx->bandEvents.size()) {
+        HIP_TRY(hipEventSynchronize(ctx->bandEvents[ctx->bandsDone]));
+        ctx->bandsDone++;
+    }
+    const uint32_t rowsDone = ctx->bandsDone ? ctx->bandEnd[ctx->bandsDone - 1] : 0;
+    *step = rowsDone >= p.height ? *total : bx * (rowsDone / p.block_size);
+    return SR_OK;
+}
+
+int sr_render_read(sr_ctx* ctx, float* rad, float* radFog, float* nrm, float* uv, float* depth, uint32_t* samples,
+                   float* var) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    if (!ctx->rendering) return fail(SR_ERR_STATE, "no render in progress");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->bandsDone = ctx->bandEvents.size();
+    const size_t px = (size_t)ctx->params.width * ctx->params.height;
+    if (rad) HIP_TRY(hipMemcpy(rad, ctx->dRad.ptr, px * 12, hipMemcpyDeviceToHost));
+    if (radFog) HIP_TRY(hipMemcpy(radFog, ctx->dRadFog.ptr, px * 12, hipMemcpyDeviceToHost));
+    if (nrm) HIP_TRY(hipMemcpy(nrm, ctx->dNormal.ptr, px * 12, hipMemcpyDeviceToHost));
+    if (uv) HIP_TRY(hipMemcpy(uv, ctx->dUv.ptr, px * 8, hipMemcpyDeviceToHost));
+    if (depth) HIP_TRY(hipMemcpy(depth, ctx->dDepth.ptr, px * 4, hipMemcpyDeviceToHost));
+    if (samples) HIP_TRY(hipMemcpy(samples, ctx->dSamples.ptr, px * 4, hipMemcpyDeviceToHost));
+    if (var) HIP_TRY(hipMemcpy(var, ctx->dVar.ptr, px * 4, hipMemcpyDeviceToHost));
+    return SR_OK;
+}
+
+int sr_render_device_aovs(sr_ctx* ctx, void** rad, void** radFog, void** nrm, void** uv, void** depth, void** samples,
+                          void** var) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    if (!ctx->rendering) return fail(SR_ERR_STATE, "no render in progress");
+    if (rad) *rad = ctx->dRad.ptr;
+    if (radFog) *radFog = ctx->dRadFog.ptr;
+    if (nrm) *nrm = ctx->dNormal.ptr;
+    if (uv) *uv = ctx->dUv.ptr;
+    if (depth) *depth = ctx->dDepth.ptr;
+    if (samples) *samples = ctx->dSamples.ptr;
+    if (var) *var = ctx->dVar.ptr;
+    return SR_OK;
+}
+
+int sr_render_gpu_seconds(sr_ctx* ctx, double* out) {
+    if (!ctx || !out) return fail(SR_ERR_INVALID, "null argument");
+    if (!ctx->rendering) return fail(SR_ERR_STATE, "no render in progress");
+    HIP_TRY(hipEventSynchronize(ctx->evEnd));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->evStart, ctx->evEnd));
+    *out = ms * 1e-3;
+    return SR_OK;
+}
+
+int sr_profile_enable(sr_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    ctx->profileOn = enable != 0;
+    return SR_OK;
+}
+
+int sr_profile_read(sr_ctx* ctx, double* ms, uint32_t* launches) {
+    if (!ctx || !ms || !launches) return fail(SR_ERR_INVALID, "null argument");
+    if (!ctx->rendering || !ctx->profileOn) return fail(SR_ERR_STATE, "kernel timing was not enabled");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int k = 0; k < 4; ++k) { ms[k] = 0.0; launches[k] = 0; }
+    for (size_t i = 1; i < ctx->profUsed; ++i) {
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, ctx->profEvents[i - 1], ctx->profEvents[i]));
+        const int k = ctx->profKinds[i];
+        ms[k] += t;
+        launches[k] += 1;
+    }
+    return SR_OK;
+}
+
+int sr_counters_enable(sr_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    ctx->countersOn = enable != 0;
+    return SR_OK;
+}
+
+int sr_counters_read(sr_ctx* ctx, int64_t* out) {
+    if (!ctx || !out) return fail(SR_ERR_INVALID, "null argument");
+    if (!ctx->rendering || !ctx->countersOn) return fail(SR_ERR_STATE, "counters were not collected");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    unsigned long long v[SR_COUNTER_COUNT];
+    HIP_TRY(hipMemcpy(v, ctx->dCtr.ptr, sizeof(v), hipMemcpyDeviceToHost));
+    for (int i = 0; i < SR_COUNTER_COUNT; ++i) out[i] = (int64_t)v[i];
+    // Host-known counters: Pixel, Sample (per pixel: its sample count, capped at maxSamples).
+    return SR_OK;
+}
+
+int sr_trace_primary(sr_ctx* ctx, const sr_view* view, uint32_t w, uint32_t h, int32_t* obj, int32_t* tri, float* dist) {
+    if (!ctx || !view || !obj || !tri || !dist || w == 0 || h == 0) return fail(SR_ERR_INVALID, "trace_primary: bad arguments");
+    if (ctx->device < 0) return fail(SR_ERR_STATE, "host-only context (device -1) cannot render");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->built) {
+        int rc = sr_scene_build(ctx);
+        if (rc) return rc;
+    }
+    if (ctx->rendering) HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const size_t px = (size_t)w * h;
+    int32_t *dObj = nullptr, *dTri = nullptr;
+    float* dDist = nullptr;
+    HIP_TRY(hipMalloc((void**)&dObj, px * 4));
+    HIP_TRY(hipMalloc((void**)&dTri, px * 4));
+    HIP_TRY(hipMalloc((void**)&dDist, px * 4));
+    sr_render_params saved = ctx->params;
+    ctx->params.width = w;
+    ctx->params.height = h;
+    Frame F = frame_for(ctx, 0, h);
+    ctx->params = saved;
+    F.view = make_view(view, w, h);
+    hipLaunchKernelGGL(k_primary, dim3((F.nSlots + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, F, ctx->dev, dObj,
+                       dTri, dDist);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(obj, dObj, px * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(tri, dTri, px * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(dist, dDist, px * 4, hipMemcpyDeviceToHost);
+    (void)hipFree(dObj);
+    (void)hipFree(dTri);
+    (void)hipFree(dDist);
+    if (e != hipSuccess) return fail(SR_ERR_HIP, hipGetErrorString(e));
+    return SR_OK;
+}
+
+int sr_bvh_export(sr_ctx* ctx, int32_t mesh, float* nodes, int32_t* items, uint32_t* node_count, uint32_t* item_count) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    const BvhResult* b = nullptr;
+    if (mesh < 0) {
+        if (!ctx->built) return fail(SR_ERR_STATE, "Scene not built");
+        b = &ctx->sceneBvh;
+    } else {
+        if (mesh >= (int32_t)ctx->meshes.size()) return fail(SR_ERR_INVALID, "bad mesh id");
+        b = &ctx->meshes[mesh].bvh;
+    }
+    if (node_count) *node_count = (uint32_t)b->used;
+    if (item_count) *item_count = (uint32_t)b->items.size();
+    if (nodes) {
+        for (int i = 0; i < b->used; ++i) {
+            const RefNode& nd = b->nodes[i];
+            float* o = nodes + (size_t)i * 8;
+            o[0] = nd.bounds.mn.x; o[1] = nd.bounds.mn.y; o[2] = nd.bounds.mn.z;
+            o[3] = nd.bounds.mx.x; o[4] = nd.bounds.mx.y; o[5] = nd.bounds.mx.z;
+            memcpy(o + 6, &nd.child, 4);
+            memcpy(o + 7, &nd.count, 4);
+        }
+    }
+    if (items) memcpy(items, b->items.data(), b->items.size() * 4);
+    return SR_OK;
+}
+
+int sr_denoise_bilateral_device(sr_ctx* ctx, const sr_denoise_params* p, const float* rad, const float* nrm,
+                                const float* dep, const float* var, uint32_t W, uint32_t H, float* out) {
+    if (!ctx || !p || !rad || !nrm || !dep || !var || !out || W == 0 || H == 0)
+        return fail(SR_ERR_INVALID, "denoise: bad arguments");
+    if (ctx->device < 0) return fail(SR_ERR_STATE, "host-only context (device -1) cannot render");
+    HIP_TRY(hipSetDevice(ctx->device));
+    FilterParams fp;
+    // Compositor ctor (compose.cs:42-44) and Filter's per-image constants (compose.cs:113-115).
+    fp.strength = p->strength;
+    fp.strengthMax = p->strength_max;
+    fp.lumBoost = p->luminance_boost;
+    fp.lumLimitInv = 1.0f / (p->luminance_limit * p->luminance_limit * 2.0f);
+    fp.normalLimitInv = 1.0f / (p->normal_limit * p->normal_limit * 2.0f);
+    fp.depthLimitInv = 1.0f / (p->depth_limit * p->depth_limit * 2.0f);
+    const float radiusPixels = p->radius * sqrtf((float)(W * H));
+    fp.radiusInv = 1.0f / (radiusPixels * radiusPixels * 2.0f);
+    fp.kr = (int32_t)ceilf(radiusPixels * 3.0f);
+    hipLaunchKernelGGL(k_filter, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, ctx->stream, fp, rad, nrm, dep, var,
+                       (int32_t)W, (int32_t)H, out);
+    HIP_TRY(hipGetLastError());
+    return SR_OK;
+}
+
+int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* p, float* out_rgb) {
+    if (!ctx || !p || !out_rgb) return fail(SR_ERR_INVALID, "denoise: null argument");
+    if (!ctx->rendering) return fail(SR_ERR_STATE, "no render to denoise");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const uint32_t W = ctx->params.width, H = ctx->params.height;
+    float* dOut = nullptr;
+    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 12));
+    int rc = sr_denoise_bilateral_device(ctx, p, ctx->dRad.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, ctx->dVar.ptr, W, H, dOut);
+    hipError_t e = rc == SR_OK ? hipStreamSynchronize(ctx->stream) : hipSuccess;
+    if (rc == SR_OK && e == hipSuccess) e = hipMemcpy(out_rgb, dOut, (size_t)W * H * 12, hipMemcpyDeviceToHost);
+    (void)hipFree(dOut);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SR_ERR_HIP, hipGetErrorString(e));
+    return SR_OK;
+}
+
+int sr_triangles_build(const float* pos, const float* nrm, const float* uv, uint32_t n, int32_t smooth, float* out) {
+    if ((n && (!pos || !out))) return fail(SR_ERR_INVALID, "triangles: null argument");
+    build_triangles(pos, nrm, uv, n, smooth != 0, out);
+    return SR_OK;
+}
+
+}  // extern "C"
