@@ -25,9 +25,11 @@ _SIGS = {
     "oc_rng_uints": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _U]),
     "oc_rng_floats": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
     "oc_splitmix64": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_uint32]),
+    "oc_philox": (None, [_U, ctypes.c_uint32, ctypes.c_uint32, _U]),
     "oc_rng_sphere": (None, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
     "oc_tri_intersect": (None, [_F, _F, ctypes.c_uint32, _F]),
     "oc_box_intersect": (None, [_F, _F, ctypes.c_uint32, _F]),
+    "oc_mesh_intersect": (None, [_V, ctypes.c_int, _F, ctypes.c_uint32, _I, _F]),
     "oc_brdf_scalar": (None, [ctypes.c_int, _F, _F, ctypes.c_uint32, _F]),
     "oc_ggx_sample": (None, [ctypes.c_float, _F, ctypes.c_uint32, _F]),
     "oc_hg_scatter": (None, [_F, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
@@ -49,7 +51,7 @@ _SIGS = {
                                 _F, _F]),
     "oc_render": (ctypes.c_int, [_V, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_float, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
-                                 ctypes.c_int, _F, _F, _F, _F, _F, _U, _F, _L]),
+                                 ctypes.c_int, _F, _F, _F, _F, _F, _U, _F, _L, ctypes.c_int, ctypes.c_uint32]),
     "oc_filter": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _F, _L]),
     "oc_sky_eval": (None, [_V, _F, ctypes.c_uint32, _F, _F]),
     "oc_sky_sample": (None, [_V, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
@@ -140,6 +142,14 @@ class OracleScene:
         self.L.oc_bvh_export(self.h, level, mid, nodes.ctypes.data_as(_F), items.ctypes.data_as(_I))
         return nodes, items
 
+    def mesh_intersect(self, mesh, rays):
+        rays = _keep(rays, np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        idx = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        self.L.oc_mesh_intersect(self.h, mesh, f(rays), n, idx.ctypes.data_as(_I), t.ctypes.data_as(_F))
+        return idx, t
+
     def bvh_stats(self, mesh=None):
         level, mid = (0, 0) if mesh is None else (1, mesh)
         ints = np.zeros(7, np.int32)
@@ -162,7 +172,7 @@ class OracleScene:
         shp = (y1 - y0, s.width)
         return obj.reshape(shp), prim.reshape(shp), t.reshape(shp), nrm.reshape(shp + (3,))
 
-    def render(self, y0=0, y1=None, threads=None, counters=False):
+    def render(self, y0=0, y1=None, threads=None, counters=False, rng_policy=0, rng_seed=0):
         s = self.spec
         y1 = s.height if y1 is None else y1
         threads = threads or os.cpu_count() or 1
@@ -176,7 +186,8 @@ class OracleScene:
                               out["radiance"].ctypes.data_as(_F), out["radiance_fog"].ctypes.data_as(_F),
                               out["normals"].ctypes.data_as(_F), out["uv"].ctypes.data_as(_F),
                               out["depth"].ctypes.data_as(_F), out["samples"].ctypes.data_as(_U),
-                              out["variance"].ctypes.data_as(_F), ctr.ctypes.data_as(_L) if counters else None)
+                              out["variance"].ctypes.data_as(_F), ctr.ctypes.data_as(_L) if counters else None,
+                              int(rng_policy), int(rng_seed))
         if rc != 0:
             raise RuntimeError("oracle: render on an unbuilt scene")
         if counters:

@@ -170,8 +170,36 @@ struct Rng {
         s2 = (uint32_t)v2v; s3 = (uint32_t)(v2v >> 32);
         s4 = 0;
     }
-    void seed_xy(uint32_t a, uint32_t b) { seed(((uint64_t)a << 32) | b); }
+    void seed_xy(uint32_t a, uint32_t b) { seed(((uint64_t)a << 32) | b); counter = false; }
+    // Counter policy (no reference counterpart): draw k of sample `sample` of pixel `pixel` is word
+    // k mod 4 of Philox4x32-10(counter = {sample, k / 4, 0, 0}, key = {seed, pixel}).
+    bool counter = false;
+    uint32_t key0 = 0, key1 = 0, sample = 0, draw = 0;
+    void seed_counter(uint32_t seedv, uint32_t pixel, uint32_t sampleIndex) {
+        counter = true; key0 = seedv; key1 = pixel; sample = sampleIndex; draw = 0;
+    }
+    // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of two 32x32->64 multiplies
+    // with Weyl-incremented keys.
+    static void philox(uint32_t ctr[4], uint32_t k0, uint32_t k1) {
+        const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+        for (int round = 0; round < 10; ++round) {
+            uint64_t a = (uint64_t)M0 * ctr[0];
+            uint64_t b = (uint64_t)M1 * ctr[2];
+            uint32_t x0 = (uint32_t)(b >> 32) ^ ctr[1] ^ k0;
+            uint32_t x1 = (uint32_t)b;
+            uint32_t x2 = (uint32_t)(a >> 32) ^ ctr[3] ^ k1;
+            uint32_t x3 = (uint32_t)a;
+            ctr[0] = x0; ctr[1] = x1; ctr[2] = x2; ctr[3] = x3;
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+    }
     uint32_t next_uint() {
+        if (counter) {
+            uint32_t c[4] = {sample, draw / 4u, 0u, 0u};
+            philox(c, key0, key1);
+            return c[draw++ % 4u];
+        }
         uint32_t t = s3;
         uint32_t s = s0;
         s3 = s2; s2 = s1; s1 = s;
@@ -1332,6 +1360,8 @@ struct View {
 struct RenderParams {
     uint32_t width, height, minSamples, maxSamples, bounces;
     real varianceThreshold, indirectClamp;
+    int policy;     // 0 reference xorwow stream per pixel, 1 counter (Philox) per sample
+    uint32_t seed;  // counter policy key
 };
 
 // Renderer.RelativeStdErr, renderer.cs:232-237.
@@ -1366,6 +1396,7 @@ PixelOut render_pixel(const Scene& sc, const View& view, const RenderParams& p, 
     uint32_t depthCount = 0;
     uint32_t sampleIndex = 0;
     for (; sampleIndex != p.maxSamples; ++sampleIndex) {
+        if (p.policy == 1) rng.seed_counter(p.seed, y * p.width + x, sampleIndex);
         real jx = (real)rng.next_float();
         real jy = (real)rng.next_float();
         V2 pos = v2(((real)x + jx) / (real)p.width, ((real)y + jy) / (real)p.height);
@@ -1511,6 +1542,12 @@ void oc_rng_floats(uint32_t a, uint32_t b, uint32_t n, float* out) {
     Rng r; r.seed_xy(a, b);
     for (uint32_t i = 0; i != n; ++i) out[i] = r.next_float();
 }
+// Philox4x32-10 block for known-answer tests: ctr[4] in, out[4].
+void oc_philox(const uint32_t* ctr, uint32_t k0, uint32_t k1, uint32_t* out) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    Rng::philox(c, k0, k1);
+    for (int i = 0; i != 4; ++i) out[i] = c[i];
+}
 uint64_t oc_splitmix64(uint64_t state, uint32_t steps) {
     uint64_t v = 0;
     for (uint32_t i = 0; i != steps; ++i) v = Rng::split_mix64(state);
@@ -1544,6 +1581,18 @@ void oc_box_intersect(const float* box6, const float* rays6, uint32_t n, float* 
         bool ok = box_intersect_dist(b, r, t);
         out2[i * 2] = ok ? 1.0f : 0.0f;
         out2[i * 2 + 1] = (float)t;
+    }
+}
+
+// Mesh.Intersect through the mesh BVH (bvh.cs:165-216) for a batch of rays; idx = -1 on a miss.
+void oc_mesh_intersect(void* s, int mesh, const float* rays6, uint32_t n, int32_t* idxOut, float* tOut) {
+    const Mesh& m = ((Scene*)s)->meshes[mesh];
+    for (uint32_t i = 0; i != n; ++i) {
+        Ray r = make_ray(ld3(rays6 + i * 6), ld3(rays6 + i * 6 + 3));
+        ShapeHit h{};
+        int idx = -1;
+        if (m.intersect(r, nullptr, h, idx)) { idxOut[i] = idx; tOut[i] = (float)h.dist; }
+        else { idxOut[i] = -1; tOut[i] = INFINITY; }
     }
 }
 
@@ -1768,11 +1817,11 @@ void oc_trace_primary(void* s, const float* view9, uint32_t W, uint32_t H, uint3
 int oc_render(void* s, const float* view9, uint32_t W, uint32_t H, uint32_t minS, uint32_t maxS, float threshold,
               uint32_t bounces, float indirectClamp, uint32_t y0, uint32_t y1, int threads,
               float* radiance, float* radianceFog, float* normals, float* uvs, float* depth, uint32_t* samples,
-              float* variance, int64_t* counters) {
+              float* variance, int64_t* counters, int policy, uint32_t seed) {
     Scene* sc = (Scene*)s;
     if (!sc->built) return -1;
     View vw = ld_view(view9);
-    RenderParams p{W, H, minS, maxS, bounces, (real)threshold, (real)indirectClamp};
+    RenderParams p{W, H, minS, maxS, bounces, (real)threshold, (real)indirectClamp, policy, seed};
     real tanFov = std::tan(vw.fov * 0.5f);
     const uint32_t bs = 32;
     uint32_t rows = y1 - y0;

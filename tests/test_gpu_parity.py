@@ -86,6 +86,20 @@ def test_radiance_parity_plain_cpu(plain_cpu, oracle_lib):
     assert np.all(rel_err(r.variance[ok, None], ref["variance"][ok, None]) <= 1e-2)
 
 
+def test_radiance_parity_counter_rng(plain_cpu, oracle_lib):
+    """Counter policy: each sample draws from Philox(seed, pixel, sample), so a divergent path cannot
+    shift the draws of the pixel's other samples."""
+    from sharpray_amd import RNG_COUNTER, scenes
+    spec = plain_cpu
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=1234)
+    r.wait()
+    ref = oracle_lib.OracleScene(spec).render(rng_policy=1, rng_seed=1234)
+    err = rel_err(r.radiance, ref["radiance"])
+    frac = float((err > RAD_TOL).mean())
+    print(f"plain_cpu counter-rng radiance: divergent fraction {frac:.5f}", flush=True)
+    assert frac <= MAX_DIVERGENT
+
+
 def test_radiance_parity_analytic_shapes(oracle_lib):
     from sharpray_amd import scenes
     spec = scenes.config_analytic(scale=1.0, spp=4)
