@@ -67,7 +67,18 @@ def main():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (torch clone), to calibrate HBM counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        for _ in range(3):
+            y = x.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "checksum": float(y[:16].sum())}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (pixels shard trivially across GPUs; see the "
               "sharding notes); run with --gpus 1", file=sys.stderr)

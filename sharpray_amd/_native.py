@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsharpray_hip.so")
+# SHARPRAY_LIB selects another build of the same library (e.g. to compare compile options).
+LIB_PATH = os.environ.get("SHARPRAY_LIB") or os.path.join(_HERE, "libsharpray_hip.so")
 
 SR_OK = 0
 SR_RNG_REFERENCE = 0

@@ -6,10 +6,10 @@
 #  -packed-fp32-ops disabled    no v_pk_*_f32 from the vectoriser
 set -euo pipefail
 here="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
-out="${here}/../libsharpray_hip.so"
+out="${SR_OUT:-${here}/../libsharpray_hip.so}"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 ARCH="${SR_OFFLOAD_ARCH:-gfx950}"
-COMMON=(-std=c++17 -O3 -fPIC -ffp-contract=off -fno-fast-math -Wall -Wno-unused-function)
+COMMON=(-std=c++17 -O3 -fPIC -ffp-contract=off -fno-fast-math -Wall -Wno-unused-function ${SR_EXTRA_FLAGS:-})
 DEVICE=(--offload-arch="${ARCH}" -fhip-fp32-correctly-rounded-divide-sqrt -fno-gpu-flush-denormals-to-zero
         -Xclang -target-feature -Xclang -packed-fp32-ops)
 tmp="$(mktemp -d)"

@@ -1,48 +1,55 @@
 // gfx950 wavefront path tracer behind the sharpray.h C ABI.
 //
-// One camera path per pixel is in flight. A frame is rendered sample-major: every sample round
-// runs raygen -> (extend -> shade -> shadow) x (bounces + 1) -> accumulate over a band of pixels.
-// Paths that terminate are compacted out of the ray queue by shade (wave-aggregated atomic
-// appends), so extend and shadow only ever see live rays. The per-pixel RNG state persists in HBM
-// between rounds, which keeps the reference's sequential per-pixel xorwow stream intact
-// (renderer.cs:156-212), and the per-pixel sums are updated in sample order.
+// Every pixel of a band owns one camera path at a time and the frame is rendered sample-major:
+// for each sample, (bounces + 1) segments of three kernels run over compacted queues,
+//   extend  - closest hit for every queued ray (two-level BVH traversal);
+//   shade   - one path segment (scene.cs:641-737): fog scatter, surface hit or sky miss; the
+//             direct-light sample becomes a deferred shadow ray; live paths are appended to the
+//             next segment's queue;
+//   shadow  - shadow transmittance and the deferred direct-light contribution;
+// then `next` accumulates the sample, applies the convergence test and emits the pixel's next
+// camera ray (renderer.cs:173-229). All rays of a launch sit at the same path depth and camera
+// rays are queued in 8x8-tile order, which keeps waves coherent (mixing depths by regenerating
+// paths as soon as they end measured slower on this workload). The per-pixel RNG state persists
+// in HBM between samples, which keeps the reference's sequential xorwow stream per pixel
+// (renderer.cs:156) and its sample-order sums. extend, shade and shadow run over a fixed grid with
+// a wave-aligned grid-stride loop (one shared atomic work cursor measured slower).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
 
 #include <algorithm>
+#include <condition_variable>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sharpray.h"
 #include "sr_device.h"
 #include "sr_host.h"
+#include "sr_trace.h"
 
 using namespace sr;
 
 namespace {
 
-constexpr float kInf = __builtin_huge_valf();
-constexpr int kBlock = 256;
-constexpr uint32_t kMaxChild = (1u << 24) - 1;
-constexpr int kStack = 64;  // bvh.cs:436 caps depth at 64, so at most 64 deferred siblings
+constexpr int kBlock = kTravBlock;
+// Per-lane LDS columns of the mesh traversal stack (kTravLdsBytes per block).
+extern __shared__ uint2 sr_lds[];
 
 // ---- Frame state --------------------------------------------------------------------------------
-struct alignas(16) PathRec {  // 128 B per path
-    vec3 o, d;
+struct alignas(16) PathRec {  // 96 B per path: everything shade keeps between segments
     vec3 energy;
-    vec3 rad, radFog;
     float scatterPdf;
-    vec3 medium;
+    vec3 rad;
     uint32_t flags;  // bit0: scatterPdf valid, bit1: inside a medium
+    vec3 radFog;
+    uint32_t seg;    // segment index of the queued ray (0 = camera ray)
+    vec3 medium;
+    uint32_t pad0;
     uint32_t rng[5];
-    uint32_t pad[6];
-};
-struct alignas(16) HitRec {  // 32 B
-    float t, u, v;
-    int32_t obj, slot;
-    int32_t pad[3];
+    uint32_t pad1[3];
 };
 struct alignas(16) ShadowRec {  // 80 B
     vec3 o, d;
@@ -70,14 +77,21 @@ struct ViewD {
     float nearPlane, tanFov, aspect;
 };
 
+// Queue counters (uint32 each).
+enum : int {
+    kRayCount0 = 0, kRayCount1 = 1, kShadowCount = 2, kCounterWords = 4
+};
+
 struct Frame {
+    float4* rayO;       // origin per slot (xyz)
+    float4* rayD;       // direction per slot (xyz)
+    float4* hitA;       // t, u, v, object (int bits)
+    int32_t* hitSlot;   // mesh item slot
     PathRec* paths;
-    HitRec* hits;
     ShadowRec* shadows;
     PixAcc* acc;
     uint32_t* queue[2];
-    uint32_t* shadowQueue;
-    uint32_t* counts;  // [0],[1] ray queues, [2] shadow queue
+    uint32_t* counts;   // kCounterWords words
     unsigned long long* ctr;  // SR_COUNTER_COUNT counters or null
     float* outRad;
     float* outRadFog;
@@ -124,296 +138,59 @@ template <> SR_DEV CounterRng rng_load<CounterRng>(const PathRec& p) {
     r.key0 = p.rng[0]; r.key1 = p.rng[1]; r.sample = p.rng[2]; r.draw = p.rng[3];
     return r;
 }
-SR_DEV void rng_store(PathRec* p, const XorwowRng& r) {
-    p->rng[0] = r.s0; p->rng[1] = r.s1; p->rng[2] = r.s2; p->rng[3] = r.s3; p->rng[4] = r.s4;
+SR_DEV void rng_store(PathRec& p, const XorwowRng& r) {
+    p.rng[0] = r.s0; p.rng[1] = r.s1; p.rng[2] = r.s2; p.rng[3] = r.s3; p.rng[4] = r.s4;
 }
-SR_DEV void rng_store(PathRec* p, const CounterRng& r) {
-    p->rng[0] = r.key0; p->rng[1] = r.key1; p->rng[2] = r.sample; p->rng[3] = r.draw;
+SR_DEV void rng_store(PathRec& p, const CounterRng& r) {
+    p.rng[0] = r.key0; p.rng[1] = r.key1; p.rng[2] = r.sample; p.rng[3] = r.draw;
 }
-
-// ---- Traversal ----------------------------------------------------------------------------------
-struct Tally {
-    uint32_t box, shape, mesh;
-};
-
-SR_DEV uint32_t pack_node(float childBits, float countBits) {
-    return ((uint32_t)__float_as_int(countBits) << 24) | (uint32_t)__float_as_int(childBits);
+// The RNG that draws sample `sample` of pixel (x, y): the pixel's persistent xorwow stream, or a
+// fresh Philox counter for that (pixel, sample).
+template <class R> SR_DEV R rng_for_sample(const Frame& F, const PathRec& p, uint32_t x, uint32_t y, uint32_t sample);
+template <> SR_DEV XorwowRng rng_for_sample<XorwowRng>(const Frame&, const PathRec& p, uint32_t, uint32_t, uint32_t) {
+    return rng_load<XorwowRng>(p);
 }
-
-// Closest hit in one mesh BVH (bvh.cs:165-216 + mesh.cs:51-62). Nearer child first, ties to
-// child B; prune when the entry distance is >= the best hit; accept strictly closer hits.
-template <bool COUNT>
-SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, float& bestT,
-                         float& bu, float& bv, int32_t& bslot, Tally& tl) {
-    const float4* N = S.nodes;
-    uint32_t stk[kStack];
-    float stt[kStack];
-    int sp = 0;
-    bestT = kInf;
-    bool found = false;
-    const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
-    float curT = 0.0f;
-    for (;;) {
-        bool next = false;
-        if (curT < bestT) {
-            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
-            if (cnt == 0) {
-                if (COUNT) tl.box += 2;
-                const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)child) * 2;
-                const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
-                float tA = 0.0f, tB = 0.0f;
-                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= bestT;
-                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= bestT;
-                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
-                if (hA && hB) {
-                    const bool aNear = tA < tB;
-                    stk[sp] = aNear ? pB : pA;
-                    stt[sp] = aNear ? tB : tA;
-                    ++sp;
-                    cur = aNear ? pA : pB;
-                    curT = aNear ? tA : tB;
-                    next = true;
-                } else if (hA) {
-                    cur = pA; curT = tA; next = true;
-                } else if (hB) {
-                    cur = pB; curT = tB; next = true;
-                }
-            } else {
-                if (COUNT) tl.shape += cnt;
-                const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
-                for (uint32_t i = 0; i < cnt; ++i, T += 9) {
-                    float t, u, v;
-                    if (tri_hit(T, r, t, u, v) && t < bestT) {
-                        bestT = t; bu = u; bv = v;
-                        bslot = (int32_t)(child + i);
-                        found = true;
-                    }
-                }
-            }
-        }
-        if (!next) {
-            if (sp == 0) break;
-            --sp;
-            cur = stk[sp];
-            curT = stt[sp];
-        }
-    }
-    return found;
+template <> SR_DEV CounterRng rng_for_sample<CounterRng>(const Frame& F, const PathRec&, uint32_t x, uint32_t y,
+                                                         uint32_t sample) {
+    CounterRng r;
+    r.key0 = F.seed;
+    r.key1 = y * F.W + x;
+    r.sample = sample;
+    r.draw = 0;
+    return r;
 }
 
-// Whether any primitive of a mesh is hit (the same node visits as the closest-hit traversal up
-// to its first accepted hit, which is where the existence of a hit is decided).
-SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r) {
-    const float4* N = S.nodes;
-    uint32_t stk[kStack];
-    int sp = 0;
-    const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
-    for (;;) {
-        bool next = false;
-        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
-        if (cnt == 0) {
-            const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)child) * 2;
-            const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
-            float tA = 0.0f, tB = 0.0f;
-            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
-            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
-            const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
-            if (hA && hB) {
-                const bool aNear = tA < tB;
-                stk[sp++] = aNear ? pB : pA;
-                cur = aNear ? pA : pB;
-                next = true;
-            } else if (hA) {
-                cur = pA; next = true;
-            } else if (hB) {
-                cur = pB; next = true;
-            }
-        } else {
-            const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
-            for (uint32_t i = 0; i < cnt; ++i, T += 9) {
-                float t, u, v;
-                if (tri_hit(T, r, t, u, v)) return true;
-            }
-        }
-        if (!next) {
-            if (sp == 0) return false;
-            cur = stk[--sp];
-        }
-    }
+// Camera ray of one sample: jitter X then Y (renderer.cs:175), View.Ray (math.cs:1613-1636).
+template <class R>
+SR_DEV void camera_ray(const ViewD& v, uint32_t W, uint32_t H, uint32_t x, uint32_t y, R& rng, vec3& o, vec3& d) {
+    const float jx = next_float(rng);
+    const float jy = next_float(rng);
+    const vec2 sp = mk2(((float)x + jx) / (float)W, ((float)y + jy) / (float)H);
+    const float ndcX = sp.x * 2.0f - 1.0f;
+    const float ndcY = -sp.y * 2.0f + 1.0f;
+    float th, tv;
+    if (v.aspect >= 1.0f) { th = v.tanFov; tv = th / v.aspect; }
+    else { tv = v.tanFov; th = tv * v.aspect; }
+    o = v.pos + rotate(v.rot, mk3(0.0f, 0.0f, v.nearPlane));
+    d = rotate(v.rot, normalized(mk3(ndcX * th, ndcY * tv, 1.0f)));
 }
 
-// Object.Intersect (scene.cs:227 -> math.cs:67-79): distance in world units only; the hit
-// record keeps (u, v, slot) so shading can rebuild the full ShapeHit.
-template <bool COUNT>
-SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r, float& dist, float& u, float& v,
-                           int32_t& slot, Tally& tl) {
-    float scale;
-    const ray3 lr = to_object(ob, r, scale);
-    float t = 0.0f;
-    slot = -1;
-    u = v = 0.0f;
-    bool hit;
-    if (ob.kind == 0) {
-        if (COUNT) tl.mesh += 1;
-        hit = ob.nodeBase >= 0 && mesh_closest<COUNT>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, tl);
-    } else {
-        ShapeHitD h;
-        hit = ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
-        t = h.dist;
-    }
-    if (!hit) return false;
-    dist = t / scale;
-    return true;
-}
-
-SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r) {
-    float scale;
-    const ray3 lr = to_object(ob, r, scale);
-    if (ob.kind == 0) return ob.nodeBase >= 0 && mesh_any(S, ob.nodeBase, ob.triBase, lr);
-    ShapeHitD h;
-    return ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
-}
-
-// Scene BVH over objects (costI 10, 32 bins; scene.cs:528-533), two-level closest hit.
-template <bool COUNT>
-SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& tl) {
-    if (S.sceneNodes == 0) return false;
-    const float4* N = S.nodes;
-    uint32_t stk[kStack];
-    float stt[kStack];
-    int sp = 0;
-    float best = kInf;
-    bool found = false;
-    const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
-    float curT = 0.0f;
-    for (;;) {
-        bool next = false;
-        if (curT < best) {
-            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
-            if (cnt == 0) {
-                if (COUNT) tl.box += 2;
-                const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)child) * 2;
-                const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
-                float tA = 0.0f, tB = 0.0f;
-                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= best;
-                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= best;
-                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
-                if (hA && hB) {
-                    const bool aNear = tA < tB;
-                    stk[sp] = aNear ? pB : pA;
-                    stt[sp] = aNear ? tB : tA;
-                    ++sp;
-                    cur = aNear ? pA : pB;
-                    curT = aNear ? tA : tB;
-                    next = true;
-                } else if (hA) {
-                    cur = pA; curT = tA; next = true;
-                } else if (hB) {
-                    cur = pB; curT = tB; next = true;
-                }
-            } else {
-                if (COUNT) tl.shape += cnt;
-                for (uint32_t i = 0; i < cnt; ++i) {
-                    const int32_t oi = S.sceneItems[child + i];
-                    float d, u, v;
-                    int32_t slot;
-                    if (object_closest<COUNT>(S, S.objects[oi], r, d, u, v, slot, tl) && d < best) {
-                        best = d;
-                        out.t = d; out.u = u; out.v = v; out.obj = oi; out.slot = slot;
-                        found = true;
-                    }
-                }
-            }
-        }
-        if (!next) {
-            if (sp == 0) break;
-            --sp;
-            cur = stk[sp];
-            curT = stt[sp];
-        }
-    }
-    return found;
-}
-
-SR_DEV bool scene_any(const DevScene& S, const ray3& r) {
-    if (S.sceneNodes == 0) return false;
-    const float4* N = S.nodes;
-    uint32_t stk[kStack];
-    int sp = 0;
-    const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
-    for (;;) {
-        bool next = false;
-        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
-        if (cnt == 0) {
-            const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)child) * 2;
-            const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
-            float tA = 0.0f, tB = 0.0f;
-            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
-            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
-            const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
-            if (hA && hB) {
-                const bool aNear = tA < tB;
-                stk[sp++] = aNear ? pB : pA;
-                cur = aNear ? pA : pB;
-                next = true;
-            } else if (hA) {
-                cur = pA; next = true;
-            } else if (hB) {
-                cur = pB; next = true;
-            }
-        } else {
-            for (uint32_t i = 0; i < cnt; ++i)
-                if (object_any(S, S.objects[S.sceneItems[child + i]], r)) return true;
-        }
-        if (!next) {
-            if (sp == 0) return false;
-            cur = stk[--sp];
-        }
-    }
-}
-
-// Scene.Trace (scene.cs:602-625) from a hit record: world-space ShapeHit, material sampling,
-// normal mapping and tangent re-orthogonalisation.
-SR_DEV void make_surface(const DevScene& S, const ray3& r, const HitRec& h, SurfaceD& s) {
-    const DevObject& ob = S.objects[h.obj];
-    ShapeHitD sh;
-    if (ob.kind == 0) {
-        const int32_t tri = S.slotTri[ob.triBase + h.slot];
-        inflate(S.tris + (size_t)(ob.triBase + h.slot) * 9, S.attrs + (size_t)(ob.triBase + tri) * 27, h.u, h.v, sh);
-    } else {
-        float scale;
-        const ray3 lr = to_object(ob, r, scale);
-        if (ob.kind == 1) sphere_hit(ob.shape, lr, sh, true);
-        else box_hit(ob.shape, lr, sh, true);
-    }
-    const vec3 wn = normalized(rotate(ob.rot, sh.norm / ob.scale));
-    const vec3 wng = normalized(rotate(ob.rot, sh.normGeo / ob.scale));
-    const vec3 wt = normalized(rotate(ob.rot, xyz(sh.tan)));
-    const vec2 uv = sh.uv;
-    const vec3 texC = ob.texColor >= 0 ? tex_sample(S, ob.texColor, uv) : splat3(1.0f);
-    s.albedo = texC * ob.albedo;
-    s.roughness = (ob.texRough >= 0 ? tex_sample(S, ob.texRough, uv).x : 1.0f) * ob.roughness;
-    s.metallic = (ob.texMetal >= 0 ? tex_sample(S, ob.texMetal, uv).x : 1.0f) * ob.metallic;
-    const vec4 htan = mk4(wt.x, wt.y, wt.z, sh.tan.w);
-    vec3 n = wn;
-    if (ob.texNormal >= 0) {
-        const vec3 c = tex_sample(S, ob.texNormal, uv);
-        const vec3 tn = normalized_or(mk3(c.x * 2.0f - 1.0f, -(c.y * 2.0f - 1.0f), c.z * 2.0f - 1.0f), up3());
-        n = normalized_or(from_tangent(tn, htan, wn), wn);
-    }
-    const vec3 td = normalized_or(wt - dot3(wt, n) * n, wt);
-    s.radiance = ob.radiance;
-    s.transparency = ob.transparency;
-    s.ior = ob.ior;
-    s.f0 = ob.f0;
-    s.normal = n;
-    s.normalGeo = wng;
-    s.tangent = mk4(td.x, td.y, td.z, sh.tan.w);
-    s.uv = uv;
+// Starts sample `sample` of slot q: camera ray plus a fresh path record carrying the RNG.
+template <class R>
+SR_DEV void begin_sample(const Frame& F, uint32_t q, uint32_t x, uint32_t y, uint32_t sample, PathRec& p) {
+    R rng = rng_for_sample<R>(F, p, x, y, sample);
+    vec3 o, d;
+    camera_ray(F.view, F.W, F.H, x, y, rng, o, d);
+    F.rayO[q] = make_float4(o.x, o.y, o.z, 0.0f);
+    F.rayD[q] = make_float4(d.x, d.y, d.z, 0.0f);
+    p.energy = splat3(1.0f);
+    p.scatterPdf = 0.0f;
+    p.rad = splat3(0.0f);
+    p.flags = 0;
+    p.radFog = splat3(0.0f);
+    p.seg = 0;
+    p.medium = splat3(0.0f);
+    rng_store(p, rng);
 }
 
 template <bool COUNT>
@@ -428,121 +205,99 @@ SR_DEV void bump(const Frame& F, int idx, uint32_t n) {
 }
 
 // ---- Kernels ------------------------------------------------------------------------------------
-__global__ void k_init(Frame F) {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0) { F.counts[0] = 0; F.counts[1] = 0; F.counts[2] = 0; }
-    if (q >= F.nSlots) return;
-    uint32_t x, y;
-    PixAcc a;
-    memset(&a, 0, sizeof(a));
-    a.active = slot_pixel(F, q, x, y) ? 1u : 0u;
-    F.acc[q] = a;
-    if (a.active) {
-        const XorwowRng r = xorwow_seed(x, y);  // new Rng(x, y), renderer.cs:156
-        rng_store(&F.paths[q], r);
-        if (F.ctr) atomicAdd(&F.ctr[0], 1ull);
-    }
+__global__ void k_reset(Frame F) {
+    if (threadIdx.x < kCounterWords) F.counts[threadIdx.x] = 0;
 }
 
-// Camera ray of sample `s` for every active pixel: jitter (2 draws), View.Ray (math.cs:1613-1636).
+// new Rng(x, y) per pixel (renderer.cs:156), zeroed sums, and sample 0's camera ray.
 template <class R>
-__global__ void k_raygen(Frame F) {
+__global__ void k_start(Frame F) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     bool live = false;
-    if (q < F.nSlots && F.acc[q].active) {
+    if (q < F.nSlots) {
         uint32_t x, y;
-        slot_pixel(F, q, x, y);
-        PathRec* p = &F.paths[q];
-        R rng;
-        if constexpr (__is_same(R, CounterRng)) {
-            rng.key0 = F.seed;
-            rng.key1 = y * F.W + x;
-            rng.sample = F.acc[q].sampleIndex;
-            rng.draw = 0;
-        } else {
-            rng = rng_load<R>(*p);
+        PixAcc a;
+        memset(&a, 0, sizeof(a));
+        live = slot_pixel(F, q, x, y);
+        a.active = live ? 1u : 0u;
+        F.acc[q] = a;
+        if (live) {
+            PathRec p;
+            rng_store(p, xorwow_seed(x, y));
+            begin_sample<R>(F, q, x, y, 0, p);
+            F.paths[q] = p;
+            if (F.ctr) { atomicAdd(&F.ctr[0], 1ull); atomicAdd(&F.ctr[1], 1ull); }
         }
-        const float jx = next_float(rng);
-        const float jy = next_float(rng);
-        const vec2 sp = mk2(((float)x + jx) / (float)F.W, ((float)y + jy) / (float)F.H);
-        const float ndcX = sp.x * 2.0f - 1.0f;
-        const float ndcY = -sp.y * 2.0f + 1.0f;
-        float th, tv;
-        if (F.view.aspect >= 1.0f) { th = F.view.tanFov; tv = th / F.view.aspect; }
-        else { tv = F.view.tanFov; th = tv * F.view.aspect; }
-        const vec3 local = mk3(ndcX * th, ndcY * tv, 1.0f);
-        const vec3 origin = F.view.pos + rotate(F.view.rot, mk3(0.0f, 0.0f, F.view.nearPlane));
-        const vec3 dir = rotate(F.view.rot, normalized(local));
-        p->o = origin;
-        p->d = dir;
-        p->energy = splat3(1.0f);
-        p->rad = splat3(0.0f);
-        p->radFog = splat3(0.0f);
-        p->scatterPdf = 0.0f;
-        p->flags = 0;
-        rng_store(p, rng);
-        live = true;
     }
-    const uint32_t slot = wave_append(&F.counts[0], live);
-    if (live) {
-        F.queue[0][slot] = q;
-        if (F.ctr) atomicAdd(&F.ctr[1], 1ull);
-    }
+    const uint32_t slot = wave_append(&F.counts[kRayCount0], live);
+    if (live) F.queue[0][slot] = q;
 }
 
 // Closest hit for every queued ray.
 template <bool COUNT>
 __global__ void __launch_bounds__(kBlock) k_extend(Frame F, DevScene S, int cur) {
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gtid == 0) { F.counts[cur ^ 1] = 0; F.counts[2] = 0; }
+    // Counters the next two kernels fill start at zero; nothing else touches them meanwhile.
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        F.counts[cur ^ 1] = 0;
+        F.counts[kShadowCount] = 0;
+    }
     const uint32_t n = F.counts[cur];
-    const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t* Q = F.queue[cur];
-    if (COUNT && gtid == 0 && F.ctr) {
+    if (COUNT && blockIdx.x == 0 && threadIdx.x == 0 && F.ctr) {
         atomicAdd(&F.ctr[2], (unsigned long long)n);
         atomicAdd(&F.ctr[10], (unsigned long long)n);
     }
     Tally tl = {0, 0, 0};
-    for (uint32_t i = gtid; i < n; i += stride) {
-        const uint32_t q = Q[i];
-        const PathRec* p = &F.paths[q];
-        const ray3 r = mk_ray(p->o, p->d);
-        HitRec h;
-        h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
-        scene_closest<COUNT>(S, r, h, tl);
-        F.hits[q] = h;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // Grid-stride over the queue with a wave-uniform base, so every lane of a wave runs the same
+    // number of iterations (the wave-level appends need the whole wave).
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const uint32_t q = Q[i];
+            const float4 o = F.rayO[q], d = F.rayD[q];
+            const ray3 r = mk_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z));
+            HitRec h;
+            h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
+            scene_closest<COUNT>(S, r, h, tl, sr_lds + threadIdx.x);
+            F.hitA[q] = make_float4(h.t, h.u, h.v, __int_as_float(h.obj));
+            F.hitSlot[q] = h.slot;
+        }
     }
     flush_tally<COUNT>(F, tl);
 }
 
-// One path segment (scene.cs:641-737): fog scatter, surface hit or sky miss. Direct sky light is
-// resolved as a deferred shadow ray whose contribution the shadow kernel adds before the next
-// segment, which keeps the reference's summation order.
+// One path segment (scene.cs:641-737). Direct sky light becomes a deferred shadow ray whose
+// contribution the shadow kernel adds before the path's next segment, which keeps the reference's
+// summation order.
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur, uint32_t seg) {
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) {
     const uint32_t n = F.counts[cur];
-    const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t* Q = F.queue[cur];
-    const bool primary = seg == 0;
-    const bool last = seg == F.bounces;
     uint32_t cHit = 0, cMiss = 0, cTerm = 0, cFog = 0, cEsc = 0, cOcc = 0;
-    // Loop until every lane of the wave is done so the wave-level appends stay convergent.
-    const uint32_t iters = (n + stride - 1) / stride;
-    for (uint32_t it = 0; it < iters; ++it) {
-        const uint32_t i = gtid + it * stride;
-        const bool valid = i < n;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // Grid-stride over the queue with a wave-uniform base, so every lane of a wave runs the same
+    // number of iterations (the wave-level appends need the whole wave).
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t i = base + lane_id();
         bool cont = false, shadow = false;
         ShadowRec sr;
         uint32_t q = 0;
-        if (valid) {
+        if (i < n) {
             q = Q[i];
             PathRec p = F.paths[q];
-            const HitRec h = F.hits[q];
+            const float4 ho = F.hitA[q];
+            const float4 ro = F.rayO[q], rd = F.rayD[q];
+            HitRec h;
+            h.t = ho.x; h.u = ho.y; h.v = ho.z; h.obj = __float_as_int(ho.w); h.slot = F.hitSlot[q];
             R rng = rng_load<R>(p);
-            const ray3 ray = mk_ray(p.o, p.d);
+            const ray3 ray = mk_ray(mk3(ro.x, ro.y, ro.z), mk3(rd.x, rd.y, rd.z));
+            const uint32_t seg = p.seg;
+            const bool primary = seg == 0;
+            const bool last = seg == F.bounces;
             const bool hit = h.obj >= 0;
             float sd = 0.0f;
+            vec3 no = ray.o, nd = ray.d;
             if (S.fog.enabled && fog_scatter(S.fog, ray, 0.0f, hit ? h.t : kInf, rng, sd)) {
                 ++cFog;
                 p.energy = p.energy * S.fog.color;
@@ -567,11 +322,11 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur, 
                     else p.energy = p.energy / survive;
                 }
                 if (!term) {
-                    const vec3 nd = hg_scatter(ray.d, S.fog.anisotropy, rng);
+                    const vec3 dir = hg_scatter(ray.d, S.fog.anisotropy, rng);
                     p.flags |= 1u;
-                    p.scatterPdf = hg_pdf(dot3(ray.d, nd), S.fog.anisotropy);
-                    p.o = pos;
-                    p.d = nd;
+                    p.scatterPdf = hg_pdf(dot3(ray.d, dir), S.fog.anisotropy);
+                    no = pos;
+                    nd = dir;
                     cont = !last;
                 }
             } else if (hit) {
@@ -631,8 +386,8 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur, 
                     }
                     if (sc.hasPdf) { p.flags |= 1u; p.scatterPdf = sc.pdf; }
                     else p.flags &= ~1u;
-                    p.o = at(ray, dist) + sc.dir * bias;
-                    p.d = sc.dir;
+                    no = at(ray, dist) + sc.dir * bias;
+                    nd = sc.dir;
                     cont = !last;
                 }
             } else {
@@ -646,11 +401,16 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur, 
                 }
                 p.rad = p.rad + sky_radiance(S, ray.d) * mis * p.energy;
             }
-            rng_store(&p, rng);
+            if (cont) {
+                p.seg = seg + 1;
+                F.rayO[q] = make_float4(no.x, no.y, no.z, 0.0f);
+                F.rayD[q] = make_float4(nd.x, nd.y, nd.z, 0.0f);
+            }
+            rng_store(p, rng);
             F.paths[q] = p;
         }
         if (shadow) ++cOcc;
-        const uint32_t ns = wave_append(&F.counts[2], shadow);
+        const uint32_t ns = wave_append(&F.counts[kShadowCount], shadow);
         if (shadow) F.shadows[ns] = sr;
         const uint32_t nc = wave_append(&F.counts[cur ^ 1], cont);
         if (cont) F.queue[cur ^ 1][nc] = q;
@@ -662,23 +422,32 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur, 
 }
 
 // ShadowTransmittance (scene.cs:576-600) and the deferred direct-light contribution.
-__global__ void __launch_bounds__(kBlock) k_shadow(Frame F, DevScene S) {
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n = F.counts[2];
-    const uint32_t stride = gridDim.x * blockDim.x;
+__global__ void __launch_bounds__(kBlock) k_shadow(Frame F, DevScene S, int lastSegment) {
+    // After a sample's last segment both ray queues are spent (the last shade appends nothing);
+    // the next sample's camera rays start in queue 0.
+    if (lastSegment && blockIdx.x == 0 && threadIdx.x == 0) {
+        F.counts[kRayCount0] = 0;
+        F.counts[kRayCount1] = 0;
+    }
+    const uint32_t n = F.counts[kShadowCount];
     uint32_t extra = 0;
-    for (uint32_t i = gtid; i < n; i += stride) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // Grid-stride over the queue with a wave-uniform base, so every lane of a wave runs the same
+    // number of iterations (the wave-level appends need the whole wave).
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t i = base + lane_id();
+        if (i >= n) continue;
         const ShadowRec sr = F.shadows[i];
         ray3 ray = mk_ray(sr.o, sr.d);
         vec3 tr = splat3(1.0f);
         if (!S.anyTransparent) {
-            if (scene_any(S, ray)) tr = splat3(0.0f);
+            if (scene_any(S, ray, sr_lds + threadIdx.x)) tr = splat3(0.0f);
         } else {
             Tally tl = {0, 0, 0};
             for (;;) {
                 HitRec h;
                 h.obj = -1;
-                if (!scene_closest<false>(S, ray, h, tl)) break;
+                if (!scene_closest<false>(S, ray, h, tl, sr_lds + threadIdx.x)) break;
                 const DevObject& ob = S.objects[h.obj];
                 if (ob.transparency <= 0.0f) { tr = splat3(0.0f); break; }
                 SurfaceD s;
@@ -710,57 +479,66 @@ SR_DEV float rel_std_err(float sum, float sumSqr, uint32_t n) {
     return sqrtf(variance / (float)n) / net_max(mean, 1e-4f);
 }
 
-// Per-pixel sample accumulation, convergence test and AOV write-back (renderer.cs:180-229).
-__global__ void k_accumulate(Frame F) {
+// After a sample's last segment, per active pixel: accumulation and the convergence test
+// (renderer.cs:180-212); then either the pixel's next camera ray, appended to queue 0 in slot
+// order so the next sample's camera rays stay tile-coherent, or its final AOVs
+// (renderer.cs:214-229, 142-148).
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_next(Frame F) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0) { F.counts[0] = 0; F.counts[1] = 0; F.counts[2] = 0; }
-    if (q >= F.nSlots) return;
-    PixAcc a = F.acc[q];
-    if (!a.active) return;
-    const PathRec& p = F.paths[q];
-    const vec3 rad = p.rad, radFog = p.radFog;
-    a.radSum = a.radSum + rad;
-    a.radFogSum = a.radFogSum + radFog;
-    const float lum = luminance(rad) + luminance(radFog);
-    a.lumSum += lum;
-    a.lumSumSqr += lum * lum;
-    const float lumSurf = luminance(rad);
-    a.lumSurf += lumSurf;
-    a.lumSurfSqr += lumSurf * lumSurf;
-    const uint32_t si = a.sampleIndex;
-    bool done = false;
-    uint32_t count = 0;
-    if (si >= F.minS - 1u && rel_std_err(a.lumSum, a.lumSumSqr, si + 1) < F.threshold) {
-        done = true;
-        count = si + 1;
-    } else if (si + 1 == F.maxS) {
-        done = true;
-        count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
-    }
-    if (!done) {
-        a.sampleIndex = si + 1;
+    bool again = false;
+    if (q < F.nSlots && F.acc[q].active) {
+        PixAcc a = F.acc[q];
+        PathRec p = F.paths[q];
+        const vec3 rad = p.rad, radFog = p.radFog;
+        a.radSum = a.radSum + rad;
+        a.radFogSum = a.radFogSum + radFog;
+        const float lum = luminance(rad) + luminance(radFog);
+        a.lumSum += lum;
+        a.lumSumSqr += lum * lum;
+        const float lumSurf = luminance(rad);
+        a.lumSurf += lumSurf;
+        a.lumSurfSqr += lumSurf * lumSurf;
+        const uint32_t si = a.sampleIndex;
+        bool done = false;
+        uint32_t count = 0;
+        if (si >= F.minS - 1u && rel_std_err(a.lumSum, a.lumSumSqr, si + 1) < F.threshold) {
+            done = true;
+            count = si + 1;
+        } else if (si + 1 == F.maxS) {
+            done = true;
+            count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
+        }
+        uint32_t x, y;
+        slot_pixel(F, q, x, y);
+        if (!done) {
+            a.sampleIndex = si + 1;
+            begin_sample<R>(F, q, x, y, si + 1, p);
+            F.paths[q] = p;
+            again = true;
+            if (F.ctr) atomicAdd(&F.ctr[1], 1ull);
+        } else {
+            a.active = 0;
+            const size_t o = (size_t)y * F.W + x;
+            const float var = rel_std_err(a.lumSurf, a.lumSurfSqr, count);
+            const vec3 r = a.radSum / (float)count;
+            const vec3 rf = a.radFogSum / (float)count;
+            const vec3 nrm = a.normalCount > 0 ? normalized_or(a.normalSum, a.normalFallback) : splat3(0.0f);
+            const float depth = a.depthCount > 0 ? a.depthSum / (float)a.depthCount : kInf;
+            F.outRad[o * 3] = r.x; F.outRad[o * 3 + 1] = r.y; F.outRad[o * 3 + 2] = r.z;
+            F.outRadFog[o * 3] = rf.x; F.outRadFog[o * 3 + 1] = rf.y; F.outRadFog[o * 3 + 2] = rf.z;
+            F.outNormal[o * 3] = nrm.x; F.outNormal[o * 3 + 1] = nrm.y; F.outNormal[o * 3 + 2] = nrm.z;
+            F.outUv[o * 2] = a.hasUv ? a.uv.x : 0.0f;
+            F.outUv[o * 2 + 1] = a.hasUv ? a.uv.y : 0.0f;
+            F.outDepth[o] = depth;
+            F.outSamples[o] = count;
+            F.outVar[o] = var;
+            if (F.ctr) atomicAdd(&F.ctr[count < F.maxS ? 8 : 9], 1ull);
+        }
         F.acc[q] = a;
-        return;
     }
-    a.active = 0;
-    F.acc[q] = a;
-    uint32_t x, y;
-    slot_pixel(F, q, x, y);
-    const size_t o = (size_t)y * F.W + x;
-    const float var = rel_std_err(a.lumSurf, a.lumSurfSqr, count);
-    const vec3 r = a.radSum / (float)count;
-    const vec3 rf = a.radFogSum / (float)count;
-    const vec3 nrm = a.normalCount > 0 ? normalized_or(a.normalSum, a.normalFallback) : splat3(0.0f);
-    const float depth = a.depthCount > 0 ? a.depthSum / (float)a.depthCount : kInf;
-    F.outRad[o * 3] = r.x; F.outRad[o * 3 + 1] = r.y; F.outRad[o * 3 + 2] = r.z;
-    F.outRadFog[o * 3] = rf.x; F.outRadFog[o * 3 + 1] = rf.y; F.outRadFog[o * 3 + 2] = rf.z;
-    F.outNormal[o * 3] = nrm.x; F.outNormal[o * 3 + 1] = nrm.y; F.outNormal[o * 3 + 2] = nrm.z;
-    F.outUv[o * 2] = a.hasUv ? a.uv.x : 0.0f;
-    F.outUv[o * 2 + 1] = a.hasUv ? a.uv.y : 0.0f;
-    F.outDepth[o] = depth;
-    F.outSamples[o] = count;
-    F.outVar[o] = var;
-    if (F.ctr) atomicAdd(&F.ctr[count < F.maxS ? 8 : 9], 1ull);
+    const uint32_t slot = wave_append(&F.counts[kRayCount0], again);
+    if (again) F.queue[0][slot] = q;
 }
 
 // Primary-hit diagnostic: sample 0 of every pixel of a band.
@@ -770,27 +548,19 @@ __global__ void k_primary(Frame F, DevScene S, int32_t* obj, int32_t* tri, float
     uint32_t x, y;
     if (!slot_pixel(F, q, x, y)) return;
     XorwowRng rng = xorwow_seed(x, y);
-    const float jx = next_float(rng);
-    const float jy = next_float(rng);
-    const vec2 sp = mk2(((float)x + jx) / (float)F.W, ((float)y + jy) / (float)F.H);
-    const float ndcX = sp.x * 2.0f - 1.0f;
-    const float ndcY = -sp.y * 2.0f + 1.0f;
-    float th, tv;
-    if (F.view.aspect >= 1.0f) { th = F.view.tanFov; tv = th / F.view.aspect; }
-    else { tv = F.view.tanFov; th = tv * F.view.aspect; }
-    const vec3 origin = F.view.pos + rotate(F.view.rot, mk3(0.0f, 0.0f, F.view.nearPlane));
-    const vec3 dir = rotate(F.view.rot, normalized(mk3(ndcX * th, ndcY * tv, 1.0f)));
+    vec3 o, d;
+    camera_ray(F.view, F.W, F.H, x, y, rng, o, d);
     HitRec h;
     h.t = kInf; h.obj = -1; h.slot = -1;
     Tally tl = {0, 0, 0};
-    const size_t o = (size_t)y * F.W + x;
-    if (scene_closest<false>(S, mk_ray(origin, dir), h, tl)) {
+    const size_t idx = (size_t)y * F.W + x;
+    if (scene_closest<false>(S, mk_ray(o, d), h, tl, sr_lds + threadIdx.x)) {
         const DevObject& ob = S.objects[h.obj];
-        obj[o] = h.obj;
-        tri[o] = ob.kind == 0 ? S.slotTri[ob.triBase + h.slot] : -1;
-        dist[o] = h.t;
+        obj[idx] = h.obj;
+        tri[idx] = ob.kind == 0 ? S.slotTri[ob.triBase + h.slot] : -1;
+        dist[idx] = h.t;
     } else {
-        obj[o] = -1; tri[o] = -1; dist[o] = kInf;
+        obj[idx] = -1; tri[idx] = -1; dist[idx] = kInf;
     }
 }
 
@@ -928,20 +698,32 @@ struct sr_ctx {
     DevBuf<DevObject> dObjects;
     DevBuf<DevTexture> dTextures;
     // frame
+    DevBuf<float4> dRayO, dRayD, dHitA;
+    DevBuf<int32_t> dHitSlot;
     DevBuf<PathRec> dPaths;
-    DevBuf<HitRec> dHits;
     DevBuf<ShadowRec> dShadows;
     DevBuf<PixAcc> dAcc;
-    DevBuf<uint32_t> dQueue0, dQueue1, dShadowQueue, dCounts;
+    DevBuf<uint32_t> dQueue0, dQueue1, dCounts;
     DevBuf<unsigned long long> dCtr;
     DevBuf<float> dRad, dRadFog, dNormal, dUv, dDepth, dVar;
     DevBuf<uint32_t> dSamples;
+    uint32_t* hostCount = nullptr;  // pinned: the live-ray count read back between chunks
     bool rendering = false;
     bool countersOn = false;
     sr_render_params params{};
-    std::vector<hipEvent_t> bandEvents;
+    ViewD view{};
+    // Background driver (the analogue of the reference's worker threads, renderer.cs:88-122):
+    // enqueues the iterations band by band; Tick waits on bandsDone.
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv;
     std::vector<uint32_t> bandEnd;  // last row (exclusive) per band
     size_t bandsDone = 0;
+    size_t bandsReported = 0;
+    bool workerDone = true;
+    int workerStatus = SR_OK;
+    std::string workerError;
+    uint32_t iterations = 0;
     hipEvent_t evStart = nullptr, evEnd = nullptr;
     int gridCUs = 256;
     // Kernel timing: one event after every launch of a frame; kernel i spans events i-1..i.
@@ -950,15 +732,21 @@ struct sr_ctx {
     std::vector<int> profKinds;  // 0 extend, 1 shade, 2 shadow, 3 other
     size_t profUsed = 0;
 
+    void join() {
+        if (worker.joinable()) worker.join();
+    }
+
     ~sr_ctx() {
-        for (auto e : bandEvents) (void)hipEventDestroy(e);
+        join();
         for (auto e : profEvents) (void)hipEventDestroy(e);
+        if (hostCount) (void)hipHostFree(hostCount);
         if (evStart) (void)hipEventDestroy(evStart);
         if (evEnd) (void)hipEventDestroy(evEnd);
         dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release();
         dSlotTri.release(); dSceneItems.release(); dObjects.release(); dTextures.release();
-        dPaths.release(); dHits.release(); dShadows.release(); dAcc.release();
-        dQueue0.release(); dQueue1.release(); dShadowQueue.release(); dCounts.release(); dCtr.release();
+        dRayO.release(); dRayD.release(); dHitA.release(); dHitSlot.release();
+        dPaths.release(); dShadows.release(); dAcc.release();
+        dQueue0.release(); dQueue1.release(); dCounts.release(); dCtr.release();
         dRad.release(); dRadFog.release(); dNormal.release(); dUv.release(); dDepth.release(); dVar.release();
         dSamples.release();
         if (stream) (void)hipStreamDestroy(stream);
@@ -1101,6 +889,7 @@ int sr_create(int32_t device, sr_ctx** out) {
 
 int sr_destroy(sr_ctx* ctx) {
     if (!ctx) return SR_OK;
+    ctx->join();
     if (ctx->device >= 0) {
         (void)hipSetDevice(ctx->device);
         if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
@@ -1364,13 +1153,15 @@ ViewD make_view(const sr_view* v, uint32_t W, uint32_t H) {
 Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows) {
     Frame F;
     memset(&F, 0, sizeof(F));
+    F.rayO = c->dRayO.ptr;
+    F.rayD = c->dRayD.ptr;
+    F.hitA = c->dHitA.ptr;
+    F.hitSlot = c->dHitSlot.ptr;
     F.paths = c->dPaths.ptr;
-    F.hits = c->dHits.ptr;
     F.shadows = c->dShadows.ptr;
     F.acc = c->dAcc.ptr;
     F.queue[0] = c->dQueue0.ptr;
     F.queue[1] = c->dQueue1.ptr;
-    F.shadowQueue = c->dShadowQueue.ptr;
     F.counts = c->dCounts.ptr;
     F.ctr = c->countersOn ? c->dCtr.ptr : nullptr;
     F.outRad = c->dRad.ptr;
@@ -1392,19 +1183,22 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows) {
     F.seed = c->params.rng_seed;
     F.threshold = c->params.variance_threshold;
     F.clamp = c->params.indirect_clamp;
+    F.view = c->view;
     return F;
 }
 
 int alloc_frame(sr_ctx* c, uint32_t maxSlots) {
     const size_t px = (size_t)c->params.width * c->params.height;
     int rc;
-    if ((rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dHits, maxSlots)) || (rc = ensure(c->dShadows, maxSlots)) ||
+    if ((rc = ensure(c->dRayO, maxSlots)) || (rc = ensure(c->dRayD, maxSlots)) || (rc = ensure(c->dHitA, maxSlots)) ||
+        (rc = ensure(c->dHitSlot, maxSlots)) || (rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dShadows, maxSlots)) ||
         (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxSlots)) || (rc = ensure(c->dQueue1, maxSlots)) ||
-        (rc = ensure(c->dShadowQueue, maxSlots)) || (rc = ensure(c->dCounts, 4)) || (rc = ensure(c->dCtr, SR_COUNTER_COUNT)) ||
-        (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) || (rc = ensure(c->dNormal, px * 3)) ||
-        (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) || (rc = ensure(c->dVar, px)) ||
-        (rc = ensure(c->dSamples, px)))
+        (rc = ensure(c->dCounts, kCounterWords)) ||
+        (rc = ensure(c->dCtr, SR_COUNTER_COUNT)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
+        (rc = ensure(c->dNormal, px * 3)) || (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) ||
+        (rc = ensure(c->dVar, px)) || (rc = ensure(c->dSamples, px)))
         return rc;
+    if (!c->hostCount) HIP_TRY(hipHostMalloc((void**)&c->hostCount, sizeof(uint32_t), hipHostMallocDefault));
     return SR_OK;
 }
 
@@ -1420,30 +1214,82 @@ void mark(sr_ctx* c, int kind) {
     (void)hipEventRecord(c->profEvents[c->profUsed++], c->stream);
 }
 
+// Enqueues one band: sample 0's camera rays, then per sample (bounces + 1) segments of extend,
+// shade and shadow followed by the accumulate / next-camera-ray pass. Queues that run empty
+// (converged pixels, terminated paths) make the remaining launches exit at once.
 template <class R, bool COUNT>
-void enqueue_band(sr_ctx* c, const Frame& F) {
+int run_band(sr_ctx* c, const Frame& F) {
     hipStream_t s = c->stream;
     const int slotBlocks = (int)((F.nSlots + kBlock - 1) / kBlock);
-    const int g = grid_for(c, F.nSlots);
+    const int g = c->gridCUs * 8;
     mark(c, 3);
-    hipLaunchKernelGGL(k_init, dim3(slotBlocks), dim3(kBlock), 0, s, F);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, F);
+    hipLaunchKernelGGL(k_start<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
     mark(c, 3);
     for (uint32_t smp = 0; smp < F.maxS; ++smp) {
-        hipLaunchKernelGGL(k_raygen<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
-        mark(c, 3);
         int cur = 0;
         for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
-            hipLaunchKernelGGL(k_extend<COUNT>, dim3(g), dim3(kBlock), 0, s, F, c->dev, cur);
+            hipLaunchKernelGGL(k_extend<COUNT>, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
             mark(c, 0);
-            hipLaunchKernelGGL(k_shade<R>, dim3(g), dim3(kBlock), 0, s, F, c->dev, cur, seg);
+            hipLaunchKernelGGL(k_shade<R>, dim3(g), dim3(kBlock), 0, s, F, c->dev, cur);
             mark(c, 1);
-            hipLaunchKernelGGL(k_shadow, dim3(g), dim3(kBlock), 0, s, F, c->dev);
+            hipLaunchKernelGGL(k_shadow, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, seg == F.bounces ? 1 : 0);
             mark(c, 2);
             cur ^= 1;
         }
-        hipLaunchKernelGGL(k_accumulate, dim3(slotBlocks), dim3(kBlock), 0, s, F);
+        hipLaunchKernelGGL(k_next<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
         mark(c, 3);
+        c->iterations++;
     }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(s));
+    return SR_OK;
+}
+
+template <class R, bool COUNT>
+int run_frame(sr_ctx* c) {
+    const sr_render_params& p = c->params;
+    for (size_t b = 0; b < c->bandEnd.size(); ++b) {
+        const uint32_t y0 = b == 0 ? 0 : c->bandEnd[b - 1];
+        const Frame F = frame_for(c, y0, c->bandEnd[b] - y0);
+        int rc = run_band<R, COUNT>(c, F);
+        if (rc) return rc;
+        std::lock_guard<std::mutex> lk(c->mu);
+        c->bandsDone = b + 1;
+        c->cv.notify_all();
+    }
+    (void)p;
+    return SR_OK;
+}
+
+void drive(sr_ctx* c) {
+    int rc = SR_OK;
+    std::string msg;
+    if (hipSetDevice(c->device) != hipSuccess || hipEventRecord(c->evStart, c->stream) != hipSuccess) {
+        rc = SR_ERR_HIP;
+        msg = "render worker: could not start";
+    } else {
+        const bool counter = c->params.rng_policy == SR_RNG_COUNTER;
+        if (counter) rc = c->countersOn ? run_frame<CounterRng, true>(c) : run_frame<CounterRng, false>(c);
+        else rc = c->countersOn ? run_frame<XorwowRng, true>(c) : run_frame<XorwowRng, false>(c);
+        if (rc) msg = g_err;
+        hipError_t e = hipEventRecord(c->evEnd, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(c->evEnd);
+        if (!rc && e != hipSuccess) { rc = SR_ERR_HIP; msg = hipGetErrorString(e); }
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->workerStatus = rc;
+    c->workerError = msg;
+    c->workerDone = true;
+    c->bandsDone = c->bandEnd.size();
+    c->cv.notify_all();
+}
+
+// Waits for the background render; reports its error on the calling thread.
+int wait_render(sr_ctx* c) {
+    c->join();
+    if (c->workerStatus != SR_OK) return fail(c->workerStatus, "render: " + c->workerError);
+    return SR_OK;
 }
 
 }  // namespace
@@ -1457,50 +1303,35 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
     if (p->min_samples == 0 || p->min_samples > p->max_samples) return fail(SR_ERR_INVALID, "render: need 0 < minSamples <= maxSamples");
     if (p->rng_policy > 1) return fail(SR_ERR_INVALID, "render: unknown rng policy");
     if (p->width > 32768 || p->height > 32768) return fail(SR_ERR_LIMIT, "render: image larger than 32768");
+    ctx->join();  // a previous render finishes first
     HIP_TRY(hipSetDevice(ctx->device));
     if (!ctx->built) {
         int rc = sr_scene_build(ctx);
         if (rc) return rc;
     }
-    if (ctx->rendering) HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->params = *p;
     if (ctx->params.tick_bands == 0) ctx->params.tick_bands = 1;
     const uint32_t bands = std::min(ctx->params.tick_bands, (p->height + 7) / 8);
-    uint32_t rowsPer = ((p->height + bands - 1) / bands + 7) / 8 * 8;
+    const uint32_t rowsPer = ((p->height + bands - 1) / bands + 7) / 8 * 8;
     const uint32_t tilesX = (p->width + 7) / 8;
     const uint32_t maxSlots = tilesX * ((rowsPer + 7) / 8) * 64;
     int rc = alloc_frame(ctx, maxSlots);
     if (rc) return rc;
-    for (auto e : ctx->bandEvents) (void)hipEventDestroy(e);
-    ctx->bandEvents.clear();
-    ctx->bandEnd.clear();
     if (!ctx->evStart) HIP_TRY(hipEventCreate(&ctx->evStart));
     if (!ctx->evEnd) HIP_TRY(hipEventCreate(&ctx->evEnd));
     if (ctx->countersOn) HIP_TRY(hipMemsetAsync(ctx->dCtr.ptr, 0, SR_COUNTER_COUNT * sizeof(unsigned long long), ctx->stream));
-    HIP_TRY(hipEventRecord(ctx->evStart, ctx->stream));
     ctx->profUsed = 0;
-    const ViewD vd = make_view(view, p->width, p->height);
-    for (uint32_t y0 = 0; y0 < p->height; y0 += rowsPer) {
-        const uint32_t rows = std::min(rowsPer, p->height - y0);
-        Frame F = frame_for(ctx, y0, rows);
-        F.view = vd;
-        if (p->rng_policy == SR_RNG_COUNTER) {
-            if (ctx->countersOn) enqueue_band<CounterRng, true>(ctx, F);
-            else enqueue_band<CounterRng, false>(ctx, F);
-        } else {
-            if (ctx->countersOn) enqueue_band<XorwowRng, true>(ctx, F);
-            else enqueue_band<XorwowRng, false>(ctx, F);
-        }
-        HIP_TRY(hipGetLastError());
-        hipEvent_t ev;
-        HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        HIP_TRY(hipEventRecord(ev, ctx->stream));
-        ctx->bandEvents.push_back(ev);
-        ctx->bandEnd.push_back(y0 + rows);
-    }
-    HIP_TRY(hipEventRecord(ctx->evEnd, ctx->stream));
+    ctx->iterations = 0;
+    ctx->view = make_view(view, p->width, p->height);
+    ctx->bandEnd.clear();
+    for (uint32_t y0 = 0; y0 < p->height; y0 += rowsPer) ctx->bandEnd.push_back(std::min(p->height, y0 + rowsPer));
     ctx->bandsDone = 0;
+    ctx->bandsReported = 0;
+    ctx->workerDone = false;
+    ctx->workerStatus = SR_OK;
+    ctx->workerError.clear();
     ctx->rendering = true;
+    ctx->worker = std::thread(drive, ctx);
     return SR_OK;
 }
 
@@ -1511,11 +1342,15 @@ int sr_render_tick(sr_ctx* ctx, uint32_t* step, uint32_t* total) {
     const uint32_t bx = (p.width + p.block_size - 1) / p.block_size;
     const uint32_t by = (p.height + p.block_size - 1) / p.block_size;
     *total = bx * by;
-    if (ctx->bandsDone < ctx->bandEvents.size()) {
-        HIP_TRY(hipEventSynchronize(ctx->bandEvents[ctx->bandsDone]));
-        ctx->bandsDone++;
+    size_t done;
+    {
+        std::unique_lock<std::mutex> lk(ctx->mu);
+        ctx->cv.wait(lk, [&] { return ctx->workerDone || ctx->bandsDone > ctx->bandsReported || ctx->bandsReported == ctx->bandEnd.size(); });
+        if (ctx->workerDone && ctx->workerStatus != SR_OK) return fail(ctx->workerStatus, "render: " + ctx->workerError);
+        ctx->bandsReported = ctx->bandsDone;
+        done = ctx->bandsDone;
     }
-    const uint32_t rowsDone = ctx->bandsDone ? ctx->bandEnd[ctx->bandsDone - 1] : 0;
+    const uint32_t rowsDone = done ? ctx->bandEnd[done - 1] : 0;
     *step = rowsDone >= p.height ? *total : bx * (rowsDone / p.block_size);
     return SR_OK;
 }
@@ -1524,9 +1359,9 @@ int sr_render_read(sr_ctx* ctx, float* rad, float* radFog, float* nrm, float* uv
                    float* var) {
     if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
     if (!ctx->rendering) return fail(SR_ERR_STATE, "no render in progress");
+    int rc = wait_render(ctx);
+    if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    ctx->bandsDone = ctx->bandEvents.size();
     const size_t px = (size_t)ctx->params.width * ctx->params.height;
     if (rad) HIP_TRY(hipMemcpy(rad, ctx->dRad.ptr, px * 12, hipMemcpyDeviceToHost));
     if (radFog) HIP_TRY(hipMemcpy(radFog, ctx->dRadFog.ptr, px * 12, hipMemcpyDeviceToHost));
@@ -1542,6 +1377,8 @@ int sr_render_device_aovs(sr_ctx* ctx, void** rad, void** radFog, void** nrm, vo
                           void** var) {
     if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
     if (!ctx->rendering) return fail(SR_ERR_STATE, "no render in progress");
+    int rc = wait_render(ctx);
+    if (rc) return rc;
     if (rad) *rad = ctx->dRad.ptr;
     if (radFog) *radFog = ctx->dRadFog.ptr;
     if (nrm) *nrm = ctx->dNormal.ptr;
@@ -1555,7 +1392,8 @@ int sr_render_device_aovs(sr_ctx* ctx, void** rad, void** radFog, void** nrm, vo
 int sr_render_gpu_seconds(sr_ctx* ctx, double* out) {
     if (!ctx || !out) return fail(SR_ERR_INVALID, "null argument");
     if (!ctx->rendering) return fail(SR_ERR_STATE, "no render in progress");
-    HIP_TRY(hipEventSynchronize(ctx->evEnd));
+    int rc = wait_render(ctx);
+    if (rc) return rc;
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, ctx->evStart, ctx->evEnd));
     *out = ms * 1e-3;
@@ -1571,7 +1409,8 @@ int sr_profile_enable(sr_ctx* ctx, int32_t enable) {
 int sr_profile_read(sr_ctx* ctx, double* ms, uint32_t* launches) {
     if (!ctx || !ms || !launches) return fail(SR_ERR_INVALID, "null argument");
     if (!ctx->rendering || !ctx->profileOn) return fail(SR_ERR_STATE, "kernel timing was not enabled");
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int rc = wait_render(ctx);
+    if (rc) return rc;
     for (int k = 0; k < 4; ++k) { ms[k] = 0.0; launches[k] = 0; }
     for (size_t i = 1; i < ctx->profUsed; ++i) {
         float t = 0.0f;
@@ -1592,7 +1431,8 @@ int sr_counters_enable(sr_ctx* ctx, int32_t enable) {
 int sr_counters_read(sr_ctx* ctx, int64_t* out) {
     if (!ctx || !out) return fail(SR_ERR_INVALID, "null argument");
     if (!ctx->rendering || !ctx->countersOn) return fail(SR_ERR_STATE, "counters were not collected");
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int rc = wait_render(ctx);
+    if (rc) return rc;
     unsigned long long v[SR_COUNTER_COUNT];
     HIP_TRY(hipMemcpy(v, ctx->dCtr.ptr, sizeof(v), hipMemcpyDeviceToHost));
     for (int i = 0; i < SR_COUNTER_COUNT; ++i) out[i] = (int64_t)v[i];
@@ -1608,7 +1448,10 @@ int sr_trace_primary(sr_ctx* ctx, const sr_view* view, uint32_t w, uint32_t h, i
         int rc = sr_scene_build(ctx);
         if (rc) return rc;
     }
-    if (ctx->rendering) HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->rendering) {
+        int rc = wait_render(ctx);
+        if (rc) return rc;
+    }
     const size_t px = (size_t)w * h;
     int32_t *dObj = nullptr, *dTri = nullptr;
     float* dDist = nullptr;
@@ -1621,7 +1464,7 @@ int sr_trace_primary(sr_ctx* ctx, const sr_view* view, uint32_t w, uint32_t h, i
     Frame F = frame_for(ctx, 0, h);
     ctx->params = saved;
     F.view = make_view(view, w, h);
-    hipLaunchKernelGGL(k_primary, dim3((F.nSlots + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream, F, ctx->dev, dObj,
+    hipLaunchKernelGGL(k_primary, dim3((F.nSlots + kBlock - 1) / kBlock), dim3(kBlock), kTravLdsBytes, ctx->stream, F, ctx->dev, dObj,
                        dTri, dDist);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -1687,8 +1530,9 @@ int sr_denoise_bilateral_device(sr_ctx* ctx, const sr_denoise_params* p, const f
 int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* p, float* out_rgb) {
     if (!ctx || !p || !out_rgb) return fail(SR_ERR_INVALID, "denoise: null argument");
     if (!ctx->rendering) return fail(SR_ERR_STATE, "no render to denoise");
+    int rc0 = wait_render(ctx);
+    if (rc0) return rc0;
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
     const uint32_t W = ctx->params.width, H = ctx->params.height;
     float* dOut = nullptr;
     HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 12));

@@ -1,0 +1,332 @@
+// Two-level closest-hit and any-hit traversal of the reference's BVHs on gfx950, plus the
+// reconstruction of the hit surface (Scene.Trace). Shared by the tracer's kernels.
+#pragma once
+
+#include "sr_device.h"
+
+namespace sr {
+
+constexpr float kInf = __builtin_huge_valf();
+constexpr uint32_t kMaxChild = (1u << 24) - 1;
+constexpr int kStack = 64;  // bvh.cs:436 caps depth at 64, so at most 64 deferred siblings
+
+// Closest-hit record of a ray: world distance, barycentrics, object and item slot (-1 = miss).
+struct HitRec {
+    float t, u, v;
+    int32_t obj, slot;
+};
+
+struct Tally {
+    uint32_t box, shape, mesh;
+};
+
+#ifndef SR_LDS_DEPTH
+#define SR_LDS_DEPTH 0
+#endif
+constexpr int kTravBlock = 256;        // threads per block of the traversal kernels
+constexpr int kLdsDepth = SR_LDS_DEPTH;  // stack entries per lane kept in LDS
+
+// Mesh traversal stack: the first kLdsDepth entries of each lane live in LDS (one column per
+// lane, so a wave's accesses are bank-conflict free); deeper entries spill to private memory.
+struct TravStack {
+    uint2* col;  // this lane's LDS column (stride kTravBlock entries)
+    uint32_t node[kStack - kLdsDepth];
+    float t[kStack - kLdsDepth];
+    SR_DEV void put(int i, uint32_t n, float tt) {
+        if (kLdsDepth > 0 && i < kLdsDepth) col[i * kTravBlock] = make_uint2(n, __float_as_uint(tt));
+        else { node[i - kLdsDepth] = n; t[i - kLdsDepth] = tt; }
+    }
+    SR_DEV void get(int i, uint32_t& n, float& tt) const {
+        if (kLdsDepth > 0 && i < kLdsDepth) {
+            const uint2 e = col[i * kTravBlock];
+            n = e.x;
+            tt = __uint_as_float(e.y);
+        } else {
+            n = node[i - kLdsDepth];
+            tt = t[i - kLdsDepth];
+        }
+    }
+};
+constexpr size_t kTravLdsBytes = (size_t)kLdsDepth * kTravBlock * sizeof(uint2);
+
+SR_DEV uint32_t pack_node(float childBits, float countBits) {
+    return ((uint32_t)__float_as_int(countBits) << 24) | (uint32_t)__float_as_int(childBits);
+}
+
+// Closest hit in one mesh BVH (bvh.cs:165-216 + mesh.cs:51-62). Nearer child first, ties to
+// child B; prune when the entry distance is >= the best hit; accept strictly closer hits.
+template <bool COUNT>
+SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, float& bestT,
+                         float& bu, float& bv, int32_t& bslot, Tally& tl, uint2* lds) {
+    const float4* N = S.nodes;
+    TravStack st;
+    st.col = lds;
+    int sp = 0;
+    bestT = kInf;
+    bool found = false;
+    const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    float curT = 0.0f;
+    for (;;) {
+        bool next = false;
+        if (curT < bestT) {
+            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+            if (cnt == 0) {
+                if (COUNT) tl.box += 2;
+                const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)child) * 2;
+                const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+                float tA = 0.0f, tB = 0.0f;
+                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= bestT;
+                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= bestT;
+                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+                if (hA && hB) {
+                    const bool aNear = tA < tB;
+                    st.put(sp, aNear ? pB : pA, aNear ? tB : tA);
+                    ++sp;
+                    cur = aNear ? pA : pB;
+                    curT = aNear ? tA : tB;
+                    next = true;
+                } else if (hA) {
+                    cur = pA; curT = tA; next = true;
+                } else if (hB) {
+                    cur = pB; curT = tB; next = true;
+                }
+            } else {
+                if (COUNT) tl.shape += cnt;
+                const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
+                for (uint32_t i = 0; i < cnt; ++i, T += 9) {
+                    float t, u, v;
+                    if (tri_hit(T, r, t, u, v) && t < bestT) {
+                        bestT = t; bu = u; bv = v;
+                        bslot = (int32_t)(child + i);
+                        found = true;
+                    }
+                }
+            }
+        }
+        if (!next) {
+            if (sp == 0) break;
+            --sp;
+            st.get(sp, cur, curT);
+        }
+    }
+    return found;
+}
+
+// Whether any primitive of a mesh is hit (the same node visits as the closest-hit traversal up
+// to its first accepted hit, which is where the existence of a hit is decided).
+SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, uint2* lds) {
+    const float4* N = S.nodes;
+    TravStack st;
+    st.col = lds;
+    int sp = 0;
+    const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    for (;;) {
+        bool next = false;
+        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+        if (cnt == 0) {
+            const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)child) * 2;
+            const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+            float tA = 0.0f, tB = 0.0f;
+            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
+            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
+            const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+            if (hA && hB) {
+                const bool aNear = tA < tB;
+                st.put(sp++, aNear ? pB : pA, 0.0f);
+                cur = aNear ? pA : pB;
+                next = true;
+            } else if (hA) {
+                cur = pA; next = true;
+            } else if (hB) {
+                cur = pB; next = true;
+            }
+        } else {
+            const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
+            for (uint32_t i = 0; i < cnt; ++i, T += 9) {
+                float t, u, v;
+                if (tri_hit(T, r, t, u, v)) return true;
+            }
+        }
+        if (!next) {
+            if (sp == 0) return false;
+            float unused;
+            st.get(--sp, cur, unused);
+        }
+    }
+}
+
+// Object.Intersect (scene.cs:227 -> math.cs:67-79): distance in world units only; the hit
+// record keeps (u, v, slot) so shading can rebuild the full ShapeHit.
+template <bool COUNT>
+SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r, float& dist, float& u, float& v,
+                           int32_t& slot, Tally& tl, uint2* lds) {
+    float scale;
+    const ray3 lr = to_object(ob, r, scale);
+    float t = 0.0f;
+    slot = -1;
+    u = v = 0.0f;
+    bool hit;
+    if (ob.kind == 0) {
+        if (COUNT) tl.mesh += 1;
+        hit = ob.nodeBase >= 0 && mesh_closest<COUNT>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, tl, lds);
+    } else {
+        ShapeHitD h;
+        hit = ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
+        t = h.dist;
+    }
+    if (!hit) return false;
+    dist = t / scale;
+    return true;
+}
+
+SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, uint2* lds) {
+    float scale;
+    const ray3 lr = to_object(ob, r, scale);
+    if (ob.kind == 0) return ob.nodeBase >= 0 && mesh_any(S, ob.nodeBase, ob.triBase, lr, lds);
+    ShapeHitD h;
+    return ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
+}
+
+// Scene BVH over objects (costI 10, 32 bins; scene.cs:528-533), two-level closest hit.
+template <bool COUNT>
+SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& tl, uint2* lds) {
+    if (S.sceneNodes == 0) return false;
+    const float4* N = S.nodes;
+    uint32_t stk[kStack];
+    float stt[kStack];
+    int sp = 0;
+    float best = kInf;
+    bool found = false;
+    const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    float curT = 0.0f;
+    for (;;) {
+        bool next = false;
+        if (curT < best) {
+            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+            if (cnt == 0) {
+                if (COUNT) tl.box += 2;
+                const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)child) * 2;
+                const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+                float tA = 0.0f, tB = 0.0f;
+                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= best;
+                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= best;
+                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+                if (hA && hB) {
+                    const bool aNear = tA < tB;
+                    stk[sp] = aNear ? pB : pA;
+                    stt[sp] = aNear ? tB : tA;
+                    ++sp;
+                    cur = aNear ? pA : pB;
+                    curT = aNear ? tA : tB;
+                    next = true;
+                } else if (hA) {
+                    cur = pA; curT = tA; next = true;
+                } else if (hB) {
+                    cur = pB; curT = tB; next = true;
+                }
+            } else {
+                if (COUNT) tl.shape += cnt;
+                for (uint32_t i = 0; i < cnt; ++i) {
+                    const int32_t oi = S.sceneItems[child + i];
+                    float d, u, v;
+                    int32_t slot;
+                    if (object_closest<COUNT>(S, S.objects[oi], r, d, u, v, slot, tl, lds) && d < best) {
+                        best = d;
+                        out.t = d; out.u = u; out.v = v; out.obj = oi; out.slot = slot;
+                        found = true;
+                    }
+                }
+            }
+        }
+        if (!next) {
+            if (sp == 0) break;
+            --sp;
+            cur = stk[sp];
+            curT = stt[sp];
+        }
+    }
+    return found;
+}
+
+SR_DEV bool scene_any(const DevScene& S, const ray3& r, uint2* lds) {
+    if (S.sceneNodes == 0) return false;
+    const float4* N = S.nodes;
+    uint32_t stk[kStack];
+    int sp = 0;
+    const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    for (;;) {
+        bool next = false;
+        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+        if (cnt == 0) {
+            const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)child) * 2;
+            const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+            float tA = 0.0f, tB = 0.0f;
+            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
+            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
+            const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+            if (hA && hB) {
+                const bool aNear = tA < tB;
+                stk[sp++] = aNear ? pB : pA;
+                cur = aNear ? pA : pB;
+                next = true;
+            } else if (hA) {
+                cur = pA; next = true;
+            } else if (hB) {
+                cur = pB; next = true;
+            }
+        } else {
+            for (uint32_t i = 0; i < cnt; ++i)
+                if (object_any(S, S.objects[S.sceneItems[child + i]], r, lds)) return true;
+        }
+        if (!next) {
+            if (sp == 0) return false;
+            cur = stk[--sp];
+        }
+    }
+}
+
+// Scene.Trace (scene.cs:602-625) from a hit record: world-space ShapeHit, material sampling,
+// normal mapping and tangent re-orthogonalisation.
+SR_DEV void make_surface(const DevScene& S, const ray3& r, const HitRec& h, SurfaceD& s) {
+    const DevObject& ob = S.objects[h.obj];
+    ShapeHitD sh;
+    if (ob.kind == 0) {
+        const int32_t tri = S.slotTri[ob.triBase + h.slot];
+        inflate(S.tris + (size_t)(ob.triBase + h.slot) * 9, S.attrs + (size_t)(ob.triBase + tri) * 27, h.u, h.v, sh);
+    } else {
+        float scale;
+        const ray3 lr = to_object(ob, r, scale);
+        if (ob.kind == 1) sphere_hit(ob.shape, lr, sh, true);
+        else box_hit(ob.shape, lr, sh, true);
+    }
+    const vec3 wn = normalized(rotate(ob.rot, sh.norm / ob.scale));
+    const vec3 wng = normalized(rotate(ob.rot, sh.normGeo / ob.scale));
+    const vec3 wt = normalized(rotate(ob.rot, xyz(sh.tan)));
+    const vec2 uv = sh.uv;
+    const vec3 texC = ob.texColor >= 0 ? tex_sample(S, ob.texColor, uv) : splat3(1.0f);
+    s.albedo = texC * ob.albedo;
+    s.roughness = (ob.texRough >= 0 ? tex_sample(S, ob.texRough, uv).x : 1.0f) * ob.roughness;
+    s.metallic = (ob.texMetal >= 0 ? tex_sample(S, ob.texMetal, uv).x : 1.0f) * ob.metallic;
+    const vec4 htan = mk4(wt.x, wt.y, wt.z, sh.tan.w);
+    vec3 n = wn;
+    if (ob.texNormal >= 0) {
+        const vec3 c = tex_sample(S, ob.texNormal, uv);
+        const vec3 tn = normalized_or(mk3(c.x * 2.0f - 1.0f, -(c.y * 2.0f - 1.0f), c.z * 2.0f - 1.0f), up3());
+        n = normalized_or(from_tangent(tn, htan, wn), wn);
+    }
+    const vec3 td = normalized_or(wt - dot3(wt, n) * n, wt);
+    s.radiance = ob.radiance;
+    s.transparency = ob.transparency;
+    s.ior = ob.ior;
+    s.f0 = ob.f0;
+    s.normal = n;
+    s.normalGeo = wng;
+    s.tangent = mk4(td.x, td.y, td.z, sh.tan.w);
+    s.uv = uv;
+}
+
+}  // namespace sr
