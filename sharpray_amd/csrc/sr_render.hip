@@ -79,7 +79,7 @@ struct ViewD {
 
 // Queue counters (uint32 each).
 enum : int {
-    kRayCount0 = 0, kRayCount1 = 1, kShadowCount = 2, kCounterWords = 4
+    kRayCount0 = 0, kRayCount1 = 1, kShadowCount0 = 2, kShadowCount1 = 3, kCounterWords = 4
 };
 
 struct Frame {
@@ -88,7 +88,7 @@ struct Frame {
     float4* hitA;       // t, u, v, object (int bits)
     int32_t* hitSlot;   // mesh item slot
     PathRec* paths;
-    ShadowRec* shadows;
+    ShadowRec* shadows[2];  // shadow rays of even / odd segments
     PixAcc* acc;
     uint32_t* queue[2];
     uint32_t* counts;   // kCounterWords words
@@ -233,43 +233,9 @@ __global__ void k_start(Frame F) {
     if (live) F.queue[0][slot] = q;
 }
 
-// Closest hit for every queued ray.
-template <bool COUNT>
-__global__ void __launch_bounds__(kBlock) k_extend(Frame F, DevScene S, int cur) {
-    // Counters the next two kernels fill start at zero; nothing else touches them meanwhile.
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        F.counts[cur ^ 1] = 0;
-        F.counts[kShadowCount] = 0;
-    }
-    const uint32_t n = F.counts[cur];
-    const uint32_t* Q = F.queue[cur];
-    if (COUNT && blockIdx.x == 0 && threadIdx.x == 0 && F.ctr) {
-        atomicAdd(&F.ctr[2], (unsigned long long)n);
-        atomicAdd(&F.ctr[10], (unsigned long long)n);
-    }
-    Tally tl = {0, 0, 0};
-    const uint32_t stride = gridDim.x * blockDim.x;
-    // Grid-stride over the queue with a wave-uniform base, so every lane of a wave runs the same
-    // number of iterations (the wave-level appends need the whole wave).
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
-        const uint32_t i = base + lane_id();
-        if (i < n) {
-            const uint32_t q = Q[i];
-            const float4 o = F.rayO[q], d = F.rayD[q];
-            const ray3 r = mk_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z));
-            HitRec h;
-            h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
-            scene_closest<COUNT>(S, r, h, tl, sr_lds + threadIdx.x);
-            F.hitA[q] = make_float4(h.t, h.u, h.v, __int_as_float(h.obj));
-            F.hitSlot[q] = h.slot;
-        }
-    }
-    flush_tally<COUNT>(F, tl);
-}
-
 // One path segment (scene.cs:641-737). Direct sky light becomes a deferred shadow ray whose
-// contribution the shadow kernel adds before the path's next segment, which keeps the reference's
-// summation order.
+// contribution the next trace launch adds before the path's next segment, which keeps the
+// reference's summation order.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) {
     const uint32_t n = F.counts[cur];
@@ -410,8 +376,8 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) 
             F.paths[q] = p;
         }
         if (shadow) ++cOcc;
-        const uint32_t ns = wave_append(&F.counts[kShadowCount], shadow);
-        if (shadow) F.shadows[ns] = sr;
+        const uint32_t ns = wave_append(&F.counts[kShadowCount0 + cur], shadow);
+        if (shadow) F.shadows[cur][ns] = sr;
         const uint32_t nc = wave_append(&F.counts[cur ^ 1], cont);
         if (cont) F.queue[cur ^ 1][nc] = q;
     }
@@ -421,54 +387,85 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) 
     }
 }
 
-// ShadowTransmittance (scene.cs:576-600) and the deferred direct-light contribution.
-__global__ void __launch_bounds__(kBlock) k_shadow(Frame F, DevScene S, int lastSegment) {
-    // After a sample's last segment both ray queues are spent (the last shade appends nothing);
-    // the next sample's camera rays start in queue 0.
-    if (lastSegment && blockIdx.x == 0 && threadIdx.x == 0) {
-        F.counts[kRayCount0] = 0;
-        F.counts[kRayCount1] = 0;
+// ShadowTransmittance (scene.cs:576-600) for one deferred shadow ray, then its direct-light
+// contribution ((((sky * T) * f) * w) / pdf) * energy (scene.cs:793, 809, 659-661, 699).
+template <bool TRANSP>
+SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, uint32_t& extra) {
+    ray3 ray = mk_ray(sr.o, sr.d);
+    vec3 tr = splat3(1.0f);
+    if (!TRANSP) {
+        if (scene_any(S, ray, sr_lds + threadIdx.x)) tr = splat3(0.0f);
+    } else {
+        Tally tl = {0, 0, 0};
+        for (;;) {
+            HitRec h;
+            h.obj = -1;
+            if (!scene_closest<false>(S, ray, h, tl, sr_lds + threadIdx.x)) break;
+            const DevObject& ob = S.objects[h.obj];
+            if (ob.transparency <= 0.0f) { tr = splat3(0.0f); break; }
+            SurfaceD s;
+            make_surface(S, ray, h, s);
+            const float nDotV = fabsf(dot3(s.normalGeo, ray.d));
+            const vec3 fr = fresnel3(nDotV, base_refl(ob.f0, s.albedo, ob.metallic));
+            tr = tr * (s.albedo * ob.transparency * (splat3(1.0f) - fr));
+            ray = mk_ray(at(ray, h.t + net_max(1e-4f, h.t * 1e-4f)), ray.d);
+            ++extra;
+            if (!(max_comp(tr) > 1e-4f)) break;
+        }
     }
-    const uint32_t n = F.counts[kShadowCount];
+    if (max_comp(tr) > 0.0f) {
+        if (S.fog.enabled) tr = tr * expf(-fog_optical_depth(S.fog, mk_ray(sr.o, sr.d), 0.0f, kInf));
+        const vec3 c = sr.skyRad * tr * sr.f * sr.w / sr.pdf * sr.energy;
+        const uint32_t q = sr.path & 0x7fffffffu;
+        PathRec* p = &F.paths[q];
+        if (sr.path & 0x80000000u) p->radFog = p->radFog + c;
+        else p->rad = p->rad + c;
+    }
+}
+
+// One traversal launch per segment: the closest hit of every ray queued for this segment, plus
+// the shadow rays the previous segment's shade deferred. The previous segment's direct light thus
+// lands in the path's radiance before this segment's shade adds anything (the reference's
+// summation order), and the short tail launches of late segments share the GPU with the shadow
+// rays instead of running alone. Each wave takes only one kind of item.
+template <bool COUNT, bool TRANSP>
+__global__ void __launch_bounds__(kBlock) k_trace(Frame F, DevScene S, int cur) {
+    // The two queues this segment's shade fills start empty; nothing else touches them meanwhile.
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        F.counts[cur ^ 1] = 0;
+        F.counts[kShadowCount0 + cur] = 0;
+    }
+    const uint32_t nE = F.counts[cur];
+    const uint32_t nS = F.counts[kShadowCount0 + (cur ^ 1)];
+    const uint32_t nE64 = (nE + 63u) & ~63u;
+    const uint32_t total = nE64 + nS;
+    const uint32_t* Q = F.queue[cur];
+    const ShadowRec* SR = F.shadows[cur ^ 1];
+    if (COUNT && blockIdx.x == 0 && threadIdx.x == 0 && F.ctr) {
+        atomicAdd(&F.ctr[2], (unsigned long long)nE);
+        atomicAdd(&F.ctr[10], (unsigned long long)nE);
+    }
+    Tally tl = {0, 0, 0};
     uint32_t extra = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
-    // Grid-stride over the queue with a wave-uniform base, so every lane of a wave runs the same
-    // number of iterations (the wave-level appends need the whole wave).
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < total; base += stride) {
         const uint32_t i = base + lane_id();
-        if (i >= n) continue;
-        const ShadowRec sr = F.shadows[i];
-        ray3 ray = mk_ray(sr.o, sr.d);
-        vec3 tr = splat3(1.0f);
-        if (!S.anyTransparent) {
-            if (scene_any(S, ray, sr_lds + threadIdx.x)) tr = splat3(0.0f);
-        } else {
-            Tally tl = {0, 0, 0};
-            for (;;) {
+        if (base < nE64) {
+            if (i < nE) {
+                const uint32_t q = Q[i];
+                const float4 o = F.rayO[q], d = F.rayD[q];
+                const ray3 r = mk_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z));
                 HitRec h;
-                h.obj = -1;
-                if (!scene_closest<false>(S, ray, h, tl, sr_lds + threadIdx.x)) break;
-                const DevObject& ob = S.objects[h.obj];
-                if (ob.transparency <= 0.0f) { tr = splat3(0.0f); break; }
-                SurfaceD s;
-                make_surface(S, ray, h, s);
-                const float nDotV = fabsf(dot3(s.normalGeo, ray.d));
-                const vec3 fr = fresnel3(nDotV, base_refl(ob.f0, s.albedo, ob.metallic));
-                tr = tr * (s.albedo * ob.transparency * (splat3(1.0f) - fr));
-                ray = mk_ray(at(ray, h.t + net_max(1e-4f, h.t * 1e-4f)), ray.d);
-                ++extra;
-                if (!(max_comp(tr) > 1e-4f)) break;
+                h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
+                scene_closest<COUNT>(S, r, h, tl, sr_lds + threadIdx.x);
+                F.hitA[q] = make_float4(h.t, h.u, h.v, __int_as_float(h.obj));
+                F.hitSlot[q] = h.slot;
             }
-        }
-        if (max_comp(tr) > 0.0f) {
-            if (S.fog.enabled) tr = tr * expf(-fog_optical_depth(S.fog, mk_ray(sr.o, sr.d), 0.0f, kInf));
-            const vec3 c = sr.skyRad * tr * sr.f * sr.w / sr.pdf * sr.energy;
-            const uint32_t q = sr.path & 0x7fffffffu;
-            PathRec* p = &F.paths[q];
-            if (sr.path & 0x80000000u) p->radFog = p->radFog + c;
-            else p->rad = p->rad + c;
+        } else if (i - nE64 < nS) {
+            shadow_item<TRANSP>(F, S, SR[i - nE64], extra);
         }
     }
+    flush_tally<COUNT>(F, tl);
     if (F.ctr && extra) atomicAdd(&F.ctr[11], (unsigned long long)extra);
 }
 
@@ -486,6 +483,10 @@ SR_DEV float rel_std_err(float sum, float sumSqr, uint32_t n) {
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_next(Frame F) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) {  // both shadow queues are spent; the next sample's first trace reads one of them
+        F.counts[kShadowCount0] = 0;
+        F.counts[kShadowCount1] = 0;
+    }
     bool again = false;
     if (q < F.nSlots && F.acc[q].active) {
         PixAcc a = F.acc[q];
@@ -1158,7 +1159,8 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows) {
     F.hitA = c->dHitA.ptr;
     F.hitSlot = c->dHitSlot.ptr;
     F.paths = c->dPaths.ptr;
-    F.shadows = c->dShadows.ptr;
+    F.shadows[0] = c->dShadows.ptr;
+    F.shadows[1] = c->dShadows.ptr + c->dShadows.n / 2;
     F.acc = c->dAcc.ptr;
     F.queue[0] = c->dQueue0.ptr;
     F.queue[1] = c->dQueue1.ptr;
@@ -1191,7 +1193,7 @@ int alloc_frame(sr_ctx* c, uint32_t maxSlots) {
     const size_t px = (size_t)c->params.width * c->params.height;
     int rc;
     if ((rc = ensure(c->dRayO, maxSlots)) || (rc = ensure(c->dRayD, maxSlots)) || (rc = ensure(c->dHitA, maxSlots)) ||
-        (rc = ensure(c->dHitSlot, maxSlots)) || (rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dShadows, maxSlots)) ||
+        (rc = ensure(c->dHitSlot, maxSlots)) || (rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dShadows, (size_t)maxSlots * 2)) ||
         (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxSlots)) || (rc = ensure(c->dQueue1, maxSlots)) ||
         (rc = ensure(c->dCounts, kCounterWords)) ||
         (rc = ensure(c->dCtr, SR_COUNTER_COUNT)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
@@ -1226,17 +1228,19 @@ int run_band(sr_ctx* c, const Frame& F) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, F);
     hipLaunchKernelGGL(k_start<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
     mark(c, 3);
+    auto trace = c->dev.anyTransparent ? k_trace<COUNT, true> : k_trace<COUNT, false>;
     for (uint32_t smp = 0; smp < F.maxS; ++smp) {
         int cur = 0;
         for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
-            hipLaunchKernelGGL(k_extend<COUNT>, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
+            hipLaunchKernelGGL(trace, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
             mark(c, 0);
             hipLaunchKernelGGL(k_shade<R>, dim3(g), dim3(kBlock), 0, s, F, c->dev, cur);
             mark(c, 1);
-            hipLaunchKernelGGL(k_shadow, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, seg == F.bounces ? 1 : 0);
-            mark(c, 2);
             cur ^= 1;
         }
+        // The last segment's shadow rays (its ray queue is empty).
+        hipLaunchKernelGGL(trace, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
+        mark(c, 2);
         hipLaunchKernelGGL(k_next<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
         mark(c, 3);
         c->iterations++;
