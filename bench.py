@@ -6,10 +6,10 @@ fBm-displaced sphere plus ground under a 2048x1024 log-normal environment map (C
 scene is uploaded once; each timed step renders into HBM-resident AOVs (no host copies inside the
 timed region).
 
-Prints one JSON line: Msamples/s, the dominant kernel's roofline (closest-hit extend kernel:
-algorithmic bytes per launch from the reference's traversal counters over its average launch time,
-measured with HIP events on the render stream) and the CPU oracle's throughput on a bounded row band
-of the same frame.
+Prints one JSON line: Msamples/s, the dominant kernel's roofline (the trace kernel: closest hit of
+a segment's rays plus the previous segment's shadow rays; algorithmic bytes per launch from the
+reference's traversal counters over its average launch time, measured with HIP events on the render
+stream) and the CPU oracle's throughput on a bounded row band of the same frame.
 """
 import argparse
 import json
@@ -21,11 +21,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# Algorithmic bytes of the extend kernel per unit of work (per-unit figures of the reference's
+# Algorithmic bytes of the trace kernel per unit of work (per-unit figures of the reference's
 # counters, bvh.cs:188-203): 32 B per child-box test (a 64-byte sibling pair per interior visit),
-# 40 B per triangle test (TriangleLean 36 B + item index 4 B), 48 B per ray (queue index 4 B,
-# origin + direction 24 B, hit record 20 B).
-BYTES_PER_BOX, BYTES_PER_TRI, BYTES_PER_RAY = 32, 40, 48
+# 40 B per triangle test (TriangleLean 36 B + item index 4 B), 48 B per closest-hit ray (queue
+# index 4 B, origin + direction 24 B, hit record 20 B), 104 B per shadow ray (80 B deferred record,
+# read-modify-write of the path's 12 B radiance).
+BYTES_PER_BOX, BYTES_PER_TRI, BYTES_PER_RAY, BYTES_PER_SHADOW = 32, 40, 48, 104
 
 
 def log(msg):
@@ -115,27 +116,28 @@ def main():
         walls.append(time.perf_counter() - t0)
         gpus.append(r.gpu_seconds())
         kt = r.kernel_times()
-        ext_ms += kt["extend"][0]
-        ext_launch += kt["extend"][1]
+        ext_ms += kt["trace"][0] + kt["trace_tail"][0]
+        ext_launch += kt["trace"][1] + kt["trace_tail"][1]
         log(f"[bench] step {i}: {walls[-1] * 1e3:.1f} ms wall, {gpus[-1] * 1e3:.1f} ms GPU, "
-            f"extend {kt['extend'][0]:.1f} ms / {kt['extend'][1]} launches, shade {kt['shade'][0]:.1f} ms, "
-            f"shadow {kt['shadow'][0]:.1f} ms, other {kt['other'][0]:.1f} ms")
+            f"trace {kt['trace'][0]:.1f} ms / {kt['trace'][1]} launches, shade {kt['shade'][0]:.1f} ms, "
+            f"trace tail {kt['trace_tail'][0]:.1f} ms, other {kt['other'][0]:.1f} ms")
     samples = spec.width * spec.height * spec.max_samples
     ms_per_step = 1e3 * sum(walls) / len(walls)
     msps = samples / (sum(walls) / len(walls)) / 1e6
 
-    # Traversal counters of one (untimed) frame give the algorithmic bytes of the extend kernel.
+    # Traversal counters of one (untimed) frame give the algorithmic bytes of the trace kernel
+    # (its node and triangle tests include the shadow rays').
     r = frame(counters=True)
     r.wait()
     c = r.counters()
     ext_bytes = (BYTES_PER_BOX * c["BvhIntersectNode"] + BYTES_PER_TRI * c["BvhIntersectShape"] +
-                 BYTES_PER_RAY * c["SceneTrace"])
+                 BYTES_PER_RAY * c["SceneTrace"] + BYTES_PER_SHADOW * c["SceneOcclude"])
     launches_per_frame = ext_launch / args.steps
     avg_launch_s = ext_ms / ext_launch / 1e3
     bytes_per_launch = ext_bytes / launches_per_frame
     achieved = bytes_per_launch / avg_launch_s / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "extend_hbm_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "trace_hbm_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -164,7 +166,7 @@ def main():
                    "bounces": spec.bounces, "triangles": spec.triangle_count, "env_map": "2048x1024 f32",
                    "rng": "reference xorwow per pixel"},
         "gpu_ms_per_step": round(1e3 * sum(gpus) / len(gpus), 3),
-        "roofline": {"kernel": "k_extend (closest hit)", "bound": "hbm", "achieved": round(achieved, 2),
+        "roofline": {"kernel": "k_trace (closest hit + deferred shadow rays)", "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic, "avg_launch_us": round(avg_launch_s * 1e6, 2),
                      "algorithmic_bytes_per_launch": round(bytes_per_launch)},

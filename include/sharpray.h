@@ -195,7 +195,8 @@ int sr_counters_enable(sr_ctx* ctx, int32_t enable);
 int sr_counters_read(sr_ctx* ctx, int64_t* out);
 /* Kernel timing of the next renders: with timing enabled, an event is recorded on the render
  * stream after every launch; sr_profile_read sums the spans per kernel kind of the last render:
- * ms[4] / launches[4] for [0] extend (closest hit), [1] shade, [2] shadow, [3] raygen + accumulate.
+ * ms[4] / launches[4] for [0] trace (closest hit of a segment's rays plus the previous segment's
+ * shadow rays), [1] shade, [2] trace of a sample's last shadow rays, [3] raygen + accumulate.
  * No counterpart in the reference (its TimeScope covers whole phases, counters.cs:4-20). */
 int sr_profile_enable(sr_ctx* ctx, int32_t enable);
 int sr_profile_read(sr_ctx* ctx, double* ms, uint32_t* launches);

@@ -430,11 +430,13 @@ class Renderer:
         return out.value
 
     def kernel_times(self):
-        """{kind: (total ms, launches)} for extend, shade, shadow and other kernels of this render."""
+        """{kind: (total ms, launches)} of this render: "trace" (closest hit of a segment's rays plus
+        the previous segment's shadow rays), "shade", "trace_tail" (the same trace kernel on a
+        sample's last shadow rays) and "other" (camera rays, accumulation)."""
         ms = (ctypes.c_double * 4)()
         n = (ctypes.c_uint32 * 4)()
         check(N.lib().sr_profile_read(self.scene.ctx, ms, n), "kernel times")
-        return {k: (ms[i], n[i]) for i, k in enumerate(["extend", "shade", "shadow", "other"])}
+        return {k: (ms[i], n[i]) for i, k in enumerate(["trace", "shade", "trace_tail", "other"])}
 
     def counters(self):
         out = (ctypes.c_int64 * N.SR_COUNTER_COUNT)()

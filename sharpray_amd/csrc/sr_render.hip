@@ -203,6 +203,12 @@ SR_DEV void flush_tally(const Frame& F, const Tally& tl) {
 SR_DEV void bump(const Frame& F, int idx, uint32_t n) {
     if (F.ctr != nullptr && n) atomicAdd(&F.ctr[idx], (unsigned long long)n);
 }
+// Counts the lanes of the wave where pred holds with one atomic; every lane of the wave calls it.
+SR_DEV void bump_wave(const Frame& F, int idx, bool pred) {
+    if (F.ctr == nullptr) return;
+    const unsigned long long mask = __ballot(pred);
+    if (mask && (int)lane_id() == __ffsll(mask) - 1) atomicAdd(&F.ctr[idx], (unsigned long long)__popcll(mask));
+}
 
 // ---- Kernels ------------------------------------------------------------------------------------
 __global__ void k_reset(Frame F) {
@@ -226,9 +232,10 @@ __global__ void k_start(Frame F) {
             rng_store(p, xorwow_seed(x, y));
             begin_sample<R>(F, q, x, y, 0, p);
             F.paths[q] = p;
-            if (F.ctr) { atomicAdd(&F.ctr[0], 1ull); atomicAdd(&F.ctr[1], 1ull); }
         }
     }
+    bump_wave(F, 0, live);
+    bump_wave(F, 1, live);
     const uint32_t slot = wave_append(&F.counts[kRayCount0], live);
     if (live) F.queue[0][slot] = q;
 }
@@ -389,18 +396,17 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) 
 
 // ShadowTransmittance (scene.cs:576-600) for one deferred shadow ray, then its direct-light
 // contribution ((((sky * T) * f) * w) / pdf) * energy (scene.cs:793, 809, 659-661, 699).
-template <bool TRANSP>
-SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, uint32_t& extra) {
+template <bool COUNT, bool TRANSP>
+SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, Tally& tl, uint32_t& extra) {
     ray3 ray = mk_ray(sr.o, sr.d);
     vec3 tr = splat3(1.0f);
     if (!TRANSP) {
-        if (scene_any(S, ray, sr_lds + threadIdx.x)) tr = splat3(0.0f);
+        if (scene_any<COUNT>(S, ray, tl, sr_lds + threadIdx.x)) tr = splat3(0.0f);
     } else {
-        Tally tl = {0, 0, 0};
         for (;;) {
             HitRec h;
             h.obj = -1;
-            if (!scene_closest<false>(S, ray, h, tl, sr_lds + threadIdx.x)) break;
+            if (!scene_closest<COUNT>(S, ray, h, tl, sr_lds + threadIdx.x)) break;
             const DevObject& ob = S.objects[h.obj];
             if (ob.transparency <= 0.0f) { tr = splat3(0.0f); break; }
             SurfaceD s;
@@ -462,7 +468,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(Frame F, DevScene S, int cur) 
                 F.hitSlot[q] = h.slot;
             }
         } else if (i - nE64 < nS) {
-            shadow_item<TRANSP>(F, S, SR[i - nE64], extra);
+            shadow_item<COUNT, TRANSP>(F, S, SR[i - nE64], tl, extra);
         }
     }
     flush_tally<COUNT>(F, tl);
@@ -487,7 +493,7 @@ __global__ void __launch_bounds__(kBlock) k_next(Frame F) {
         F.counts[kShadowCount0] = 0;
         F.counts[kShadowCount1] = 0;
     }
-    bool again = false;
+    bool again = false, converged = false, maxed = false;
     if (q < F.nSlots && F.acc[q].active) {
         PixAcc a = F.acc[q];
         PathRec p = F.paths[q];
@@ -517,7 +523,6 @@ __global__ void __launch_bounds__(kBlock) k_next(Frame F) {
             begin_sample<R>(F, q, x, y, si + 1, p);
             F.paths[q] = p;
             again = true;
-            if (F.ctr) atomicAdd(&F.ctr[1], 1ull);
         } else {
             a.active = 0;
             const size_t o = (size_t)y * F.W + x;
@@ -534,10 +539,14 @@ __global__ void __launch_bounds__(kBlock) k_next(Frame F) {
             F.outDepth[o] = depth;
             F.outSamples[o] = count;
             F.outVar[o] = var;
-            if (F.ctr) atomicAdd(&F.ctr[count < F.maxS ? 8 : 9], 1ull);
+            converged = count < F.maxS;
+            maxed = !converged;
         }
         F.acc[q] = a;
     }
+    bump_wave(F, 1, again);
+    bump_wave(F, 8, converged);
+    bump_wave(F, 9, maxed);
     const uint32_t slot = wave_append(&F.counts[kRayCount0], again);
     if (again) F.queue[0][slot] = q;
 }

@@ -115,7 +115,8 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
 
 // Whether any primitive of a mesh is hit (the same node visits as the closest-hit traversal up
 // to its first accepted hit, which is where the existence of a hit is decided).
-SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, uint2* lds) {
+template <bool COUNT>
+SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, Tally& tl, uint2* lds) {
     const float4* N = S.nodes;
     TravStack st;
     st.col = lds;
@@ -126,6 +127,7 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
         bool next = false;
         const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
         if (cnt == 0) {
+            if (COUNT) tl.box += 2;
             const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)child) * 2;
             const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
             float tA = 0.0f, tB = 0.0f;
@@ -145,6 +147,7 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
         } else {
             const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
             for (uint32_t i = 0; i < cnt; ++i, T += 9) {
+                if (COUNT) tl.shape += 1;
                 float t, u, v;
                 if (tri_hit(T, r, t, u, v)) return true;
             }
@@ -181,10 +184,14 @@ SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r
     return true;
 }
 
-SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, uint2* lds) {
+template <bool COUNT>
+SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, Tally& tl, uint2* lds) {
     float scale;
     const ray3 lr = to_object(ob, r, scale);
-    if (ob.kind == 0) return ob.nodeBase >= 0 && mesh_any(S, ob.nodeBase, ob.triBase, lr, lds);
+    if (ob.kind == 0) {
+        if (COUNT) tl.mesh += 1;
+        return ob.nodeBase >= 0 && mesh_any<COUNT>(S, ob.nodeBase, ob.triBase, lr, tl, lds);
+    }
     ShapeHitD h;
     return ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
 }
@@ -251,7 +258,10 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
     return found;
 }
 
-SR_DEV bool scene_any(const DevScene& S, const ray3& r, uint2* lds) {
+// Whether a ray reaches no surface at all: for scenes without transparent objects this decides
+// ShadowTransmittance (scene.cs:576-600) exactly as its closest-hit trace does.
+template <bool COUNT>
+SR_DEV bool scene_any(const DevScene& S, const ray3& r, Tally& tl, uint2* lds) {
     if (S.sceneNodes == 0) return false;
     const float4* N = S.nodes;
     uint32_t stk[kStack];
@@ -262,6 +272,7 @@ SR_DEV bool scene_any(const DevScene& S, const ray3& r, uint2* lds) {
         bool next = false;
         const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
         if (cnt == 0) {
+            if (COUNT) tl.box += 2;
             const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)child) * 2;
             const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
             float tA = 0.0f, tB = 0.0f;
@@ -279,8 +290,10 @@ SR_DEV bool scene_any(const DevScene& S, const ray3& r, uint2* lds) {
                 cur = pB; next = true;
             }
         } else {
-            for (uint32_t i = 0; i < cnt; ++i)
-                if (object_any(S, S.objects[S.sceneItems[child + i]], r, lds)) return true;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                if (COUNT) tl.shape += 1;
+                if (object_any<COUNT>(S, S.objects[S.sceneItems[child + i]], r, tl, lds)) return true;
+            }
         }
         if (!next) {
             if (sp == 0) return false;
