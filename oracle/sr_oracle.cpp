@@ -10,7 +10,7 @@
 // are sequenced in C#'s left-to-right order, and MathF.Min/Max keep .NET 8's NaN-propagating,
 // -0 < +0 semantics. Transcendentals go to the host libm (glibc), which is what .NET's MathF
 // calls on Linux; the exact bits of those calls are the one part that no reference test pins
-// ("parity unpinned"; see the oracle section of the design notes).
+// ("parity unpinned": no reference test or fixture holds their outputs).
 //
 // Compiled twice: SR_REAL=float (the parity oracle) and SR_REAL=double (the FP64 oracle used to
 // judge converged images). In the double build the BVH and all shading run in double precision.
