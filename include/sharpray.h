@@ -86,9 +86,14 @@ typedef struct sr_view {
  * (renderer.cs:156); SR_RNG_COUNTER draws from a counter-based hash of (seed, pixel, sample,
  * draw index) so one path's divergence cannot shift another sample's draws (no reference
  * counterpart). tick_bands: number of row bands the frame is split into (>= 1); sr_render_tick
- * returns after each band. */
+ * returns after each band. light_sampler picks how a texture sky's light directions are drawn:
+ * SR_LIGHT_CDF walks the reference's row/column CDFs (Cdf2, cdf.cs:62-98); SR_LIGHT_ALIAS draws
+ * the same discrete distribution from a Walker alias table with one 16-byte load (no reference
+ * counterpart; same two uniforms, same pdf). */
 #define SR_RNG_REFERENCE 0
 #define SR_RNG_COUNTER 1
+#define SR_LIGHT_CDF 0
+#define SR_LIGHT_ALIAS 1
 typedef struct sr_render_params {
     uint32_t width;
     uint32_t height;
@@ -101,6 +106,7 @@ typedef struct sr_render_params {
     uint32_t rng_policy;
     uint32_t rng_seed;
     uint32_t tick_bands;
+    uint32_t light_sampler;
 } sr_render_params;
 
 /* Compositor ctor denoise arguments (compose.cs:22-34). */
@@ -165,11 +171,19 @@ int sr_render_read(sr_ctx* ctx, float* radiance, float* radiance_fog, float* nor
  * for callers that keep the image resident in HBM. No counterpart in the reference. */
 int sr_render_device_aovs(sr_ctx* ctx, void** radiance, void** radiance_fog, void** normals, void** uv,
                           void** depth, void** samples, void** variance);
+/* Albedo AOV (w*h*3): mean of the primary surface's Surface.Albedo over the samples whose camera
+ * ray hit a surface (the samples Depth averages over), 0 where none did. A denoiser guide with no
+ * counterpart in the reference (its Fragment, scene.cs:154-159, has no albedo). */
+int sr_render_read_albedo(sr_ctx* ctx, float* albedo);
 
 /* ---- Compositor (compose.cs:101-204) ------------------------------------------------------- */
 /* Compositor.Filter over the last render's AOVs (joint bilateral, variance-driven). Writes w*h*3
  * floats to out_rgb (host). */
 int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* params, float* out_rgb);
+/* The same filter with the albedo AOV as a fourth guide: on pixels with a surface, a neighbour's
+ * weight is also multiplied by exp(-|albedo_c - albedo_n|^2 / (2 albedo_limit^2)), applied after
+ * the depth term (albedo_limit > 0). No counterpart in the reference. */
+int sr_denoise_bilateral_guided(sr_ctx* ctx, const sr_denoise_params* params, float albedo_limit, float* out_rgb);
 /* Same filter on caller-provided device arrays (radiance/normals: w*h*3, depth/variance: w*h). */
 int sr_denoise_bilateral_device(sr_ctx* ctx, const sr_denoise_params* params, const float* d_radiance,
                                 const float* d_normals, const float* d_depth, const float* d_variance,

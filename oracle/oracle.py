@@ -51,10 +51,12 @@ _SIGS = {
                                 _F, _F]),
     "oc_render": (ctypes.c_int, [_V, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                  ctypes.c_float, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
-                                 ctypes.c_int, _F, _F, _F, _F, _F, _U, _F, _L, ctypes.c_int, ctypes.c_uint32]),
-    "oc_filter": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _F, _L]),
+                                 ctypes.c_int, _F, _F, _F, _F, _F, _U, _F, _L, ctypes.c_int, ctypes.c_uint32,
+                                 ctypes.c_int, _F]),
+    "oc_filter": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _F, _L, _F,
+                          ctypes.c_float]),
     "oc_sky_eval": (None, [_V, _F, ctypes.c_uint32, _F, _F]),
-    "oc_sky_sample": (None, [_V, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
+    "oc_sky_sample": (None, [_V, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _F]),
     "oc_surface_eval": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, _F]),
     "oc_surface_scatter": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
     "oc_fog_optical_depth": (None, [_F, _F, _F, ctypes.c_uint32, _F]),
@@ -172,14 +174,15 @@ class OracleScene:
         shp = (y1 - y0, s.width)
         return obj.reshape(shp), prim.reshape(shp), t.reshape(shp), nrm.reshape(shp + (3,))
 
-    def render(self, y0=0, y1=None, threads=None, counters=False, rng_policy=0, rng_seed=0):
+    def render(self, y0=0, y1=None, threads=None, counters=False, rng_policy=0, rng_seed=0, light_sampler=0):
         s = self.spec
         y1 = s.height if y1 is None else y1
         threads = threads or os.cpu_count() or 1
         n = (y1 - y0) * s.width
         out = dict(radiance=np.empty((n, 3), np.float32), radiance_fog=np.empty((n, 3), np.float32),
                    normals=np.empty((n, 3), np.float32), uv=np.empty((n, 2), np.float32),
-                   depth=np.empty(n, np.float32), samples=np.empty(n, np.uint32), variance=np.empty(n, np.float32))
+                   depth=np.empty(n, np.float32), samples=np.empty(n, np.uint32), variance=np.empty(n, np.float32),
+                   albedo=np.empty((n, 3), np.float32))
         ctr = np.zeros(len(COUNTERS), np.int64)
         rc = self.L.oc_render(self.h, f(s.view), s.width, s.height, s.min_samples, s.max_samples,
                               s.variance_threshold, s.bounces, s.indirect_clamp, y0, y1, threads,
@@ -187,7 +190,8 @@ class OracleScene:
                               out["normals"].ctypes.data_as(_F), out["uv"].ctypes.data_as(_F),
                               out["depth"].ctypes.data_as(_F), out["samples"].ctypes.data_as(_U),
                               out["variance"].ctypes.data_as(_F), ctr.ctypes.data_as(_L) if counters else None,
-                              int(rng_policy), int(rng_seed))
+                              int(rng_policy), int(rng_seed), int(light_sampler),
+                              out["albedo"].ctypes.data_as(_F))
         if rc != 0:
             raise RuntimeError("oracle: render on an unbuilt scene")
         if counters:
@@ -205,9 +209,11 @@ def triangles_make(pos, normals=None, uv=None, smooth=False, precision="f32"):
     return out
 
 
-def bilateral(params7, radiance, normals, depth, variance, width, height, threads=None):
+def bilateral(params7, radiance, normals, depth, variance, width, height, threads=None, albedo=None,
+              albedo_limit=0.0):
     out = np.empty((width * height, 3), np.float32)
     samples = np.zeros(1, np.int64)
     lib().oc_filter(f(params7), f(radiance), f(normals), f(depth), f(variance), width, height,
-                    threads or os.cpu_count() or 1, out.ctypes.data_as(_F), samples.ctypes.data_as(_L))
+                    threads or os.cpu_count() or 1, out.ctypes.data_as(_F), samples.ctypes.data_as(_L), f(albedo),
+                    float(albedo_limit))
     return out, int(samples[0])

@@ -779,11 +779,52 @@ inline Quat quat_look(V3 forward, V3 upRef) {
     return Quat{(c2.x + c0.z) / s, (c2.y + c1.z) / s, s * 0.25f, (c0.y - c1.x) / s};
 }
 
+// Walker/Vose alias table over the environment's texel weights (no reference counterpart; the
+// reference samples the same discrete distribution with Cdf2, cdf.cs:62-98). Built in double:
+// scaled_i = w_i * n / sum(w); the worklists are stacks filled in index order; a small entry s takes
+// the last large entry l as its alias and l keeps (scaled_l + scaled_s) - 1. Leftovers keep prob 1.
+struct AliasTable {
+    std::vector<float> prob;
+    std::vector<int32_t> alias;
+    void build(const std::vector<real>& w) {
+        size_t n = w.size();
+        double total = 0;
+        for (size_t i = 0; i != n; ++i) total += (double)w[i];
+        prob.assign(n, 1.0f);
+        alias.resize(n);
+        for (size_t i = 0; i != n; ++i) alias[i] = (int32_t)i;
+        if (!(total > 0)) return;
+        std::vector<double> scaled(n);
+        std::vector<int32_t> small, large;
+        for (size_t i = 0; i != n; ++i) {
+            scaled[i] = (double)w[i] * (double)n / total;
+            (scaled[i] < 1.0 ? small : large).push_back((int32_t)i);
+        }
+        while (!small.empty() && !large.empty()) {
+            int32_t a = small.back(); small.pop_back();
+            int32_t b = large.back(); large.pop_back();
+            prob[a] = (float)scaled[a];
+            alias[a] = b;
+            scaled[b] = (scaled[b] + scaled[a]) - 1.0;
+            (scaled[b] < 1.0 ? small : large).push_back(b);
+        }
+    }
+    // Two uniforms (u.x picks the column, u.y accepts it or takes its alias).
+    uint32_t sample(real ux, real uy) const {
+        uint32_t n = (uint32_t)prob.size();
+        uint32_t i = (uint32_t)(float)((float)ux * (float)n);
+        if (i >= n) i = n - 1;
+        return (float)uy < prob[i] ? i : (uint32_t)alias[i];
+    }
+};
+
 struct Sky {
     int kind = 0;  // 0 none, 1 texture, 2 procedural
+    int sampler = 0;  // texture sky light sampling: 0 Cdf2 (reference), 1 alias table
     // SkyTexture, scene.cs:379-435.
     Texture tex;
     Cdf2 cdf;
+    AliasTable aliasTab;
     real pdfScale = 0, multiplier = 1, yaw = 0;
     Quat rot{0, 0, 0, 1}, rotInv{0, 0, 0, 1};
     // SkyProcedural / SunProcedural, scene.cs:281-377.
@@ -795,13 +836,16 @@ struct Sky {
     void init_texture() {
         int W = (int)tex.w, H = (int)tex.h;
         cdf.cond.resize(H);
-        std::vector<real> rowW(H), colW(W);
+        std::vector<real> rowW(H), colW(W), all((size_t)W * H);
         for (int y = 0; y != H; ++y) {
-            for (int x = 0; x != W; ++x)
+            for (int x = 0; x != W; ++x) {
                 colW[x] = luminance(tex.get(x, y)) * std::sin(kPi * ((real)y + 0.5f) / (real)H);
+                all[(size_t)y * W + x] = colW[x];
+            }
             cdf.cond[y].build(colW.data(), W);
             rowW[y] = cdf.cond[y].total;
         }
+        aliasTab.build(all);
         cdf.marginal.build(rowW.data(), H);
         pdfScale = (real)(W * H) / (cdf.total() * 2.0f * kPi * kPi);
         rot = angle_axis(yaw, kUp);
@@ -830,8 +874,15 @@ struct Sky {
         if (kind == 1) {
             if (cdf.total() <= 0.0f) return false;
             V2 c = rng.rand2();
-            int y = (int)cdf.marginal.sample(c.y);
-            int x = (int)cdf.cond[y].sample(c.x);
+            int x, y;
+            if (sampler == 1) {
+                uint32_t t = aliasTab.sample(c.x, c.y);
+                x = (int)(t % tex.w);
+                y = (int)(t / tex.w);
+            } else {
+                y = (int)cdf.marginal.sample(c.y);
+                x = (int)cdf.cond[y].sample(c.x);
+            }
             V2 uv = v2(((real)x + 0.5f) / (real)tex.w, ((real)y + 0.5f) / (real)tex.h);
             V3 dir = qrot(rot, from_equirect_uv(uv));
             real pdf = luminance(tex.get(x, y)) * pdfScale;
@@ -1031,6 +1082,7 @@ struct Object {
 
 struct Fragment {
     Color radiance, radianceFog;
+    Color albedo;  // primary surface albedo (valid with hasDepth); no reference counterpart
     bool hasNormal; V3 normal;
     bool hasUv; V2 uv;
     bool hasDepth; real depth;
@@ -1301,6 +1353,7 @@ struct Scene {
                     fr.hasNormal = true; fr.normal = surf.normal;
                     fr.hasUv = true; fr.uv = surf.uv;
                     fr.hasDepth = true; fr.depth = dist;
+                    fr.albedo = surf.albedo;
                 }
                 radiance = radiance + sample_sky_direct(surf, hitPos, viewDir, rng, ctr) * energy;
                 if (russian_roulette(energy, i, rng, ctr)) break;
@@ -1373,6 +1426,7 @@ inline real relative_std_err(real sum, real sumSqr, uint32_t n) {
 
 struct PixelOut {
     Color radiance, radianceFog;
+    Color albedo;  // mean primary-hit albedo over samples with a primary hit (0 if none)
     V3 normal;
     V2 uv;
     real depth;
@@ -1394,6 +1448,7 @@ PixelOut render_pixel(const Scene& sc, const View& view, const RenderParams& p, 
     V2 uv = v2(0, 0);
     real depthSum = 0;
     uint32_t depthCount = 0;
+    Color albedoSum = kBlack;
     uint32_t sampleIndex = 0;
     for (; sampleIndex != p.maxSamples; ++sampleIndex) {
         if (p.policy == 1) rng.seed_counter(p.seed, y * p.width + x, sampleIndex);
@@ -1418,6 +1473,7 @@ PixelOut render_pixel(const Scene& sc, const View& view, const RenderParams& p, 
         if (fr.hasDepth) {
             depthSum += fr.depth;
             depthCount++;
+            albedoSum = albedoSum + fr.albedo;
         }
         if (!hasUv && fr.hasUv) { hasUv = true; uv = fr.uv; }
         if (sampleIndex >= p.minSamples - 1) {
@@ -1432,6 +1488,7 @@ PixelOut render_pixel(const Scene& sc, const View& view, const RenderParams& p, 
     o.normal = normalCount > 0 ? normalize_or(normalSum, normalFallback) : v3s(0);
     o.uv = hasUv ? uv : v2(0, 0);
     o.depth = depthCount > 0 ? depthSum / (real)depthCount : kInf;
+    o.albedo = depthCount > 0 ? albedoSum / (real)depthCount : kBlack;
     o.samples = sampleCount;
     if (ctr) ctr->v[sampleCount < p.maxSamples ? C_SamplePixelConverged : C_SamplePixelMaxed]++;
     return o;
@@ -1439,6 +1496,8 @@ PixelOut render_pixel(const Scene& sc, const View& view, const RenderParams& p, 
 
 struct Denoise {
     real radius, strength, strengthMax, lumBoost, lumLimitInv, normalLimitInv, depthLimitInv;
+    const float* albedo = nullptr;  // optional albedo guide (no reference counterpart)
+    real albedoLimitInv = 0;
 };
 
 // Compositor.Filter, compose.cs:101-204 (ctor constants compose.cs:42-44).
@@ -1485,6 +1544,12 @@ Color filter_pixel(const Denoise& dn, const float* radiance, const float* normal
             if (hasCD && w > 0.0f) {
                 real dd = cd - nd;
                 w *= std::exp(-(dd * dd) * dn.depthLimitInv);
+            }
+            // Albedo guide (an extension): after the depth term, on surface pixels only.
+            if (dn.albedo && hasCN && w > 0.0f) {
+                V3 da = v3(dn.albedo[ci * 3], dn.albedo[ci * 3 + 1], dn.albedo[ci * 3 + 2]) -
+                        v3(dn.albedo[ni * 3], dn.albedo[ni * 3 + 1], dn.albedo[ni * 3 + 2]);
+                w *= std::exp(-mag_sqr(da) * dn.albedoLimitInv);
             }
             if (w > 0.0f) {
                 real ld = nmax(0.0f, luminance(nr) - lum);
@@ -1817,9 +1882,10 @@ void oc_trace_primary(void* s, const float* view9, uint32_t W, uint32_t H, uint3
 int oc_render(void* s, const float* view9, uint32_t W, uint32_t H, uint32_t minS, uint32_t maxS, float threshold,
               uint32_t bounces, float indirectClamp, uint32_t y0, uint32_t y1, int threads,
               float* radiance, float* radianceFog, float* normals, float* uvs, float* depth, uint32_t* samples,
-              float* variance, int64_t* counters, int policy, uint32_t seed) {
+              float* variance, int64_t* counters, int policy, uint32_t seed, int lightSampler, float* albedo) {
     Scene* sc = (Scene*)s;
     if (!sc->built) return -1;
+    sc->sky.sampler = lightSampler;
     View vw = ld_view(view9);
     RenderParams p{W, H, minS, maxS, bounces, (real)threshold, (real)indirectClamp, policy, seed};
     real tanFov = std::tan(vw.fov * 0.5f);
@@ -1847,6 +1913,10 @@ int oc_render(void* s, const float* view9, uint32_t W, uint32_t H, uint32_t minS
                     depth[o] = (float)po.depth;
                     samples[o] = po.samples;
                     variance[o] = (float)po.variance;
+                    if (albedo) {
+                        albedo[o * 3] = (float)po.albedo.x; albedo[o * 3 + 1] = (float)po.albedo.y;
+                        albedo[o * 3 + 2] = (float)po.albedo.z;
+                    }
                 }
         }
     };
@@ -1868,8 +1938,11 @@ int oc_render(void* s, const float* view9, uint32_t W, uint32_t H, uint32_t minS
 // Compositor.Filter over a whole image. params: radius strength strengthMax lumBoost lumLimit
 // normalLimit depthLimit (the Compositor ctor arguments, compose.cs:22-34).
 void oc_filter(const float* params7, const float* radiance, const float* normals, const float* depth,
-               const float* variance, uint32_t W, uint32_t H, int threads, float* out, int64_t* filterSamples) {
+               const float* variance, uint32_t W, uint32_t H, int threads, float* out, int64_t* filterSamples,
+               const float* albedo, float albedoLimit) {
     Denoise dn;
+    dn.albedo = albedo;
+    if (albedo) dn.albedoLimitInv = 1.0f / ((real)albedoLimit * (real)albedoLimit * 2.0f);
     dn.radius = params7[0]; dn.strength = params7[1]; dn.strengthMax = params7[2]; dn.lumBoost = params7[3];
     dn.lumLimitInv = 1.0f / ((real)params7[4] * (real)params7[4] * 2.0f);
     dn.normalLimitInv = 1.0f / ((real)params7[5] * (real)params7[5] * 2.0f);
@@ -1911,9 +1984,11 @@ void oc_sky_eval(void* s, const float* dirs, uint32_t n, float* rad3, float* pdf
         pdf[i] = sc->sky.light_dir(d, p) ? (float)p : 0.0f;
     }
 }
-// Sky light samples from seed (a,b): dir3 + pdf (pdf = 0 when LightDirRand returns null).
-void oc_sky_sample(void* s, uint32_t a, uint32_t b, uint32_t n, float* out4) {
+// Sky light samples from seed (a,b): dir3 + pdf (pdf = 0 when LightDirRand returns null);
+// lightSampler 0 = Cdf2, 1 = alias table.
+void oc_sky_sample(void* s, uint32_t a, uint32_t b, uint32_t n, int lightSampler, float* out4) {
     Scene* sc = (Scene*)s;
+    sc->sky.sampler = lightSampler;
     Rng r; r.seed_xy(a, b);
     for (uint32_t i = 0; i != n; ++i) {
         SampleLight l{v3s(0), 0};

@@ -21,6 +21,8 @@ __all__ = [
 
 RNG_REFERENCE = N.SR_RNG_REFERENCE
 RNG_COUNTER = N.SR_RNG_COUNTER
+LIGHT_CDF = N.SR_LIGHT_CDF
+LIGHT_ALIAS = N.SR_LIGHT_ALIAS
 
 
 def _f32(a, shape_tail=None):
@@ -369,7 +371,7 @@ class Renderer:
 
     def __init__(self, scene, view, width, height, block_size, min_samples, max_samples, variance_threshold, bounces,
                  indirect_clamp, rng_policy=RNG_REFERENCE, rng_seed=0, tick_bands=1, counters=False,
-                 kernel_timing=False):
+                 kernel_timing=False, light_sampler=LIGHT_CDF):
         if width <= 0 or height <= 0 or block_size <= 0:
             raise ValueError("size must be positive")
         if not (0 < min_samples <= max_samples):
@@ -383,6 +385,7 @@ class Renderer:
         p.variance_threshold = float(variance_threshold)
         p.bounces, p.indirect_clamp = int(bounces), float(indirect_clamp)
         p.rng_policy, p.rng_seed, p.tick_bands = int(rng_policy), int(rng_seed), int(tick_bands)
+        p.light_sampler = int(light_sampler)
         self.params = p
         check(N.lib().sr_counters_enable(scene.ctx, 1 if counters else 0), "counters")
         check(N.lib().sr_profile_enable(scene.ctx, 1 if kernel_timing else 0), "kernel timing")
@@ -413,6 +416,8 @@ class Renderer:
             check(N.lib().sr_render_read(self.scene.ctx, fptr(a["radiance"]), fptr(a["radiance_fog"]),
                                          fptr(a["normals"]), fptr(a["uv"]), fptr(a["depth"]), uptr(a["samples"]),
                                          fptr(a["variance"])), "render_read")
+            a["albedo"] = np.empty((n, 3), np.float32)
+            check(N.lib().sr_render_read_albedo(self.scene.ctx, fptr(a["albedo"])), "render_read_albedo")
             self._aovs = a
         return self._aovs
 
@@ -423,6 +428,7 @@ class Renderer:
     depth = property(lambda self: self._read()["depth"])
     samples = property(lambda self: self._read()["samples"])
     variance = property(lambda self: self._read()["variance"])
+    albedo = property(lambda self: self._read()["albedo"])  # primary-hit albedo guide (no reference counterpart)
 
     def gpu_seconds(self):
         out = ctypes.c_double()
@@ -460,7 +466,13 @@ class Compositor:
         return np.array([p.radius, p.strength, p.strength_max, p.luminance_boost, p.luminance_limit, p.normal_limit,
                          p.depth_limit], dtype=np.float32)
 
-    def filter(self, renderer):
+    def filter(self, renderer, albedo_limit=None):
+        """Filter (compose.cs:101-204) of the renderer's surface radiance; with albedo_limit the
+        albedo AOV joins normal and depth as a guide (no reference counterpart)."""
         out = np.empty((renderer.width * renderer.height, 3), np.float32)
-        check(N.lib().sr_denoise_bilateral(renderer.scene.ctx, ctypes.byref(self.params), fptr(out)), "denoise")
+        if albedo_limit is None:
+            check(N.lib().sr_denoise_bilateral(renderer.scene.ctx, ctypes.byref(self.params), fptr(out)), "denoise")
+        else:
+            check(N.lib().sr_denoise_bilateral_guided(renderer.scene.ctx, ctypes.byref(self.params),
+                                                      float(albedo_limit), fptr(out)), "denoise")
         return out

@@ -13,6 +13,8 @@ LIB_PATH = os.environ.get("SHARPRAY_LIB") or os.path.join(_HERE, "libsharpray_hi
 SR_OK = 0
 SR_RNG_REFERENCE = 0
 SR_RNG_COUNTER = 1
+SR_LIGHT_CDF = 0
+SR_LIGHT_ALIAS = 1
 SR_SHAPE_MESH, SR_SHAPE_SPHERE, SR_SHAPE_BOX = 0, 1, 2
 SR_COUNTER_COUNT = 15
 COUNTER_NAMES = [
@@ -62,7 +64,7 @@ class RenderParams_(ctypes.Structure):
         ("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("block_size", ctypes.c_uint32),
         ("min_samples", ctypes.c_uint32), ("max_samples", ctypes.c_uint32), ("variance_threshold", ctypes.c_float),
         ("bounces", ctypes.c_uint32), ("indirect_clamp", ctypes.c_float), ("rng_policy", ctypes.c_uint32),
-        ("rng_seed", ctypes.c_uint32), ("tick_bands", ctypes.c_uint32),
+        ("rng_seed", ctypes.c_uint32), ("tick_bands", ctypes.c_uint32), ("light_sampler", ctypes.c_uint32),
     ]
 
 
@@ -93,7 +95,10 @@ SIGNATURES = {
     "sr_render_read": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, c_float_p, c_float_p,
                                       c_uint32_p, c_float_p]),
     "sr_render_device_aovs": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_void_p)] * 7),
+    "sr_render_read_albedo": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
     "sr_denoise_bilateral": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DenoiseParams_), c_float_p]),
+    "sr_denoise_bilateral_guided": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DenoiseParams_), ctypes.c_float,
+                                                   c_float_p]),
     "sr_denoise_bilateral_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DenoiseParams_), ctypes.c_void_p,
                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                    ctypes.c_uint32, ctypes.c_void_p]),

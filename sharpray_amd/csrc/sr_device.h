@@ -50,6 +50,7 @@ struct DevSky {
     quat rot, rotInv;
     int32_t w, h;
     int32_t cdfOff;  // marginal CDF (h+1 floats) followed by h conditional CDFs (w+1 floats each)
+    int32_t sampler;  // 0: row/column CDFs, 1: alias table (DevScene::alias)
     float total, pdfScale;
     vec3 top, middle, bottom, sunDir, sunRad;
     float sunCos, sunPdfDenom;
@@ -74,6 +75,7 @@ struct DevScene {
     const float* texels;
     const DevTexture* textures;
     const float* cdf;
+    const float4* alias;  // per texel: acceptance threshold, alias texel (int bits), pdf, alias pdf
     int32_t sceneBase;   // pad node of the scene BVH
     int32_t sceneNodes;  // node count of the scene BVH (0 = empty scene)
     int32_t anyTransparent;
@@ -410,14 +412,28 @@ SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf) {
     if (k.kind == 1) {
         if (k.total <= 0.0f) return false;
         const vec2 c = next_vec2(rng);
-        const float* marg = S.cdf + k.cdfOff;
-        const uint32_t y = cdf_sample(marg, (uint32_t)k.h, c.y);
-        const float* cond = marg + (k.h + 1) + (size_t)y * (k.w + 1);
-        const uint32_t x = cdf_sample(cond, (uint32_t)k.w, c.x);
+        uint32_t x, y;
+        if (k.sampler == 1) {
+            // One 16-byte entry decides the texel and carries both candidates' pdfs.
+            const uint32_t n = (uint32_t)k.w * (uint32_t)k.h;
+            uint32_t i = (uint32_t)(c.x * (float)n);
+            if (i >= n) i = n - 1;
+            const float4 e = S.alias[i];
+            const bool keep = c.y < e.x;
+            const uint32_t texel = keep ? i : (uint32_t)__float_as_int(e.y);
+            x = texel % (uint32_t)k.w;
+            y = texel / (uint32_t)k.w;
+            pdf = keep ? e.z : e.w;
+        } else {
+            const float* marg = S.cdf + k.cdfOff;
+            y = cdf_sample(marg, (uint32_t)k.h, c.y);
+            const float* cond = marg + (k.h + 1) + (size_t)y * (k.w + 1);
+            x = cdf_sample(cond, (uint32_t)k.w, c.x);
+            const DevTexture t = S.textures[k.tex];
+            pdf = luminance(tex_get(S, t, y * (uint32_t)k.w + x)) * k.pdfScale;
+        }
         const vec2 uv = mk2(((float)(int32_t)x + 0.5f) / (float)k.w, ((float)(int32_t)y + 0.5f) / (float)k.h);
         dir = rotate(k.rot, from_equirect_uv(uv));
-        const DevTexture t = S.textures[k.tex];
-        pdf = luminance(tex_get(S, t, y * (uint32_t)k.w + x)) * k.pdfScale;
         return pdf > 0.0f;
     }
     if (k.kind == 2) {

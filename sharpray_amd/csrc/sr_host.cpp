@@ -289,6 +289,35 @@ void build_triangles(const float* pos, const float* nrm, const float* uv, uint32
     }
 }
 
+void build_alias(const float* w, size_t n, float* prob, int32_t* alias) {
+    double sum = 0.0;
+    for (size_t i = 0; i < n; ++i) sum += (double)w[i];
+    for (size_t i = 0; i < n; ++i) { prob[i] = 1.0f; alias[i] = (int32_t)i; }
+    if (!(sum > 0.0)) return;
+    // Columns below the mean donate their deficit to one column above it; the worklists are used
+    // as stacks in index order so the table is a pure function of the weights.
+    std::vector<double> excess(n);
+    std::vector<int32_t> under, over;
+    under.reserve(n);
+    over.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+        excess[i] = (double)w[i] * (double)n / sum;
+        if (excess[i] < 1.0) under.push_back((int32_t)i);
+        else over.push_back((int32_t)i);
+    }
+    while (!under.empty() && !over.empty()) {
+        const int32_t lo = under.back();
+        under.pop_back();
+        const int32_t hi = over.back();
+        over.pop_back();
+        prob[lo] = (float)excess[lo];
+        alias[lo] = hi;
+        excess[hi] = (excess[hi] + excess[lo]) - 1.0;
+        if (excess[hi] < 1.0) under.push_back(hi);
+        else over.push_back(hi);
+    }
+}
+
 float build_cdf1(const float* w, int n, float* cdf) {
     cdf[0] = 0.0f;
     for (int i = 0; i < n; ++i) cdf[i + 1] = cdf[i] + w[i];

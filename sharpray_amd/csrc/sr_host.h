@@ -38,5 +38,9 @@ box3 lean_bounds(const float* t9);
 
 // Cdf1 (cdf.cs:7-57): prefix sums normalised by the total; returns the total weight.
 float build_cdf1(const float* w, int n, float* cdf_out /* n+1 */);
+// Walker alias table of n weights (Vose's construction in double precision): prob[i] is the
+// acceptance threshold of column i, alias[i] the column taken otherwise. All-zero weights leave
+// prob = 1, alias = i.
+void build_alias(const float* w, size_t n, float* prob, int32_t* alias);
 
 }  // namespace sr

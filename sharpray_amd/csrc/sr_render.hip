@@ -58,7 +58,7 @@ struct alignas(16) ShadowRec {  // 80 B
     vec3 energy;
     uint32_t path;  // bit31: add to radianceFog
 };
-struct alignas(16) PixAcc {  // 96 B
+struct alignas(16) PixAcc {  // 112 B
     vec3 radSum, radFogSum;
     float lumSum, lumSumSqr, lumSurf, lumSurfSqr;
     vec3 normalSum, normalFallback;
@@ -69,6 +69,8 @@ struct alignas(16) PixAcc {  // 96 B
     uint32_t depthCount;
     uint32_t sampleIndex;
     uint32_t active;
+    vec3 albedoSum;  // primary-hit Surface.Albedo, averaged over depthCount
+    uint32_t pad;
 };
 
 struct ViewD {
@@ -98,6 +100,7 @@ struct Frame {
     float* outNormal;
     float* outUv;
     float* outDepth;
+    float* outAlbedo;
     uint32_t* outSamples;
     float* outVar;
     uint32_t W, H, y0, rows, tilesX, nSlots;
@@ -320,6 +323,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) 
                     a->normalCount += 1;
                     a->depthSum += dist;
                     a->depthCount += 1;
+                    a->albedoSum = a->albedoSum + s.albedo;
                     if (!a->hasUv) { a->hasUv = 1; a->uv = s.uv; }
                 }
                 // SampleSkyDirect (scene.cs:774-794): the sides are tested before the shadow ray,
@@ -537,6 +541,8 @@ __global__ void __launch_bounds__(kBlock) k_next(Frame F) {
             F.outUv[o * 2] = a.hasUv ? a.uv.x : 0.0f;
             F.outUv[o * 2 + 1] = a.hasUv ? a.uv.y : 0.0f;
             F.outDepth[o] = depth;
+            const vec3 alb = a.depthCount > 0 ? a.albedoSum / (float)a.depthCount : splat3(0.0f);
+            F.outAlbedo[o * 3] = alb.x; F.outAlbedo[o * 3 + 1] = alb.y; F.outAlbedo[o * 3 + 2] = alb.z;
             F.outSamples[o] = count;
             F.outVar[o] = var;
             converged = count < F.maxS;
@@ -577,10 +583,15 @@ __global__ void k_primary(Frame F, DevScene S, int32_t* obj, int32_t* tri, float
 // Compositor.Filter (compose.cs:101-204): joint bilateral filter over the circular window.
 struct FilterParams {
     float strength, strengthMax, lumBoost, lumLimitInv, normalLimitInv, depthLimitInv, radiusInv;
+    float albedoLimitInv;  // ALB only: exp(-|albedo delta|^2 * albedoLimitInv) on surface pixels
     int32_t kr;
 };
+// Compositor.Filter (compose.cs:101-204), one pixel per lane. ALB adds an albedo guide term (no
+// reference counterpart) after the depth term.
+template <bool ALB>
 __global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* rad, const float* nrm, const float* dep,
-                                                 const float* var, int32_t W, int32_t H, float* out) {
+                                                 const float* var, const float* alb, int32_t W, int32_t H,
+                                                 float* out) {
     const int32_t cx = blockIdx.x * 16 + (threadIdx.x & 15);
     const int32_t cy = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (cx >= W || cy >= H) return;
@@ -590,6 +601,7 @@ __global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* ra
     const float cd = dep[ci];
     const bool hasCN = len_sqr(cn) > 0.0f;
     const bool hasCD = !isinf(cd);
+    const vec3 ca = ALB ? mk3(alb[ci * 3], alb[ci * 3 + 1], alb[ci * 3 + 2]) : splat3(0.0f);
     const float lum = luminance(cr);
     const float lb = 1.0f + lum * fp.lumBoost;
     const float dw = net_min(var[ci] * lb * fp.strength, fp.strengthMax);
@@ -621,6 +633,10 @@ __global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* ra
             if (hasCD && w > 0.0f) {
                 const float dd = cd - nd;
                 w *= expf(-(dd * dd) * fp.depthLimitInv);
+            }
+            if (ALB && hasCN && w > 0.0f) {
+                const vec3 na = mk3(alb[ni * 3], alb[ni * 3 + 1], alb[ni * 3 + 2]);
+                w *= expf(-len_sqr(ca - na) * fp.albedoLimitInv);
             }
             if (w > 0.0f) {
                 const float ld = net_max(0.0f, luminance(nr) - lum);
@@ -704,6 +720,7 @@ struct sr_ctx {
     // device scene arrays
     DevBuf<float4> dNodes;
     DevBuf<float> dTris, dAttrs, dTexels, dCdf;
+    DevBuf<float4> dAlias;
     DevBuf<int32_t> dSlotTri, dSceneItems;
     DevBuf<DevObject> dObjects;
     DevBuf<DevTexture> dTextures;
@@ -715,7 +732,7 @@ struct sr_ctx {
     DevBuf<PixAcc> dAcc;
     DevBuf<uint32_t> dQueue0, dQueue1, dCounts;
     DevBuf<unsigned long long> dCtr;
-    DevBuf<float> dRad, dRadFog, dNormal, dUv, dDepth, dVar;
+    DevBuf<float> dRad, dRadFog, dNormal, dUv, dDepth, dVar, dAlbedo;
     DevBuf<uint32_t> dSamples;
     uint32_t* hostCount = nullptr;  // pinned: the live-ray count read back between chunks
     bool rendering = false;
@@ -752,12 +769,13 @@ struct sr_ctx {
         if (hostCount) (void)hipHostFree(hostCount);
         if (evStart) (void)hipEventDestroy(evStart);
         if (evEnd) (void)hipEventDestroy(evEnd);
-        dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release();
+        dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release(); dAlias.release();
         dSlotTri.release(); dSceneItems.release(); dObjects.release(); dTextures.release();
         dRayO.release(); dRayD.release(); dHitA.release(); dHitSlot.release();
         dPaths.release(); dShadows.release(); dAcc.release();
         dQueue0.release(); dQueue1.release(); dCounts.release(); dCtr.release();
         dRad.release(); dRadFog.release(); dNormal.release(); dUv.release(); dDepth.release(); dVar.release();
+        dAlbedo.release();
         dSamples.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1039,6 +1057,7 @@ int sr_scene_build(sr_ctx* ctx) {
     if (rc) return rc;
 
     std::vector<float> texels, cdf;
+    std::vector<float4> aliasTab;
     std::vector<DevTexture> dtex;
     for (const HostTexture& t : ctx->textures) {
         DevTexture d;
@@ -1066,17 +1085,37 @@ int sr_scene_build(sr_ctx* ctx) {
         K.w = W; K.h = H;
         K.cdfOff = 0;
         cdf.assign((size_t)(H + 1) + (size_t)H * (W + 1), 0.0f);
-        std::vector<float> rowW(H), colW(W);
+        const size_t n = (size_t)W * H;
+        std::vector<float> rowW(H), colW(W), allW(n);
         for (int y = 0; y < H; ++y) {
             const float s = sinf(kPi * ((float)y + 0.5f) / (float)H);
             for (int x = 0; x < W; ++x) {
                 const float* c = t.rgb.data() + ((size_t)y * W + x) * 3;
                 colW[x] = luminance(mk3(c[0], c[1], c[2])) * s;
+                allW[(size_t)y * W + x] = colW[x];
             }
             rowW[y] = build_cdf1(colW.data(), W, cdf.data() + (H + 1) + (size_t)y * (W + 1));
         }
         K.total = build_cdf1(rowW.data(), H, cdf.data());
         K.pdfScale = (float)(W * H) / (K.total * 2.0f * kPi * kPi);
+        // Alias table over the same weights; each entry carries the pdf LightDirRand returns for
+        // either outcome (luminance * pdfScale, scene.cs:414).
+        std::vector<float> prob(n);
+        std::vector<int32_t> alias(n);
+        build_alias(allW.data(), n, prob.data(), alias.data());
+        aliasTab.resize(n);
+        auto texel_pdf = [&](size_t i) {
+            const float* c = t.rgb.data() + i * 3;
+            return luminance(mk3(c[0], c[1], c[2])) * K.pdfScale;
+        };
+        for (size_t i = 0; i < n; ++i) {
+            float4 e;
+            e.x = prob[i];
+            memcpy(&e.y, &alias[i], sizeof(float));
+            e.z = texel_pdf(i);
+            e.w = texel_pdf((size_t)alias[i]);
+            aliasTab[i] = e;
+        }
     } else {
         const sr_sky_procedural& p = ctx->skyProc;
         K.top = mk3(p.radiance_top[0], p.radiance_top[1], p.radiance_top[2]);
@@ -1132,6 +1171,7 @@ int sr_scene_build(sr_ctx* ctx) {
     if ((rc = upload(ctx->dTexels, texels.data(), texels.size(), s))) return rc;
     if ((rc = upload(ctx->dTextures, dtex.data(), dtex.size(), s))) return rc;
     if ((rc = upload(ctx->dCdf, cdf.data(), cdf.size(), s))) return rc;
+    if ((rc = upload(ctx->dAlias, aliasTab.data(), aliasTab.size(), s))) return rc;
     HIP_TRY(hipStreamSynchronize(s));
     S.nodes = ctx->dNodes.ptr;
     S.tris = ctx->dTris.ptr;
@@ -1142,6 +1182,7 @@ int sr_scene_build(sr_ctx* ctx) {
     S.texels = ctx->dTexels.ptr;
     S.textures = ctx->dTextures.ptr;
     S.cdf = ctx->dCdf.ptr;
+    S.alias = ctx->dAlias.ptr;
     ctx->built = true;
     return SR_OK;
 }
@@ -1180,6 +1221,7 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows) {
     F.outNormal = c->dNormal.ptr;
     F.outUv = c->dUv.ptr;
     F.outDepth = c->dDepth.ptr;
+    F.outAlbedo = c->dAlbedo.ptr;
     F.outSamples = c->dSamples.ptr;
     F.outVar = c->dVar.ptr;
     F.W = c->params.width;
@@ -1206,7 +1248,7 @@ int alloc_frame(sr_ctx* c, uint32_t maxSlots) {
         (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxSlots)) || (rc = ensure(c->dQueue1, maxSlots)) ||
         (rc = ensure(c->dCounts, kCounterWords)) ||
         (rc = ensure(c->dCtr, SR_COUNTER_COUNT)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
-        (rc = ensure(c->dNormal, px * 3)) || (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) ||
+        (rc = ensure(c->dNormal, px * 3)) || (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) || (rc = ensure(c->dAlbedo, px * 3)) ||
         (rc = ensure(c->dVar, px)) || (rc = ensure(c->dSamples, px)))
         return rc;
     if (!c->hostCount) HIP_TRY(hipHostMalloc((void**)&c->hostCount, sizeof(uint32_t), hipHostMallocDefault));
@@ -1315,6 +1357,7 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
     if (p->width == 0 || p->height == 0 || p->block_size == 0) return fail(SR_ERR_INVALID, "render: zero size");
     if (p->min_samples == 0 || p->min_samples > p->max_samples) return fail(SR_ERR_INVALID, "render: need 0 < minSamples <= maxSamples");
     if (p->rng_policy > 1) return fail(SR_ERR_INVALID, "render: unknown rng policy");
+    if (p->light_sampler > 1) return fail(SR_ERR_INVALID, "render: unknown light sampler");
     if (p->width > 32768 || p->height > 32768) return fail(SR_ERR_LIMIT, "render: image larger than 32768");
     ctx->join();  // a previous render finishes first
     HIP_TRY(hipSetDevice(ctx->device));
@@ -1323,6 +1366,7 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
         if (rc) return rc;
     }
     ctx->params = *p;
+    ctx->dev.sky.sampler = (int32_t)p->light_sampler;
     if (ctx->params.tick_bands == 0) ctx->params.tick_bands = 1;
     const uint32_t bands = std::min(ctx->params.tick_bands, (p->height + 7) / 8);
     const uint32_t rowsPer = ((p->height + bands - 1) / bands + 7) / 8 * 8;
@@ -1383,6 +1427,17 @@ int sr_render_read(sr_ctx* ctx, float* rad, float* radFog, float* nrm, float* uv
     if (depth) HIP_TRY(hipMemcpy(depth, ctx->dDepth.ptr, px * 4, hipMemcpyDeviceToHost));
     if (samples) HIP_TRY(hipMemcpy(samples, ctx->dSamples.ptr, px * 4, hipMemcpyDeviceToHost));
     if (var) HIP_TRY(hipMemcpy(var, ctx->dVar.ptr, px * 4, hipMemcpyDeviceToHost));
+    return SR_OK;
+}
+
+int sr_render_read_albedo(sr_ctx* ctx, float* albedo) {
+    if (!ctx || !albedo) return fail(SR_ERR_INVALID, "read_albedo: null argument");
+    if (!ctx->rendering) return fail(SR_ERR_STATE, "no render in progress");
+    int rc = wait_render(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t px = (size_t)ctx->params.width * ctx->params.height;
+    HIP_TRY(hipMemcpy(albedo, ctx->dAlbedo.ptr, px * 12, hipMemcpyDeviceToHost));
     return SR_OK;
 }
 
@@ -1517,10 +1572,12 @@ int sr_bvh_export(sr_ctx* ctx, int32_t mesh, float* nodes, int32_t* items, uint3
     return SR_OK;
 }
 
-int sr_denoise_bilateral_device(sr_ctx* ctx, const sr_denoise_params* p, const float* rad, const float* nrm,
-                                const float* dep, const float* var, uint32_t W, uint32_t H, float* out) {
+namespace {
+int launch_filter(sr_ctx* ctx, const sr_denoise_params* p, const float* rad, const float* nrm, const float* dep,
+                  const float* var, const float* alb, float albedoLimit, uint32_t W, uint32_t H, float* out) {
     if (!ctx || !p || !rad || !nrm || !dep || !var || !out || W == 0 || H == 0)
         return fail(SR_ERR_INVALID, "denoise: bad arguments");
+    if (alb && !(albedoLimit > 0.0f)) return fail(SR_ERR_INVALID, "denoise: albedo limit must be > 0");
     if (ctx->device < 0) return fail(SR_ERR_STATE, "host-only context (device -1) cannot render");
     HIP_TRY(hipSetDevice(ctx->device));
     FilterParams fp;
@@ -1534,13 +1591,20 @@ int sr_denoise_bilateral_device(sr_ctx* ctx, const sr_denoise_params* p, const f
     const float radiusPixels = p->radius * sqrtf((float)(W * H));
     fp.radiusInv = 1.0f / (radiusPixels * radiusPixels * 2.0f);
     fp.kr = (int32_t)ceilf(radiusPixels * 3.0f);
-    hipLaunchKernelGGL(k_filter, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, ctx->stream, fp, rad, nrm, dep, var,
-                       (int32_t)W, (int32_t)H, out);
+    fp.albedoLimitInv = alb ? 1.0f / (albedoLimit * albedoLimit * 2.0f) : 0.0f;
+    const dim3 grid((W + 15) / 16, (H + 15) / 16);
+    if (alb)
+        hipLaunchKernelGGL(k_filter<true>, grid, dim3(256), 0, ctx->stream, fp, rad, nrm, dep, var, alb, (int32_t)W,
+                           (int32_t)H, out);
+    else
+        hipLaunchKernelGGL(k_filter<false>, grid, dim3(256), 0, ctx->stream, fp, rad, nrm, dep, var, alb, (int32_t)W,
+                           (int32_t)H, out);
     HIP_TRY(hipGetLastError());
     return SR_OK;
 }
 
-int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* p, float* out_rgb) {
+// Filters the last render's AOVs into host memory.
+int filter_to_host(sr_ctx* ctx, const sr_denoise_params* p, bool guided, float albedoLimit, float* out_rgb) {
     if (!ctx || !p || !out_rgb) return fail(SR_ERR_INVALID, "denoise: null argument");
     if (!ctx->rendering) return fail(SR_ERR_STATE, "no render to denoise");
     int rc0 = wait_render(ctx);
@@ -1549,13 +1613,28 @@ int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* p, float* out_rgb
     const uint32_t W = ctx->params.width, H = ctx->params.height;
     float* dOut = nullptr;
     HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 12));
-    int rc = sr_denoise_bilateral_device(ctx, p, ctx->dRad.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, ctx->dVar.ptr, W, H, dOut);
+    int rc = launch_filter(ctx, p, ctx->dRad.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, ctx->dVar.ptr,
+                           guided ? ctx->dAlbedo.ptr : nullptr, albedoLimit, W, H, dOut);
     hipError_t e = rc == SR_OK ? hipStreamSynchronize(ctx->stream) : hipSuccess;
     if (rc == SR_OK && e == hipSuccess) e = hipMemcpy(out_rgb, dOut, (size_t)W * H * 12, hipMemcpyDeviceToHost);
     (void)hipFree(dOut);
     if (rc) return rc;
     if (e != hipSuccess) return fail(SR_ERR_HIP, hipGetErrorString(e));
     return SR_OK;
+}
+}  // namespace
+
+int sr_denoise_bilateral_device(sr_ctx* ctx, const sr_denoise_params* p, const float* rad, const float* nrm,
+                                const float* dep, const float* var, uint32_t W, uint32_t H, float* out) {
+    return launch_filter(ctx, p, rad, nrm, dep, var, nullptr, 0.0f, W, H, out);
+}
+
+int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* p, float* out_rgb) {
+    return filter_to_host(ctx, p, false, 0.0f, out_rgb);
+}
+
+int sr_denoise_bilateral_guided(sr_ctx* ctx, const sr_denoise_params* p, float albedo_limit, float* out_rgb) {
+    return filter_to_host(ctx, p, true, albedo_limit, out_rgb);
 }
 
 int sr_triangles_build(const float* pos, const float* nrm, const float* uv, uint32_t n, int32_t smooth, float* out) {

@@ -13,7 +13,7 @@ Bars (per the tracer's parity contract):
 import numpy as np
 import pytest
 
-from tests.helpers import bits, rel_err
+from tests.helpers import bits, rel_err, rel_mse
 
 pytestmark = pytest.mark.gpu
 
@@ -120,4 +120,101 @@ def test_bilateral_filter_parity(plain_cpu, oracle_lib):
                                    spec.height)
     err = rel_err(out, oref, floor=1e-3)
     print(f"bilateral: max rel err {err.max():.2e}", flush=True)
+    assert err.max() <= 1e-4
+
+
+def test_radiance_parity_height_fog(oracle_lib):
+    """Exponential height fog (scene.cs:438-505): free flight, fog NEE with HG MIS, the fog
+    radiance AOV of primary segments, fog transmittance of shadow rays. Counter RNG policy, so a
+    path whose decision flips cannot shift the rest of the pixel."""
+    from sharpray_amd import RNG_COUNTER, scenes
+    spec = scenes.config_plain_cpu(scale=0.5, spp=8)
+    spec.fog = (0.35, (0.9, 0.85, 0.8), 0.3, 0.6)
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=77)
+    r.wait()
+    ref = oracle_lib.OracleScene(spec).render(rng_policy=1, rng_seed=77)
+    assert float(np.abs(ref["radiance_fog"]).max()) > 0.0
+    err = rel_err(r.radiance, ref["radiance"])
+    errf = rel_err(r.radiance_fog, ref["radiance_fog"])
+    frac = float((err > RAD_TOL).mean())
+    fracf = float((errf > RAD_TOL).mean())
+    print(f"height fog: divergent fraction radiance {frac:.5f}, fog radiance {fracf:.5f}", flush=True)
+    assert frac <= MAX_DIVERGENT and fracf <= MAX_DIVERGENT
+
+
+def test_radiance_parity_transparent_objects(oracle_lib):
+    """Transparent spheres: TransparentScatter (scene.cs:121-143) and the shadow walk through
+    transparent surfaces (scene.cs:576-600), i.e. the closest-hit shadow path of the trace kernel."""
+    from sharpray_amd import RNG_COUNTER, scenes
+    spec = scenes.config_analytic(scale=1.0, spp=8)
+    for o in spec.objects[1:6]:
+        o.material = o.material.copy()
+        o.material[4] = 0.0   # metallic
+        o.material[5] = 0.85  # transparency
+        o.material[6] = 1.45  # ior
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=5)
+    r.wait()
+    ref = oracle_lib.OracleScene(spec).render(rng_policy=1, rng_seed=5)
+    err = rel_err(r.radiance, ref["radiance"])
+    frac = float((err > RAD_TOL).mean())
+    print(f"transparent: divergent fraction {frac:.5f}", flush=True)
+    assert frac <= MAX_DIVERGENT
+
+
+def test_converged_image_matches_float64_oracle(oracle_lib):
+    """The GPU's float32 estimate converges to the float64 oracle's: its relMSE against a float64
+    render (independent random numbers) is no larger than the noise between two GPU renders with
+    different seeds, up to a 25 % margin. relMSE = mean (x - r)^2 / (r^2 + 1e-2)."""
+    from sharpray_amd import RNG_COUNTER, scenes
+    spec = scenes.config_plain_cpu(scale=0.25, spp=512)
+    _, _, a = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=1)
+    a.wait()
+    ra = a.radiance.copy()
+    _, _, b = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=2)
+    b.wait()
+    rb = b.radiance.copy()
+    ref = oracle_lib.OracleScene(spec, precision="f64").render(rng_policy=1, rng_seed=3)["radiance"]
+    noise = rel_mse(ra, rb)
+    vs64 = rel_mse(ra, ref)
+    print(f"converged 64x64 @512spp: relMSE gpu-vs-fp64 {vs64:.3e}, gpu-vs-gpu {noise:.3e}", flush=True)
+    assert vs64 <= 1.25 * noise
+
+
+def test_alias_light_sampler_parity(plain_cpu, oracle_lib):
+    """The alias-table light sampler (same two uniforms, same pdf) against the oracle's alias
+    sampler: the host-built tables must agree entry for entry for the per-sample parity to hold."""
+    from sharpray_amd import LIGHT_ALIAS, RNG_COUNTER, scenes
+    spec = plain_cpu
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=9, light_sampler=LIGHT_ALIAS)
+    r.wait()
+    ref = oracle_lib.OracleScene(spec).render(rng_policy=1, rng_seed=9, light_sampler=1)
+    err = rel_err(r.radiance, ref["radiance"])
+    frac = float((err > RAD_TOL).mean())
+    print(f"alias sampler radiance: divergent fraction {frac:.5f}", flush=True)
+    assert frac <= MAX_DIVERGENT
+
+
+def test_albedo_aov(plain_cpu, oracle_lib):
+    """Mean primary-hit albedo over the samples that hit a surface (counter policy: the same camera
+    jitter per sample on both sides, so the hits and hence the averages agree)."""
+    from sharpray_amd import RNG_COUNTER, scenes
+    spec = plain_cpu
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=21)
+    r.wait()
+    ref = oracle_lib.OracleScene(spec).render(rng_policy=1, rng_seed=21)
+    assert np.any(ref["albedo"] > 0) and np.any(ref["albedo"] == 0)
+    np.testing.assert_allclose(r.albedo, ref["albedo"], rtol=1e-6, atol=1e-7)
+
+
+def test_bilateral_filter_albedo_guided_parity(plain_cpu, oracle_lib):
+    from sharpray_amd import Compositor, RNG_COUNTER, scenes
+    spec = plain_cpu
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=4)
+    r.wait()
+    comp = Compositor(0.01, 0.1, 1.75, 0.25, 2.0, 0.125, 0.2)
+    out = comp.filter(r, albedo_limit=0.15)
+    oref, _ = oracle_lib.bilateral(comp.as_array(), r.radiance, r.normals, r.depth, r.variance, spec.width,
+                                   spec.height, albedo=r.albedo, albedo_limit=0.15)
+    err = rel_err(out, oref, floor=1e-3)
+    print(f"albedo-guided bilateral: max rel err {err.max():.2e}", flush=True)
     assert err.max() <= 1e-4

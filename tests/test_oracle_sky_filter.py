@@ -81,9 +81,11 @@ def test_sky_radiance_and_mis_pdf_closed_form(oracle_lib, yaw):
         L.oc_scene_free(h)
 
 
-def test_sky_light_sampling_pmf_chi_square(oracle_lib):
-    """LightDirRand draws texel centres with probability lum * sin(theta) / total (Cdf2, cdf.cs:62-98)
-    and returns pdf = lum * pdfScale: a chi-square test of 200k draws against the closed-form pmf."""
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_sky_light_sampling_pmf_chi_square(oracle_lib, sampler):
+    """LightDirRand draws texel centres with probability lum * sin(theta) / total (Cdf2, cdf.cs:62-98;
+    sampler 1: the alias table over the same weights) and returns pdf = lum * pdfScale: a chi-square
+    test of 200k draws against the closed-form pmf."""
     L = oracle_lib.lib()
     rng = np.random.default_rng(5)
     H, W = 6, 12
@@ -94,7 +96,7 @@ def test_sky_light_sampling_pmf_chi_square(oracle_lib):
     try:
         n = 200_000
         out = np.empty((n, 4), np.float32)
-        L.oc_sky_sample(h, 3, 9, n, O.f(out))
+        L.oc_sky_sample(h, 3, 9, n, sampler, O.f(out))
     finally:
         L.oc_scene_free(h)
     t64 = tex.astype(np.float64)
@@ -118,8 +120,9 @@ def test_sky_light_sampling_pmf_chi_square(oracle_lib):
     np.testing.assert_allclose(out[:, 3], _lum(t64[y, x]) * scale, rtol=2e-5)
 
 
-def _filter_ref(params, rad, nrm, dep, var, W, H):
-    """Compositor.Filter (compose.cs:101-204) restated per pixel in float64."""
+def _filter_ref(params, rad, nrm, dep, var, W, H, alb=None, alim=0.0):
+    """Compositor.Filter (compose.cs:101-204) restated per pixel in float64; alb adds the albedo
+    guide term exp(-|da|^2 / (2 alim^2)) on surface pixels."""
     radius, strength, smax, lboost, llim, nlim, dlim = params
     rp = radius * np.sqrt(W * H)
     kr = int(np.ceil(rp * 3.0))
@@ -152,6 +155,9 @@ def _filter_ref(params, rad, nrm, dep, var, W, H):
                         w *= np.exp(-((N[cy, cx] - N[ny, nx]) ** 2).sum() / (2 * nlim * nlim))
                     if hasD[cy, cx]:
                         w *= np.exp(-(D[cy, cx] - D[ny, nx]) ** 2 / (2 * dlim * dlim))
+                    if alb is not None and hasN[cy, cx]:
+                        A = alb.reshape(H, W, 3)
+                        w *= np.exp(-((A[cy, cx] - A[ny, nx]) ** 2).sum() / (2 * alim * alim))
                     ld = max(0.0, lum[ny, nx] - lum[cy, cx])
                     w *= np.exp(-ld * ld / (2 * llim * llim))
                     wsum += w
@@ -160,7 +166,8 @@ def _filter_ref(params, rad, nrm, dep, var, W, H):
     return out
 
 
-def test_bilateral_filter_oracle_vs_float64_restatement(oracle_lib):
+@pytest.mark.parametrize("guided", [False, True])
+def test_bilateral_filter_oracle_vs_float64_restatement(oracle_lib, guided):
     rng = np.random.default_rng(2)
     W, H = 14, 11
     rad = rng.uniform(0, 2, (W * H, 3)).astype(np.float32)
@@ -174,8 +181,10 @@ def test_bilateral_filter_oracle_vs_float64_restatement(oracle_lib):
     var = rng.uniform(0, 0.3, W * H).astype(np.float32)
     var[7] = 0.0  # early out
     params = np.array([0.15, 2.0, 0.9, 0.5, 0.6, 0.5, 0.4], np.float32)
-    got, _ = O.bilateral(params, rad, nrm, dep, var, W, H, threads=2)
+    alb = rng.uniform(0, 1, (W * H, 3)).astype(np.float32) if guided else None
+    got, _ = O.bilateral(params, rad, nrm, dep, var, W, H, threads=2, albedo=alb, albedo_limit=0.3)
     ref = _filter_ref(params.astype(np.float64), rad.astype(np.float64), nrm.astype(np.float64),
-                      dep.astype(np.float64), var.astype(np.float64), W, H)
+                      dep.astype(np.float64), var.astype(np.float64), W, H,
+                      None if alb is None else alb.astype(np.float64), 0.3)
     np.testing.assert_allclose(got, ref, rtol=1e-4, atol=1e-6)
     assert np.array_equal(got[7], rad[7])
