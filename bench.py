@@ -65,8 +65,13 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="image scale (1.0 = 1920x1080)")
-    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default 64; 4 for --workload fog)")
+    ap.add_argument("--workload", choices=["plain", "fog"], default="plain",
+                    help="plain: the 1080p / 1M-triangle forward frame; fog: the same scene inside a 128^3 fp16 "
+                         "density grid (delta tracking)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--light-sampler", choices=["cdf", "alias"], default="cdf",
+                    help="environment light sampling: the reference's row/column CDFs or the alias table")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (torch clone), to calibrate HBM counters")
@@ -89,7 +94,10 @@ def main():
     from sharpray_amd import scenes
 
     t0 = time.perf_counter()
-    spec = scenes.config_plain_gpu(scale=args.scale, spp=args.spp)
+    if args.workload == "fog":
+        spec = scenes.config_fog(scale=args.scale, spp=args.spp or 4)
+    else:
+        spec = scenes.config_plain_gpu(scale=args.scale, spp=args.spp or 64)
     log(f"[bench] scene spec: {spec.triangle_count} triangles, {spec.width}x{spec.height}, "
         f"{spec.max_samples} spp ({time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
@@ -97,9 +105,12 @@ def main():
     scene.build()
     log(f"[bench] BVH build + upload {time.perf_counter() - t0:.1f} s")
 
-    from sharpray_amd import Renderer
+    from sharpray_amd import LIGHT_ALIAS, LIGHT_CDF, Renderer
+
+    light = LIGHT_ALIAS if args.light_sampler == "alias" else LIGHT_CDF
 
     def frame(**kw):
+        kw.setdefault("light_sampler", light)
         return Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
                         spec.variance_threshold, spec.bounces, spec.indirect_clamp, **kw)
 
@@ -150,7 +161,7 @@ def main():
         log(f"[bench] CPU: {cpu}")
 
     line = {
-        "metric": "Msamples/s @1080p/1M tris",
+        "metric": "Msamples/s @1080p/1M tris" + (" + 128^3 grid fog" if args.workload == "fog" else ""),
         "value": round(msps, 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
@@ -164,7 +175,8 @@ def main():
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": spec.name, "width": spec.width, "height": spec.height, "spp": spec.max_samples,
                    "bounces": spec.bounces, "triangles": spec.triangle_count, "env_map": "2048x1024 f32",
-                   "rng": "reference xorwow per pixel"},
+                   "rng": "reference xorwow per pixel",
+                   "light_sampler": args.light_sampler},
         "gpu_ms_per_step": round(1e3 * sum(gpus) / len(gpus), 3),
         "roofline": {"kernel": "k_trace (closest hit + deferred shadow rays)", "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),

@@ -72,6 +72,26 @@ typedef struct sr_fog {
     float height_falloff;
 } sr_fog;
 
+/* A bounded participating medium (no reference counterpart; it takes the place of Fog in
+ * Scene.Sample, scene.cs:641-672, with the same color / anisotropy meaning). kind
+ * SR_MEDIUM_HOMOGENEOUS: extinction sigma inside the box bounds_min..bounds_max; kind
+ * SR_MEDIUM_GRID: extinction sigma * value of the voxel containing the point, from a
+ * res[0] x res[1] x res[2] grid of IEEE half-precision values (x fastest) spanning the box.
+ * Free flight is sampled by delta tracking against sigma * max(grid); shadow rays take the
+ * exact voxel-by-voxel transmittance. The grid is copied at the call. */
+#define SR_MEDIUM_HOMOGENEOUS 1
+#define SR_MEDIUM_GRID 2
+typedef struct sr_fog_volume {
+    uint32_t kind;
+    float sigma;
+    float color[3];
+    float anisotropy;
+    float bounds_min[3];
+    float bounds_max[3];
+    uint32_t res[3];
+    const uint16_t* density;
+} sr_fog_volume;
+
 /* View (math.cs:1582-1611): transform without scale, horizontal fov (radians), near plane. */
 typedef struct sr_view {
     float pos[3];
@@ -143,6 +163,8 @@ int sr_scene_set_sky_texture(sr_ctx* ctx, int32_t texture, float multiplier, flo
 int sr_scene_set_sky_procedural(sr_ctx* ctx, const sr_sky_procedural* sky);
 /* Scene.SetFog (scene.cs:553-561). */
 int sr_scene_set_fog(sr_ctx* ctx, const sr_fog* fog);
+/* Scene medium from sr_fog_volume (replaces a height fog set before; no reference counterpart). */
+int sr_scene_set_fog_volume(sr_ctx* ctx, const sr_fog_volume* volume);
 /* Scene.AddObject(new Object(name, trans, material, shape)) (scene.cs:563-571, 214-223).
  * shape_kind 0 = mesh (mesh id), 1 = Sphere (shape_params = center xyz, radius; math.cs:1171),
  * 2 = AABox (shape_params = min xyz, max xyz; math.cs:936). */

@@ -45,6 +45,11 @@ _SIGS = {
     "oc_scene_set_sky_procedural": (None, [_V, _F]),
     "oc_scene_set_fog": (None, [_V, ctypes.c_float, _F, ctypes.c_float, ctypes.c_float]),
     "oc_scene_build": (ctypes.c_int, [_V]),
+    "oc_scene_set_fog_volume": (None, [_V, ctypes.c_int, ctypes.c_float, _F, ctypes.c_float, _F, _F, _U,
+                                       ctypes.POINTER(ctypes.c_uint16)]),
+    "oc_medium_optical_depth": (None, [_V, _F, _F, ctypes.c_uint32, _F]),
+    "oc_medium_scatter": (None, [_V, _F, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                 ctypes.c_uint32, _F]),
     "oc_bvh_export": (ctypes.c_int, [_V, ctypes.c_int, ctypes.c_int, _F, _I]),
     "oc_bvh_stats": (ctypes.c_float, [_V, ctypes.c_int, ctypes.c_int, _I]),
     "oc_trace_primary": (None, [_V, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _I, _I,
@@ -127,6 +132,18 @@ class OracleScene:
         if spec.fog is not None:
             d, c, g, fo = spec.fog
             self.L.oc_scene_set_fog(self.h, d, f(np.asarray(c, np.float32)), g, fo)
+        vol = getattr(spec, "fog_volume", None)
+        if vol is not None:
+            grid = None
+            if vol.density is not None:
+                grid = np.ascontiguousarray(vol.density, np.float16).view(np.uint16)
+                self._refs.append(grid)
+            res = np.array(vol.res, np.uint32)
+            self._refs.append(res)
+            self.L.oc_scene_set_fog_volume(self.h, vol.kind, vol.sigma, f(np.asarray(vol.color, np.float32)),
+                                           vol.anisotropy, f(np.asarray(vol.bounds_min, np.float32)),
+                                           f(np.asarray(vol.bounds_max, np.float32)), res.ctypes.data_as(_U),
+                                           None if grid is None else grid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
         if self.L.oc_scene_build(self.h) != 0:
             raise RuntimeError("oracle: scene build failed (sky required)")
 

@@ -743,6 +743,109 @@ struct Fog {  // height fog, scene.cs:438-505
     }
 };
 
+// Bounded media (no reference counterpart): a homogeneous box (kind 1) or a box holding a
+// piecewise-constant grid of half-precision densities (kind 2; sigma = scale * voxel value).
+// Free flight: kind 1 inverts T = exp(-sigma d) with one exponential draw; kind 2 is delta
+// tracking (tentative collisions at the majorant rate, accepted with probability
+// sigma / majorant, at most 4096 tentative steps). Transmittance: exp of the exact optical
+// depth, summed voxel by voxel along the ray (next boundary x, then y, then z on ties).
+inline real half_bits(uint16_t h) {
+    int e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    real v = e == 0 ? std::ldexp((real)m, -24) : (e == 31 ? (m ? (real)NAN : kInf) : std::ldexp((real)(m | 0x400), e - 25));
+    return (h & 0x8000) ? -v : v;
+}
+struct Medium {
+    int kind = 0;
+    real sigma = 0;
+    V3 bmin, bmax, cell, cellInv;
+    int n[3] = {0, 0, 0};
+    std::vector<real> grid;  // decoded values
+    real majorant = 0;
+    void init(int k, real s, V3 lo, V3 hi, const uint32_t* res, const uint16_t* bits) {
+        kind = k; sigma = s; bmin = lo; bmax = hi;
+        if (kind == 2) {
+            for (int i = 0; i != 3; ++i) n[i] = (int)res[i];
+            cell = v3((hi.x - lo.x) / (real)n[0], (hi.y - lo.y) / (real)n[1], (hi.z - lo.z) / (real)n[2]);
+            cellInv = v3((real)1 / cell.x, (real)1 / cell.y, (real)1 / cell.z);
+            size_t cnt = (size_t)n[0] * n[1] * n[2];
+            grid.resize(cnt);
+            real mx = 0;
+            for (size_t i = 0; i != cnt; ++i) { grid[i] = half_bits(bits[i]); mx = std::max(mx, grid[i]); }
+            majorant = sigma * mx;
+        }
+    }
+    bool span(const Ray& r, real start, real end, real& a, real& b) const {
+        real t1 = (bmin.x - r.o.x) * r.inv.x, t2 = (bmax.x - r.o.x) * r.inv.x;
+        real t3 = (bmin.y - r.o.y) * r.inv.y, t4 = (bmax.y - r.o.y) * r.inv.y;
+        real t5 = (bmin.z - r.o.z) * r.inv.z, t6 = (bmax.z - r.o.z) * r.inv.z;
+        real lo = nmax(nmax(nmin(t1, t2), nmin(t3, t4)), nmin(t5, t6));
+        real hi = nmin(nmin(nmax(t1, t2), nmax(t3, t4)), nmax(t5, t6));
+        a = nmax(lo, start);
+        b = nmin(hi, end);
+        return a < b;
+    }
+    static int coord(real p, real lo, real inv, int cnt) {
+        int i = (int)std::floor((p - lo) * inv);
+        return i < 0 ? 0 : (i >= cnt ? cnt - 1 : i);
+    }
+    real sigma_cell(int x, int y, int z) const { return sigma * grid[((size_t)z * n[1] + y) * n[0] + x]; }
+    real sigma_at(V3 p) const {
+        return sigma_cell(coord(p.x, bmin.x, cellInv.x, n[0]), coord(p.y, bmin.y, cellInv.y, n[1]),
+                          coord(p.z, bmin.z, cellInv.z, n[2]));
+    }
+    real grid_depth(const Ray& r, real a, real b) const {
+        V3 p = ray_at(r, a);
+        int c[3] = {coord(p.x, bmin.x, cellInv.x, n[0]), coord(p.y, bmin.y, cellInv.y, n[1]),
+                    coord(p.z, bmin.z, cellInv.z, n[2])};
+        real o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z}, inv[3] = {r.inv.x, r.inv.y, r.inv.z};
+        real lo[3] = {bmin.x, bmin.y, bmin.z}, cs[3] = {cell.x, cell.y, cell.z};
+        real next[3], delta[3];
+        int step[3];
+        for (int k = 0; k != 3; ++k) {
+            step[k] = d[k] > 0 ? 1 : -1;
+            real edge = lo[k] + (real)(c[k] + (d[k] > 0 ? 1 : 0)) * cs[k];
+            next[k] = d[k] == 0 ? kInf : (edge - o[k]) * inv[k];
+            delta[k] = d[k] == 0 ? kInf : cs[k] * std::fabs(inv[k]);
+        }
+        real t = a, tau = 0;
+        for (int it = 0, lim = n[0] + n[1] + n[2] + 3; it != lim; ++it) {
+            int k = next[0] <= next[1] ? (next[0] <= next[2] ? 0 : 2) : (next[1] <= next[2] ? 1 : 2);
+            real tEnd = nmin(next[k], b);
+            tau += sigma_cell(c[0], c[1], c[2]) * nmax((real)0, tEnd - t);
+            if (tEnd >= b) break;
+            t = tEnd;
+            c[k] += step[k];
+            if (c[k] < 0 || c[k] >= n[k]) break;
+            next[k] += delta[k];
+        }
+        return tau;
+    }
+    real optical_depth(const Ray& r, real start, real end) const {
+        real a, b;
+        if (!span(r, start, end, a, b)) return 0;
+        return kind == 1 ? sigma * (b - a) : grid_depth(r, a, b);
+    }
+    bool scatter_distance(const Ray& r, real start, real end, Rng& rng, real& tOut) const {
+        real a, b;
+        if (!span(r, start, end, a, b)) return false;
+        if (kind == 1) {
+            if (!(sigma > 0)) return false;
+            real t = a + (real)rng.next_exponential() / sigma;
+            if (t < b) { tOut = t; return true; }
+            return false;
+        }
+        if (!(majorant > 0)) return false;
+        real t = a;
+        for (int it = 0; it != 4096; ++it) {
+            t += (real)rng.next_exponential() / majorant;
+            if (!(t < b)) return false;
+            real s = sigma_at(ray_at(r, t));
+            if ((real)rng.next_float() < s / majorant) { tOut = t; return true; }
+        }
+        return false;
+    }
+};
+
 struct SampleLight { V3 dir; real pdf; };
 
 // Quat.AngleAxis, math.cs:621-626.
@@ -1094,7 +1197,14 @@ struct Scene {
     std::vector<Object> objects;
     Sky sky;
     bool hasFog = false;
-    Fog fog{};
+    Fog fog{};        // height fog, or color / anisotropy of a bounded medium
+    Medium medium;    // kind != 0: bounded medium in place of the height fog
+    real fog_depth(const Ray& r, real start, real end) const {
+        return medium.kind ? medium.optical_depth(r, start, end) : fog.optical_depth(r, start, end);
+    }
+    bool fog_scatter(const Ray& r, real start, real end, Rng& rng, real& t) const {
+        return medium.kind ? medium.scatter_distance(r, start, end, rng, t) : fog.scatter_distance(r, start, end, rng, t);
+    }
     Bvh bvh;
     bool built = false;
 
@@ -1274,7 +1384,7 @@ struct Scene {
         Ray shadow = make_ray(pos, light.dir);
         tr = shadow_transmittance(shadow, ctr);
         if (max_component(tr) <= 0.0f) return false;
-        if (hasFog) tr = tr * fog.transmittance(shadow, 0.0f, kInf);
+        if (hasFog) tr = tr * std::exp(-fog_depth(shadow, 0.0f, kInf));
         return true;
     }
     // SampleSkyDirect, scene.cs:774-794.
@@ -1328,7 +1438,7 @@ struct Scene {
             real dist = 0;
             bool hit = trace(ray, ctr, surf, dist, nullptr, nullptr);
             real scatterDist = 0;
-            if (hasFog && fog.scatter_distance(ray, 0.0f, hit ? dist : kInf, rng, scatterDist)) {
+            if (hasFog && fog_scatter(ray, 0.0f, hit ? dist : kInf, rng, scatterDist)) {
                 if (ctr) ctr->v[C_SampleFogScatter]++;
                 energy = energy * fog.color;
                 Color sr = sample_sky_direct_fog(ray_at(ray, scatterDist), ray.d, rng, ctr) * energy;
@@ -1805,6 +1915,35 @@ void oc_scene_set_fog(void* s, float density, const float* color, float anisotro
     Scene* sc = (Scene*)s;
     sc->hasFog = true;
     sc->fog = Fog{density, ld3(color), anisotropy, heightFalloff};
+    sc->medium = Medium{};
+}
+// Bounded medium: kind 1 homogeneous, 2 grid (res3, half-precision bits, x fastest).
+void oc_scene_set_fog_volume(void* s, int kind, float sigma, const float* color, float anisotropy, const float* bmin,
+                             const float* bmax, const uint32_t* res, const uint16_t* density) {
+    Scene* sc = (Scene*)s;
+    sc->hasFog = true;
+    sc->fog = Fog{sigma, ld3(color), anisotropy, 0};
+    sc->medium = Medium{};
+    sc->medium.init(kind, sigma, ld3(bmin), ld3(bmax), res, density);
+}
+// Medium queries for pinning tests: optical depth of [start, end) segments of rays, and
+// free-flight draws from seed (a, b) (t, or -1 when the ray leaves the medium unscattered).
+void oc_medium_optical_depth(void* s, const float* rays6, const float* se2, uint32_t n, float* out) {
+    Scene* sc = (Scene*)s;
+    for (uint32_t i = 0; i != n; ++i) {
+        Ray r = make_ray(ld3(rays6 + i * 6), ld3(rays6 + i * 6 + 3));
+        out[i] = (float)sc->fog_depth(r, se2[i * 2], se2[i * 2 + 1]);
+    }
+}
+void oc_medium_scatter(void* s, const float* ray6, float start, float end, uint32_t a, uint32_t b, uint32_t n,
+                       float* out) {
+    Scene* sc = (Scene*)s;
+    Ray r = make_ray(ld3(ray6), ld3(ray6 + 3));
+    Rng rng; rng.seed_xy(a, b);
+    for (uint32_t i = 0; i != n; ++i) {
+        real t = 0;
+        out[i] = sc->fog_scatter(r, start, end, rng, t) ? (float)t : -1.0f;
+    }
 }
 int oc_scene_build(void* s) { return ((Scene*)s)->build() ? 0 : -1; }
 

@@ -237,6 +237,27 @@ class Fog:
         self.height_falloff = float(height_falloff)
 
 
+MEDIUM_HOMOGENEOUS = 1
+MEDIUM_GRID = 2
+
+
+class FogVolume:
+    """A bounded participating medium in place of Fog (no reference counterpart): a homogeneous box
+    (kind MEDIUM_HOMOGENEOUS, extinction sigma) or a half-precision density grid spanning the box
+    (kind MEDIUM_GRID, extinction sigma * voxel value; density shaped (nz, ny, nx)). color and
+    anisotropy mean what they mean for Fog."""
+
+    def __init__(self, kind, sigma, color, anisotropy, bounds_min, bounds_max, density=None):
+        self.kind = int(kind)
+        self.sigma = float(sigma)
+        self.color = np.asarray(color, dtype=np.float32)
+        self.anisotropy = float(anisotropy)
+        self.bounds_min = np.asarray(bounds_min, dtype=np.float32)
+        self.bounds_max = np.asarray(bounds_max, dtype=np.float32)
+        self.density = None if density is None else np.ascontiguousarray(density, dtype=np.float16)
+        self.res = (0, 0, 0) if density is None else tuple(int(x) for x in self.density.shape[::-1])
+
+
 class Scene:
     """Scene (scene.cs:507-881) backed by one device context."""
 
@@ -302,6 +323,18 @@ class Scene:
 
     def set_fog(self, fog):
         self._guard()
+        if isinstance(fog, FogVolume):
+            v = N.FogVolume_()
+            v.kind, v.sigma, v.anisotropy = fog.kind, fog.sigma, fog.anisotropy
+            v.color[:] = fog.color.tolist()
+            v.bounds_min[:] = fog.bounds_min.tolist()
+            v.bounds_max[:] = fog.bounds_max.tolist()
+            v.res[:] = list(fog.res)
+            if fog.density is not None:
+                v.density = fog.density.view(np.uint16).ctypes.data_as(ctypes.POINTER(ctypes.c_uint16))
+            check(N.lib().sr_scene_set_fog_volume(self._ctx, ctypes.byref(v)), "set_fog_volume")
+            self.fog = fog
+            return
         f = N.Fog_()
         f.density = fog.density
         f.color[:] = fog.color.tolist()

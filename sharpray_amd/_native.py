@@ -54,6 +54,12 @@ class Fog_(ctypes.Structure):
                 ("height_falloff", ctypes.c_float)]
 
 
+class FogVolume_(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("sigma", ctypes.c_float), ("color", ctypes.c_float * 3),
+                ("anisotropy", ctypes.c_float), ("bounds_min", ctypes.c_float * 3), ("bounds_max", ctypes.c_float * 3),
+                ("res", ctypes.c_uint32 * 3), ("density", ctypes.POINTER(ctypes.c_uint16))]
+
+
 class View_(ctypes.Structure):
     _fields_ = [("pos", ctypes.c_float * 3), ("rot", ctypes.c_float * 4), ("fov", ctypes.c_float),
                 ("near_plane", ctypes.c_float)]
@@ -87,6 +93,7 @@ SIGNATURES = {
     "sr_scene_set_sky_texture": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_float]),
     "sr_scene_set_sky_procedural": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SkyProcedural_)]),
     "sr_scene_set_fog": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Fog_)]),
+    "sr_scene_set_fog_volume": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FogVolume_)]),
     "sr_scene_add_object": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Transform_), ctypes.POINTER(Material_),
                                            ctypes.c_int32, ctypes.c_int32, c_float_p, c_int32_p]),
     "sr_scene_build": (ctypes.c_int, [ctypes.c_void_p]),

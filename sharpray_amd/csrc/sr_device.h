@@ -59,10 +59,16 @@ struct DevSky {
 
 struct DevFog {
     int32_t enabled;
-    float density;
+    int32_t kind;  // 0 height fog (scene.cs:438-505), 1 homogeneous box, 2 fp16 density grid
+    float density;  // height fog density / box and grid extinction scale
     vec3 color;
     float anisotropy;
     float heightFalloff;
+    vec3 bmin, bmax;   // medium box (kinds 1, 2)
+    vec3 cell, cellInv;  // grid voxel size and its reciprocal
+    int32_t nx, ny, nz;
+    float majorant;    // density * max grid value
+    const uint16_t* grid;  // fp16 bits, x fastest
 };
 
 struct DevScene {
@@ -498,6 +504,99 @@ SR_DEV bool fog_scatter(const DevFog& f, const ray3& r, float start, float end, 
     if (t < end) {
         tOut = t;
         return true;
+    }
+    return false;
+}
+
+// ---- Bounded media (no reference counterpart) ---------------------------------------------------
+// A homogeneous box or a box holding a piecewise-constant fp16 density grid (extinction
+// sigma = density * value of the voxel containing the point). Free flight in the grid is drawn by
+// delta tracking against the grid's majorant; transmittance is the exact optical depth summed
+// voxel by voxel along the ray (3D DDA), so shadow rays need no random numbers.
+constexpr float kInfF = __builtin_huge_valf();
+constexpr int32_t kMaxDeltaSteps = 4096;  // bound on tentative collisions per free-flight draw
+SR_DEV float half_bits_to_float(uint16_t h) { return (float)__builtin_bit_cast(_Float16, h); }
+
+// The part [a, b) of [start, end) inside the medium box.
+SR_DEV bool medium_span(const DevFog& f, const ray3& r, float start, float end, float& a, float& b) {
+    const float t1 = (f.bmin.x - r.o.x) * r.inv.x, t2 = (f.bmax.x - r.o.x) * r.inv.x;
+    const float t3 = (f.bmin.y - r.o.y) * r.inv.y, t4 = (f.bmax.y - r.o.y) * r.inv.y;
+    const float t5 = (f.bmin.z - r.o.z) * r.inv.z, t6 = (f.bmax.z - r.o.z) * r.inv.z;
+    const float tmin = net_max(net_max(net_min(t1, t2), net_min(t3, t4)), net_min(t5, t6));
+    const float tmax = net_min(net_min(net_max(t1, t2), net_max(t3, t4)), net_max(t5, t6));
+    a = net_max(tmin, start);
+    b = net_min(tmax, end);
+    return a < b;
+}
+SR_DEV int32_t grid_coord(float p, float lo, float inv, int32_t n) {
+    const int32_t i = (int32_t)floorf((p - lo) * inv);
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+SR_DEV float grid_sigma(const DevFog& f, int32_t x, int32_t y, int32_t z) {
+    return f.density * half_bits_to_float(f.grid[((size_t)z * f.ny + y) * f.nx + x]);
+}
+SR_DEV float grid_sigma_at(const DevFog& f, vec3 p) {
+    return grid_sigma(f, grid_coord(p.x, f.bmin.x, f.cellInv.x, f.nx), grid_coord(p.y, f.bmin.y, f.cellInv.y, f.ny),
+                      grid_coord(p.z, f.bmin.z, f.cellInv.z, f.nz));
+}
+// Optical depth of the grid over [a, b): walk the voxels the ray crosses (Amanatides-Woo), next
+// boundary first on x, then y, then z on ties.
+SR_DEV float grid_optical_depth(const DevFog& f, const ray3& r, float a, float b) {
+    const vec3 p = at(r, a);
+    int32_t c[3] = {grid_coord(p.x, f.bmin.x, f.cellInv.x, f.nx), grid_coord(p.y, f.bmin.y, f.cellInv.y, f.ny),
+                    grid_coord(p.z, f.bmin.z, f.cellInv.z, f.nz)};
+    const int32_t n[3] = {f.nx, f.ny, f.nz};
+    const float o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z}, inv[3] = {r.inv.x, r.inv.y, r.inv.z};
+    const float lo[3] = {f.bmin.x, f.bmin.y, f.bmin.z}, cs[3] = {f.cell.x, f.cell.y, f.cell.z};
+    float tNext[3], tDelta[3];
+    int32_t step[3];
+    for (int k = 0; k < 3; ++k) {
+        step[k] = d[k] > 0.0f ? 1 : -1;
+        const float edge = lo[k] + (float)(c[k] + (d[k] > 0.0f ? 1 : 0)) * cs[k];
+        tNext[k] = d[k] == 0.0f ? kInfF : (edge - o[k]) * inv[k];
+        tDelta[k] = d[k] == 0.0f ? kInfF : cs[k] * fabsf(inv[k]);
+    }
+    float t = a, tau = 0.0f;
+    for (int32_t it = 0, lim = f.nx + f.ny + f.nz + 3; it < lim; ++it) {
+        const int k = tNext[0] <= tNext[1] ? (tNext[0] <= tNext[2] ? 0 : 2) : (tNext[1] <= tNext[2] ? 1 : 2);
+        const float tEnd = net_min(tNext[k], b);
+        tau += grid_sigma(f, c[0], c[1], c[2]) * net_max(0.0f, tEnd - t);
+        if (tEnd >= b) break;
+        t = tEnd;
+        c[k] += step[k];
+        if (c[k] < 0 || c[k] >= n[k]) break;
+        tNext[k] += tDelta[k];
+    }
+    return tau;
+}
+
+// ---- Medium dispatch ----------------------------------------------------------------------------
+SR_DEV float medium_optical_depth(const DevFog& f, const ray3& r, float start, float end) {
+    if (f.kind == 0) return fog_optical_depth(f, r, start, end);
+    float a, b;
+    if (!medium_span(f, r, start, end, a, b)) return 0.0f;
+    if (f.kind == 1) return f.density * (b - a);
+    return grid_optical_depth(f, r, a, b);
+}
+template <class R>
+SR_DEV bool medium_scatter(const DevFog& f, const ray3& r, float start, float end, R& rng, float& tOut) {
+    if (f.kind == 0) return fog_scatter(f, r, start, end, rng, tOut);
+    float a, b;
+    if (!medium_span(f, r, start, end, a, b)) return false;
+    if (f.kind == 1) {
+        if (!(f.density > 0.0f)) return false;
+        const float t = a + next_exponential(rng) / f.density;
+        if (t < b) { tOut = t; return true; }
+        return false;
+    }
+    // Delta tracking: tentative collisions at majorant rate, real with probability sigma / majorant.
+    if (!(f.majorant > 0.0f)) return false;
+    float t = a;
+    for (int32_t it = 0; it < kMaxDeltaSteps; ++it) {
+        t += next_exponential(rng) / f.majorant;
+        if (!(t < b)) return false;
+        const float s = grid_sigma_at(f, at(r, t));
+        if (next_float(rng) < s / f.majorant) { tOut = t; return true; }
     }
     return false;
 }

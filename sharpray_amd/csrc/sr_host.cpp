@@ -2,6 +2,7 @@
 #include "sr_host.h"
 
 #include <math.h>
+#include <string.h>
 
 #include <algorithm>
 #include <numeric>
@@ -316,6 +317,20 @@ void build_alias(const float* w, size_t n, float* prob, int32_t* alias) {
         if (excess[hi] < 1.0) under.push_back(hi);
         else over.push_back(hi);
     }
+}
+
+float half_to_float_host(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h >> 15) << 31;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    float v;
+    if (e == 0) v = ldexpf((float)m, -24);                      // zero and subnormals
+    else if (e == 31) v = m ? __builtin_nanf("") : __builtin_huge_valf();
+    else v = ldexpf((float)(m | 0x400u), (int)e - 25);
+    uint32_t bits;
+    memcpy(&bits, &v, 4);
+    bits |= sign;
+    memcpy(&v, &bits, 4);
+    return v;
 }
 
 float build_cdf1(const float* w, int n, float* cdf) {

@@ -42,5 +42,7 @@ float build_cdf1(const float* w, int n, float* cdf_out /* n+1 */);
 // acceptance threshold of column i, alias[i] the column taken otherwise. All-zero weights leave
 // prob = 1, alias = i.
 void build_alias(const float* w, size_t n, float* prob, int32_t* alias);
+// IEEE half-precision bits to float (exact).
+float half_to_float_host(uint16_t h);
 
 }  // namespace sr

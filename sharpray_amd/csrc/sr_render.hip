@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) 
             const bool hit = h.obj >= 0;
             float sd = 0.0f;
             vec3 no = ray.o, nd = ray.d;
-            if (S.fog.enabled && fog_scatter(S.fog, ray, 0.0f, hit ? h.t : kInf, rng, sd)) {
+            if (S.fog.enabled && medium_scatter(S.fog, ray, 0.0f, hit ? h.t : kInf, rng, sd)) {
                 ++cFog;
                 p.energy = p.energy * S.fog.color;
                 const vec3 pos = at(ray, sd);
@@ -424,7 +424,7 @@ SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, 
         }
     }
     if (max_comp(tr) > 0.0f) {
-        if (S.fog.enabled) tr = tr * expf(-fog_optical_depth(S.fog, mk_ray(sr.o, sr.d), 0.0f, kInf));
+        if (S.fog.enabled) tr = tr * expf(-medium_optical_depth(S.fog, mk_ray(sr.o, sr.d), 0.0f, kInf));
         const vec3 c = sr.skyRad * tr * sr.f * sr.w / sr.pdf * sr.energy;
         const uint32_t q = sr.path & 0x7fffffffu;
         PathRec* p = &F.paths[q];
@@ -713,6 +713,9 @@ struct sr_ctx {
     sr_sky_procedural skyProc{};
     bool hasFog = false;
     sr_fog fog{};
+    uint32_t fogKind = 0;  // 0 height fog, SR_MEDIUM_HOMOGENEOUS, SR_MEDIUM_GRID
+    sr_fog_volume volume{};
+    std::vector<uint16_t> fogGrid;
     bool built = false;
     BvhResult sceneBvh;
     std::vector<int32_t> meshNodeBase, meshTriBase;
@@ -721,6 +724,7 @@ struct sr_ctx {
     DevBuf<float4> dNodes;
     DevBuf<float> dTris, dAttrs, dTexels, dCdf;
     DevBuf<float4> dAlias;
+    DevBuf<uint16_t> dFogGrid;
     DevBuf<int32_t> dSlotTri, dSceneItems;
     DevBuf<DevObject> dObjects;
     DevBuf<DevTexture> dTextures;
@@ -769,7 +773,7 @@ struct sr_ctx {
         if (hostCount) (void)hipHostFree(hostCount);
         if (evStart) (void)hipEventDestroy(evStart);
         if (evEnd) (void)hipEventDestroy(evEnd);
-        dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release(); dAlias.release();
+        dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release(); dAlias.release(); dFogGrid.release();
         dSlotTri.release(); dSceneItems.release(); dObjects.release(); dTextures.release();
         dRayO.release(); dRayD.release(); dHitA.release(); dHitSlot.release();
         dPaths.release(); dShadows.release(); dAcc.release();
@@ -976,6 +980,34 @@ int sr_scene_set_fog(sr_ctx* ctx, const sr_fog* fog) {
     if (ctx->built) return fail(SR_ERR_STATE, "Scene already built");
     ctx->hasFog = true;
     ctx->fog = *fog;
+    ctx->fogKind = 0;
+    return SR_OK;
+}
+
+int sr_scene_set_fog_volume(sr_ctx* ctx, const sr_fog_volume* v) {
+    if (!ctx || !v) return fail(SR_ERR_INVALID, "fog volume: null");
+    if (ctx->built) return fail(SR_ERR_STATE, "Scene already built");
+    if (v->kind != SR_MEDIUM_HOMOGENEOUS && v->kind != SR_MEDIUM_GRID) return fail(SR_ERR_INVALID, "fog volume: bad kind");
+    if (!(v->sigma >= 0.0f) || !(v->anisotropy > -1.0f && v->anisotropy < 1.0f))
+        return fail(SR_ERR_INVALID, "fog volume: need sigma >= 0 and -1 < anisotropy < 1");
+    for (int k = 0; k < 3; ++k)
+        if (!(v->bounds_min[k] < v->bounds_max[k])) return fail(SR_ERR_INVALID, "fog volume: empty box");
+    std::vector<uint16_t> grid;
+    if (v->kind == SR_MEDIUM_GRID) {
+        if (!v->density || v->res[0] == 0 || v->res[1] == 0 || v->res[2] == 0) return fail(SR_ERR_INVALID, "fog volume: grid required");
+        if ((uint64_t)v->res[0] * v->res[1] * v->res[2] > (1ull << 30)) return fail(SR_ERR_LIMIT, "fog volume: grid too large");
+        const size_t n = (size_t)v->res[0] * v->res[1] * v->res[2];
+        grid.assign(v->density, v->density + n);
+        for (uint16_t h : grid) {  // negative, infinite or NaN densities are rejected
+            if ((h & 0x7c00u) == 0x7c00u) return fail(SR_ERR_INVALID, "fog volume: non-finite density");
+            if ((h & 0x8000u) && (h & 0x7fffu)) return fail(SR_ERR_INVALID, "fog volume: negative density");
+        }
+    }
+    ctx->hasFog = true;
+    ctx->fogKind = v->kind;
+    ctx->volume = *v;
+    ctx->volume.density = nullptr;
+    ctx->fogGrid.swap(grid);
     return SR_OK;
 }
 
@@ -1128,10 +1160,29 @@ int sr_scene_build(sr_ctx* ctx) {
         K.sunPdfDenom = kPi * (1.0f - K.sunCos);
     }
     S.fog.enabled = ctx->hasFog ? 1 : 0;
-    S.fog.density = ctx->fog.density;
-    S.fog.color = mk3(ctx->fog.color[0], ctx->fog.color[1], ctx->fog.color[2]);
-    S.fog.anisotropy = ctx->fog.anisotropy;
-    S.fog.heightFalloff = ctx->fog.height_falloff;
+    S.fog.kind = (int32_t)ctx->fogKind;
+    if (ctx->fogKind == 0) {
+        S.fog.density = ctx->fog.density;
+        S.fog.color = mk3(ctx->fog.color[0], ctx->fog.color[1], ctx->fog.color[2]);
+        S.fog.anisotropy = ctx->fog.anisotropy;
+        S.fog.heightFalloff = ctx->fog.height_falloff;
+    } else {
+        const sr_fog_volume& v = ctx->volume;
+        S.fog.density = v.sigma;
+        S.fog.color = mk3(v.color[0], v.color[1], v.color[2]);
+        S.fog.anisotropy = v.anisotropy;
+        S.fog.bmin = mk3(v.bounds_min[0], v.bounds_min[1], v.bounds_min[2]);
+        S.fog.bmax = mk3(v.bounds_max[0], v.bounds_max[1], v.bounds_max[2]);
+        if (ctx->fogKind == SR_MEDIUM_GRID) {
+            S.fog.nx = (int32_t)v.res[0]; S.fog.ny = (int32_t)v.res[1]; S.fog.nz = (int32_t)v.res[2];
+            S.fog.cell = mk3((S.fog.bmax.x - S.fog.bmin.x) / (float)S.fog.nx, (S.fog.bmax.y - S.fog.bmin.y) / (float)S.fog.ny,
+                             (S.fog.bmax.z - S.fog.bmin.z) / (float)S.fog.nz);
+            S.fog.cellInv = mk3(1.0f / S.fog.cell.x, 1.0f / S.fog.cell.y, 1.0f / S.fog.cell.z);
+            float mx = 0.0f;
+            for (uint16_t h : ctx->fogGrid) mx = std::max(mx, half_to_float_host(h));
+            S.fog.majorant = v.sigma * mx;
+        }
+    }
 
     std::vector<DevObject> dobj(ctx->objects.size());
     for (size_t i = 0; i < ctx->objects.size(); ++i) {
@@ -1172,6 +1223,7 @@ int sr_scene_build(sr_ctx* ctx) {
     if ((rc = upload(ctx->dTextures, dtex.data(), dtex.size(), s))) return rc;
     if ((rc = upload(ctx->dCdf, cdf.data(), cdf.size(), s))) return rc;
     if ((rc = upload(ctx->dAlias, aliasTab.data(), aliasTab.size(), s))) return rc;
+    if ((rc = upload(ctx->dFogGrid, ctx->fogGrid.data(), ctx->fogGrid.size(), s))) return rc;
     HIP_TRY(hipStreamSynchronize(s));
     S.nodes = ctx->dNodes.ptr;
     S.tris = ctx->dTris.ptr;
@@ -1183,6 +1235,7 @@ int sr_scene_build(sr_ctx* ctx) {
     S.textures = ctx->dTextures.ptr;
     S.cdf = ctx->dCdf.ptr;
     S.alias = ctx->dAlias.ptr;
+    S.fog.grid = ctx->dFogGrid.ptr;
     ctx->built = true;
     return SR_OK;
 }

@@ -21,8 +21,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from . import (AABox, Fog, Material, Mesh, Object, Quat, Renderer, Scene, SkyProcedural, SkyTexture, Sphere,
-               Texture, Transform, View, triangles_build)
+from . import (MEDIUM_GRID, AABox, Fog, FogVolume, Material, Mesh, Object, Quat, Renderer, Scene, SkyProcedural,
+               SkyTexture, Sphere, Texture, Transform, View, triangles_build)
 
 SEED_BASE = 0x5EED0000
 
@@ -49,6 +49,7 @@ class SceneSpec:
     sky_yaw: float = 0.0
     sky_procedural: np.ndarray = None               # 16 floats
     fog: tuple = None                                # (density, color3, anisotropy, falloff)
+    fog_volume: object = None                        # sharpray_amd.FogVolume (bounded medium)
     view: np.ndarray = None                          # pos3 quat4 fov near
     width: int = 0
     height: int = 0
@@ -79,6 +80,8 @@ def instantiate(spec, device=0):
     if spec.fog is not None:
         d, c, g, f = spec.fog
         scene.set_fog(Fog(d, c, g, f))
+    if spec.fog_volume is not None:
+        scene.set_fog(spec.fog_volume)
     for o in spec.objects:
         t = o.transform
         tr = Transform(t[0:3], Quat(*t[3:7]), t[7:10])
@@ -345,6 +348,27 @@ def config_plain_gpu(scale=1.0, spp=64, rings=512, segments=1024, env=(2048, 102
         s.objects.append(ObjectSpec(f"band_{b}", _identity(), _mat(albedo, rough, metal), 0, b))
     s.objects.append(ObjectSpec("ground", _identity(), _mat(0.5, 1.0), 0, bands))
     s.view = _view((0.0, 0.35, -3.4), 4.0, 60.0)
+    return s
+
+
+def fbm_density_grid(res, seed, octaves=4, freq=0.6, box=((-6.0, -1.25, -3.0), (6.0, 3.0, 9.0))):
+    """A res^3 half-precision density grid in [0, 1] (shape (nz, ny, nx)): fBm value noise sampled at
+    the voxel centres of `box`, remapped by clip(0.5 + fbm, 0, 1)."""
+    lo, hi = np.array(box[0]), np.array(box[1])
+    ax = [lo[k] + (np.arange(res) + 0.5) * (hi[k] - lo[k]) / res for k in range(3)]
+    z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    p = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    d = np.clip(0.5 + fbm(p, octaves, seed, base_freq=freq), 0.0, 1.0)
+    return d.reshape(res, res, res).astype(np.float16), box
+
+
+def config_fog(scale=1.0, spp=4, res=128, seed=SEED_BASE + 2):
+    """The forward scene with a res^3 fp16 fBm density grid (extinction 0 to 0.2 per unit), HG
+    anisotropy 0.3, delta-tracked free flight."""
+    s = config_plain_gpu(scale, spp)
+    s.name = f"fog_grid_1080p_{spp}spp_1Mtri"
+    grid, box = fbm_density_grid(res, seed)
+    s.fog_volume = FogVolume(MEDIUM_GRID, 0.2, (0.95, 0.95, 0.95), 0.3, box[0], box[1], grid)
     return s
 
 
