@@ -239,6 +239,24 @@ int sr_profile_read(sr_ctx* ctx, double* ms, uint32_t* launches);
 /* Seconds of GPU time of the last render (HIP events around the whole frame). */
 int sr_render_gpu_seconds(sr_ctx* ctx, double* out_seconds);
 
+/* ---- Adjoint rendering (no counterpart in the reference) -------------------------------------
+ * Differentiable parameters, in order: per object (in sr_scene_add_object order) albedo r, g, b
+ * and roughness (4 floats), then, for a texture sky, the sky texture's texels (w*h*3, row-major).
+ * The backward pass returns d/dθ of sum(d_image * Radiance) for the last forward render, replaying
+ * every path with its counter-RNG stream; it differentiates the BSDF values, the sky radiance, the
+ * MIS weights and the density of the reference's BSDF sampling (whose ClampToGeoSide mirror makes
+ * it differ from OpaquePdf). Opaque scenes without fog; fixed sample counts. */
+int sr_adjoint_param_count(sr_ctx* ctx, uint32_t* count);
+int sr_adjoint_params_get(sr_ctx* ctx, float* params);
+/* Sets the parameters (materials and sky texels; the sky's sampling tables are rebuilt). */
+int sr_adjoint_params_set(sr_ctx* ctx, const float* params);
+/* Renders like sr_render_begin + sr_render_read's Radiance (w*h*3, may be null) and keeps the view
+ * and parameters for the backward pass. Requires SR_RNG_COUNTER, min = max samples, threshold 0,
+ * at most 16 bounces. */
+int sr_adjoint_forward(sr_ctx* ctx, const sr_view* view, const sr_render_params* params, float* image);
+/* grad (param_count floats) = d sum(d_image * Radiance) / d params; d_image is w*h*3 (host). */
+int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad);
+
 /* ---- Host data preparation (the C# host computes these in Triangle's constructors) ---------- */
 /* Triangle ctors (math.cs:1503-1531) and Mesh.ComputeSmoothNormals (mesh.cs:99-123): pos n*9,
  * normals n*9 or null, uv n*6 or null (null = the OBJ defaults (0,0),(1,0),(0,1); obj.cs:417-427).

@@ -45,6 +45,11 @@ _SIGS = {
     "oc_scene_set_sky_procedural": (None, [_V, _F]),
     "oc_scene_set_fog": (None, [_V, ctypes.c_float, _F, ctypes.c_float, ctypes.c_float]),
     "oc_scene_build": (ctypes.c_int, [_V]),
+    "oc_param_count": (ctypes.c_uint32, [_V]),
+    "oc_params_get": (None, [_V, _F]),
+    "oc_params_set": (None, [_V, _F]),
+    "oc_adjoint": (ctypes.c_int, [_V, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                  ctypes.c_float, ctypes.c_uint32, _F, ctypes.c_int, _F]),
     "oc_scene_set_fog_volume": (None, [_V, ctypes.c_int, ctypes.c_float, _F, ctypes.c_float, _F, _F, _U,
                                        ctypes.POINTER(ctypes.c_uint16)]),
     "oc_medium_optical_depth": (None, [_V, _F, _F, ctypes.c_uint32, _F]),
@@ -63,6 +68,7 @@ _SIGS = {
     "oc_sky_eval": (None, [_V, _F, ctypes.c_uint32, _F, _F]),
     "oc_sky_sample": (None, [_V, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _F]),
     "oc_surface_eval": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, _F]),
+    "oc_surface_eval_grad": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, _F]),
     "oc_surface_scatter": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
     "oc_fog_optical_depth": (None, [_F, _F, _F, ctypes.c_uint32, _F]),
     "oc_fog_scatter": (None, [_F, _F, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
@@ -190,6 +196,27 @@ class OracleScene:
                                 prim.ctypes.data_as(_I), t.ctypes.data_as(_F), nrm.ctypes.data_as(_F))
         shp = (y1 - y0, s.width)
         return obj.reshape(shp), prim.reshape(shp), t.reshape(shp), nrm.reshape(shp + (3,))
+
+    def params(self):
+        p = np.empty(self.L.oc_param_count(self.h), np.float32)
+        self.L.oc_params_get(self.h, p.ctypes.data_as(_F))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.L.oc_param_count(self.h)
+        self.L.oc_params_set(self.h, p.ctypes.data_as(_F))
+
+    def adjoint(self, d_image, seed, threads=None):
+        """Gradient of sum(d_image * Radiance) over the parameters (fixed spp, counter RNG)."""
+        s = self.spec
+        d = np.ascontiguousarray(d_image, np.float32).reshape(-1)
+        g = np.empty(self.L.oc_param_count(self.h), np.float32)
+        rc = self.L.oc_adjoint(self.h, f(s.view), s.width, s.height, s.max_samples, s.bounces, s.indirect_clamp,
+                               int(seed), d.ctypes.data_as(_F), threads or os.cpu_count() or 1, g.ctypes.data_as(_F))
+        if rc != 0:
+            raise RuntimeError(f"oracle: adjoint failed ({rc})")
+        return g
 
     def render(self, y0=0, y1=None, threads=None, counters=False, rng_policy=0, rng_seed=0, light_sampler=0):
         s = self.spec

@@ -846,7 +846,7 @@ struct Medium {
     }
 };
 
-struct SampleLight { V3 dir; real pdf; };
+struct SampleLight { V3 dir; real pdf; int texel = -1; };
 
 // Quat.AngleAxis, math.cs:621-626.
 inline Quat angle_axis(real angle, V3 axis) {
@@ -990,7 +990,7 @@ struct Sky {
             V3 dir = qrot(rot, from_equirect_uv(uv));
             real pdf = luminance(tex.get(x, y)) * pdfScale;
             if (!(pdf > 0.0f)) return false;
-            out = SampleLight{dir, pdf};
+            out = SampleLight{dir, pdf, y * (int)tex.w + x};
             return true;
         }
         if (kind == 2) {
@@ -1003,6 +1003,66 @@ struct Sky {
             return true;
         }
         return false;
+    }
+    // Texel whose luminance LightDir's pdf reads for direction dir (texture sky).
+    int light_texel(V3 dir) const {
+        V2 uv = equirect_uv(qrot(rotInv, dir));
+        int cx = ((int)(uv.x * (real)(int)tex.w)) % (int)tex.w;
+        int cy = ((int)(uv.y * (real)(int)tex.h)) % (int)tex.h;
+        return cy * (int)tex.w + cx;
+    }
+    // Derivative of a texel-sky light pdf p = lum(texel) * W * H / (total * 2 pi^2) with respect to
+    // the texels, scaled by s: the texel's own luminance directly, every texel through total (the
+    // latter is gathered in gTotal and spread by apply_total_grad).
+    void pdf_grad(int texel, real pdf, real s, real* g, real& gTotal) const {
+        if (kind != 1 || texel < 0 || s == 0) return;
+        g[texel * 3] += s * (real)0.2126f * pdfScale;
+        g[texel * 3 + 1] += s * (real)0.7152f * pdfScale;
+        g[texel * 3 + 2] += s * (real)0.0722f * pdfScale;
+        gTotal += s * (-pdf / cdf.total());
+    }
+    void apply_total_grad(real gTotal, real* g) const {
+        if (kind != 1 || gTotal == 0) return;
+        for (uint32_t y = 0; y != tex.h; ++y) {
+            real sy = std::sin(kPi * ((real)y + 0.5f) / (real)tex.h);
+            for (uint32_t x = 0; x != tex.w; ++x) {
+                size_t t = (size_t)y * tex.w + x;
+                g[t * 3] += gTotal * (real)0.2126f * sy;
+                g[t * 3 + 1] += gTotal * (real)0.7152f * sy;
+                g[t * 3 + 2] += gTotal * (real)0.0722f * sy;
+            }
+        }
+    }
+    // Adds s_c * multiplier * (bilinear weight of texel t) to g[t * 3 + c] for the texels that
+    // Radiance(dir) interpolates: the derivative of Radiance with respect to the sky texels.
+    void radiance_grad(V3 dir, Color s, real* g) const {
+        if (kind != 1) return;
+        V2 c = equirect_uv(qrot(rotInv, dir));
+        real u = c.x * tex.tiling.x; u -= std::floor(u);
+        real v = c.y * tex.tiling.y; v -= std::floor(v);
+        uint32_t idx[4];
+        real w[4];
+        if (tex.filter == 1) {
+            idx[0] = ((uint32_t)(v * (real)tex.h) % tex.h) * tex.w + (uint32_t)(u * (real)tex.w) % tex.w;
+            w[0] = 1;
+            idx[1] = idx[2] = idx[3] = idx[0];
+            w[1] = w[2] = w[3] = 0;
+        } else {
+            real fx = u * (real)(tex.w - 1), fy = v * (real)(tex.h - 1);
+            uint32_t x0 = (uint32_t)fx, y0 = (uint32_t)fy;
+            uint32_t x1 = (x0 + 1) % tex.w, y1 = (y0 + 1) % tex.h;
+            real tx = fx - (real)x0, ty = fy - (real)y0;
+            idx[0] = y0 * tex.w + x0; w[0] = (1 - tx) * (1 - ty);
+            idx[1] = y0 * tex.w + x1; w[1] = tx * (1 - ty);
+            idx[2] = y1 * tex.w + x0; w[2] = (1 - tx) * ty;
+            idx[3] = y1 * tex.w + x1; w[3] = tx * ty;
+        }
+        for (int i = 0; i != 4; ++i) {
+            real k = w[i] * multiplier;
+            g[idx[i] * 3] += s.x * k;
+            g[idx[i] * 3 + 1] += s.y * k;
+            g[idx[i] * 3 + 2] += s.z * k;
+        }
     }
     // LightDir (MIS pdf of a direction): SkyTexture scene.cs:418-426, SunProcedural scene.cs:306-313.
     bool light_dir(V3 dir, real& pdfOut) const {
@@ -1130,6 +1190,119 @@ struct Surface {
         Color diffuse = (kWhite - fresnel(nDotV, f0)) * albedo * (1.0f - metallic) * (nDotL / kPi);
         return (specular + diffuse) * (1.0f - transparency);
     }
+    // Partial derivatives of OpaqueEval with respect to this surface's albedo (per channel: the
+    // Jacobian is diagonal) and roughness. For the adjoint renderer; no reference counterpart.
+    void opaque_eval_grad(V3 viewDir, V3 lightDir, Color& dAlbedo, Color& dRough) const {
+        dAlbedo = kBlack;
+        dRough = kBlack;
+        real nDotL = dot(normal, lightDir);
+        if (nDotL <= 0.0f) return;
+        real nDotV = nmax(1e-4f, dot(normal, viewDir));
+        real r2 = nmax(roughness * roughness, 1e-4f);
+        Color f0 = base_reflectivity(ior, albedo, metallic);
+        V3 h = normalize_or(viewDir + lightDir, normal);
+        real nDotH = dot(normal, h);
+        real hDotV = dot(h, viewDir);
+        real xh = 1 - hDotV, xv = 1 - nDotV;
+        real xh5 = xh * xh * xh * xh * xh, xv5 = xv * xv * xv * xv * xv;
+        Color fh = fresnel(hDotV, f0), fv = fresnel(nDotV, f0);
+        real g1v = smith_g1(nDotV, r2), g1l = smith_g1(nDotL, r2);
+        real dist = ggx_d(nDotH, r2);
+        real keep = 1 - transparency;
+        // d F / d albedo = metallic * (1 - x^5) for both Fresnel terms (F0 = lerp(f0, albedo, metallic)).
+        real specA = dist * g1v * g1l / (4 * nDotV) * metallic * (1 - xh5);
+        real diffK = (1 - metallic) * (nDotL / kPi);
+        dAlbedo = v3(specA + diffK * ((1 - fv.x) - albedo.x * metallic * (1 - xv5)),
+                     specA + diffK * ((1 - fv.y) - albedo.y * metallic * (1 - xv5)),
+                     specA + diffK * ((1 - fv.z) - albedo.z * metallic * (1 - xv5))) * keep;
+        if (roughness * roughness > (real)1e-4f) {  // r2 = max(r^2, 1e-4): no derivative while clamped
+            real alpha2 = r2 * r2, nh2 = nDotH * nDotH;
+            real dd = (1 - nh2) + nh2 * alpha2;
+            real dD_dAlpha2 = 1 / (kPi * dd * dd) - 2 * alpha2 * nh2 / (kPi * dd * dd * dd);
+            real dD = dD_dAlpha2 * 2 * r2;
+            real k = r2 / 2;
+            auto dG1dk = [&](real x) { real den = x * (1 - k) + k; return -x * (1 - x) / (den * den); };
+            real dG = 0.5f * (dG1dk(nDotV) * g1l + g1v * dG1dk(nDotL));
+            real dSpec = (dD * g1v * g1l + dist * dG) / (4 * nDotV) * (2 * roughness);
+            dRough = fh * dSpec * keep;
+        }
+    }
+    // d GgxD(nDotH, max(r^2, 1e-4)) / d r.
+    real ggx_d_dr(real nDotH) const {
+        if (!(roughness * roughness > (real)1e-4f)) return 0;
+        real r2 = roughness * roughness, alpha2 = r2 * r2, nh2 = nDotH * nDotH;
+        real dd = (1 - nh2) + nh2 * alpha2;
+        return (1 / (kPi * dd * dd) - 2 * alpha2 * nh2 / (kPi * dd * dd * dd)) * 2 * r2 * 2 * roughness;
+    }
+    // Density of the specular candidate direction d (GGX half vector, reflected view) at any d,
+    // and its derivative with respect to roughness.
+    void spec_candidate_pdf(V3 viewDir, V3 d, real& pdf, real& dPdf) const {
+        pdf = dPdf = 0;
+        V3 hv = viewDir + d;
+        if (mag_sqr(hv) <= 1e-12f) return;
+        V3 h = normalize(hv);
+        real nh = dot(normal, h);
+        if (nh < 0) { h = -h; nh = -nh; }
+        real vh = std::fabs(dot(h, viewDir));
+        if (!(vh > 1e-6f)) return;
+        real r2 = nmax(roughness * roughness, 1e-4f);
+        pdf = ggx_d(nh, r2) * nh / (4 * vh);
+        dPdf = ggx_d_dr(nh) * nh / (4 * vh);
+    }
+    // d SpecularProbability / d albedo: sp = clamp(max(lum F(nDotV), metallic), .001, .999) moves
+    // with albedo only where lum F is the maximum and inside the clamp.
+    Color spec_prob_grad(V3 viewDir) const {
+        real nDotV = nmax(1e-4f, dot(normal, viewDir));
+        real lumF = luminance(fresnel(nDotV, base_reflectivity(ior, albedo, metallic)));
+        if (!(lumF > metallic && lumF > 0.001f && lumF < 0.999f)) return kBlack;
+        real xv = 1 - nDotV, base = metallic * (1 - xv * xv * xv * xv * xv);
+        return v3(0.2126f * base, 0.7152f * base, 0.0722f * base);
+    }
+    // d OpaquePdf(viewDir, l, sp(albedo)) / d albedo (per channel) and / d roughness.
+    void opaque_pdf_grad(V3 viewDir, V3 l, real sp, Color& dA, real& dR) const {
+        dA = kBlack;
+        dR = 0;
+        real nDotL = dot(normal, l);
+        if (nDotL <= 0.0f) return;
+        V3 h = normalize_or(viewDir + l, normal);
+        real nh = nmax(0.0f, dot(normal, h));
+        real hv = nmax(1e-6f, dot(h, viewDir));
+        real r2 = nmax(roughness * roughness, 1e-4f);
+        real keep = 1 - transparency;
+        real pSpec = ggx_d(nh, r2) * nh / (4 * hv), pDiff = nDotL / kPi;
+        dR = sp * ggx_d_dr(nh) * nh / (4 * hv) * keep;
+        dA = spec_prob_grad(viewDir) * ((pSpec - pDiff) * keep);
+    }
+    // Score correction of a scatter direction l drawn by OpaqueScatter: d/dθ [log q(l) - log p(l)],
+    // q being the density of the procedure (lobe choice, GGX reflection or cosine lobe, then the
+    // ClampToGeoSide mirror, which folds the density at the mirror image of l onto l) and p the
+    // OpaquePdf the estimator divides by. Zero when q = p. sA: per albedo channel, sR: roughness.
+    void scatter_score(V3 viewDir, V3 l, real sp, Color& sA, real& sR) const {
+        sA = kBlack;
+        sR = 0;
+        real p = opaque_pdf_sp(viewDir, l, sp);
+        V3 hh = normalize_or(viewDir + l, normal);
+        real nh = nmax(0.0f, dot(normal, hh));
+        real hv = nmax(1e-6f, dot(hh, viewDir));
+        real r2 = nmax(roughness * roughness, 1e-4f);
+        real pSpec = ggx_d(nh, r2) * nh / (4 * hv), pDiff = dot(normal, l) / kPi;
+        real dpSpec = ggx_d_dr(nh) * nh / (4 * hv);
+        if (dot(normal, l) <= 0) { pSpec = pDiff = dpSpec = 0; }
+        V3 m = l - 2.0f * dot(l, normalGeo) * normalGeo;
+        real mSpec = 0, dmSpec = 0, mDiff = 0;
+        if (dot(m, normalGeo) < 0) {
+            spec_candidate_pdf(viewDir, m, mSpec, dmSpec);
+            mDiff = nmax(0.0f, dot(normal, m)) / kPi;
+        }
+        real mm = sp * mSpec + (1 - sp) * mDiff;
+        real q = p + mm;
+        if (!(q > 0)) return;
+        bool pLive = p >= 1e-6f;  // below that the estimator divides by the constant 1e-6
+        real dpR = sp * dpSpec, dmR = sp * dmSpec;
+        sR = (dpR + dmR) / q - (pLive ? dpR / p : 0);
+        real dpS = pSpec - pDiff, dmS = mSpec - mDiff;
+        sA = spec_prob_grad(viewDir) * ((dpS + dmS) / q - (pLive ? dpS / p : 0));
+    }
     real opaque_pdf_sp(V3 viewDir, V3 lightDir, real sp) const {
         real nDotL = dot(normal, lightDir);
         if (nDotL <= 0.0f) return 0.0f;
@@ -1195,6 +1368,7 @@ struct Scene {
     std::vector<Mesh> meshes;
     std::vector<Texture> textures;
     std::vector<Object> objects;
+    int skyTexId = -1;
     Sky sky;
     bool hasFog = false;
     Fog fog{};        // height fog, or color / anisotropy of a bounded medium
@@ -1410,6 +1584,144 @@ struct Scene {
         real fogPdf = hg_pdf(rDotL, fog.anisotropy);
         real w = power_heuristic(light.pdf, fogPdf);
         return sky.radiance(light.dir) * tr * fogPdf * w / light.pdf;
+    }
+    // Adjoint of one camera path (no reference counterpart): adds to gMat (4 per object: albedo
+    // rgb, roughness) and gEnv (3 per sky texel) the gradient of sum_c delta_c * L_c, L being this
+    // path's radiance as Sample computes it, i.e. of the estimator's expectation: a scatter's BSDF
+    // factor scales every contribution after it, so its gradient is delta * (radiance gathered
+    // downstream) * (dBSDF / BSDF + d log q - d log p), q being the density of the directions
+    // OpaqueScatter draws (ClampToGeoSide folds mirrored ones) and p the OpaquePdf divided by; the
+    // MIS weights are differentiated through both pdfs. Russian-roulette survival, the light pmf
+    // (whose score cancels the 1/pdf derivative) and the indirect clamp's scale are held fixed.
+    // The per-vertex contributions are kept and summed from the end of the path. Opaque, no fog.
+    void sample_adjoint(Ray ray, Rng& rng, uint32_t bounces, real indirectClamp, Color delta, real* gMat,
+                        real* gEnv, real& gTotal) const {
+        Color energy = kWhite;
+        bool hasScatterPdf = false;
+        real scatterPdf = 0;
+        int prevObj = -1;          // material of the scatter that produced the current ray
+        Color prevDpA = kBlack;    // its d pdf / d albedo, d pdf / d roughness (texture-scaled)
+        real prevDpR = 0;
+        std::vector<int> vObj;
+        std::vector<Color> vRa, vRr, vC;
+        std::vector<bool> vScat;
+        for (uint32_t i = 0; i != bounces + 1; ++i) {
+            Surface surf;
+            real dist = 0;
+            int obj = -1, prim = -1;
+            bool hit = trace(ray, nullptr, surf, dist, &obj, &prim);
+            if (!hit) {
+                real mis = 1.0f;
+                Color skyD = sky.radiance(ray.d);
+                if (hasScatterPdf) {
+                    real lp = 0.0f;
+                    if (!sky.light_dir(ray.d, lp)) lp = 0.0f;
+                    mis = power_heuristic(scatterPdf, lp);
+                    // d w / d pb of pb^2 / (pb^2 + pl^2), pb being the previous scatter's pdf.
+                    real pb = scatterPdf, den = pb * pb + lp * lp;
+                    real dw = den > 0 ? 2 * pb * lp * lp / (den * den) : 0;
+                    // The weight also moves with the sky pdf lp (a function of the texels).
+                    if (lp > 0 && den > 0) {
+                        Color sw = delta * skyD * energy;
+                        sky.pdf_grad(sky.light_texel(ray.d), lp, (sw.x + sw.y + sw.z) * (-2 * pb * pb * lp / (den * den)),
+                                     gEnv, gTotal);
+                    }
+                    if (prevObj >= 0 && dw != 0) {
+                        Color s = delta * skyD * energy * dw;
+                        real sum = s.x + s.y + s.z;
+                        gMat[prevObj * 4] += sum * prevDpA.x;
+                        gMat[prevObj * 4 + 1] += sum * prevDpA.y;
+                        gMat[prevObj * 4 + 2] += sum * prevDpA.z;
+                        gMat[prevObj * 4 + 3] += sum * prevDpR;
+                    }
+                }
+                Color c = skyD * mis * energy;
+                sky.radiance_grad(ray.d, delta * (energy * mis), gEnv);
+                vObj.push_back(-1); vRa.push_back(kBlack); vRr.push_back(kBlack); vC.push_back(c); vScat.push_back(false);
+                break;
+            }
+            const Material& mat = objects[obj].mat;
+            Color tA = mat.colorTex >= 0 ? textures[mat.colorTex].sample(surf.uv) : kWhite;
+            real tR = mat.roughTex >= 0 ? textures[mat.roughTex].sample(surf.uv).x : 1.0f;
+            V3 viewDir = -ray.d;
+            real distBias = nmax(1e-4f, dist * 1e-4f);
+            V3 hitPos = ray_at(ray, dist - distBias);
+            Color c = surf.radiance * energy;
+            SampleLight light;
+            Color tr;
+            if (sample_sky_light(hitPos, rng, nullptr, light, tr)) {
+                V3 n = dot(surf.normal, viewDir) >= 0.0f ? surf.normal : -surf.normal;
+                V3 ng = dot(surf.normalGeo, viewDir) >= 0.0f ? surf.normalGeo : -surf.normalGeo;
+                if (dot(n, light.dir) > 0.0f && dot(ng, light.dir) > 0.0f) {
+                    Color refl = surf.opaque_eval(viewDir, light.dir);
+                    real sp = surf.specular_probability(viewDir);
+                    real pb = surf.opaque_pdf_sp(viewDir, light.dir, sp);
+                    real w = power_heuristic(light.pdf, pb);
+                    Color skyL = sky.radiance(light.dir);
+                    c = c + skyL * tr * refl * w / light.pdf * energy;
+                    Color dA, dR, dpA;
+                    real dpR;
+                    surf.opaque_eval_grad(viewDir, light.dir, dA, dR);
+                    surf.opaque_pdf_grad(viewDir, light.dir, sp, dpA, dpR);
+                    // d w / d pb of the power heuristic pl^2 / (pl^2 + pb^2).
+                    real pl2 = light.pdf * light.pdf, den = pl2 + pb * pb;
+                    real dw = den > 0 ? -2 * pl2 * pb / (den * den) : 0;
+                    if (den > 0) {
+                        Color sw = delta * skyL * tr * refl * (1 / light.pdf) * energy;
+                        sky.pdf_grad(light.texel, light.pdf, (sw.x + sw.y + sw.z) * (2 * light.pdf * pb * pb / (den * den)),
+                                     gEnv, gTotal);
+                    }
+                    Color common = delta * skyL * tr * (1 / light.pdf) * energy;
+                    Color gA = dA * w + refl * dpA * dw, gR = dR * w + refl * (dpR * dw);
+                    gMat[obj * 4] += common.x * gA.x * tA.x;
+                    gMat[obj * 4 + 1] += common.y * gA.y * tA.y;
+                    gMat[obj * 4 + 2] += common.z * gA.z * tA.z;
+                    gMat[obj * 4 + 3] += (common.x * gR.x + common.y * gR.y + common.z * gR.z) * tR;
+                    sky.radiance_grad(light.dir, delta * tr * refl * (w / light.pdf) * energy, gEnv);
+                }
+            }
+            vObj.push_back(obj); vC.push_back(c);
+            if (russian_roulette(energy, i, rng, nullptr)) {
+                vRa.push_back(kBlack); vRr.push_back(kBlack); vScat.push_back(false);
+                break;
+            }
+            Scatter sc = surf.scatter(ray.d, viewDir, rng);
+            Color f = surf.opaque_eval(viewDir, sc.dir);
+            Color dA, dR, sA;
+            real sR;
+            surf.opaque_eval_grad(viewDir, sc.dir, dA, dR);
+            surf.scatter_score(viewDir, sc.dir, surf.specular_probability(viewDir), sA, sR);
+            auto ratio = [](real d, real v) { return v > 0 ? d / v : (real)0; };
+            vRa.push_back(v3((ratio(dA.x, f.x) + sA.x) * tA.x, (ratio(dA.y, f.y) + sA.y) * tA.y,
+                             (ratio(dA.z, f.z) + sA.z) * tA.z));
+            vRr.push_back(v3((ratio(dR.x, f.x) + sR) * tR, (ratio(dR.y, f.y) + sR) * tR, (ratio(dR.z, f.z) + sR) * tR));
+            vScat.push_back(true);
+            energy = energy * sc.weight;
+            energy = clamp_luminance(energy, indirectClamp);
+            hasScatterPdf = sc.hasPdf;
+            scatterPdf = sc.pdf;
+            {
+                Color dpA;
+                real dpR;
+                surf.opaque_pdf_grad(viewDir, sc.dir, surf.specular_probability(viewDir), dpA, dpR);
+                prevObj = obj;
+                prevDpA = dpA * tA;
+                prevDpR = dpR * tR;
+            }
+            ray = make_ray(ray_at(ray, dist) + sc.dir * distBias, sc.dir);
+        }
+        Color after = kBlack;  // radiance gathered after the current vertex's scatter
+        for (size_t j = vC.size(); j-- > 0;) {
+            if (vScat[j]) {
+                Color s = delta * after;
+                int o = vObj[j];
+                gMat[o * 4] += s.x * vRa[j].x;
+                gMat[o * 4 + 1] += s.y * vRa[j].y;
+                gMat[o * 4 + 2] += s.z * vRa[j].z;
+                gMat[o * 4 + 3] += s.x * vRr[j].x + s.y * vRr[j].y + s.z * vRr[j].z;
+            }
+            after = after + vC[j];
+        }
     }
     // RussianRoulette, scene.cs:744-756.
     static bool russian_roulette(Color& energy, uint32_t bounce, Rng& rng, Counters* ctr) {
@@ -1899,6 +2211,7 @@ void oc_scene_set_sky_texture(void* s, int texId, float multiplier, float yaw) {
     Scene* sc = (Scene*)s;
     sc->sky.kind = 1;
     sc->sky.tex = sc->textures[texId];
+    sc->skyTexId = texId;
     sc->sky.multiplier = multiplier;
     sc->sky.yaw = yaw;
     sc->sky.init_texture();
@@ -2112,6 +2425,91 @@ void oc_filter(const float* params7, const float* radiance, const float* normals
     }
 }
 
+// ---- Adjoint (no reference counterpart) -----------------------------------------------------
+// Parameters: per object albedo rgb + roughness (4 floats), then the sky texture's texels (3 per
+// texel, row-major) when the sky is a texture.
+uint32_t oc_param_count(void* s) {
+    Scene* sc = (Scene*)s;
+    return (uint32_t)sc->objects.size() * 4 + (sc->sky.kind == 1 ? sc->sky.tex.w * sc->sky.tex.h * 3 : 0);
+}
+void oc_params_get(void* s, float* p) {
+    Scene* sc = (Scene*)s;
+    size_t k = 0;
+    for (auto& o : sc->objects) {
+        p[k++] = (float)o.mat.albedo.x; p[k++] = (float)o.mat.albedo.y; p[k++] = (float)o.mat.albedo.z;
+        p[k++] = (float)o.mat.roughness;
+    }
+    if (sc->sky.kind == 1)
+        for (auto& t : sc->sky.tex.texels) { p[k++] = (float)t.x; p[k++] = (float)t.y; p[k++] = (float)t.z; }
+}
+// Sets the parameters and rebuilds the sky's sampling tables (a material using the sky's texture
+// sees the new texels too).
+void oc_params_set(void* s, const float* p) {
+    Scene* sc = (Scene*)s;
+    size_t k = 0;
+    for (auto& o : sc->objects) {
+        o.mat.albedo = v3(p[k], p[k + 1], p[k + 2]);
+        o.mat.roughness = p[k + 3];
+        k += 4;
+    }
+    if (sc->sky.kind == 1) {
+        for (auto& t : sc->sky.tex.texels) { t = v3(p[k], p[k + 1], p[k + 2]); k += 3; }
+        if (sc->skyTexId >= 0) sc->textures[sc->skyTexId].texels = sc->sky.tex.texels;
+        sc->sky.init_texture();
+    }
+}
+// Gradient of sum_pixels sum_c dImage[p,c] * Radiance[p,c] for a fixed-spp render with the
+// counter RNG policy (Radiance = sum of spp samples / (spp + 1), renderer.cs:173,215).
+int oc_adjoint(void* s, const float* view9, uint32_t W, uint32_t H, uint32_t spp, uint32_t bounces, float clamp,
+               uint32_t seed, const float* dImage, int threads, float* grad) {
+    Scene* sc = (Scene*)s;
+    if (!sc->built) return -1;
+    for (auto& o : sc->objects)
+        if (o.mat.transparency > 0.0f) return -2;
+    if (sc->hasFog) return -2;
+    View vw = ld_view(view9);
+    real tanFov = std::tan(vw.fov * 0.5f);
+    real aspect = (real)((float)W / (float)H);
+    size_t nMat = sc->objects.size() * 4, n = oc_param_count(s);
+    int nt = std::max(threads, 1);
+    std::vector<std::vector<real>> acc(nt, std::vector<real>(n, 0));
+    std::vector<real> accTotal(nt, 0);
+    std::atomic<uint32_t> next{0};
+    auto worker = [&](int wi) {
+        real* gm = acc[wi].data();
+        real* ge = gm + nMat;
+        for (;;) {
+            uint32_t y = next.fetch_add(1);
+            if (y >= H) break;
+            for (uint32_t x = 0; x != W; ++x) {
+                size_t o = (size_t)y * W + x;
+                Color delta = v3(dImage[o * 3], dImage[o * 3 + 1], dImage[o * 3 + 2]) / (real)(spp + 1);
+                for (uint32_t smp = 0; smp != spp; ++smp) {
+                    Rng rng;
+                    rng.seed_counter(seed, y * W + x, smp);
+                    real jx = (real)rng.next_float();
+                    real jy = (real)rng.next_float();
+                    V2 pos = v2(((real)x + jx) / (real)W, ((real)y + jy) / (real)H);
+                    sc->sample_adjoint(vw.ray(pos, aspect, tanFov), rng, bounces, (real)clamp, delta, gm, ge,
+                                       accTotal[wi]);
+                }
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nt; ++i) pool.emplace_back(worker, i);
+    worker(0);
+    for (auto& t : pool) t.join();
+    real total = 0;
+    for (int t = 0; t != nt; ++t) total += accTotal[t];
+    std::vector<real> sum(n, 0);
+    for (size_t i = 0; i != n; ++i)
+        for (int t = 0; t != nt; ++t) sum[i] += acc[t][i];
+    sc->sky.apply_total_grad(total, sum.data() + nMat);
+    for (size_t i = 0; i != n; ++i) grad[i] = (float)sum[i];
+    return 0;
+}
+
 // Sky queries for pinning tests: radiance(dir) and the MIS pdf LightDir(dir); pdf = 0 when null.
 void oc_sky_eval(void* s, const float* dirs, uint32_t n, float* rad3, float* pdf) {
     Scene* sc = (Scene*)s;
@@ -2148,6 +2546,20 @@ void oc_surface_eval(const float* material, const float* n3, const float* ng3, c
         Color e = s.opaque_eval(v, l);
         out4[i * 4] = (float)e.x; out4[i * 4 + 1] = (float)e.y; out4[i * 4 + 2] = (float)e.z;
         out4[i * 4 + 3] = (float)s.opaque_pdf(v, l);
+    }
+}
+// d OpaqueEval / d albedo (3, per channel) and d OpaqueEval / d roughness (3) per (view, light) pair.
+void oc_surface_eval_grad(const float* material, const float* n3, const float* ng3, const float* t4,
+                          const float* vl6, uint32_t cnt, float* out6) {
+    Material m = ld_material(material, nullptr);
+    Surface s{m.radiance, m.albedo, m.roughness, m.metallic, m.transparency, m.ior,
+              ld3(n3), ld3(ng3), V4{t4[0], t4[1], t4[2], t4[3]}, v2(0, 0)};
+    for (uint32_t i = 0; i != cnt; ++i) {
+        Color dA, dR;
+        s.opaque_eval_grad(ld3(vl6 + i * 6), ld3(vl6 + i * 6 + 3), dA, dR);
+        float* o = out6 + i * 6;
+        o[0] = (float)dA.x; o[1] = (float)dA.y; o[2] = (float)dA.z;
+        o[3] = (float)dR.x; o[4] = (float)dR.y; o[5] = (float)dR.z;
     }
 }
 // Surface.Scatter samples from seed (a,b) for incoming direction inc3: dir3 weight3 pdf refracted.

@@ -483,6 +483,56 @@ class Renderer:
         return dict(zip(N.COUNTER_NAMES, list(out)))
 
 
+class Adjoint:
+    """Adjoint rendering of a built scene (no reference counterpart): parameters are each object's
+    albedo rgb and roughness, then the texture sky's texels; backward(d_image) returns
+    d sum(d_image * Radiance) / d parameters for the last forward render (counter RNG policy,
+    fixed sample count)."""
+
+    def __init__(self, scene):
+        self.scene = scene
+        scene.build()
+        n = ctypes.c_uint32()
+        check(N.lib().sr_adjoint_param_count(scene.ctx, ctypes.byref(n)), "adjoint param count")
+        self.param_count = n.value
+        self._wh = None
+
+    def params(self):
+        p = np.empty(self.param_count, np.float32)
+        check(N.lib().sr_adjoint_params_get(self.scene.ctx, fptr(p)), "adjoint params get")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.param_count:
+            raise ValueError(f"expected {self.param_count} parameters")
+        check(N.lib().sr_adjoint_params_set(self.scene.ctx, fptr(p)), "adjoint params set")
+
+    def forward(self, view, width, height, spp, bounces, indirect_clamp, seed):
+        rp = N.RenderParams_()
+        rp.width, rp.height, rp.block_size = int(width), int(height), 32
+        rp.min_samples = rp.max_samples = int(spp)
+        rp.variance_threshold = 0.0
+        rp.bounces, rp.indirect_clamp = int(bounces), float(indirect_clamp)
+        rp.rng_policy, rp.rng_seed, rp.tick_bands, rp.light_sampler = RNG_COUNTER, int(seed), 1, LIGHT_CDF
+        img = np.empty((int(width) * int(height), 3), np.float32)
+        v = view._c()
+        check(N.lib().sr_adjoint_forward(self.scene.ctx, ctypes.byref(v), ctypes.byref(rp), fptr(img)),
+              "adjoint forward")
+        self._wh = (int(width), int(height))
+        return img
+
+    def backward(self, d_image):
+        if self._wh is None:
+            raise RuntimeError("forward first")
+        d = np.ascontiguousarray(d_image, np.float32).reshape(-1)
+        if d.size != self._wh[0] * self._wh[1] * 3:
+            raise ValueError("d_image must be width * height * 3 floats")
+        g = np.empty(self.param_count, np.float32)
+        check(N.lib().sr_adjoint_backward(self.scene.ctx, fptr(d), fptr(g)), "adjoint backward")
+        return g
+
+
 class Compositor:
     """Compositor (compose.cs:7-267): the joint bilateral denoise of Filter (compose.cs:101-204)."""
 

@@ -117,6 +117,12 @@ SIGNATURES = {
     "sr_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "sr_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), c_uint32_p]),
     "sr_render_gpu_seconds": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "sr_adjoint_param_count": (ctypes.c_int, [ctypes.c_void_p, c_uint32_p]),
+    "sr_adjoint_params_get": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "sr_adjoint_params_set": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "sr_adjoint_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(View_), ctypes.POINTER(RenderParams_),
+                                          c_float_p]),
+    "sr_adjoint_backward": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p]),
     "sr_triangles_build": (ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_uint32, ctypes.c_int32, c_float_p]),
 }
 

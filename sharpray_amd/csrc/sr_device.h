@@ -413,7 +413,7 @@ SR_DEV uint32_t cdf_sample(const float* cdf, uint32_t n, float x) {
     return lo < n - 1 ? lo : n - 1;
 }
 template <class R>
-SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf) {
+SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf, uint32_t* texelOut = nullptr) {
     const DevSky& k = S.sky;
     if (k.kind == 1) {
         if (k.total <= 0.0f) return false;
@@ -440,6 +440,7 @@ SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf) {
         }
         const vec2 uv = mk2(((float)(int32_t)x + 0.5f) / (float)k.w, ((float)(int32_t)y + 0.5f) / (float)k.h);
         dir = rotate(k.rot, from_equirect_uv(uv));
+        if (texelOut) *texelOut = y * (uint32_t)k.w + x;
         return pdf > 0.0f;
     }
     if (k.kind == 2) {

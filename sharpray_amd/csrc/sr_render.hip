@@ -29,6 +29,7 @@
 #include "sr_device.h"
 #include "sr_host.h"
 #include "sr_trace.h"
+#include "sr_adjoint.h"
 
 using namespace sr;
 
@@ -558,6 +559,51 @@ __global__ void __launch_bounds__(kBlock) k_next(Frame F) {
 }
 
 // Primary-hit diagnostic: sample 0 of every pixel of a band.
+// Adjoint backward pass: one lane per (pixel, sample) path, replayed with the forward render's
+// counter-RNG stream (key (seed, y * W + x), counter (sample, draw)). Material gradients gather in
+// LDS per block; sky-texel gradients go straight to global memory.
+struct AdjointArgs {
+    ViewD view;
+    uint32_t W, H, spp, bounces, seed, nObj;
+    float clampLum;
+    const float* dImage;  // W * H * 3
+    float* gMat;          // nObj * 4
+    float* gEnv;          // sky texels * 3
+    float* gTotal;        // one float: the through-total part of the sky-pdf derivatives
+};
+__global__ void __launch_bounds__(kBlock) k_adjoint(AdjointArgs A, DevScene S) {
+    __shared__ float gl[kAdjLdsObjects * 4];
+    __shared__ float glTotal;
+    const uint32_t nl = (A.nObj < (uint32_t)kAdjLdsObjects ? A.nObj : (uint32_t)kAdjLdsObjects) * 4;
+    for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) gl[i] = 0.0f;
+    if (threadIdx.x == 0) glTotal = 0.0f;
+    __syncthreads();
+    MatGrad gm{gl, A.gMat, A.nObj, 0.0f};
+    const uint64_t id = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < (uint64_t)A.W * A.H * A.spp) {
+        const uint32_t smp = (uint32_t)(id % A.spp);
+        const uint32_t pix = (uint32_t)(id / A.spp);
+        const uint32_t x = pix % A.W, y = pix / A.W;
+        const vec3 dI = mk3(A.dImage[pix * 3], A.dImage[pix * 3 + 1], A.dImage[pix * 3 + 2]);
+        if (dI.x != 0.0f || dI.y != 0.0f || dI.z != 0.0f) {
+            const vec3 delta = dI / (float)(A.spp + 1);  // Radiance = sum / (spp + 1) (renderer.cs:173,215)
+            CounterRng rng;
+            rng.key0 = A.seed;
+            rng.key1 = y * A.W + x;
+            rng.sample = smp;
+            rng.draw = 0;
+            vec3 o, d;
+            camera_ray(A.view, A.W, A.H, x, y, rng, o, d);
+            adjoint_path(S, mk_ray(o, d), rng, A.bounces, A.clampLum, delta, gm, A.gEnv, sr_lds + threadIdx.x);
+        }
+    }
+    if (gm.gTotal != 0.0f) atomicAdd(&glTotal, gm.gTotal);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x)
+        if (gl[i] != 0.0f) unsafeAtomicAdd(&A.gMat[i], gl[i]);
+    if (threadIdx.x == 0 && glTotal != 0.0f) unsafeAtomicAdd(A.gTotal, glTotal);
+}
+
 __global__ void k_primary(Frame F, DevScene S, int32_t* obj, int32_t* tri, float* dist) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= F.nSlots) return;
@@ -725,6 +771,11 @@ struct sr_ctx {
     DevBuf<float> dTris, dAttrs, dTexels, dCdf;
     DevBuf<float4> dAlias;
     DevBuf<uint16_t> dFogGrid;
+    // Adjoint: the last forward render's view and parameters, and the backward pass's buffers.
+    bool adjReady = false;
+    sr_view adjView{};
+    sr_render_params adjParams{};
+    DevBuf<float> dAdjImage, dAdjGrad;
     DevBuf<int32_t> dSlotTri, dSceneItems;
     DevBuf<DevObject> dObjects;
     DevBuf<DevTexture> dTextures;
@@ -775,6 +826,7 @@ struct sr_ctx {
         if (evEnd) (void)hipEventDestroy(evEnd);
         dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release(); dAlias.release(); dFogGrid.release();
         dSlotTri.release(); dSceneItems.release(); dObjects.release(); dTextures.release();
+        dAdjImage.release(); dAdjGrad.release();
         dRayO.release(); dRayD.release(); dHitA.release(); dHitSlot.release();
         dPaths.release(); dShadows.release(); dAcc.release();
         dQueue0.release(); dQueue1.release(); dCounts.release(); dCtr.release();
@@ -888,6 +940,95 @@ int grid_for(const sr_ctx* c, uint32_t maxItems) {
     const uint32_t want = (maxItems + kBlock - 1) / kBlock;
     const uint32_t cap = (uint32_t)c->gridCUs * 8u;
     return (int)std::max(1u, std::min(want, cap));
+}
+
+}  // namespace
+
+namespace {
+// Sky tables (scene.cs:389-404 / 341-352 / 290-298): the texture sky's row/column CDFs, alias
+// table and pdf scale, or the procedural sky's constants.
+void build_sky(sr_ctx* ctx, DevSky& K, std::vector<float>& cdf, std::vector<float4>& aliasTab) {
+    K.kind = ctx->skyKind;
+    if (ctx->skyKind == 1) {
+        const HostTexture& t = ctx->textures[ctx->skyTex];
+        const int W = (int)t.w, H = (int)t.h;
+        K.tex = ctx->skyTex;
+        K.multiplier = ctx->skyMult;
+        K.rot = angle_axis(ctx->skyYaw, up3());
+        K.rotInv = conj(K.rot);
+        K.w = W; K.h = H;
+        K.cdfOff = 0;
+        cdf.assign((size_t)(H + 1) + (size_t)H * (W + 1), 0.0f);
+        const size_t n = (size_t)W * H;
+        std::vector<float> rowW(H), colW(W), allW(n);
+        for (int y = 0; y < H; ++y) {
+            const float s = sinf(kPi * ((float)y + 0.5f) / (float)H);
+            for (int x = 0; x < W; ++x) {
+                const float* c = t.rgb.data() + ((size_t)y * W + x) * 3;
+                colW[x] = luminance(mk3(c[0], c[1], c[2])) * s;
+                allW[(size_t)y * W + x] = colW[x];
+            }
+            rowW[y] = build_cdf1(colW.data(), W, cdf.data() + (H + 1) + (size_t)y * (W + 1));
+        }
+        K.total = build_cdf1(rowW.data(), H, cdf.data());
+        K.pdfScale = (float)(W * H) / (K.total * 2.0f * kPi * kPi);
+        // Alias table over the same weights; each entry carries the pdf LightDirRand returns for
+        // either outcome (luminance * pdfScale, scene.cs:414).
+        std::vector<float> prob(n);
+        std::vector<int32_t> alias(n);
+        build_alias(allW.data(), n, prob.data(), alias.data());
+        aliasTab.resize(n);
+        auto texel_pdf = [&](size_t i) {
+            const float* c = t.rgb.data() + i * 3;
+            return luminance(mk3(c[0], c[1], c[2])) * K.pdfScale;
+        };
+        for (size_t i = 0; i < n; ++i) {
+            float4 e;
+            e.x = prob[i];
+            memcpy(&e.y, &alias[i], sizeof(float));
+            e.z = texel_pdf(i);
+            e.w = texel_pdf((size_t)alias[i]);
+            aliasTab[i] = e;
+        }
+    } else {
+        const sr_sky_procedural& p = ctx->skyProc;
+        K.top = mk3(p.radiance_top[0], p.radiance_top[1], p.radiance_top[2]);
+        K.middle = mk3(p.radiance_middle[0], p.radiance_middle[1], p.radiance_middle[2]);
+        K.bottom = mk3(p.radiance_bottom[0], p.radiance_bottom[1], p.radiance_bottom[2]);
+        K.sunDir = mk3(p.sun_dir[0], p.sun_dir[1], p.sun_dir[2]);
+        K.sunRad = mk3(p.sun_radiance[0], p.sun_radiance[1], p.sun_radiance[2]);
+        K.sunCos = cosf(p.sun_angle);
+        K.sunRot = quat_look(K.sunDir, up3());
+        K.sunPdfDenom = kPi * (1.0f - K.sunCos);
+    }
+}
+
+// Device object records (transform, material, BVH bases; Object ctor scene.cs:214-223).
+void build_objects(sr_ctx* ctx, std::vector<DevObject>& dobj, int32_t& anyTransparent) {
+    dobj.assign(ctx->objects.size(), DevObject{});
+    anyTransparent = 0;
+    for (size_t i = 0; i < ctx->objects.size(); ++i) {
+        const HostObject& o = ctx->objects[i];
+        DevObject& d = dobj[i];
+        memset(&d, 0, sizeof(d));
+        d.rot = mkq(o.trans.rot);
+        d.pos = mk3(o.trans.pos[0], o.trans.pos[1], o.trans.pos[2]);
+        d.scale = mk3(o.trans.scale[0], o.trans.scale[1], o.trans.scale[2]);
+        d.kind = o.kind;
+        if (o.kind == 0) {
+            d.nodeBase = ctx->meshes[o.mesh].bvh.used > 0 ? ctx->meshNodeBase[o.mesh] : -1;
+            d.triBase = ctx->meshTriBase[o.mesh];
+        }
+        d.texColor = o.mat.color_texture; d.texRough = o.mat.roughness_texture;
+        d.texMetal = o.mat.metallic_texture; d.texNormal = o.mat.normal_texture;
+        d.albedo = mk3(o.mat.albedo[0], o.mat.albedo[1], o.mat.albedo[2]);
+        d.roughness = o.mat.roughness; d.metallic = o.mat.metallic;
+        d.transparency = o.mat.transparency; d.ior = o.mat.ior;
+        d.radiance = mk3(o.mat.radiance[0], o.mat.radiance[1], o.mat.radiance[2]);
+        d.f0 = libm_powf((o.mat.ior - 1.0f) / (o.mat.ior + 1.0f), 2.0f);
+        memcpy(d.shape, o.shape, sizeof(d.shape));
+        if (o.mat.transparency > 0.0f) anyTransparent = 1;
+    }
 }
 
 }  // namespace
@@ -1105,60 +1246,7 @@ int sr_scene_build(sr_ctx* ctx) {
     S.sceneBase = sceneBase;
     S.sceneNodes = ctx->sceneBvh.used;
     // Sky (scene.cs:389-404 / 341-352 / 290-298).
-    DevSky& K = S.sky;
-    K.kind = ctx->skyKind;
-    if (ctx->skyKind == 1) {
-        const HostTexture& t = ctx->textures[ctx->skyTex];
-        const int W = (int)t.w, H = (int)t.h;
-        K.tex = ctx->skyTex;
-        K.multiplier = ctx->skyMult;
-        K.rot = angle_axis(ctx->skyYaw, up3());
-        K.rotInv = conj(K.rot);
-        K.w = W; K.h = H;
-        K.cdfOff = 0;
-        cdf.assign((size_t)(H + 1) + (size_t)H * (W + 1), 0.0f);
-        const size_t n = (size_t)W * H;
-        std::vector<float> rowW(H), colW(W), allW(n);
-        for (int y = 0; y < H; ++y) {
-            const float s = sinf(kPi * ((float)y + 0.5f) / (float)H);
-            for (int x = 0; x < W; ++x) {
-                const float* c = t.rgb.data() + ((size_t)y * W + x) * 3;
-                colW[x] = luminance(mk3(c[0], c[1], c[2])) * s;
-                allW[(size_t)y * W + x] = colW[x];
-            }
-            rowW[y] = build_cdf1(colW.data(), W, cdf.data() + (H + 1) + (size_t)y * (W + 1));
-        }
-        K.total = build_cdf1(rowW.data(), H, cdf.data());
-        K.pdfScale = (float)(W * H) / (K.total * 2.0f * kPi * kPi);
-        // Alias table over the same weights; each entry carries the pdf LightDirRand returns for
-        // either outcome (luminance * pdfScale, scene.cs:414).
-        std::vector<float> prob(n);
-        std::vector<int32_t> alias(n);
-        build_alias(allW.data(), n, prob.data(), alias.data());
-        aliasTab.resize(n);
-        auto texel_pdf = [&](size_t i) {
-            const float* c = t.rgb.data() + i * 3;
-            return luminance(mk3(c[0], c[1], c[2])) * K.pdfScale;
-        };
-        for (size_t i = 0; i < n; ++i) {
-            float4 e;
-            e.x = prob[i];
-            memcpy(&e.y, &alias[i], sizeof(float));
-            e.z = texel_pdf(i);
-            e.w = texel_pdf((size_t)alias[i]);
-            aliasTab[i] = e;
-        }
-    } else {
-        const sr_sky_procedural& p = ctx->skyProc;
-        K.top = mk3(p.radiance_top[0], p.radiance_top[1], p.radiance_top[2]);
-        K.middle = mk3(p.radiance_middle[0], p.radiance_middle[1], p.radiance_middle[2]);
-        K.bottom = mk3(p.radiance_bottom[0], p.radiance_bottom[1], p.radiance_bottom[2]);
-        K.sunDir = mk3(p.sun_dir[0], p.sun_dir[1], p.sun_dir[2]);
-        K.sunRad = mk3(p.sun_radiance[0], p.sun_radiance[1], p.sun_radiance[2]);
-        K.sunCos = cosf(p.sun_angle);
-        K.sunRot = quat_look(K.sunDir, up3());
-        K.sunPdfDenom = kPi * (1.0f - K.sunCos);
-    }
+    build_sky(ctx, S.sky, cdf, aliasTab);
     S.fog.enabled = ctx->hasFog ? 1 : 0;
     S.fog.kind = (int32_t)ctx->fogKind;
     if (ctx->fogKind == 0) {
@@ -1184,29 +1272,8 @@ int sr_scene_build(sr_ctx* ctx) {
         }
     }
 
-    std::vector<DevObject> dobj(ctx->objects.size());
-    for (size_t i = 0; i < ctx->objects.size(); ++i) {
-        const HostObject& o = ctx->objects[i];
-        DevObject& d = dobj[i];
-        memset(&d, 0, sizeof(d));
-        d.rot = mkq(o.trans.rot);
-        d.pos = mk3(o.trans.pos[0], o.trans.pos[1], o.trans.pos[2]);
-        d.scale = mk3(o.trans.scale[0], o.trans.scale[1], o.trans.scale[2]);
-        d.kind = o.kind;
-        if (o.kind == 0) {
-            d.nodeBase = ctx->meshes[o.mesh].bvh.used > 0 ? ctx->meshNodeBase[o.mesh] : -1;
-            d.triBase = ctx->meshTriBase[o.mesh];
-        }
-        d.texColor = o.mat.color_texture; d.texRough = o.mat.roughness_texture;
-        d.texMetal = o.mat.metallic_texture; d.texNormal = o.mat.normal_texture;
-        d.albedo = mk3(o.mat.albedo[0], o.mat.albedo[1], o.mat.albedo[2]);
-        d.roughness = o.mat.roughness; d.metallic = o.mat.metallic;
-        d.transparency = o.mat.transparency; d.ior = o.mat.ior;
-        d.radiance = mk3(o.mat.radiance[0], o.mat.radiance[1], o.mat.radiance[2]);
-        d.f0 = libm_powf((o.mat.ior - 1.0f) / (o.mat.ior + 1.0f), 2.0f);
-        memcpy(d.shape, o.shape, sizeof(d.shape));
-        if (o.mat.transparency > 0.0f) S.anyTransparent = 1;
-    }
+    std::vector<DevObject> dobj;
+    build_objects(ctx, dobj, S.anyTransparent);
 
     if (ctx->device < 0) {  // host-only context
         ctx->built = true;
@@ -1688,6 +1755,149 @@ int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* p, float* out_rgb
 
 int sr_denoise_bilateral_guided(sr_ctx* ctx, const sr_denoise_params* p, float albedo_limit, float* out_rgb) {
     return filter_to_host(ctx, p, true, albedo_limit, out_rgb);
+}
+
+// ---- Adjoint -------------------------------------------------------------------------------------
+int sr_adjoint_param_count(sr_ctx* ctx, uint32_t* count) {
+    if (!ctx || !count) return fail(SR_ERR_INVALID, "adjoint: null argument");
+    size_t n = ctx->objects.size() * 4;
+    if (ctx->skyKind == 1) n += (size_t)ctx->textures[ctx->skyTex].w * ctx->textures[ctx->skyTex].h * 3;
+    if (n > 0xffffffffu) return fail(SR_ERR_LIMIT, "adjoint: too many parameters");
+    *count = (uint32_t)n;
+    return SR_OK;
+}
+
+int sr_adjoint_params_get(sr_ctx* ctx, float* params) {
+    if (!ctx || !params) return fail(SR_ERR_INVALID, "adjoint: null argument");
+    size_t k = 0;
+    for (const HostObject& o : ctx->objects) {
+        params[k++] = o.mat.albedo[0]; params[k++] = o.mat.albedo[1]; params[k++] = o.mat.albedo[2];
+        params[k++] = o.mat.roughness;
+    }
+    if (ctx->skyKind == 1) {
+        const std::vector<float>& rgb = ctx->textures[ctx->skyTex].rgb;
+        memcpy(params + k, rgb.data(), rgb.size() * sizeof(float));
+    }
+    return SR_OK;
+}
+
+int sr_adjoint_params_set(sr_ctx* ctx, const float* params) {
+    if (!ctx || !params) return fail(SR_ERR_INVALID, "adjoint: null argument");
+    ctx->join();
+    uint32_t n = 0;
+    int rc = sr_adjoint_param_count(ctx, &n);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; ++i)
+        if (!std::isfinite(params[i])) return fail(SR_ERR_INVALID, "adjoint: non-finite parameter");
+    size_t k = 0;
+    for (HostObject& o : ctx->objects) {
+        o.mat.albedo[0] = params[k]; o.mat.albedo[1] = params[k + 1]; o.mat.albedo[2] = params[k + 2];
+        o.mat.roughness = params[k + 3];
+        k += 4;
+    }
+    if (ctx->skyKind == 1) {
+        std::vector<float>& rgb = ctx->textures[ctx->skyTex].rgb;
+        memcpy(rgb.data(), params + k, rgb.size() * sizeof(float));
+    }
+    ctx->adjReady = false;
+    if (!ctx->built) return SR_OK;
+    // Refresh what depends on the parameters: object records, the sky texture's texels and the sky
+    // sampling tables (the BVHs do not change).
+    std::vector<DevObject> dobj;
+    build_objects(ctx, dobj, ctx->dev.anyTransparent);
+    std::vector<float> cdf;
+    std::vector<float4> aliasTab;
+    const int32_t sampler = ctx->dev.sky.sampler;
+    build_sky(ctx, ctx->dev.sky, cdf, aliasTab);
+    ctx->dev.sky.sampler = sampler;
+    if (ctx->device < 0) return SR_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    if ((rc = upload(ctx->dObjects, dobj.data(), dobj.size(), s))) return rc;
+    if (ctx->skyKind == 1) {
+        size_t off = 0;
+        for (int32_t t = 0; t < ctx->skyTex; ++t) off += ctx->textures[t].rgb.size();
+        const std::vector<float>& rgb = ctx->textures[ctx->skyTex].rgb;
+        HIP_TRY(hipMemcpyAsync(ctx->dTexels.ptr + off, rgb.data(), rgb.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        if ((rc = upload(ctx->dCdf, cdf.data(), cdf.size(), s))) return rc;
+        if ((rc = upload(ctx->dAlias, aliasTab.data(), aliasTab.size(), s))) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    ctx->dev.objects = ctx->dObjects.ptr;
+    ctx->dev.cdf = ctx->dCdf.ptr;
+    ctx->dev.alias = ctx->dAlias.ptr;
+    return SR_OK;
+}
+
+int sr_adjoint_forward(sr_ctx* ctx, const sr_view* view, const sr_render_params* p, float* image) {
+    if (!ctx || !view || !p) return fail(SR_ERR_INVALID, "adjoint forward: null argument");
+    if (p->rng_policy != SR_RNG_COUNTER) return fail(SR_ERR_INVALID, "adjoint forward: needs the counter RNG policy");
+    if (p->min_samples != p->max_samples || p->variance_threshold != 0.0f)
+        return fail(SR_ERR_INVALID, "adjoint forward: needs a fixed sample count (min = max, threshold 0)");
+    if (p->bounces + 1 > (uint32_t)kAdjMaxVertices) return fail(SR_ERR_LIMIT, "adjoint forward: at most 16 bounces");
+    if (ctx->hasFog) return fail(SR_ERR_STATE, "adjoint: scenes with fog are not differentiated");
+    for (const HostObject& o : ctx->objects)
+        if (o.mat.transparency > 0.0f) return fail(SR_ERR_STATE, "adjoint: transparent objects are not differentiated");
+    int rc = sr_render_begin(ctx, view, p);
+    if (rc) return rc;
+    if ((rc = wait_render(ctx))) return rc;
+    if (image) {
+        HIP_TRY(hipMemcpy(image, ctx->dRad.ptr, (size_t)p->width * p->height * 12, hipMemcpyDeviceToHost));
+    }
+    ctx->adjView = *view;
+    ctx->adjParams = *p;
+    ctx->adjReady = true;
+    return SR_OK;
+}
+
+int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad) {
+    if (!ctx || !d_image || !grad) return fail(SR_ERR_INVALID, "adjoint backward: null argument");
+    if (!ctx->adjReady) return fail(SR_ERR_STATE, "adjoint backward: no forward render with the current parameters");
+    ctx->join();
+    HIP_TRY(hipSetDevice(ctx->device));
+    const sr_render_params& p = ctx->adjParams;
+    uint32_t n = 0;
+    int rc = sr_adjoint_param_count(ctx, &n);
+    if (rc) return rc;
+    const size_t px = (size_t)p.width * p.height;
+    if ((rc = ensure(ctx->dAdjImage, px * 3)) || (rc = ensure(ctx->dAdjGrad, (size_t)n + 1))) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(ctx->dAdjImage.ptr, d_image, px * 12, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(ctx->dAdjGrad.ptr, 0, ((size_t)n + 1) * sizeof(float), s));
+    AdjointArgs A;
+    A.view = make_view(&ctx->adjView, p.width, p.height);
+    A.W = p.width; A.H = p.height; A.spp = p.max_samples; A.bounces = p.bounces; A.seed = p.rng_seed;
+    A.nObj = (uint32_t)ctx->objects.size();
+    A.clampLum = p.indirect_clamp;
+    A.dImage = ctx->dAdjImage.ptr;
+    A.gMat = ctx->dAdjGrad.ptr;
+    A.gEnv = ctx->dAdjGrad.ptr + ctx->objects.size() * 4;
+    A.gTotal = ctx->dAdjGrad.ptr + n;
+    const uint64_t total = (uint64_t)px * p.max_samples;
+    const uint64_t blocks = (total + kBlock - 1) / kBlock;
+    if (blocks > 0x7fffffffu) return fail(SR_ERR_LIMIT, "adjoint backward: too many paths");
+    hipLaunchKernelGGL(k_adjoint, dim3((uint32_t)blocks), dim3(kBlock), kTravLdsBytes, s, A, ctx->dev);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(grad, ctx->dAdjGrad.ptr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+    float gTotal = 0.0f;
+    HIP_TRY(hipMemcpyAsync(&gTotal, ctx->dAdjGrad.ptr + n, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // The sky pdf's dependence on the CDF total reaches every texel: d total / d texel(x, y, c) =
+    // luminance weight c * sin(pi (y + 0.5) / H).
+    if (ctx->skyKind == 1 && gTotal != 0.0f) {
+        const HostTexture& t = ctx->textures[ctx->skyTex];
+        float* g = grad + ctx->objects.size() * 4;
+        for (uint32_t y = 0; y < t.h; ++y) {
+            const float sy = sinf(kPi * ((float)y + 0.5f) / (float)t.h);
+            for (uint32_t x = 0; x < t.w; ++x) {
+                const size_t i = ((size_t)y * t.w + x) * 3;
+                g[i] += gTotal * 0.2126f * sy;
+                g[i + 1] += gTotal * 0.7152f * sy;
+                g[i + 2] += gTotal * 0.0722f * sy;
+            }
+        }
+    }
+    return SR_OK;
 }
 
 int sr_triangles_build(const float* pos, const float* nrm, const float* uv, uint32_t n, int32_t smooth, float* out) {

@@ -372,6 +372,60 @@ def config_fog(scale=1.0, spp=4, res=128, seed=SEED_BASE + 2):
     return s
 
 
+def config_adjoint(scale=1.0, spp=16, bounces=4, rings=224, segments=224, materials=16, seed=SEED_BASE + 3):
+    """The adjoint workload: 512x512, a ~100k-triangle fBm noise sphere split into `materials`
+    latitude bands (one material each: albedo, roughness, metallic 0 or 1) over a ground quad,
+    under a 256x128 log-normal environment map. Parameters: 4 per object + 256*128*3 texels."""
+    s = SceneSpec(name=f"adjoint_512px_{spp}spp_100ktri")
+    s.width = max(8, int(round(512 * scale)))
+    s.height = max(8, int(round(512 * scale)))
+    s.min_samples = s.max_samples = spp
+    s.bounces = bounces
+    s.indirect_clamp = 10.0
+    rng = np.random.default_rng(seed)
+    s.textures.append(lognormal_env(256, 128, 1.0, seed + 100))
+    s.sky_texture = 0
+    pos, uv, ring = fbm_sphere(rings, segments, 0.15, 5, seed + 200)
+    tris = triangles_build(pos, None, uv, smooth=True)
+    per = rings // materials
+    for b in range(materials):
+        sel = (ring >= b * per) & (ring < (b + 1) * per) if b < materials - 1 else ring >= b * per
+        s.meshes.append(np.ascontiguousarray(tris[sel]))
+        s.mesh_bins.append(8)
+    gp, guv = quad(-1.25, 12.0)
+    s.meshes.append(triangles_build(gp, None, guv))
+    s.mesh_bins.append(8)
+    for b in range(materials):
+        s.objects.append(ObjectSpec(f"band_{b}", _identity(),
+                                    _mat(rng.uniform(0.1, 0.9, size=3), rng.uniform(0.2, 0.9),
+                                         float(rng.integers(0, 2))), 0, b))
+    s.objects.append(ObjectSpec("ground", _identity(), _mat(0.5, 0.8), 0, materials))
+    s.view = _view((0.0, 0.3, -3.0), 4.0, 50.0)
+    return s
+
+
+def config_adjoint_small(width=32, height=24, spp=16, bounces=2):
+    """A small analytic scene for gradient checks: a box floor and four spheres (dielectric and
+    metal, rough and glossy) under a 32x16 log-normal sky; bounces 2 keeps Russian roulette off."""
+    s = SceneSpec(name="adjoint_small")
+    s.textures.append(lognormal_env(32, 16, 1.0, 123))
+    s.sky_texture = 0
+    s.width, s.height = width, height
+    s.min_samples = s.max_samples = spp
+    s.bounces = bounces
+    s.indirect_clamp = 1e30
+    s.objects.append(ObjectSpec("floor", _identity(), _mat((0.6, 0.5, 0.4), 0.8, 0.0), 2, -1,
+                                np.array([-5, -1.2, -3, 5, -1.0, 8], np.float32)))
+    mats = [((0.8, 0.2, 0.2), 0.5, 0.0), ((0.2, 0.7, 0.3), 0.3, 1.0), ((0.3, 0.3, 0.9), 0.7, 0.0),
+            ((0.9, 0.8, 0.2), 0.25, 1.0)]
+    for i, (a, r, m) in enumerate(mats):
+        c = (-1.5 + i * 1.0, -0.55 + 0.1 * i, 2.5 + 0.4 * (i % 2))
+        s.objects.append(ObjectSpec(f"s{i}", _identity(), _mat(a, r, m), 1, -1,
+                                    np.array(list(c) + [0.45, 0, 0], np.float32)))
+    s.view = _view((0.0, 0.2, -1.0), 5.0, 60.0)
+    return s
+
+
 def config_analytic(scale=1.0, spp=4):
     """program.cs-like scene (spheres and a box floor under a procedural sky with sun) exercising
     the analytic shapes and the procedural sky."""
