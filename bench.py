@@ -59,6 +59,62 @@ def cpu_baseline(spec, seconds_target=15.0):
                       f"({int(samples)} samples, {dt:.1f} s, scene build {setup:.1f} s)"}
 
 
+def bench_adjoint(args, spec, scene, view):
+    """Inverse-rendering step: forward render at perturbed parameters (materials +-30 %, sky texels
+    +-30 %) and the backward pass of 0.5 * |image - target|^2, the target being rendered at the
+    scene's own parameters. value = backward paths per second."""
+    import numpy as np
+    from sharpray_amd import Adjoint
+    adj = Adjoint(scene)
+    fw = (view, spec.width, spec.height, spec.max_samples, spec.bounces, spec.indirect_clamp)
+    p_true = adj.params()
+    target = adj.forward(*fw, seed=1)
+    rng = np.random.default_rng(0)
+    p0 = (p_true * rng.uniform(0.7, 1.3, size=p_true.shape)).astype(np.float32)
+    n_obj = len(spec.objects)
+    mat = p0[:n_obj * 4].reshape(n_obj, 4)
+    mat[:, :3] = np.clip(mat[:, :3], 0.0, 1.0)
+    mat[:, 3] = np.clip(mat[:, 3], 0.05, 1.0)
+    adj.set_params(p0)
+    for i in range(args.warmup):
+        img = adj.forward(*fw, seed=2)
+        adj.backward(img - target)
+    tf, tb = [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        img = adj.forward(*fw, seed=2)
+        t1 = time.perf_counter()
+        g = adj.backward(img - target)
+        t2 = time.perf_counter()
+        tf.append(t1 - t0)
+        tb.append(t2 - t1)
+        log(f"[bench] adjoint step {i}: forward {1e3 * tf[-1]:.1f} ms, backward {1e3 * tb[-1]:.1f} ms, "
+            f"loss {0.5 * float(((img - target) ** 2).sum()):.4e}, |grad| {float(np.linalg.norm(g)):.4e}")
+    paths = spec.width * spec.height * spec.max_samples
+    mb = sum(tb) / len(tb)
+    mf = sum(tf) / len(tf)
+    line = {
+        "metric": "adjoint backward Mpaths/s @512px/100k tris",
+        "value": round(paths / mb / 1e6, 3),
+        "unit": "Mpaths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * (mb + mf), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": spec.name, "width": spec.width, "height": spec.height, "spp": spec.max_samples,
+                   "bounces": spec.bounces, "triangles": spec.triangle_count, "params": adj.param_count},
+        "forward_ms": round(1e3 * mf, 3),
+        "backward_ms": round(1e3 * mb, 3),
+        "backward_over_forward": round(mb / mf, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,9 +122,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="image scale (1.0 = 1920x1080)")
     ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default 64; 4 for --workload fog)")
-    ap.add_argument("--workload", choices=["plain", "fog"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "fog", "adjoint"], default="plain",
                     help="plain: the 1080p / 1M-triangle forward frame; fog: the same scene inside a 128^3 fp16 "
-                         "density grid (delta tracking)")
+                         "density grid (delta tracking); adjoint: forward + backward at 512x512, 16 spp, 100k triangles")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--light-sampler", choices=["cdf", "alias"], default="cdf",
                     help="environment light sampling: the reference's row/column CDFs or the alias table")
@@ -96,6 +152,8 @@ def main():
     t0 = time.perf_counter()
     if args.workload == "fog":
         spec = scenes.config_fog(scale=args.scale, spp=args.spp or 4)
+    elif args.workload == "adjoint":
+        spec = scenes.config_adjoint(scale=args.scale, spp=args.spp or 16)
     else:
         spec = scenes.config_plain_gpu(scale=args.scale, spp=args.spp or 64)
     log(f"[bench] scene spec: {spec.triangle_count} triangles, {spec.width}x{spec.height}, "
@@ -104,6 +162,9 @@ def main():
     scene, view = scenes.instantiate(spec)
     scene.build()
     log(f"[bench] BVH build + upload {time.perf_counter() - t0:.1f} s")
+    if args.workload == "adjoint":
+        bench_adjoint(args, spec, scene, view)
+        return
 
     from sharpray_amd import LIGHT_ALIAS, LIGHT_CDF, Renderer
 
