@@ -115,6 +115,40 @@ def bench_adjoint(args, spec, scene, view):
     print(json.dumps(line), flush=True)
 
 
+def bench_cnn(args, spec, scene, view):
+    """The kernel-predicting denoiser on device-resident AOVs of one render (random He-initialised
+    weights: the cost does not depend on their values). value = frames per second."""
+    from sharpray_amd import KpnDenoiser, Renderer
+    r = Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
+                 spec.variance_threshold, spec.bounces, spec.indirect_clamp)
+    r.wait()
+    den = KpnDenoiser(scene)
+    den.load(KpnDenoiser.random_weights(0))
+    den.benchmark(r, max(1, args.warmup))
+    ms = den.benchmark(r, max(1, args.steps))
+    flops = KpnDenoiser.flops(spec.width, spec.height)
+    tflops = flops / (ms * 1e-3) / 1e12
+    line = {
+        "metric": "kpn denoiser frames/s @1080p",
+        "value": round(1e3 / ms, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded procedural scene, random weights)",
+        "config": {"workload": "kpn_unet_1080p", "width": spec.width, "height": spec.height,
+                   "input_spp": spec.max_samples, "gflop_per_frame": round(flops / 1e9, 2)},
+        "roofline": {"kernel": "k_conv3x3 (bf16 MFMA implicit GEMM)", "bound": "mfma", "achieved": round(tflops, 2),
+                     "peak": 2500.0, "unit": "TFLOP/s", "frac": round(tflops / 2500.0, 4), "traffic": None},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,9 +156,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="image scale (1.0 = 1920x1080)")
     ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default 64; 4 for --workload fog)")
-    ap.add_argument("--workload", choices=["plain", "fog", "adjoint"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "fog", "adjoint", "cnn"], default="plain",
                     help="plain: the 1080p / 1M-triangle forward frame; fog: the same scene inside a 128^3 fp16 "
-                         "density grid (delta tracking); adjoint: forward + backward at 512x512, 16 spp, 100k triangles")
+                         "density grid (delta tracking); adjoint: forward + backward at 512x512, 16 spp, 100k triangles; "
+                         "cnn: the U-Net denoiser on a 1080p 4 spp render")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--light-sampler", choices=["cdf", "alias"], default="cdf",
                     help="environment light sampling: the reference's row/column CDFs or the alias table")
@@ -154,6 +189,8 @@ def main():
         spec = scenes.config_fog(scale=args.scale, spp=args.spp or 4)
     elif args.workload == "adjoint":
         spec = scenes.config_adjoint(scale=args.scale, spp=args.spp or 16)
+    elif args.workload == "cnn":
+        spec = scenes.config_plain_gpu(scale=args.scale, spp=args.spp or 4)
     else:
         spec = scenes.config_plain_gpu(scale=args.scale, spp=args.spp or 64)
     log(f"[bench] scene spec: {spec.triangle_count} triangles, {spec.width}x{spec.height}, "
@@ -164,6 +201,9 @@ def main():
     log(f"[bench] BVH build + upload {time.perf_counter() - t0:.1f} s")
     if args.workload == "adjoint":
         bench_adjoint(args, spec, scene, view)
+        return
+    if args.workload == "cnn":
+        bench_cnn(args, spec, scene, view)
         return
 
     from sharpray_amd import LIGHT_ALIAS, LIGHT_CDF, Renderer

@@ -257,6 +257,30 @@ int sr_adjoint_forward(sr_ctx* ctx, const sr_view* view, const sr_render_params*
 /* grad (param_count floats) = d sum(d_image * Radiance) / d params; d_image is w*h*3 (host). */
 int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad);
 
+/* ---- Kernel-predicting denoiser (no counterpart in the reference) ----------------------------
+ * A 5-level U-Net (32/48/64/96/128 channels, two 3x3 conv + ReLU per level, 2x2 average pooling,
+ * nearest upsampling with skip concatenation) on bf16 MFMA with fp32 accumulation. Inputs per
+ * pixel: log(1 + radiance), albedo, normal, 1 / (1 + depth) (0 without a surface); the last conv
+ * predicts 25 logits whose softmax weights the pixel's 5x5 neighbourhood of the noisy radiance
+ * (edge-clamped). Weights: float blob, per layer [cout][cin][3][3] then bias[cout], layers in
+ * the order of sharpray_amd.cnn_layers(); concatenated inputs list the upsampled channels first. */
+int sr_cnn_weight_count(uint64_t* count);
+int sr_cnn_load(sr_ctx* ctx, const float* weights, uint64_t count);
+/* Denoises the last render (radiance, albedo, normal, depth AOVs) into out_rgb (host, w*h*3). */
+int sr_cnn_denoise(sr_ctx* ctx, float* out_rgb);
+/* Same on caller device arrays (rad/alb/nrm: w*h*3, dep: w*h), out device w*h*3; asynchronous on
+ * the context's stream. */
+int sr_cnn_denoise_device(sr_ctx* ctx, const float* d_radiance, const float* d_albedo, const float* d_normals,
+                          const float* d_depth, uint32_t width, uint32_t height, float* d_out);
+/* Activations of a layer after the last denoise (layer -1: the input features) as floats,
+ * [(y * width + x) * channels + c]; out may be null to query the sizes. */
+int sr_cnn_layer_read(sr_ctx* ctx, int32_t layer, float* out, uint32_t* channels, uint32_t* width, uint32_t* height);
+/* Milliseconds per denoise of the last render, averaged over `iterations` back-to-back device runs
+ * (HIP events on the context's stream). */
+int sr_cnn_benchmark(sr_ctx* ctx, uint32_t iterations, double* ms_per_run);
+/* Floating-point operations of one denoise at width x height (2 per multiply-add). */
+int sr_cnn_flops(uint32_t width, uint32_t height, double* flops);
+
 /* ---- Host data preparation (the C# host computes these in Triangle's constructors) ---------- */
 /* Triangle ctors (math.cs:1503-1531) and Mesh.ComputeSmoothNormals (mesh.cs:99-123): pos n*9,
  * normals n*9 or null, uv n*6 or null (null = the OBJ defaults (0,0),(1,0),(0,1); obj.cs:417-427).

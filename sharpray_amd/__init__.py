@@ -533,6 +533,80 @@ class Adjoint:
         return g
 
 
+# Kernel-predicting denoiser topology (mirrors sr_cnn.hip): (cinA, cinB, cout, mode, srcA, srcB, level,
+# relu); mode 0 same resolution, 1 2x2 average pool of srcA, 2 concat(nearest-up(srcA), srcB); -1 = the
+# 10 input features.
+CNN_LAYERS = [
+    (10, 0, 32, 0, -1, -1, 0, True), (32, 0, 32, 0, 0, -1, 0, True),
+    (32, 0, 48, 1, 1, -1, 1, True), (48, 0, 48, 0, 2, -1, 1, True),
+    (48, 0, 64, 1, 3, -1, 2, True), (64, 0, 64, 0, 4, -1, 2, True),
+    (64, 0, 96, 1, 5, -1, 3, True), (96, 0, 96, 0, 6, -1, 3, True),
+    (96, 0, 128, 1, 7, -1, 4, True), (128, 0, 128, 0, 8, -1, 4, True),
+    (128, 96, 96, 2, 9, 7, 3, True), (96, 0, 96, 0, 10, -1, 3, True),
+    (96, 64, 64, 2, 11, 5, 2, True), (64, 0, 64, 0, 12, -1, 2, True),
+    (64, 48, 48, 2, 13, 3, 1, True), (48, 0, 48, 0, 14, -1, 1, True),
+    (48, 32, 32, 2, 15, 1, 0, True), (32, 0, 32, 0, 16, -1, 0, True),
+    (32, 0, 25, 0, 17, -1, 0, False),
+]
+
+
+def cnn_weight_count():
+    n = ctypes.c_uint64()
+    check(N.lib().sr_cnn_weight_count(ctypes.byref(n)), "cnn weight count")
+    return n.value
+
+
+class KpnDenoiser:
+    """Kernel-predicting U-Net denoiser of a scene's renders (no reference counterpart; the
+    reference's Compositor.Filter is the bilateral alternative)."""
+
+    def __init__(self, scene):
+        self.scene = scene
+
+    @staticmethod
+    def random_weights(seed=0):
+        """He-initialised weights (zero biases) in the blob layout sr_cnn_load expects."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for cin_a, cin_b, cout, *_ in CNN_LAYERS:
+            cin = cin_a + cin_b
+            parts.append(rng.normal(0.0, np.sqrt(2.0 / (9 * cin)), size=cout * cin * 9).astype(np.float32))
+            parts.append(np.zeros(cout, np.float32))
+        return np.concatenate(parts)
+
+    def load(self, weights):
+        w = np.ascontiguousarray(weights, np.float32)
+        check(N.lib().sr_cnn_load(self.scene.ctx, fptr(w), w.size), "cnn load")
+
+    def denoise(self, renderer):
+        out = np.empty((renderer.width * renderer.height, 3), np.float32)
+        check(N.lib().sr_cnn_denoise(renderer.scene.ctx, fptr(out)), "cnn denoise")
+        return out
+
+    def layer(self, index):
+        """Activations of layer `index` (-1: input features) of the last denoise, shape (h, w, c)."""
+        c, w, h = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        L = N.lib()
+        check(L.sr_cnn_layer_read(self.scene.ctx, index, None, ctypes.byref(c), ctypes.byref(w), ctypes.byref(h)),
+              "cnn layer")
+        out = np.empty((h.value, w.value, c.value), np.float32)
+        check(L.sr_cnn_layer_read(self.scene.ctx, index, fptr(out), ctypes.byref(c), ctypes.byref(w), ctypes.byref(h)),
+              "cnn layer")
+        return out
+
+    def benchmark(self, renderer, iterations=10):
+        """Milliseconds per device-resident denoise of the renderer's AOVs."""
+        ms = ctypes.c_double()
+        check(N.lib().sr_cnn_benchmark(renderer.scene.ctx, iterations, ctypes.byref(ms)), "cnn benchmark")
+        return ms.value
+
+    @staticmethod
+    def flops(width, height):
+        f = ctypes.c_double()
+        check(N.lib().sr_cnn_flops(width, height, ctypes.byref(f)), "cnn flops")
+        return f.value
+
+
 class Compositor:
     """Compositor (compose.cs:7-267): the joint bilateral denoise of Filter (compose.cs:101-204)."""
 

@@ -123,6 +123,14 @@ SIGNATURES = {
     "sr_adjoint_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(View_), ctypes.POINTER(RenderParams_),
                                           c_float_p]),
     "sr_adjoint_backward": (ctypes.c_int, [ctypes.c_void_p, c_float_p, c_float_p]),
+    "sr_cnn_weight_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_uint64)]),
+    "sr_cnn_load": (ctypes.c_int, [ctypes.c_void_p, c_float_p, ctypes.c_uint64]),
+    "sr_cnn_denoise": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "sr_cnn_denoise_device": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 4 + [ctypes.c_uint32, ctypes.c_uint32,
+                                                                                       ctypes.c_void_p]),
+    "sr_cnn_layer_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_uint32_p, c_uint32_p, c_uint32_p]),
+    "sr_cnn_benchmark": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    "sr_cnn_flops": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "sr_triangles_build": (ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_uint32, ctypes.c_int32, c_float_p]),
 }
 

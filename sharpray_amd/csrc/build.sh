@@ -16,6 +16,7 @@ tmp="$(mktemp -d)"
 trap 'rm -rf "${tmp}"' EXIT
 "${HIPCC}" "${COMMON[@]}" -x hip --offload-host-only -c "${here}/sr_host.cpp" -o "${tmp}/sr_host.o" -fno-builtin-powf
 "${HIPCC}" "${COMMON[@]}" "${DEVICE[@]}" -c "${here}/sr_render.hip" -o "${tmp}/sr_render.o"
-"${HIPCC}" -shared --offload-arch="${ARCH}" "${tmp}/sr_host.o" "${tmp}/sr_render.o" -o "${out}.tmp"
+"${HIPCC}" "${COMMON[@]}" "${DEVICE[@]}" -c "${here}/sr_cnn.hip" -o "${tmp}/sr_cnn.o"
+"${HIPCC}" -shared --offload-arch="${ARCH}" "${tmp}/sr_host.o" "${tmp}/sr_render.o" "${tmp}/sr_cnn.o" -o "${out}.tmp"
 mv "${out}.tmp" "${out}"
 echo "built ${out}"

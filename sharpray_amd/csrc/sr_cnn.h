@@ -1,0 +1,61 @@
+// Kernel-predicting U-Net denoiser (no reference counterpart; the reference denoises with the
+// joint bilateral Compositor.Filter, compose.cs:101-204, which stays available beside it).
+//
+// Input per pixel: log(1 + radiance) (3), albedo (3), normal (3), 1 / (1 + depth) (0 where no
+// surface) - 10 channels. Five levels with 32 / 48 / 64 / 96 / 128 channels: two 3x3 conv + ReLU
+// per level, 2x2 average pooling down, nearest upsampling and skip concatenation up, and a last
+// 3x3 conv to 25 logits whose softmax weights the 5x5 neighbourhood of the noisy radiance.
+// Convolutions run as implicit GEMMs on bf16 MFMA (v_mfma_f32_16x16x32_bf16) with fp32
+// accumulation; activations are bf16, NHWC with the channel stride padded to a multiple of 32.
+#pragma once
+
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+namespace sr {
+
+struct CnnLayerSpec {
+    int cinA, cinB;  // true input channels (cinB: skip channels of a concatenation)
+    int cout;
+    int mode;        // 0 same resolution, 1 2x2 average pool of the source, 2 concat(up(source), skip)
+    int srcA, srcB;  // source activations: -1 = input features, else a layer index
+    int level;       // resolution level of the output (0 = full)
+    bool relu;
+};
+
+// The fixed topology (19 conv layers).
+const std::vector<CnnLayerSpec>& cnn_layers();
+// Floats in a weight blob: per layer, weights [cout][cinA + cinB][3][3] then bias [cout].
+uint64_t cnn_weight_count();
+
+class CnnNet {
+public:
+    ~CnnNet();
+    // Converts a float weight blob to the padded bf16 device layout. Returns false on a HIP error.
+    bool load(const float* weights, hipStream_t s);
+    bool loaded() const { return loaded_; }
+    // Denoises W x H: rad/alb/nrm (3 floats per pixel), dep (1 float, +inf = no surface), out rgb.
+    // All pointers are device memory. Returns false on a HIP error.
+    bool run(const float* rad, const float* alb, const float* nrm, const float* dep, uint32_t W, uint32_t H,
+             float* out, hipStream_t s);
+    // Activations of `layer` from the last run (layer -1: input features) as float, true channels
+    // only: out[(y * w + x) * c + k]. Returns false if unavailable.
+    bool read_layer(int layer, std::vector<float>& out, uint32_t& c, uint32_t& w, uint32_t& h) const;
+    // Algorithmic floating-point operations of one run at W x H (2 per multiply-add, true channels).
+    static double flops(uint32_t W, uint32_t H);
+
+private:
+    bool ensure(uint32_t W, uint32_t H);
+    void release_acts();
+    bool loaded_ = false;
+    std::vector<uint16_t*> w_;  // per layer [9][coutPad][cinPad] bf16
+    std::vector<float*> b_;     // per layer [coutPad]
+    uint16_t* feat_ = nullptr;  // input features, stride 32
+    std::vector<uint16_t*> act_;
+    uint32_t W_ = 0, H_ = 0;
+};
+
+}  // namespace sr

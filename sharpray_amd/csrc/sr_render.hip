@@ -30,6 +30,7 @@
 #include "sr_host.h"
 #include "sr_trace.h"
 #include "sr_adjoint.h"
+#include "sr_cnn.h"
 
 using namespace sr;
 
@@ -776,6 +777,7 @@ struct sr_ctx {
     sr_view adjView{};
     sr_render_params adjParams{};
     DevBuf<float> dAdjImage, dAdjGrad;
+    sr::CnnNet* cnn = nullptr;  // kernel-predicting denoiser (weights loaded by sr_cnn_load)
     DevBuf<int32_t> dSlotTri, dSceneItems;
     DevBuf<DevObject> dObjects;
     DevBuf<DevTexture> dTextures;
@@ -827,6 +829,8 @@ struct sr_ctx {
         dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release(); dAlias.release(); dFogGrid.release();
         dSlotTri.release(); dSceneItems.release(); dObjects.release(); dTextures.release();
         dAdjImage.release(); dAdjGrad.release();
+        delete cnn;
+        cnn = nullptr;
         dRayO.release(); dRayD.release(); dHitA.release(); dHitSlot.release();
         dPaths.release(); dShadows.release(); dAcc.release();
         dQueue0.release(); dQueue1.release(); dCounts.release(); dCtr.release();
@@ -1897,6 +1901,102 @@ int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad) {
             }
         }
     }
+    return SR_OK;
+}
+
+// ---- Kernel-predicting denoiser --------------------------------------------------------------------
+int sr_cnn_weight_count(uint64_t* count) {
+    if (!count) return fail(SR_ERR_INVALID, "cnn: null argument");
+    *count = sr::cnn_weight_count();
+    return SR_OK;
+}
+
+int sr_cnn_load(sr_ctx* ctx, const float* weights, uint64_t count) {
+    if (!ctx || !weights) return fail(SR_ERR_INVALID, "cnn: null argument");
+    if (count != sr::cnn_weight_count()) return fail(SR_ERR_INVALID, "cnn: weight count mismatch");
+    if (ctx->device < 0) return fail(SR_ERR_STATE, "host-only context (device -1) cannot denoise");
+    for (uint64_t i = 0; i < count; ++i)
+        if (!std::isfinite(weights[i])) return fail(SR_ERR_INVALID, "cnn: non-finite weight");
+    ctx->join();
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->cnn) ctx->cnn = new sr::CnnNet();
+    if (!ctx->cnn->load(weights, ctx->stream)) return fail(SR_ERR_HIP, "cnn: weight upload failed");
+    return SR_OK;
+}
+
+int sr_cnn_denoise_device(sr_ctx* ctx, const float* rad, const float* alb, const float* nrm, const float* dep,
+                          uint32_t W, uint32_t H, float* out) {
+    if (!ctx || !rad || !alb || !nrm || !dep || !out || W == 0 || H == 0) return fail(SR_ERR_INVALID, "cnn: bad arguments");
+    if (!ctx->cnn || !ctx->cnn->loaded()) return fail(SR_ERR_STATE, "cnn: no weights loaded");
+    if (W > 32768 || H > 32768) return fail(SR_ERR_LIMIT, "cnn: image larger than 32768");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->cnn->run(rad, alb, nrm, dep, W, H, out, ctx->stream)) return fail(SR_ERR_HIP, "cnn: launch failed");
+    return SR_OK;
+}
+
+int sr_cnn_denoise(sr_ctx* ctx, float* out_rgb) {
+    if (!ctx || !out_rgb) return fail(SR_ERR_INVALID, "cnn: null argument");
+    if (!ctx->rendering) return fail(SR_ERR_STATE, "no render to denoise");
+    int rc = wait_render(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t W = ctx->params.width, H = ctx->params.height;
+    float* dOut = nullptr;
+    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 12));
+    rc = sr_cnn_denoise_device(ctx, ctx->dRad.ptr, ctx->dAlbedo.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, W, H, dOut);
+    hipError_t e = rc == SR_OK ? hipStreamSynchronize(ctx->stream) : hipSuccess;
+    if (rc == SR_OK && e == hipSuccess) e = hipMemcpy(out_rgb, dOut, (size_t)W * H * 12, hipMemcpyDeviceToHost);
+    (void)hipFree(dOut);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SR_ERR_HIP, hipGetErrorString(e));
+    return SR_OK;
+}
+
+int sr_cnn_layer_read(sr_ctx* ctx, int32_t layer, float* out, uint32_t* channels, uint32_t* width, uint32_t* height) {
+    if (!ctx || !channels || !width || !height) return fail(SR_ERR_INVALID, "cnn: null argument");
+    if (!ctx->cnn) return fail(SR_ERR_STATE, "cnn: no weights loaded");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<float> v;
+    if (!ctx->cnn->read_layer(layer, v, *channels, *width, *height)) return fail(SR_ERR_STATE, "cnn: no such layer or no run yet");
+    if (out) memcpy(out, v.data(), v.size() * sizeof(float));
+    return SR_OK;
+}
+
+int sr_cnn_benchmark(sr_ctx* ctx, uint32_t iterations, double* ms_per_run) {
+    if (!ctx || !ms_per_run || iterations == 0) return fail(SR_ERR_INVALID, "cnn benchmark: bad arguments");
+    if (!ctx->rendering) return fail(SR_ERR_STATE, "no render to denoise");
+    int rc = wait_render(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t W = ctx->params.width, H = ctx->params.height;
+    float* dOut = nullptr;
+    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 12));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // One untimed run sizes the activations; the timed runs are back to back on the stream.
+    if (e == hipSuccess)
+        rc = sr_cnn_denoise_device(ctx, ctx->dRad.ptr, ctx->dAlbedo.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, W, H, dOut);
+    if (e == hipSuccess && rc == SR_OK) e = hipEventRecord(e0, ctx->stream);
+    for (uint32_t i = 0; e == hipSuccess && rc == SR_OK && i < iterations; ++i)
+        rc = sr_cnn_denoise_device(ctx, ctx->dRad.ptr, ctx->dAlbedo.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, W, H, dOut);
+    if (e == hipSuccess && rc == SR_OK) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess && rc == SR_OK) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess && rc == SR_OK) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(dOut);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SR_ERR_HIP, hipGetErrorString(e));
+    *ms_per_run = (double)ms / iterations;
+    return SR_OK;
+}
+
+int sr_cnn_flops(uint32_t width, uint32_t height, double* flops) {
+    if (!flops) return fail(SR_ERR_INVALID, "cnn: null argument");
+    *flops = sr::CnnNet::flops(width, height);
     return SR_OK;
 }
 

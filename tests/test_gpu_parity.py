@@ -297,3 +297,40 @@ def test_adjoint_albedo_gradients_match_gpu_finite_differences():
     adj.set_params(p0)
     print(f"adjoint albedo vs GPU finite differences: worst rel err {worst:.2e}", flush=True)
     assert worst <= 1e-2
+
+
+def test_kpn_denoiser_layers_match_cpu_restatement():
+    """The bf16-MFMA U-Net against the float32 restatement (oracle/cnn_ref.py): every layer
+    recomputed on the CPU from the GPU's own input activations agrees to one bf16 rounding step;
+    the whole network end to end stays within 2 % of the radiance scale."""
+    from oracle import cnn_ref as C
+    from sharpray_amd import KpnDenoiser, RNG_COUNTER, scenes
+    spec = scenes.config_plain_cpu(scale=0.16, spp=2)  # 41 x 41: odd sizes at every level
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=3)
+    r.wait()
+    den = KpnDenoiser(scene)
+    blob = KpnDenoiser.random_weights(17)
+    den.load(blob)
+    out = den.denoise(r)
+    H, W = spec.height, spec.width
+    rad = r.radiance.reshape(H, W, 3)
+    alb, nrm, dep = r.albedo.reshape(H, W, 3), r.normals.reshape(H, W, 3), r.depth.reshape(H, W)
+    feat = den.layer(-1)
+    ref_feat = C.features(rad, alb, nrm, dep)
+    assert np.all(np.abs(feat - ref_feat) <= np.abs(ref_feat) * 2.0 ** -7 + 1e-7)
+    wb = C.split_weights(blob)
+    gpu = {-1: feat}
+    worst = 0.0
+    for i in range(len(C.LAYERS)):
+        gpu[i] = den.layer(i)
+        ref = C.layer_forward(i, wb, gpu)
+        assert ref.shape == gpu[i].shape, (i, ref.shape, gpu[i].shape)
+        scale = max(float(np.abs(ref).max()), 1e-6)
+        err = np.abs(gpu[i] - ref)
+        tol = np.abs(ref) * 2.0 ** -7 + 1e-4 * scale
+        worst = max(worst, float((err / (np.abs(ref) + 1e-4 * scale)).max()))
+        assert np.all(err <= tol), (i, float(err.max()), scale)
+    acts, ref_out = C.run(blob, rad, alb, nrm, dep)
+    rel = np.abs(out.reshape(H, W, 3) - ref_out) / (np.abs(ref_out) + 1e-2)
+    print(f"kpn denoiser: worst per-layer rel diff {worst:.2e}, end-to-end max rel diff {rel.max():.2e}", flush=True)
+    assert rel.max() <= 2e-2

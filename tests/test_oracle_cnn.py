@@ -1,0 +1,70 @@
+"""Pins of the denoiser's CPU restatement (oracle/cnn_ref.py) against direct loops and closed forms."""
+import numpy as np
+
+from oracle import cnn_ref as C
+
+
+def test_bf16_rounding_known_values():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.0e-3, 65504.0], np.float32)
+    expect = np.array([1.0, 1.0, 1.0 + 2 ** -6, -2.5, np.float32(3.0e-3), 65536.0], np.float32)
+    got = C.bf16(x)
+    assert got[0] == 1.0 and got[1] == 1.0 and got[2] == np.float32(1.0 + 2 ** -6) and got[3] == -2.5
+    assert abs(got[4] - 3.0e-3) <= 3.0e-3 * 2 ** -8 and got[5] == expect[5]
+
+
+def test_conv3x3_matches_direct_loops():
+    rng = np.random.default_rng(0)
+    h, w, cin, cout = 5, 7, 6, 4
+    x = C.bf16(rng.normal(size=(h, w, cin)).astype(np.float32))
+    wt = C.bf16(rng.normal(size=(cout, cin, 3, 3)).astype(np.float32))
+    b = rng.normal(size=cout).astype(np.float32)
+    got = C.conv3x3(x, wt, b, relu=True)
+    ref = np.zeros((h, w, cout))
+    for y in range(h):
+        for xx in range(w):
+            for co in range(cout):
+                s = float(b[co])
+                for ky in range(3):
+                    for kx in range(3):
+                        yy, xk = y + ky - 1, xx + kx - 1
+                        if 0 <= yy < h and 0 <= xk < w:
+                            s += float(np.dot(x[yy, xk].astype(np.float64), wt[co, :, ky, kx].astype(np.float64)))
+                ref[y, xx, co] = max(s, 0.0)
+    ulp = np.maximum(np.abs(ref), 1e-30) * 2.0 ** -7
+    assert np.all(np.abs(got - ref) <= ulp + 1e-6)
+
+
+def test_pool_and_up():
+    x = C.bf16(np.arange(5 * 3 * 2, dtype=np.float32).reshape(5, 3, 2))
+    p = C.pool(x)
+    assert p.shape == (3, 2, 2)
+    # (0,0): mean of rows 0-1, cols 0-1; (2,1): last row and column repeated.
+    assert np.allclose(p[0, 0], (x[0, 0] + x[0, 1] + x[1, 0] + x[1, 1]) / 4)
+    assert np.allclose(p[2, 1], x[4, 2], rtol=2 ** -7)
+    u = C.up(p, 5, 3)
+    assert u.shape == (5, 3, 2) and np.array_equal(u[4, 2], p[2, 1]) and np.array_equal(u[1, 1], p[0, 0])
+
+
+def test_kpn_apply_identity_and_box():
+    rng = np.random.default_rng(1)
+    rad = rng.uniform(0, 3, (6, 7, 3)).astype(np.float32)
+    logits = np.full((6, 7, 32), -30.0, np.float32)
+    logits[..., 12] = 30.0
+    assert np.allclose(C.kpn_apply(logits, rad), rad, rtol=1e-6)
+    box = C.kpn_apply(np.zeros((6, 7, 32), np.float32), rad)
+    ref = np.zeros_like(rad)
+    for y in range(6):
+        for x in range(7):
+            acc = 0
+            for dy in range(-2, 3):
+                for dx in range(-2, 3):
+                    acc = acc + rad[min(max(y + dy, 0), 5), min(max(x + dx, 0), 6)]
+            ref[y, x] = acc / 25
+    assert np.allclose(box, ref, rtol=1e-5)
+
+
+def test_weight_count_matches_library():
+    import sharpray_amd as A
+    n = sum(co * (ca + cb) * 9 + co for ca, cb, co, *_ in C.LAYERS)
+    assert n == A.cnn_weight_count() == A.KpnDenoiser.random_weights(0).size
+    assert [tuple(l) for l in C.LAYERS] == [tuple(l) for l in A.CNN_LAYERS]
