@@ -273,11 +273,25 @@ int sr_cnn_denoise(sr_ctx* ctx, float* out_rgb);
 int sr_cnn_denoise_device(sr_ctx* ctx, const float* d_radiance, const float* d_albedo, const float* d_normals,
                           const float* d_depth, uint32_t width, uint32_t height, float* d_out);
 /* Activations of a layer after the last denoise (layer -1: the input features) as floats,
- * [(y * width + x) * channels + c]; out may be null to query the sizes. */
+ * [(y * width + x) * channels + c]; out may be null to query the sizes. Layer 1000 + L reads
+ * layer L's output gradient (after its ReLU) from the last training step. */
 int sr_cnn_layer_read(sr_ctx* ctx, int32_t layer, float* out, uint32_t* channels, uint32_t* width, uint32_t* height);
 /* Milliseconds per denoise of the last render, averaged over `iterations` back-to-back device runs
  * (HIP events on the context's stream). */
 int sr_cnn_benchmark(sr_ctx* ctx, uint32_t iterations, double* ms_per_run);
+/* Training with hand-written kernels: Adam on fp32 master weights starting from the loaded ones;
+ * loss mean (out - target)^2 / (target^2 + 0.01) of the denoised radiance against a reference.
+ * sr_cnn_train_add keeps one example (noisy AOVs + target radiance, host arrays) in device
+ * memory; sr_cnn_train_run takes `steps` steps over the examples in a seeded shuffled order and
+ * returns their mean loss; sr_cnn_train_export writes the weights (sr_cnn_weight_count floats);
+ * the context's denoiser uses the trained weights meanwhile. */
+int sr_cnn_train_begin(sr_ctx* ctx);
+int sr_cnn_train_add(sr_ctx* ctx, const float* radiance, const float* albedo, const float* normals, const float* depth,
+                     const float* target, uint32_t width, uint32_t height);
+int sr_cnn_train_run(sr_ctx* ctx, uint32_t steps, float learning_rate, uint32_t seed, float* mean_loss);
+int sr_cnn_train_export(sr_ctx* ctx, float* weights);
+/* Loss and gradient (blob layout) of one stored example, without an update. */
+int sr_cnn_train_gradient(sr_ctx* ctx, uint32_t example, float* gradient, float* loss);
 /* Floating-point operations of one denoise at width x height (2 per multiply-add). */
 int sr_cnn_flops(uint32_t width, uint32_t height, double* flops);
 

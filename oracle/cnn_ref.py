@@ -36,41 +36,51 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32)
 
 
-def split_weights(blob):
-    """[(w (cout, cin, 3, 3), b (cout,))] per layer, weights rounded to bf16."""
+def exact(x):
+    """No rounding: the float64 mode used to pin the backward pass by finite differences."""
+    return np.asarray(x, np.float64)
+
+
+def split_weights(blob, rnd=bf16):
+    """[(w (cout, cin, 3, 3), b (cout,))] per layer, weights rounded by rnd (bf16 by default)."""
     out, k = [], 0
     for cin_a, cin_b, cout, *_ in LAYERS:
         cin = cin_a + cin_b
-        w = np.asarray(blob[k:k + cout * cin * 9], np.float32).reshape(cout, cin, 3, 3)
+        w = np.asarray(blob[k:k + cout * cin * 9]).reshape(cout, cin, 3, 3)
         k += cout * cin * 9
-        b = np.asarray(blob[k:k + cout], np.float32)
+        b = np.asarray(blob[k:k + cout])
         k += cout
-        out.append((bf16(w), b))
+        out.append((rnd(w), b.astype(np.float32) if rnd is bf16 else exact(b)))
     assert k == len(blob)
     return out
 
 
-def features(rad, alb, nrm, dep):
+def features(rad, alb, nrm, dep, rnd=bf16):
     """(h, w, 10) input features from (h, w, 3) radiance, albedo, normals and (h, w) depth."""
     f = np.concatenate([np.log1p(np.maximum(rad, 0.0)), alb, nrm,
                         np.where(np.isinf(dep), 0.0, 1.0 / (1.0 + dep))[..., None]], axis=-1)
-    return bf16(f.astype(np.float32))
+    return rnd(f.astype(np.float32) if rnd is bf16 else f)
 
 
-def conv3x3(x, w, b, relu):
+def _cols(x):
+    h, wd, cin = x.shape
+    xp = np.zeros((h + 2, wd + 2, cin), x.dtype)
+    xp[1:-1, 1:-1] = x
+    return np.concatenate([xp[ky:ky + h, kx:kx + wd] for ky in range(3) for kx in range(3)], axis=-1)
+
+
+def conv3x3(x, w, b, relu, rnd=bf16):
     """Zero-padded 3x3 cross-correlation of x (h, w, cin) with w (cout, cin, 3, 3), float32 sums."""
     h, wd, cin = x.shape
-    xp = np.zeros((h + 2, wd + 2, cin), np.float32)
-    xp[1:-1, 1:-1] = x
-    cols = np.concatenate([xp[ky:ky + h, kx:kx + wd] for ky in range(3) for kx in range(3)], axis=-1)
     wm = w.transpose(2, 3, 1, 0).reshape(9 * cin, -1)  # (tap, cin) x cout
-    y = cols.reshape(h * wd, 9 * cin) @ wm + b
+    y = _cols(x).reshape(h * wd, 9 * cin) @ wm + b
     if relu:
         y = np.maximum(y, 0.0)
-    return bf16(y.reshape(h, wd, -1).astype(np.float32))
+    y = y.reshape(h, wd, -1)
+    return rnd(y.astype(np.float32) if rnd is bf16 else y)
 
 
-def pool(x):
+def pool(x, rnd=bf16):
     h, wd, _ = x.shape
     h2, w2 = (h + 1) // 2, (wd + 1) // 2
     y0 = np.arange(h2) * 2
@@ -78,34 +88,38 @@ def pool(x):
     y1 = np.minimum(y0 + 1, h - 1)
     x1 = np.minimum(x0 + 1, wd - 1)
     s = ((x[y0][:, x0] + x[y0][:, x1]) + x[y1][:, x0]) + x[y1][:, x1]
-    return bf16((s * np.float32(0.25)).astype(np.float32))
+    return rnd((s * np.float32(0.25)).astype(np.float32) if rnd is bf16 else s * 0.25)
 
 
 def up(x, h, wd):
     return x[np.arange(h) // 2][:, np.arange(wd) // 2]
 
 
-def layer_forward(i, wb, inputs):
-    """Layer i from its input activations (inputs: dict index -> (h, w, c); -1 = features)."""
+def staged_input(i, inputs, rnd=bf16):
+    """The input of layer i's convolution (after pooling or upsampling + concatenation)."""
     cin_a, cin_b, cout, mode, sa, sb, level, relu = LAYERS[i]
     a = inputs[sa]
     if mode == 0:
-        x = a
-    elif mode == 1:
-        x = pool(a)
-    else:
-        skip = inputs[sb]
-        x = np.concatenate([up(a, skip.shape[0], skip.shape[1]), skip], axis=-1)
+        return a
+    if mode == 1:
+        return pool(a, rnd)
+    skip = inputs[sb]
+    return np.concatenate([up(a, skip.shape[0], skip.shape[1]), skip], axis=-1)
+
+
+def layer_forward(i, wb, inputs, rnd=bf16):
+    """Layer i from its input activations (inputs: dict index -> (h, w, c); -1 = features)."""
     w, b = wb[i]
-    return conv3x3(x, w, b, relu)
+    return conv3x3(staged_input(i, inputs, rnd), w, b, LAYERS[i][7], rnd)
 
 
 def kpn_apply(logits, rad):
     h, wd, _ = rad.shape
-    l = logits[..., :25].astype(np.float32)
+    dt = np.float64 if logits.dtype == np.float64 else np.float32
+    l = logits[..., :25].astype(dt)
     e = np.exp(l - l.max(-1, keepdims=True))
-    s = np.zeros((h, wd), np.float32)
-    acc = np.zeros((h, wd, 3), np.float32)
+    s = np.zeros((h, wd), dt)
+    acc = np.zeros((h, wd, 3), dt)
     ys, xs = np.arange(h), np.arange(wd)
     for k in range(25):
         dy, dx = k // 5 - 2, k % 5 - 2
@@ -115,10 +129,87 @@ def kpn_apply(logits, rad):
     return acc / s[..., None]
 
 
-def run(blob, rad, alb, nrm, dep):
+def run(blob, rad, alb, nrm, dep, rnd=bf16):
     """Whole network: returns (list of layer activations, denoised radiance (h, w, 3))."""
-    wb = split_weights(blob)
-    acts = {-1: features(rad, alb, nrm, dep)}
+    wb = split_weights(blob, rnd)
+    acts = {-1: features(rad, alb, nrm, dep, rnd)}
     for i in range(len(LAYERS)):
-        acts[i] = layer_forward(i, wb, acts)
-    return acts, kpn_apply(acts[len(LAYERS) - 1], rad.astype(np.float32))
+        acts[i] = layer_forward(i, wb, acts, rnd)
+    return acts, kpn_apply(acts[len(LAYERS) - 1], rad.astype(np.float32) if rnd is bf16 else exact(rad))
+
+
+def loss(out, target):
+    """mean (out - t)^2 / (t^2 + 0.01) over pixels and channels."""
+    return float(np.mean((out - target) ** 2 / (target ** 2 + 0.01)))
+
+
+def backward(blob, rad, alb, nrm, dep, target, rnd=bf16):
+    """Loss and its gradient with respect to every layer's (w, b), by reverse-mode differentiation
+    of the network above. With rnd = bf16 the gradients are rounded where the GPU rounds them
+    (the logit gradient, every layer's masked output gradient and every data gradient)."""
+    wb = split_weights(blob, rnd)
+    acts = {-1: features(rad, alb, nrm, dep, rnd)}
+    staged = {}
+    for i in range(len(LAYERS)):
+        staged[i] = staged_input(i, acts, rnd)
+        w, b = wb[i]
+        acts[i] = conv3x3(staged[i], w, b, LAYERS[i][7], rnd)
+    r = rad.astype(np.float32) if rnd is bf16 else exact(rad)
+    h, wd, _ = r.shape
+    last = len(LAYERS) - 1
+    l = acts[last][..., :25].astype(np.float64)
+    e = np.exp(l - l.max(-1, keepdims=True))
+    wsoft = e / e.sum(-1, keepdims=True)
+    ys, xs = np.arange(h), np.arange(wd)
+    nb = [r[np.clip(ys + k // 5 - 2, 0, h - 1)][:, np.clip(xs + k % 5 - 2, 0, wd - 1)].astype(np.float64)
+          for k in range(25)]
+    out = sum(wsoft[..., k:k + 1] * nb[k] for k in range(25))
+    t = np.asarray(target, np.float64)
+    n = out.size
+    g = 2.0 * (out - t) / (t ** 2 + 0.01) / n
+    dwk = np.stack([(g * nb[k]).sum(-1) for k in range(25)], -1)
+    dl = wsoft * (dwk - (wsoft * dwk).sum(-1, keepdims=True))
+    gm = (lambda v: bf16(v.astype(np.float32))) if rnd is bf16 else exact
+    dact = {i: None for i in range(-1, len(LAYERS))}
+    dact[last] = gm(dl)
+    grads = [None] * len(LAYERS)
+    for i in range(last, -1, -1):
+        cin_a, cin_b, cout, mode, sa, sb, level, relu = LAYERS[i]
+        dy = dact[i].astype(np.float64)
+        if relu:
+            dy = dy * (acts[i] > 0)
+        dy = gm(dy).astype(np.float64)
+        x = staged[i].astype(np.float64)
+        hh, ww, cin = x.shape
+        dwm = _cols(x).reshape(hh * ww, 9 * cin).T @ dy.reshape(hh * ww, cout)  # (tap, cin) x cout
+        grads[i] = (dwm.reshape(3, 3, cin, cout).transpose(3, 2, 0, 1), dy.sum((0, 1)))
+        if i == 0:
+            break
+        w = wb[i][0].astype(np.float64)
+        wt = w[:, :, ::-1, ::-1].transpose(1, 0, 2, 3)  # flipped, (cin, cout, 3, 3)
+        dx = gm(conv3x3(dy, wt, np.zeros(cin), False, exact)).astype(np.float64)
+        def add(k, v):
+            dact[k] = v if dact[k] is None else dact[k] + v
+        if mode == 0:
+            add(sa, dx)
+        elif mode == 1:
+            fh, fw = acts[sa].shape[:2]
+            up_ = dx[np.arange(fh) // 2][:, np.arange(fw) // 2] * 0.25
+            if fw % 2:
+                up_[:, -1] *= 2.0
+            if fh % 2:
+                up_[-1] *= 2.0
+            add(sa, up_)
+        else:
+            lh, lw = acts[sa].shape[:2]
+            da = np.zeros((lh, lw, cin_a))
+            np.add.at(da, (np.arange(hh)[:, None] // 2, np.arange(ww)[None, :] // 2), dx[..., :cin_a])
+            add(sa, da)
+            add(sb, dx[..., cin_a:])
+    lossv = float(np.mean((out - t) ** 2 / (t ** 2 + 0.01)))
+    return lossv, grads
+
+
+def flatten(grads):
+    """Per-layer (dW, db) in the weight blob's order."""
+    return np.concatenate([np.concatenate([gw.reshape(-1), gb.reshape(-1)]) for gw, gb in grads])

@@ -594,6 +594,32 @@ class KpnDenoiser:
               "cnn layer")
         return out
 
+    # ---- training ------------------------------------------------------------------------------
+    def train_begin(self):
+        check(N.lib().sr_cnn_train_begin(self.scene.ctx), "cnn train begin")
+
+    def train_add(self, radiance, albedo, normals, depth, target, width, height):
+        a = [np.ascontiguousarray(x, np.float32) for x in (radiance, albedo, normals, depth, target)]
+        check(N.lib().sr_cnn_train_add(self.scene.ctx, *[fptr(x) for x in a], int(width), int(height)), "cnn train add")
+
+    def train(self, steps, learning_rate=1e-3, seed=0):
+        loss = ctypes.c_float()
+        check(N.lib().sr_cnn_train_run(self.scene.ctx, int(steps), float(learning_rate), int(seed), ctypes.byref(loss)),
+              "cnn train")
+        return loss.value
+
+    def gradient(self, example=0):
+        """(loss, gradient blob) of a stored example without an update."""
+        g = np.empty(cnn_weight_count(), np.float32)
+        loss = ctypes.c_float()
+        check(N.lib().sr_cnn_train_gradient(self.scene.ctx, int(example), fptr(g), ctypes.byref(loss)), "cnn gradient")
+        return loss.value, g
+
+    def export(self):
+        w = np.empty(cnn_weight_count(), np.float32)
+        check(N.lib().sr_cnn_train_export(self.scene.ctx, fptr(w)), "cnn export")
+        return w
+
     def benchmark(self, renderer, iterations=10):
         """Milliseconds per device-resident denoise of the renderer's AOVs."""
         ms = ctypes.c_double()

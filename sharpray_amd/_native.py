@@ -130,6 +130,11 @@ SIGNATURES = {
                                                                                        ctypes.c_void_p]),
     "sr_cnn_layer_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_uint32_p, c_uint32_p, c_uint32_p]),
     "sr_cnn_benchmark": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    "sr_cnn_train_begin": (ctypes.c_int, [ctypes.c_void_p]),
+    "sr_cnn_train_add": (ctypes.c_int, [ctypes.c_void_p] + [c_float_p] * 5 + [ctypes.c_uint32, ctypes.c_uint32]),
+    "sr_cnn_train_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint32, c_float_p]),
+    "sr_cnn_train_export": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),
+    "sr_cnn_train_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_float_p, c_float_p]),
     "sr_cnn_flops": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "sr_triangles_build": (ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_uint32, ctypes.c_int32, c_float_p]),
 }

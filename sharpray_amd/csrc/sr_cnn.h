@@ -26,6 +26,8 @@ struct CnnLayerSpec {
     bool relu;
 };
 
+// read_layer index of layer L's masked output gradient from the last training step.
+constexpr int kGradLayerBase = 1000;
 // The fixed topology (19 conv layers).
 const std::vector<CnnLayerSpec>& cnn_layers();
 // Floats in a weight blob: per layer, weights [cout][cinA + cinB][3][3] then bias [cout].
@@ -47,9 +49,39 @@ public:
     // Algorithmic floating-point operations of one run at W x H (2 per multiply-add, true channels).
     static double flops(uint32_t W, uint32_t H);
 
+    // ---- Training: Adam on fp32 master weights, bf16 copies refreshed after every step. ----------
+    // Starts from the loaded weights with zero Adam moments.
+    bool train_init(hipStream_t s);
+    bool training() const { return !mw_.empty(); }
+    // One step on one example (device arrays; target: w*h*3 radiance): forward, loss
+    // mean (out - target)^2 / (target^2 + 0.01) over pixels and channels (added to *lossDev),
+    // backward, Adam with learning rate lr.
+    bool train_step(const float* rad, const float* alb, const float* nrm, const float* dep, const float* target,
+                    uint32_t W, uint32_t H, float lr, float* lossDev, hipStream_t s, bool apply = true);
+    // Gradients left by a train_step with apply = false, in the blob layout; cleared afterwards.
+    bool export_grads(std::vector<float>& blob, hipStream_t s);
+    // Master weights in the blob layout of cnn_weight_count().
+    bool export_weights(std::vector<float>& blob) const;
+    bool export_blob(const std::vector<float*>& w, const std::vector<float*>& b, std::vector<float>& blob) const;
+
 private:
     bool ensure(uint32_t W, uint32_t H);
+    bool ensure_train(uint32_t W, uint32_t H);
     void release_acts();
+    void release_train_acts();
+    bool pack(hipStream_t s);
+    // Training state (per layer): fp32 master weights [9][coutPad][cinPad] and biases, their
+    // gradients and Adam moments, bf16 flipped-transposed weights [9][cinPad][coutPad] for the
+    // data gradient, fp32 activation gradients, bf16 masked output gradients.
+    std::vector<float*> mw_, mb_, gw_, gb_, m1w_, m2w_, m1b_, m2b_;
+    std::vector<uint16_t*> wt_;
+    std::vector<float*> dact_;
+    std::vector<uint16_t*> dyp_;
+    uint16_t* dx_ = nullptr;
+    float* zeros_ = nullptr;
+    float* out_ = nullptr;
+    uint32_t tW_ = 0, tH_ = 0;
+    int step_ = 0;
     bool loaded_ = false;
     std::vector<uint16_t*> w_;  // per layer [9][coutPad][cinPad] bf16
     std::vector<float*> b_;     // per layer [coutPad]

@@ -182,6 +182,205 @@ __global__ void k_kpn_apply(const uint16_t* logits, const float* rad, int W, int
     out[i * 3 + 2] = b / s;
 }
 
+// ---- Training kernels --------------------------------------------------------------------------------
+// Loss mean (out - t)^2 / (t^2 + 0.01) and its gradient through the 5x5 kernel application and
+// the softmax: g_c = dL/dout_c, dL/dw_k = sum_c g_c rad_k,c, dL/dlogit_k = w_k (dL/dw_k - sum_j w_j dL/dw_j).
+__global__ void k_kpn_loss_back(const uint16_t* logits, const float* rad, const float* target, int W, int H,
+                                float invN, uint16_t* dLogits, float* loss) {
+    __shared__ float part[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float lsum = 0.0f;
+    if (i < W * H) {
+        const int x = i % W, y = i / W;
+        float l[kTaps], w[kTaps];
+        float m = -INFINITY;
+        for (int k = 0; k < kTaps; ++k) {
+            l[k] = bf2f(logits[(size_t)i * 32 + k]);
+            m = fmaxf(m, l[k]);
+        }
+        float s = 0.0f;
+        for (int k = 0; k < kTaps; ++k) {
+            w[k] = expf(l[k] - m);
+            s += w[k];
+        }
+        float o[3] = {0.0f, 0.0f, 0.0f};
+        const float* nb[kTaps];
+        for (int k = 0; k < kTaps; ++k) {
+            w[k] /= s;
+            const int yy = min(max(y + k / 5 - 2, 0), H - 1), xx = min(max(x + k % 5 - 2, 0), W - 1);
+            nb[k] = rad + ((size_t)yy * W + xx) * 3;
+            for (int c = 0; c < 3; ++c) o[c] += w[k] * nb[k][c];
+        }
+        float g[3];
+        for (int c = 0; c < 3; ++c) {
+            const float t = target[(size_t)i * 3 + c], d = o[c] - t, den = t * t + 0.01f;
+            lsum += d * d / den * invN;
+            g[c] = 2.0f * d / den * invN;
+        }
+        float dw[kTaps], dot = 0.0f;
+        for (int k = 0; k < kTaps; ++k) {
+            dw[k] = g[0] * nb[k][0] + g[1] * nb[k][1] + g[2] * nb[k][2];
+            dot += w[k] * dw[k];
+        }
+        uint32_t packed[16];
+        for (int k = 0; k < 16; ++k) packed[k] = 0u;
+        for (int k = 0; k < kTaps; ++k) {
+            const uint32_t b = f2bf(w[k] * (dw[k] - dot));
+            packed[k >> 1] |= (k & 1) ? (b << 16) : b;
+        }
+        uint4* d = (uint4*)(dLogits + (size_t)i * 32);
+        for (int q = 0; q < 4; ++q) d[q] = make_uint4(packed[q * 4], packed[q * 4 + 1], packed[q * 4 + 2], packed[q * 4 + 3]);
+    }
+    part[threadIdx.x] = lsum;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, part[0]);
+}
+
+// Output gradient of a layer through its ReLU (kept where the bf16 activation is positive), as
+// bf16; the fp32 accumulator is cleared for the next step.
+__global__ void k_relu_back(float* dact, const uint16_t* act, size_t n, int relu, uint16_t* dy) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float g = dact[i];
+    if (relu && !(bf2f(act[i]) > 0.0f)) g = 0.0f;
+    dy[i] = f2bf(g);
+    dact[i] = 0.0f;
+}
+
+// Gradient of a conv's staged input (bf16, stride cinPad, at the conv's resolution W x H) back to
+// its sources. Mode 0: same pixels. Mode 1 (2x2 average of a finer source wa x ha, last row /
+// column repeated): fine pixel (Y, X) receives 0.25 of its coarse pixel, twice along an axis
+// where it was read twice. Mode 2: the first caPad channels go to the coarse source (sum over the
+// 2x2 fine pixels that read it), the rest to the skip source.
+__global__ void k_stage_back_a(const uint16_t* dx, int W, int H, int cinPad, int mode, float* da, int wa, int ha,
+                               int caPad) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)wa * ha * caPad) return;
+    const int c = (int)(i % caPad);
+    const int p = (int)(i / caPad);
+    const int X = p % wa, Y = p / wa;
+    float g = 0.0f;
+    if (mode == 0) {
+        g = bf2f(dx[(size_t)p * cinPad + c]);
+    } else if (mode == 1) {
+        const float mx = (X == wa - 1 && (wa & 1)) ? 2.0f : 1.0f, my = (Y == ha - 1 && (ha & 1)) ? 2.0f : 1.0f;
+        g = 0.25f * mx * my * bf2f(dx[((size_t)(Y >> 1) * W + (X >> 1)) * cinPad + c]);
+    } else {
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dxx = 0; dxx < 2; ++dxx) {
+                const int yy = 2 * Y + dy, xx = 2 * X + dxx;
+                if (yy < H && xx < W) g += bf2f(dx[((size_t)yy * W + xx) * cinPad + c]);
+            }
+    }
+    da[i] += g;
+}
+__global__ void k_stage_back_b(const uint16_t* dx, int W, int H, int cinPad, int caPad, float* db, int cbPad) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)W * H * cbPad) return;
+    const int c = (int)(i % cbPad);
+    const size_t p = i / cbPad;
+    db[i] += bf2f(dx[p * cinPad + caPad + c]);
+}
+
+// Weight gradient of one layer: dW[tap][co][ci] = sum over pixels of dY[p][co] X[p + off(tap)][ci],
+// X being the conv's staged input. A workgroup owns one (tap, 32-channel chunk) pair and a strided
+// set of 16 x 8 pixel tiles; it stages dY (transposed, [co][pixel]) and the input halo
+// (transposed, [ci][halo pixel]) in LDS, and each wave multiplies its 32 pixels of the tile:
+// A = dY^T (16 co x 32 px), B = X (32 px x 16 ci) on v_mfma_f32_16x16x32_bf16.
+template <int MT>
+__global__ void __launch_bounds__(256) k_conv_wgrad(ConvArgs X, const uint16_t* dy, float* gw) {
+    __shared__ uint16_t dyT[MT * 16][kTW * kTH];
+    __shared__ uint16_t xT[32][kHalo];
+    const int tap = blockIdx.x % 9, c0 = (blockIdx.x / 9) * 32;
+    const int ky = tap / 3, kx = tap % 3;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int coutPad = X.coutPad;  // channel stride of dy
+    f32x4 acc[MT][2];
+    for (int mt = 0; mt < MT; ++mt)
+        for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int tilesX = (X.W + kTW - 1) / kTW, tilesY = (X.H + kTH - 1) / kTH;
+    for (int t = blockIdx.y; t < tilesX * tilesY; t += gridDim.y) {
+        const int tx0 = (t % tilesX) * kTW, ty0 = (t / tilesX) * kTH;
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < kTW * kTH * MT * 2; idx += 256) {
+            const int px = idx / (MT * 2), g = idx % (MT * 2);
+            const int y = ty0 + px / kTW, x = tx0 + px % kTW;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (x < X.W && y < X.H) v = *(const uint4*)(dy + ((size_t)y * X.W + x) * coutPad + g * 8);
+            const uint32_t q[4] = {v.x, v.y, v.z, v.w};
+            for (int j = 0; j < 8; ++j) dyT[g * 8 + j][px] = (uint16_t)(q[j >> 1] >> ((j & 1) * 16));
+        }
+        for (int idx = threadIdx.x; idx < kHalo * 4; idx += 256) {
+            const int p = idx >> 2, part = idx & 3;
+            const uint4 v = fetch8(X, c0 + part * 8, ty0 - 1 + p / kHW, tx0 - 1 + p % kHW);
+            const uint32_t q[4] = {v.x, v.y, v.z, v.w};
+            for (int j = 0; j < 8; ++j) xT[part * 8 + j][p] = (uint16_t)(q[j >> 1] >> ((j & 1) * 16));
+        }
+        __syncthreads();
+        const int k0 = wave * 32 + (lane >> 4) * 8;  // this lane's 8 pixels (one tile row)
+        const int h0 = (k0 / kTW + ky) * kHW + k0 % kTW + kx;
+        bf16x8 bfr[2];
+        for (int nt = 0; nt < 2; ++nt) {
+            const uint16_t* src = &xT[nt * 16 + (lane & 15)][h0];
+            uint32_t q[4];
+            for (int j = 0; j < 4; ++j) q[j] = (uint32_t)src[2 * j] | ((uint32_t)src[2 * j + 1] << 16);
+            bfr[nt] = __builtin_bit_cast(bf16x8, make_uint4(q[0], q[1], q[2], q[3]));
+        }
+        for (int mt = 0; mt < MT; ++mt) {
+            const bf16x8 afr = __builtin_bit_cast(bf16x8, *(const uint4*)&dyT[mt * 16 + (lane & 15)][k0]);
+            for (int nt = 0; nt < 2; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nt], acc[mt][nt], 0, 0, 0);
+        }
+    }
+    for (int mt = 0; mt < MT; ++mt)
+        for (int nt = 0; nt < 2; ++nt)
+            for (int r = 0; r < 4; ++r) {
+                const float v = acc[mt][nt][r];
+                if (v == 0.0f) continue;
+                const int co = mt * 16 + 4 * (lane >> 4) + r, ci = c0 + nt * 16 + (lane & 15);
+                unsafeAtomicAdd(&gw[((size_t)tap * X.coutPad + co) * X.cinPad + ci], v);
+            }
+}
+
+// Bias gradient: sum of dY over pixels, one channel per lane, 256 pixels per block.
+__global__ void k_bias_grad(const uint16_t* dy, int n, int coutPad, float* gb) {
+    const int c = threadIdx.x;
+    if (c >= coutPad) return;
+    const int p0 = blockIdx.x * 256, p1 = min(p0 + 256, n);
+    float s = 0.0f;
+    for (int p = p0; p < p1; ++p) s += bf2f(dy[(size_t)p * coutPad + c]);
+    if (s != 0.0f) unsafeAtomicAdd(&gb[c], s);
+}
+
+// Adam (bias-corrected), gradients cleared after use.
+__global__ void k_adam(float* w, float* g, float* m1, float* m2, size_t n, float lr, float bc1, float bc2) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float a = 0.9f * m1[i] + 0.1f * gi, b = 0.999f * m2[i] + 0.001f * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    w[i] -= lr * (a / bc1) / (sqrtf(b / bc2) + 1e-8f);
+    g[i] = 0.0f;
+}
+
+// bf16 forward weights [tap][co][ci] and flipped-transposed data-gradient weights
+// [8 - tap][ci][co] from the fp32 masters.
+__global__ void k_pack_weights(const float* mw, int coutPad, int cinPad, uint16_t* w, uint16_t* wt) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)9 * coutPad * cinPad) return;
+    const int ci = (int)(i % cinPad);
+    const int co = (int)((i / cinPad) % coutPad);
+    const int tap = (int)(i / ((size_t)cinPad * coutPad));
+    const uint16_t b = f2bf(mw[i]);
+    w[i] = b;
+    wt[((size_t)(8 - tap) * cinPad + ci) * coutPad + co] = b;
+}
+
 uint16_t host_f2bf(float f) {
     uint32_t u;
     memcpy(&u, &f, 4);
@@ -255,6 +454,11 @@ CnnNet::~CnnNet() {
     for (uint16_t* p : w_) (void)hipFree(p);
     for (float* p : b_) (void)hipFree(p);
     release_acts();
+    for (auto* v : {&mw_, &mb_, &gw_, &gb_, &m1w_, &m2w_, &m1b_, &m2b_})
+        for (float* p : *v) (void)hipFree(p);
+    for (uint16_t* p : wt_) (void)hipFree(p);
+    if (zeros_) (void)hipFree(zeros_);
+    release_train_acts();
 }
 
 void CnnNet::release_acts() {
@@ -318,53 +522,82 @@ bool CnnNet::ensure(uint32_t W, uint32_t H) {
     return true;
 }
 
+namespace {
+// Forward arguments of layer li at image size W x H.
+ConvArgs layer_args(int li, uint32_t W, uint32_t H, const uint16_t* feat, const std::vector<uint16_t*>& act,
+                    const std::vector<uint16_t*>& w, const std::vector<float*>& b) {
+    const std::vector<CnnLayerSpec>& Ls = cnn_layers();
+    const CnnLayerSpec& L = Ls[li];
+    ConvArgs A;
+    memset(&A, 0, sizeof(A));
+    uint32_t wd, h;
+    level_dims(W, H, L.level, wd, h);
+    const int srcLevel = L.mode == 1 ? L.level - 1 : (L.mode == 2 ? L.level + 1 : L.level);
+    uint32_t wa, ha;
+    level_dims(W, H, srcLevel, wa, ha);
+    A.a = L.srcA < 0 ? feat : act[L.srcA];
+    A.ca = L.srcA < 0 ? 32 : pad32(Ls[L.srcA].cout);
+    A.b = L.srcB < 0 ? nullptr : act[L.srcB];
+    A.cb = L.srcB < 0 ? 0 : pad32(Ls[L.srcB].cout);
+    A.wa = (int)wa;
+    A.ha = (int)ha;
+    A.mode = L.mode;
+    A.W = (int)wd;
+    A.H = (int)h;
+    A.cinPad = cin_pad(L, A.ca, A.cb);
+    A.coutPad = pad32(L.cout);
+    A.cout = L.cout;
+    A.w = w.empty() ? nullptr : w[li];
+    A.bias = b.empty() ? nullptr : b[li];
+    A.out = act[li];
+    A.relu = L.relu ? 1 : 0;
+    return A;
+}
+
+bool launch_conv(const ConvArgs& A, hipStream_t s) {
+    const dim3 grid((A.W + kTW - 1) / kTW, (A.H + kTH - 1) / kTH);
+    switch ((A.cout + 15) / 16) {
+        case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, A); break;
+        case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, s, A); break;
+        case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, A); break;
+        case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, dim3(256), 0, s, A); break;
+        case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(256), 0, s, A); break;
+        case 10: hipLaunchKernelGGL(k_conv3x3<10>, grid, dim3(256), 0, s, A); break;
+        case 14: hipLaunchKernelGGL(k_conv3x3<14>, grid, dim3(256), 0, s, A); break;
+        default: return false;
+    }
+    return true;
+}
+}  // namespace
+
 bool CnnNet::run(const float* rad, const float* alb, const float* nrm, const float* dep, uint32_t W, uint32_t H,
                  float* out, hipStream_t s) {
     if (!loaded_ || !ensure(W, H)) return false;
     const int n = (int)(W * H);
     hipLaunchKernelGGL(k_features, dim3((n + 255) / 256), dim3(256), 0, s, rad, alb, nrm, dep, (int)W, (int)H, feat_);
     const std::vector<CnnLayerSpec>& Ls = cnn_layers();
-    for (size_t li = 0; li < Ls.size(); ++li) {
-        const CnnLayerSpec& L = Ls[li];
-        ConvArgs A;
-        memset(&A, 0, sizeof(A));
-        uint32_t w, h;
-        level_dims(W, H, L.level, w, h);
-        const int srcLevel = L.mode == 1 ? L.level - 1 : (L.mode == 2 ? L.level + 1 : L.level);
-        uint32_t wa, ha;
-        level_dims(W, H, srcLevel, wa, ha);
-        A.a = L.srcA < 0 ? feat_ : act_[L.srcA];
-        A.ca = L.srcA < 0 ? 32 : pad32(Ls[L.srcA].cout);
-        A.b = L.srcB < 0 ? nullptr : act_[L.srcB];
-        A.cb = L.srcB < 0 ? 0 : pad32(Ls[L.srcB].cout);
-        A.wa = (int)wa;
-        A.ha = (int)ha;
-        A.mode = L.mode;
-        A.W = (int)w;
-        A.H = (int)h;
-        A.cinPad = cin_pad(L, A.ca, A.cb);
-        A.coutPad = pad32(L.cout);
-        A.cout = L.cout;
-        A.w = w_[li];
-        A.bias = b_[li];
-        A.out = act_[li];
-        A.relu = L.relu ? 1 : 0;
-        const dim3 grid((w + kTW - 1) / kTW, (h + kTH - 1) / kTH);
-        switch ((L.cout + 15) / 16) {
-            case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, A); break;
-            case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, s, A); break;
-            case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, A); break;
-            case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, dim3(256), 0, s, A); break;
-            case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(256), 0, s, A); break;
-            default: return false;
-        }
-    }
-    hipLaunchKernelGGL(k_kpn_apply, dim3((n + 255) / 256), dim3(256), 0, s, act_.back(), rad, (int)W, (int)H, out);
+    for (size_t li = 0; li < Ls.size(); ++li)
+        if (!launch_conv(layer_args((int)li, W, H, feat_, act_, w_, b_), s)) return false;
+    if (out)
+        hipLaunchKernelGGL(k_kpn_apply, dim3((n + 255) / 256), dim3(256), 0, s, act_.back(), rad, (int)W, (int)H, out);
     return hipGetLastError() == hipSuccess;
 }
 
 bool CnnNet::read_layer(int layer, std::vector<float>& out, uint32_t& c, uint32_t& w, uint32_t& h) const {
     const std::vector<CnnLayerSpec>& Ls = cnn_layers();
+    if (layer >= kGradLayerBase) {  // masked output gradient of a layer from the last training step
+        const int li = layer - kGradLayerBase;
+        if (dyp_.empty() || li >= (int)Ls.size()) return false;
+        level_dims(tW_, tH_, Ls[li].level, w, h);
+        c = Ls[li].cout;
+        const int stride = pad32(Ls[li].cout);
+        std::vector<uint16_t> raw((size_t)w * h * stride);
+        if (hipMemcpy(raw.data(), dyp_[li], raw.size() * 2, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        out.resize((size_t)w * h * c);
+        for (size_t p = 0; p < (size_t)w * h; ++p)
+            for (uint32_t k = 0; k < c; ++k) out[p * c + k] = host_bf2f(raw[p * stride + k]);
+        return true;
+    }
     if (!feat_ || layer < -1 || layer >= (int)Ls.size()) return false;
     const int level = layer < 0 ? 0 : Ls[layer].level;
     level_dims(W_, H_, level, w, h);
@@ -376,6 +609,203 @@ bool CnnNet::read_layer(int layer, std::vector<float>& out, uint32_t& c, uint32_
     out.resize((size_t)w * h * c);
     for (size_t p = 0; p < (size_t)w * h; ++p)
         for (uint32_t k = 0; k < c; ++k) out[p * c + k] = host_bf2f(raw[p * stride + k]);
+    return true;
+}
+
+
+// ---- Training ----------------------------------------------------------------------------------------
+namespace {
+size_t layer_weights(int li) {
+    const CnnLayerSpec& L = cnn_layers()[li];
+    const int padA = pad32(L.cinA), cinP = L.mode == 2 ? padA + pad32(L.cinB) : padA;
+    return (size_t)9 * pad32(L.cout) * cinP;
+}
+int layer_cin_pad(int li) {
+    const CnnLayerSpec& L = cnn_layers()[li];
+    const int padA = pad32(L.cinA);
+    return L.mode == 2 ? padA + pad32(L.cinB) : padA;
+}
+template <class T>
+bool dalloc(T*& p, size_t n) {
+    if (hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return false;
+    return hipMemset(p, 0, std::max<size_t>(n, 1) * sizeof(T)) == hipSuccess;
+}
+}  // namespace
+
+bool CnnNet::train_init(hipStream_t s) {
+    if (!loaded_) return false;
+    const std::vector<CnnLayerSpec>& Ls = cnn_layers();
+    const size_t nl = Ls.size();
+    mw_.assign(nl, nullptr); mb_.assign(nl, nullptr); gw_.assign(nl, nullptr); gb_.assign(nl, nullptr);
+    m1w_.assign(nl, nullptr); m2w_.assign(nl, nullptr); m1b_.assign(nl, nullptr); m2b_.assign(nl, nullptr);
+    wt_.assign(nl, nullptr);
+    for (size_t li = 0; li < nl; ++li) {
+        const size_t nw = layer_weights((int)li), nb = (size_t)pad32(Ls[li].cout);
+        if (!dalloc(mw_[li], nw) || !dalloc(gw_[li], nw) || !dalloc(m1w_[li], nw) || !dalloc(m2w_[li], nw) ||
+            !dalloc(mb_[li], nb) || !dalloc(gb_[li], nb) || !dalloc(m1b_[li], nb) || !dalloc(m2b_[li], nb) ||
+            !dalloc(wt_[li], nw))
+            return false;
+        // Masters from the loaded bf16 weights (exact) and fp32 biases.
+        std::vector<uint16_t> wb(nw);
+        std::vector<float> wf(nw);
+        if (hipMemcpy(wb.data(), w_[li], nw * 2, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        for (size_t i = 0; i < nw; ++i) wf[i] = host_bf2f(wb[i]);
+        if (hipMemcpy(mw_[li], wf.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess) return false;
+        if (hipMemcpy(mb_[li], b_[li], nb * 4, hipMemcpyDeviceToDevice) != hipSuccess) return false;
+    }
+    if (!dalloc(zeros_, 256)) return false;
+    step_ = 0;
+    return pack(s);
+}
+
+bool CnnNet::pack(hipStream_t s) {
+    const std::vector<CnnLayerSpec>& Ls = cnn_layers();
+    for (size_t li = 0; li < Ls.size(); ++li) {
+        const size_t nw = layer_weights((int)li);
+        hipLaunchKernelGGL(k_pack_weights, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, mw_[li],
+                           pad32(Ls[li].cout), layer_cin_pad((int)li), w_[li], wt_[li]);
+    }
+    return hipGetLastError() == hipSuccess;
+}
+
+void CnnNet::release_train_acts() {
+    for (float* p : dact_) (void)hipFree(p);
+    for (uint16_t* p : dyp_) (void)hipFree(p);
+    dact_.clear();
+    dyp_.clear();
+    if (dx_) (void)hipFree(dx_);
+    if (out_) (void)hipFree(out_);
+    dx_ = nullptr;
+    out_ = nullptr;
+    tW_ = tH_ = 0;
+}
+
+bool CnnNet::ensure_train(uint32_t W, uint32_t H) {
+    if (W == tW_ && H == tH_ && !dact_.empty()) return true;
+    release_train_acts();
+    const std::vector<CnnLayerSpec>& Ls = cnn_layers();
+    dact_.assign(Ls.size(), nullptr);
+    dyp_.assign(Ls.size(), nullptr);
+    size_t dxMax = 0;
+    for (size_t li = 0; li < Ls.size(); ++li) {
+        uint32_t w, h;
+        level_dims(W, H, Ls[li].level, w, h);
+        const size_t n = (size_t)w * h * pad32(Ls[li].cout);
+        if (!dalloc(dact_[li], n) || !dalloc(dyp_[li], n)) return false;
+        dxMax = std::max(dxMax, (size_t)w * h * layer_cin_pad((int)li));
+    }
+    if (!dalloc(dx_, dxMax) || !dalloc(out_, (size_t)W * H * 3)) return false;
+    tW_ = W;
+    tH_ = H;
+    return true;
+}
+
+bool CnnNet::train_step(const float* rad, const float* alb, const float* nrm, const float* dep, const float* target,
+                        uint32_t W, uint32_t H, float lr, float* lossDev, hipStream_t s, bool apply) {
+    if (!training() || !run(rad, alb, nrm, dep, W, H, nullptr, s) || !ensure_train(W, H)) return false;
+    const std::vector<CnnLayerSpec>& Ls = cnn_layers();
+    const int nl = (int)Ls.size();
+    const int n = (int)(W * H);
+    hipLaunchKernelGGL(k_kpn_loss_back, dim3((n + 255) / 256), dim3(256), 0, s, act_[nl - 1], rad, target, (int)W,
+                       (int)H, 1.0f / (3.0f * (float)n), dyp_[nl - 1], lossDev);
+    for (int li = nl - 1; li >= 0; --li) {
+        const CnnLayerSpec& L = Ls[li];
+        const ConvArgs F = layer_args(li, W, H, feat_, act_, w_, b_);
+        const size_t nact = (size_t)F.W * F.H * F.coutPad;
+        if (li < nl - 1)
+            hipLaunchKernelGGL(k_relu_back, dim3((unsigned)((nact + 255) / 256)), dim3(256), 0, s, dact_[li], act_[li],
+                               nact, L.relu ? 1 : 0, dyp_[li]);
+        // Weight and bias gradients.
+        const dim3 wg(9 * (F.cinPad / 32), std::min(64, ((F.W + kTW - 1) / kTW) * ((F.H + kTH - 1) / kTH)));
+        switch (F.coutPad / 16) {
+            case 2: hipLaunchKernelGGL(k_conv_wgrad<2>, wg, dim3(256), 0, s, F, dyp_[li], gw_[li]); break;
+            case 4: hipLaunchKernelGGL(k_conv_wgrad<4>, wg, dim3(256), 0, s, F, dyp_[li], gw_[li]); break;
+            case 6: hipLaunchKernelGGL(k_conv_wgrad<6>, wg, dim3(256), 0, s, F, dyp_[li], gw_[li]); break;
+            case 8: hipLaunchKernelGGL(k_conv_wgrad<8>, wg, dim3(256), 0, s, F, dyp_[li], gw_[li]); break;
+            default: return false;
+        }
+        const int np = F.W * F.H;
+        hipLaunchKernelGGL(k_bias_grad, dim3((np + 255) / 256), dim3(256), 0, s, dyp_[li], np, F.coutPad, gb_[li]);
+        if (li == 0) break;  // no gradient for the input features
+        // Data gradient: the same conv with flipped, transposed weights, into the staged-input space.
+        ConvArgs D;
+        memset(&D, 0, sizeof(D));
+        D.a = dyp_[li];
+        D.ca = F.coutPad;
+        D.wa = F.W;
+        D.ha = F.H;
+        D.mode = 0;
+        D.W = F.W;
+        D.H = F.H;
+        D.cinPad = F.coutPad;
+        D.coutPad = F.cinPad;
+        D.cout = F.cinPad;
+        D.w = wt_[li];
+        D.bias = zeros_;
+        D.out = dx_;
+        D.relu = 0;
+        if (!launch_conv(D, s)) return false;
+        // Back through pooling / upsampling / concatenation into the sources' gradients.
+        if (L.srcA >= 0) {
+            const size_t na = (size_t)F.wa * F.ha * F.ca;
+            hipLaunchKernelGGL(k_stage_back_a, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, dx_, F.W, F.H,
+                               F.cinPad, F.mode, dact_[L.srcA], F.wa, F.ha, F.ca);
+        }
+        if (L.mode == 2) {
+            const size_t nb = (size_t)F.W * F.H * F.cb;
+            hipLaunchKernelGGL(k_stage_back_b, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, dx_, F.W, F.H,
+                               F.cinPad, F.ca, dact_[L.srcB], F.cb);
+        }
+    }
+    if (!apply) return hipGetLastError() == hipSuccess;
+    ++step_;
+    const float bc1 = 1.0f - powf(0.9f, (float)step_), bc2 = 1.0f - powf(0.999f, (float)step_);
+    for (int li = 0; li < nl; ++li) {
+        const size_t nw = layer_weights(li), nb = (size_t)pad32(Ls[li].cout);
+        hipLaunchKernelGGL(k_adam, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, mw_[li], gw_[li], m1w_[li],
+                           m2w_[li], nw, lr, bc1, bc2);
+        hipLaunchKernelGGL(k_adam, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, mb_[li], gb_[li], m1b_[li],
+                           m2b_[li], nb, lr, bc1, bc2);
+    }
+    if (!pack(s)) return false;
+    // The forward biases are the fp32 masters.
+    for (int li = 0; li < nl; ++li)
+        if (hipMemcpyAsync(b_[li], mb_[li], (size_t)pad32(Ls[li].cout) * 4, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            return false;
+    return hipGetLastError() == hipSuccess;
+}
+
+bool CnnNet::export_weights(std::vector<float>& blob) const {
+    return training() && export_blob(mw_, mb_, blob);
+}
+
+bool CnnNet::export_grads(std::vector<float>& blob, hipStream_t s) {
+    if (!training() || hipStreamSynchronize(s) != hipSuccess || !export_blob(gw_, gb_, blob)) return false;
+    const std::vector<CnnLayerSpec>& Ls = cnn_layers();
+    for (size_t li = 0; li < Ls.size(); ++li) {
+        if (hipMemset(gw_[li], 0, layer_weights((int)li) * 4) != hipSuccess) return false;
+        if (hipMemset(gb_[li], 0, (size_t)pad32(Ls[li].cout) * 4) != hipSuccess) return false;
+    }
+    return true;
+}
+
+bool CnnNet::export_blob(const std::vector<float*>& mw, const std::vector<float*>& mb, std::vector<float>& blob) const {
+    const std::vector<CnnLayerSpec>& Ls = cnn_layers();
+    blob.clear();
+    for (size_t li = 0; li < Ls.size(); ++li) {
+        const CnnLayerSpec& L = Ls[li];
+        const size_t nw = layer_weights((int)li);
+        const int coutP = pad32(L.cout), cinP = layer_cin_pad((int)li), padA = pad32(L.cinA), cin = L.cinA + L.cinB;
+        std::vector<float> w(nw), b(coutP);
+        if (hipMemcpy(w.data(), mw[li], nw * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        if (hipMemcpy(b.data(), mb[li], coutP * 4, hipMemcpyDeviceToHost) != hipSuccess) return false;
+        for (int co = 0; co < L.cout; ++co)
+            for (int ci = 0; ci < cin; ++ci) {
+                const int cp = ci < L.cinA ? ci : padA + (ci - L.cinA);
+                for (int tap = 0; tap < 9; ++tap) blob.push_back(w[((size_t)tap * coutP + co) * cinP + cp]);
+            }
+        for (int co = 0; co < L.cout; ++co) blob.push_back(b[co]);
+    }
     return true;
 }
 

@@ -778,6 +778,12 @@ struct sr_ctx {
     sr_render_params adjParams{};
     DevBuf<float> dAdjImage, dAdjGrad;
     sr::CnnNet* cnn = nullptr;  // kernel-predicting denoiser (weights loaded by sr_cnn_load)
+    struct CnnExample {  // training pair kept in device memory
+        DevBuf<float> rad, alb, nrm, dep, target;
+        uint32_t w = 0, h = 0;
+    };
+    std::vector<CnnExample*> cnnExamples;
+    DevBuf<float> dCnnLoss;
     DevBuf<int32_t> dSlotTri, dSceneItems;
     DevBuf<DevObject> dObjects;
     DevBuf<DevTexture> dTextures;
@@ -831,6 +837,12 @@ struct sr_ctx {
         dAdjImage.release(); dAdjGrad.release();
         delete cnn;
         cnn = nullptr;
+        for (CnnExample* e : cnnExamples) {
+            e->rad.release(); e->alb.release(); e->nrm.release(); e->dep.release(); e->target.release();
+            delete e;
+        }
+        cnnExamples.clear();
+        dCnnLoss.release();
         dRayO.release(); dRayD.release(); dHitA.release(); dHitSlot.release();
         dPaths.release(); dShadows.release(); dAcc.release();
         dQueue0.release(); dQueue1.release(); dCounts.release(); dCtr.release();
@@ -1991,6 +2003,102 @@ int sr_cnn_benchmark(sr_ctx* ctx, uint32_t iterations, double* ms_per_run) {
     if (rc) return rc;
     if (e != hipSuccess) return fail(SR_ERR_HIP, hipGetErrorString(e));
     *ms_per_run = (double)ms / iterations;
+    return SR_OK;
+}
+
+int sr_cnn_train_begin(sr_ctx* ctx) {
+    if (!ctx) return fail(SR_ERR_INVALID, "cnn: null argument");
+    if (!ctx->cnn || !ctx->cnn->loaded()) return fail(SR_ERR_STATE, "cnn: load initial weights first");
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!ctx->cnn->train_init(ctx->stream)) return fail(SR_ERR_HIP, "cnn: training state allocation failed");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SR_OK;
+}
+
+int sr_cnn_train_add(sr_ctx* ctx, const float* rad, const float* alb, const float* nrm, const float* dep,
+                     const float* target, uint32_t W, uint32_t H) {
+    if (!ctx || !rad || !alb || !nrm || !dep || !target || W == 0 || H == 0) return fail(SR_ERR_INVALID, "cnn: bad arguments");
+    if (ctx->device < 0) return fail(SR_ERR_STATE, "host-only context (device -1)");
+    if (W > 8192 || H > 8192) return fail(SR_ERR_LIMIT, "cnn: training images up to 8192 x 8192");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t n = (size_t)W * H;
+    sr_ctx::CnnExample* e = new sr_ctx::CnnExample();
+    int rc;
+    hipStream_t s = ctx->stream;
+    if ((rc = upload(e->rad, rad, n * 3, s)) || (rc = upload(e->alb, alb, n * 3, s)) || (rc = upload(e->nrm, nrm, n * 3, s)) ||
+        (rc = upload(e->dep, dep, n, s)) || (rc = upload(e->target, target, n * 3, s))) {
+        e->rad.release(); e->alb.release(); e->nrm.release(); e->dep.release(); e->target.release();
+        delete e;
+        return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    e->w = W;
+    e->h = H;
+    ctx->cnnExamples.push_back(e);
+    return SR_OK;
+}
+
+int sr_cnn_train_run(sr_ctx* ctx, uint32_t steps, float learning_rate, uint32_t seed, float* mean_loss) {
+    if (!ctx || !mean_loss || steps == 0) return fail(SR_ERR_INVALID, "cnn: bad arguments");
+    if (!ctx->cnn || !ctx->cnn->training()) return fail(SR_ERR_STATE, "cnn: sr_cnn_train_begin first");
+    if (ctx->cnnExamples.empty()) return fail(SR_ERR_STATE, "cnn: no training examples");
+    if (!(learning_rate > 0.0f)) return fail(SR_ERR_INVALID, "cnn: learning rate must be > 0");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = ensure(ctx->dCnnLoss, 1);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(ctx->dCnnLoss.ptr, 0, sizeof(float), ctx->stream));
+    // Examples in a seeded shuffled order, reshuffled every pass.
+    std::vector<uint32_t> order(ctx->cnnExamples.size());
+    uint64_t st = ((uint64_t)seed << 32) ^ 0x9e3779b97f4a7c15ull;
+    size_t pos = order.size();
+    for (uint32_t k = 0; k < steps; ++k) {
+        if (pos == order.size()) {
+            for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+            for (size_t i = order.size(); i > 1; --i) {
+                st = st * 6364136223846793005ull + 1442695040888963407ull;
+                std::swap(order[i - 1], order[(size_t)((st >> 33) % i)]);
+            }
+            pos = 0;
+        }
+        const sr_ctx::CnnExample* e = ctx->cnnExamples[order[pos++]];
+        if (!ctx->cnn->train_step(e->rad.ptr, e->alb.ptr, e->nrm.ptr, e->dep.ptr, e->target.ptr, e->w, e->h, learning_rate,
+                                  ctx->dCnnLoss.ptr, ctx->stream))
+            return fail(SR_ERR_HIP, "cnn: training step failed");
+    }
+    float loss = 0.0f;
+    HIP_TRY(hipMemcpyAsync(&loss, ctx->dCnnLoss.ptr, sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *mean_loss = loss / (float)steps;
+    return SR_OK;
+}
+
+int sr_cnn_train_gradient(sr_ctx* ctx, uint32_t example, float* gradient, float* loss) {
+    if (!ctx || !gradient || !loss) return fail(SR_ERR_INVALID, "cnn: null argument");
+    if (!ctx->cnn || !ctx->cnn->training()) return fail(SR_ERR_STATE, "cnn: sr_cnn_train_begin first");
+    if (example >= ctx->cnnExamples.size()) return fail(SR_ERR_INVALID, "cnn: no such example");
+    HIP_TRY(hipSetDevice(ctx->device));
+    int rc = ensure(ctx->dCnnLoss, 1);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(ctx->dCnnLoss.ptr, 0, sizeof(float), ctx->stream));
+    const sr_ctx::CnnExample* e = ctx->cnnExamples[example];
+    if (!ctx->cnn->train_step(e->rad.ptr, e->alb.ptr, e->nrm.ptr, e->dep.ptr, e->target.ptr, e->w, e->h, 1.0f,
+                              ctx->dCnnLoss.ptr, ctx->stream, false))
+        return fail(SR_ERR_HIP, "cnn: gradient pass failed");
+    std::vector<float> blob;
+    if (!ctx->cnn->export_grads(blob, ctx->stream)) return fail(SR_ERR_HIP, "cnn: gradient export failed");
+    memcpy(gradient, blob.data(), blob.size() * sizeof(float));
+    HIP_TRY(hipMemcpy(loss, ctx->dCnnLoss.ptr, sizeof(float), hipMemcpyDeviceToHost));
+    return SR_OK;
+}
+
+int sr_cnn_train_export(sr_ctx* ctx, float* weights) {
+    if (!ctx || !weights) return fail(SR_ERR_INVALID, "cnn: null argument");
+    if (!ctx->cnn || !ctx->cnn->training()) return fail(SR_ERR_STATE, "cnn: not training");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::vector<float> blob;
+    if (!ctx->cnn->export_weights(blob)) return fail(SR_ERR_HIP, "cnn: export failed");
+    memcpy(weights, blob.data(), blob.size() * sizeof(float));
     return SR_OK;
 }
 

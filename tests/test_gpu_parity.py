@@ -334,3 +334,117 @@ def test_kpn_denoiser_layers_match_cpu_restatement():
     rel = np.abs(out.reshape(H, W, 3) - ref_out) / (np.abs(ref_out) + 1e-2)
     print(f"kpn denoiser: worst per-layer rel diff {worst:.2e}, end-to-end max rel diff {rel.max():.2e}", flush=True)
     assert rel.max() <= 2e-2
+
+
+def _kpn_pair(spec_fn, seed):
+    from sharpray_amd import RNG_COUNTER, scenes
+    spec = spec_fn()
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=seed)
+    r.wait()
+    H, W = spec.height, spec.width
+    noisy = dict(rad=r.radiance.reshape(H, W, 3).copy(), alb=r.albedo.reshape(H, W, 3).copy(),
+                 nrm=r.normals.reshape(H, W, 3).copy(), dep=r.depth.reshape(H, W).copy())
+    spec.min_samples = spec.max_samples = 64
+    _, _, rt = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=seed + 1000)
+    rt.wait()
+    return scene, noisy, rt.radiance.reshape(H, W, 3).copy(), W, H
+
+
+def test_kpn_training_gradient_matches_cpu_restatement():
+    """One backward pass of the hand-written training kernels (loss, softmax-kernel, ReLU, MFMA
+    data and weight gradients, pooling / upsampling / concatenation) against the reverse-mode
+    restatement, step by step: each layer's output gradient recomputed on the CPU from the GPU's
+    own activations and its consumers' GPU gradients, and each layer's weight and bias gradient
+    from the GPU's output gradient and staged input. (A whole-network comparison would compound
+    bf16 rounding flips of a random 19-layer network, ReLU masks included, into noise.)"""
+    from oracle import cnn_ref as C
+    from sharpray_amd import KpnDenoiser, scenes
+    scene, ex, target, W, H = _kpn_pair(lambda: scenes.config_plain_cpu(scale=0.16, spp=2), 8)
+    den = KpnDenoiser(scene)
+    blob = KpnDenoiser.random_weights(23)
+    den.load(blob)
+    den.train_begin()
+    den.train_add(ex["rad"], ex["alb"], ex["nrm"], ex["dep"], target, W, H)
+    loss, g = den.gradient(0)
+    lref, _ = C.backward(blob, ex["rad"], ex["alb"], ex["nrm"], ex["dep"], target)
+    assert abs(loss - lref) <= 1e-4 * lref, (loss, lref)
+    acts = {i: den.layer(i).astype(np.float64) for i in range(-1, 19)}
+    dys = {i: den.layer(1000 + i).astype(np.float64) for i in range(19)}
+    wb = C.split_weights(blob)
+    # Output gradient of the last layer: the softmax-kernel gradient of the loss.
+    out_logits = acts[18]
+    r = ex["rad"].astype(np.float64)
+    l = out_logits[..., :25]
+    e = np.exp(l - l.max(-1, keepdims=True))
+    ws = e / e.sum(-1, keepdims=True)
+    ys, xs = np.arange(H), np.arange(W)
+    nb = [r[np.clip(ys + k // 5 - 2, 0, H - 1)][:, np.clip(xs + k % 5 - 2, 0, W - 1)] for k in range(25)]
+    out = sum(ws[..., k:k + 1] * nb[k] for k in range(25))
+    t = target.astype(np.float64)
+    gr = 2.0 * (out - t) / (t ** 2 + 0.01) / out.size
+    dwk = np.stack([(gr * nb[k]).sum(-1) for k in range(25)], -1)
+    dl = C.bf16((ws * (dwk - (ws * dwk).sum(-1, keepdims=True))).astype(np.float32))
+    errs = {"dY18": float(np.linalg.norm(dys[18] - dl) / np.linalg.norm(dl))}
+    dact = {i: None for i in range(-1, 19)}
+
+    def add(k, v):
+        dact[k] = v if dact[k] is None else dact[k] + v
+
+    k = 0
+    offsets = []
+    for ca, cb, cout, *_ in C.LAYERS:
+        offsets.append(k)
+        k += cout * (ca + cb) * 9 + cout
+    for i in range(18, -1, -1):
+        ca, cb, cout, mode, sa, sb, lvl, relu = C.LAYERS[i]
+        if i < 18:
+            ref = C.bf16((dact[i] * (acts[i] > 0)).astype(np.float32))
+            errs[f"dY{i}"] = float(np.linalg.norm(dys[i] - ref) / max(np.linalg.norm(ref), 1e-30))
+        x = C.staged_input(i, acts, C.bf16).astype(np.float64)
+        hh, ww, cin = x.shape
+        dwm = C._cols(x).reshape(hh * ww, 9 * cin).T @ dys[i].reshape(hh * ww, cout)
+        gw_ref = dwm.reshape(3, 3, cin, cout).transpose(3, 2, 0, 1).reshape(-1)
+        gb_ref = dys[i].sum((0, 1))
+        nw = gw_ref.size
+        gw, gb = g[offsets[i]:offsets[i] + nw], g[offsets[i] + nw:offsets[i] + nw + cout]
+        errs[f"W{i}"] = float(np.linalg.norm(gw - gw_ref) / max(np.linalg.norm(gw_ref), 1e-30))
+        errs[f"b{i}"] = float(np.linalg.norm(gb - gb_ref) / max(np.linalg.norm(gb_ref), 1e-30))
+        if i == 0:
+            break
+        w = wb[i][0].astype(np.float64)
+        wt = w[:, :, ::-1, ::-1].transpose(1, 0, 2, 3)
+        dx = C.bf16(C.conv3x3(dys[i], wt, np.zeros(cin), False, C.exact).astype(np.float32)).astype(np.float64)
+        if mode == 0:
+            add(sa, dx)
+        elif mode == 1:
+            fh, fw = acts[sa].shape[:2]
+            up_ = dx[np.arange(fh) // 2][:, np.arange(fw) // 2] * 0.25
+            if fw % 2:
+                up_[:, -1] *= 2.0
+            if fh % 2:
+                up_[-1] *= 2.0
+            add(sa, up_)
+        else:
+            lh, lw = acts[sa].shape[:2]
+            da = np.zeros((lh, lw, ca))
+            np.add.at(da, (np.arange(hh)[:, None] // 2, np.arange(ww)[None, :] // 2), dx[..., :ca])
+            add(sa, da)
+            add(sb, dx[..., ca:])
+    worst = max(errs.values())
+    print(f"kpn training gradient: loss {loss:.5f} (cpu {lref:.5f}), worst step-wise rel L2 err {worst:.2e} "
+          f"({max(errs, key=errs.get)})", flush=True)
+    assert worst <= 2e-3, errs
+
+
+def test_kpn_training_reduces_loss():
+    from sharpray_amd import KpnDenoiser, scenes
+    scene, ex, target, W, H = _kpn_pair(lambda: scenes.config_plain_cpu(scale=0.25, spp=2), 9)
+    den = KpnDenoiser(scene)
+    den.load(KpnDenoiser.random_weights(1))
+    den.train_begin()
+    den.train_add(ex["rad"], ex["alb"], ex["nrm"], ex["dep"], target, W, H)
+    first = den.train(10, 1e-3, 0)
+    later = den.train(150, 1e-3, 1)
+    last = den.train(10, 1e-3, 2)
+    print(f"kpn training: loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < 0.7 * first

@@ -68,3 +68,46 @@ def test_weight_count_matches_library():
     n = sum(co * (ca + cb) * 9 + co for ca, cb, co, *_ in C.LAYERS)
     assert n == A.cnn_weight_count() == A.KpnDenoiser.random_weights(0).size
     assert [tuple(l) for l in C.LAYERS] == [tuple(l) for l in A.CNN_LAYERS]
+
+
+def _example(seed, h=12, w=10):
+    rng = np.random.default_rng(seed)
+    rad = rng.uniform(0, 2, (h, w, 3))
+    alb = rng.uniform(0, 1, (h, w, 3))
+    nrm = rng.normal(size=(h, w, 3))
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    dep = rng.uniform(1, 4, (h, w))
+    dep[0, :3] = np.inf
+    target = rng.uniform(0, 2, (h, w, 3))
+    return rad, alb, nrm, dep, target
+
+
+def test_backward_matches_finite_differences_in_float64():
+    """The reverse-mode restatement (float64, no rounding) against central differences of the
+    float64 loss, for a sample of weights and biases in every layer."""
+    import sharpray_amd as A
+    rad, alb, nrm, dep, target = _example(0)
+    blob = A.KpnDenoiser.random_weights(3).astype(np.float64)
+    rng = np.random.default_rng(1)
+    blob += rng.normal(0, 0.01, blob.size)  # non-zero biases
+    loss0, grads = C.backward(blob, rad, alb, nrm, dep, target, rnd=C.exact)
+    flat = C.flatten(grads)
+
+    def f(b):
+        _, out = C.run(b, rad, alb, nrm, dep, rnd=C.exact)
+        return C.loss(out, target)
+
+    assert abs(f(blob) - loss0) <= 1e-12 * max(1.0, loss0)
+    k = 0
+    picks = []
+    for cin_a, cin_b, cout, *_ in C.LAYERS:
+        nw = cout * (cin_a + cin_b) * 9
+        picks += list(k + rng.choice(nw, 3, replace=False)) + [k + nw + int(rng.integers(cout))]
+        k += nw + cout
+    for i in picks:
+        eps = 1e-6
+        hi, lo = blob.copy(), blob.copy()
+        hi[i] += eps
+        lo[i] -= eps
+        fd = (f(hi) - f(lo)) / (2 * eps)
+        assert abs(fd - flat[i]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(flat).max()), (i, fd, flat[i])
