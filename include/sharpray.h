@@ -261,8 +261,8 @@ int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad);
  * A 5-level U-Net (32/48/64/96/128 channels, two 3x3 conv + ReLU per level, 2x2 average pooling,
  * nearest upsampling with skip concatenation) on bf16 MFMA with fp32 accumulation. Inputs per
  * pixel: log(1 + radiance), albedo, normal, 1 / (1 + depth) (0 without a surface); the last conv
- * predicts 25 logits whose softmax weights the pixel's 5x5 neighbourhood of the noisy radiance
- * (edge-clamped). Weights: float blob, per layer [cout][cin][3][3] then bias[cout], layers in
+ * predicts 4 x 25 logits: four 5x5 kernels (softmax-normalised) applied in cascade to the noisy
+ * radiance with their taps 1, 2, 4 and 8 pixels apart (edge-clamped). Weights: float blob, per layer [cout][cin][3][3] then bias[cout], layers in
  * the order of sharpray_amd.cnn_layers(); concatenated inputs list the upsampled channels first. */
 int sr_cnn_weight_count(uint64_t* count);
 int sr_cnn_load(sr_ctx* ctx, const float* weights, uint64_t count);

@@ -8,7 +8,8 @@ Arithmetic contract shared with the GPU: weights and every activation are rounde
 (round to nearest even); a 3x3 convolution sums products in float32 and adds the bias, then ReLU,
 then rounds; 2x2 average pooling adds ((a + b) + c) + d in float32, scales by 0.25 and rounds,
 repeating the last row / column at odd sizes; upsampling is nearest; the final 25 logits are
-soft-maxed (max-subtracted) and weight the edge-clamped 5x5 neighbourhood of the noisy radiance.
+soft-maxed (max-subtracted) per group of 25 and weight, in cascade, the edge-clamped 5x5
+neighbourhoods (taps 1, 2, 4, 8 pixels apart) of the noisy radiance and of each stage's output.
 Only the float32 summation order of the convolutions differs from the GPU's.
 """
 import numpy as np
@@ -25,8 +26,9 @@ LAYERS = [
     (96, 64, 64, 2, 11, 5, 2, True), (64, 0, 64, 0, 12, -1, 2, True),
     (64, 48, 48, 2, 13, 3, 1, True), (48, 0, 48, 0, 14, -1, 1, True),
     (48, 32, 32, 2, 15, 1, 0, True), (32, 0, 32, 0, 16, -1, 0, True),
-    (32, 0, 25, 0, 17, -1, 0, False),
+    (32, 0, 100, 0, 17, -1, 0, False),
 ]
+STAGES = 4  # kernel cascade: four 5x5 kernels, taps 1, 2, 4 and 8 pixels apart
 
 
 def bf16(x):
@@ -113,20 +115,57 @@ def layer_forward(i, wb, inputs, rnd=bf16):
     return conv3x3(staged_input(i, inputs, rnd), w, b, LAYERS[i][7], rnd)
 
 
-def kpn_apply(logits, rad):
-    h, wd, _ = rad.shape
+def _stage_weights(logits, stage):
     dt = np.float64 if logits.dtype == np.float64 else np.float32
-    l = logits[..., :25].astype(dt)
+    l = logits[..., 25 * stage:25 * stage + 25].astype(dt)
     e = np.exp(l - l.max(-1, keepdims=True))
-    s = np.zeros((h, wd), dt)
-    acc = np.zeros((h, wd, 3), dt)
+    return e / e.sum(-1, keepdims=True)
+
+
+def _taps(img, stage):
+    h, wd, _ = img.shape
+    d = 1 << stage
     ys, xs = np.arange(h), np.arange(wd)
-    for k in range(25):
-        dy, dx = k // 5 - 2, k % 5 - 2
-        nb = rad[np.clip(ys + dy, 0, h - 1)][:, np.clip(xs + dx, 0, wd - 1)]
-        s += e[..., k]
-        acc += e[..., k:k + 1] * nb
-    return acc / s[..., None]
+    return [(np.clip(ys + d * (k // 5 - 2), 0, h - 1), np.clip(xs + d * (k % 5 - 2), 0, wd - 1)) for k in range(25)]
+
+
+def kpn_stage(logits, img, stage):
+    dt = np.float64 if logits.dtype == np.float64 else np.float32
+    w = _stage_weights(logits, stage)
+    acc = np.zeros(img.shape, dt)
+    for k, (ys, xs) in enumerate(_taps(img, stage)):
+        acc += w[..., k:k + 1] * img[ys][:, xs]
+    return acc
+
+
+def kpn_apply(logits, rad):
+    """The three cascade stages; returns the final image."""
+    return kpn_cascade(logits, rad)[-1]
+
+
+def kpn_cascade(logits, rad):
+    """[input, after stage 0, after stage 1, after stage 2]."""
+    out = [rad]
+    for st in range(STAGES):
+        out.append(kpn_stage(logits, out[-1], st))
+    return out
+
+
+def kpn_cascade_back(logits, rad, g):
+    """Gradient of the loss with respect to the 75 logits given g = dL/d(final image)."""
+    imgs = kpn_cascade(logits.astype(np.float64), np.asarray(rad, np.float64))
+    dl = np.zeros(logits.shape[:2] + (25 * STAGES,))
+    for st in range(STAGES - 1, -1, -1):
+        src = imgs[st]
+        w = _stage_weights(logits.astype(np.float64), st)
+        taps = _taps(src, st)
+        dw = np.stack([(g * src[ys][:, xs]).sum(-1) for ys, xs in taps], -1)
+        dl[..., 25 * st:25 * st + 25] = w * (dw - (w * dw).sum(-1, keepdims=True))
+        gin = np.zeros_like(src)
+        for k, (ys, xs) in enumerate(taps):
+            np.add.at(gin, (ys[:, None], xs[None, :]), w[..., k:k + 1] * g)
+        g = gin
+    return dl
 
 
 def run(blob, rad, alb, nrm, dep, rnd=bf16):
@@ -155,20 +194,12 @@ def backward(blob, rad, alb, nrm, dep, target, rnd=bf16):
         w, b = wb[i]
         acts[i] = conv3x3(staged[i], w, b, LAYERS[i][7], rnd)
     r = rad.astype(np.float32) if rnd is bf16 else exact(rad)
-    h, wd, _ = r.shape
     last = len(LAYERS) - 1
-    l = acts[last][..., :25].astype(np.float64)
-    e = np.exp(l - l.max(-1, keepdims=True))
-    wsoft = e / e.sum(-1, keepdims=True)
-    ys, xs = np.arange(h), np.arange(wd)
-    nb = [r[np.clip(ys + k // 5 - 2, 0, h - 1)][:, np.clip(xs + k % 5 - 2, 0, wd - 1)].astype(np.float64)
-          for k in range(25)]
-    out = sum(wsoft[..., k:k + 1] * nb[k] for k in range(25))
+    out = kpn_cascade(acts[last].astype(np.float64), r.astype(np.float64))[-1]
     t = np.asarray(target, np.float64)
     n = out.size
     g = 2.0 * (out - t) / (t ** 2 + 0.01) / n
-    dwk = np.stack([(g * nb[k]).sum(-1) for k in range(25)], -1)
-    dl = wsoft * (dwk - (wsoft * dwk).sum(-1, keepdims=True))
+    dl = kpn_cascade_back(acts[last], r, g)
     gm = (lambda v: bf16(v.astype(np.float32))) if rnd is bf16 else exact
     dact = {i: None for i in range(-1, len(LAYERS))}
     dact[last] = gm(dl)

@@ -546,7 +546,7 @@ CNN_LAYERS = [
     (96, 64, 64, 2, 11, 5, 2, True), (64, 0, 64, 0, 12, -1, 2, True),
     (64, 48, 48, 2, 13, 3, 1, True), (48, 0, 48, 0, 14, -1, 1, True),
     (48, 32, 32, 2, 15, 1, 0, True), (32, 0, 32, 0, 16, -1, 0, True),
-    (32, 0, 25, 0, 17, -1, 0, False),
+    (32, 0, 100, 0, 17, -1, 0, False),
 ]
 
 

@@ -4,7 +4,8 @@
 // Input per pixel: log(1 + radiance) (3), albedo (3), normal (3), 1 / (1 + depth) (0 where no
 // surface) - 10 channels. Five levels with 32 / 48 / 64 / 96 / 128 channels: two 3x3 conv + ReLU
 // per level, 2x2 average pooling down, nearest upsampling and skip concatenation up, and a last
-// 3x3 conv to 25 logits whose softmax weights the 5x5 neighbourhood of the noisy radiance.
+// 3x3 conv to 4 x 25 logits: four predicted 5x5 kernels (softmax-normalised) applied in cascade
+// to the noisy radiance with their taps 1, 2, 4 and 8 pixels apart (a 61-pixel footprint).
 // Convolutions run as implicit GEMMs on bf16 MFMA (v_mfma_f32_16x16x32_bf16) with fp32
 // accumulation; activations are bf16, NHWC with the channel stride padded to a multiple of 32.
 #pragma once
@@ -26,6 +27,8 @@ struct CnnLayerSpec {
     bool relu;
 };
 
+// Stages of the predicted-kernel cascade (taps 2^stage pixels apart).
+constexpr int kKpnStages = 4;
 // read_layer index of layer L's masked output gradient from the last training step.
 constexpr int kGradLayerBase = 1000;
 // The fixed topology (19 conv layers).
@@ -87,6 +90,8 @@ private:
     std::vector<float*> b_;     // per layer [coutPad]
     uint16_t* feat_ = nullptr;  // input features, stride 32
     std::vector<uint16_t*> act_;
+    float* casc_[kKpnStages - 1] = {};  // radiance after each cascade stage but the last
+    float* gcas_[kKpnStages] = {};      // loss gradients of the cascade stages' outputs
     uint32_t W_ = 0, H_ = 0;
 };
 

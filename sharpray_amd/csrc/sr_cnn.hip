@@ -30,6 +30,9 @@ constexpr int kHW = kTW + 2, kHH = kTH + 2;    // halo tile
 constexpr int kHalo = kHW * kHH;               // 180 pixels
 constexpr int kFeat = 10;                      // input feature channels
 constexpr int kTaps = 25;                      // 5 x 5 predicted kernel
+constexpr int kStages = kKpnStages;            // kernel cascade, taps 1, 2, 4 and 8 pixels apart
+constexpr int kLogits = kTaps * kStages;       // 100 logits per pixel
+constexpr int kLogitStride = 128;              // their padded channel stride
 
 int pad32(int c) { return (c + 31) & ~31; }
 
@@ -156,22 +159,24 @@ __global__ void k_features(const float* rad, const float* alb, const float* nrm,
     o[3] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-// Softmax of the 25 logits weights the 5 x 5 neighbourhood (edge-clamped) of the noisy radiance.
-__global__ void k_kpn_apply(const uint16_t* logits, const float* rad, int W, int H, float* out) {
+// One stage of the kernel cascade: out(p) = sum_k w_k(p) in(clamp(p + d (dx_k, dy_k))), w = softmax of
+// the stage's 25 logits, 5 x 5 taps spaced d = 2^stage pixels, edges clamped.
+__global__ void k_kpn_stage(const uint16_t* logits, int stage, const float* in, int W, int H, float* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= W * H) return;
-    const int x = i % W, y = i / W;
+    const int x = i % W, y = i / W, d = 1 << stage;
+    const uint16_t* lg = logits + (size_t)i * kLogitStride + stage * kTaps;
     float l[kTaps];
     float m = -INFINITY;
     for (int k = 0; k < kTaps; ++k) {
-        l[k] = bf2f(logits[(size_t)i * 32 + k]);
+        l[k] = bf2f(lg[k]);
         m = fmaxf(m, l[k]);
     }
     float s = 0.0f, r = 0.0f, g = 0.0f, b = 0.0f;
     for (int k = 0; k < kTaps; ++k) {
         const float e = expf(l[k] - m);
-        const int yy = min(max(y + k / 5 - 2, 0), H - 1), xx = min(max(x + k % 5 - 2, 0), W - 1);
-        const float* p = rad + ((size_t)yy * W + xx) * 3;
+        const int yy = min(max(y + d * (k / 5 - 2), 0), H - 1), xx = min(max(x + d * (k % 5 - 2), 0), W - 1);
+        const float* p = in + ((size_t)yy * W + xx) * 3;
         s += e;
         r += e * p[0];
         g += e * p[1];
@@ -183,53 +188,17 @@ __global__ void k_kpn_apply(const uint16_t* logits, const float* rad, int W, int
 }
 
 // ---- Training kernels --------------------------------------------------------------------------------
-// Loss mean (out - t)^2 / (t^2 + 0.01) and its gradient through the 5x5 kernel application and
-// the softmax: g_c = dL/dout_c, dL/dw_k = sum_c g_c rad_k,c, dL/dlogit_k = w_k (dL/dw_k - sum_j w_j dL/dw_j).
-__global__ void k_kpn_loss_back(const uint16_t* logits, const float* rad, const float* target, int W, int H,
-                                float invN, uint16_t* dLogits, float* loss) {
+// Loss mean (out - t)^2 / (t^2 + 0.01) and its gradient g = dL/dout.
+__global__ void k_kpn_loss(const float* out, const float* target, int n, float invN, float* g, float* loss) {
     __shared__ float part[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     float lsum = 0.0f;
-    if (i < W * H) {
-        const int x = i % W, y = i / W;
-        float l[kTaps], w[kTaps];
-        float m = -INFINITY;
-        for (int k = 0; k < kTaps; ++k) {
-            l[k] = bf2f(logits[(size_t)i * 32 + k]);
-            m = fmaxf(m, l[k]);
-        }
-        float s = 0.0f;
-        for (int k = 0; k < kTaps; ++k) {
-            w[k] = expf(l[k] - m);
-            s += w[k];
-        }
-        float o[3] = {0.0f, 0.0f, 0.0f};
-        const float* nb[kTaps];
-        for (int k = 0; k < kTaps; ++k) {
-            w[k] /= s;
-            const int yy = min(max(y + k / 5 - 2, 0), H - 1), xx = min(max(x + k % 5 - 2, 0), W - 1);
-            nb[k] = rad + ((size_t)yy * W + xx) * 3;
-            for (int c = 0; c < 3; ++c) o[c] += w[k] * nb[k][c];
-        }
-        float g[3];
+    if (i < n) {
         for (int c = 0; c < 3; ++c) {
-            const float t = target[(size_t)i * 3 + c], d = o[c] - t, den = t * t + 0.01f;
+            const float t = target[(size_t)i * 3 + c], d = out[(size_t)i * 3 + c] - t, den = t * t + 0.01f;
             lsum += d * d / den * invN;
-            g[c] = 2.0f * d / den * invN;
+            g[(size_t)i * 3 + c] = 2.0f * d / den * invN;
         }
-        float dw[kTaps], dot = 0.0f;
-        for (int k = 0; k < kTaps; ++k) {
-            dw[k] = g[0] * nb[k][0] + g[1] * nb[k][1] + g[2] * nb[k][2];
-            dot += w[k] * dw[k];
-        }
-        uint32_t packed[16];
-        for (int k = 0; k < 16; ++k) packed[k] = 0u;
-        for (int k = 0; k < kTaps; ++k) {
-            const uint32_t b = f2bf(w[k] * (dw[k] - dot));
-            packed[k >> 1] |= (k & 1) ? (b << 16) : b;
-        }
-        uint4* d = (uint4*)(dLogits + (size_t)i * 32);
-        for (int q = 0; q < 4; ++q) d[q] = make_uint4(packed[q * 4], packed[q * 4 + 1], packed[q * 4 + 2], packed[q * 4 + 3]);
     }
     part[threadIdx.x] = lsum;
     __syncthreads();
@@ -238,6 +207,48 @@ __global__ void k_kpn_loss_back(const uint16_t* logits, const float* rad, const 
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicAdd(loss, part[0]);
+}
+
+// Backward of one cascade stage: with g = dL/dout(p), dL/dw_k = sum_c g_c in(q_k)_c and the softmax
+// gives dL/dlogit_k = w_k (dL/dw_k - sum_j w_j dL/dw_j) (written as bf16 into the stage's logit
+// channels); dL/din(q_k) += w_k g is scattered into gin when it is needed.
+__global__ void k_kpn_stage_back(const uint16_t* logits, int stage, const float* in, const float* gout, int W, int H,
+                                 uint16_t* dlog, float* gin) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W * H) return;
+    const int x = i % W, y = i / W, d = 1 << stage;
+    const uint16_t* lg = logits + (size_t)i * kLogitStride + stage * kTaps;
+    float w[kTaps];
+    int q[kTaps];
+    float m = -INFINITY;
+    for (int k = 0; k < kTaps; ++k) {
+        w[k] = bf2f(lg[k]);
+        m = fmaxf(m, w[k]);
+    }
+    float s = 0.0f;
+    for (int k = 0; k < kTaps; ++k) {
+        w[k] = expf(w[k] - m);
+        s += w[k];
+    }
+    const float g0 = gout[(size_t)i * 3], g1 = gout[(size_t)i * 3 + 1], g2 = gout[(size_t)i * 3 + 2];
+    float dw[kTaps], dot = 0.0f;
+    for (int k = 0; k < kTaps; ++k) {
+        w[k] /= s;
+        const int yy = min(max(y + d * (k / 5 - 2), 0), H - 1), xx = min(max(x + d * (k % 5 - 2), 0), W - 1);
+        q[k] = yy * W + xx;
+        const float* p = in + (size_t)q[k] * 3;
+        dw[k] = g0 * p[0] + g1 * p[1] + g2 * p[2];
+        dot += w[k] * dw[k];
+    }
+    uint16_t* dl = dlog + (size_t)i * kLogitStride + stage * kTaps;
+    for (int k = 0; k < kTaps; ++k) dl[k] = f2bf(w[k] * (dw[k] - dot));
+    if (gin)
+        for (int k = 0; k < kTaps; ++k) {
+            float* t = gin + (size_t)q[k] * 3;
+            unsafeAtomicAdd(&t[0], w[k] * g0);
+            unsafeAtomicAdd(&t[1], w[k] * g1);
+            unsafeAtomicAdd(&t[2], w[k] * g2);
+        }
 }
 
 // Output gradient of a layer through its ReLU (kept where the bf16 activation is positive), as
@@ -429,7 +440,7 @@ const std::vector<CnnLayerSpec>& cnn_layers() {
         {48, 0, 48, 0, 14, -1, 1, true},      // 15
         {48, 32, 32, 2, 15, 1, 0, true},      // 16
         {32, 0, 32, 0, 16, -1, 0, true},      // 17
-        {32, 0, kTaps, 0, 17, -1, 0, false},  // 18 kernel logits
+        {32, 0, kLogits, 0, 17, -1, 0, false},  // 18 kernel logits of the four cascade stages
     };
     return L;
 }
@@ -464,6 +475,10 @@ CnnNet::~CnnNet() {
 void CnnNet::release_acts() {
     if (feat_) (void)hipFree(feat_);
     feat_ = nullptr;
+    for (float*& p : casc_) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
     for (uint16_t* p : act_) (void)hipFree(p);
     act_.clear();
     W_ = H_ = 0;
@@ -508,6 +523,8 @@ bool CnnNet::ensure(uint32_t W, uint32_t H) {
     release_acts();
     const std::vector<CnnLayerSpec>& Ls = cnn_layers();
     if (hipMalloc((void**)&feat_, (size_t)W * H * 32 * 2) != hipSuccess) return false;
+    for (float*& p : casc_)
+        if (hipMalloc((void**)&p, (size_t)W * H * 12) != hipSuccess) return false;
     act_.assign(Ls.size(), nullptr);
     for (size_t li = 0; li < Ls.size(); ++li) {
         uint32_t w, h;
@@ -560,7 +577,9 @@ bool launch_conv(const ConvArgs& A, hipStream_t s) {
         case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, A); break;
         case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, s, A); break;
         case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, A); break;
+        case 5: hipLaunchKernelGGL(k_conv3x3<5>, grid, dim3(256), 0, s, A); break;
         case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, dim3(256), 0, s, A); break;
+        case 7: hipLaunchKernelGGL(k_conv3x3<7>, grid, dim3(256), 0, s, A); break;
         case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(256), 0, s, A); break;
         case 10: hipLaunchKernelGGL(k_conv3x3<10>, grid, dim3(256), 0, s, A); break;
         case 14: hipLaunchKernelGGL(k_conv3x3<14>, grid, dim3(256), 0, s, A); break;
@@ -578,8 +597,12 @@ bool CnnNet::run(const float* rad, const float* alb, const float* nrm, const flo
     const std::vector<CnnLayerSpec>& Ls = cnn_layers();
     for (size_t li = 0; li < Ls.size(); ++li)
         if (!launch_conv(layer_args((int)li, W, H, feat_, act_, w_, b_), s)) return false;
-    if (out)
-        hipLaunchKernelGGL(k_kpn_apply, dim3((n + 255) / 256), dim3(256), 0, s, act_.back(), rad, (int)W, (int)H, out);
+    if (out) {
+        const dim3 g((n + 255) / 256);
+        for (int st = 0; st < kStages; ++st)
+            hipLaunchKernelGGL(k_kpn_stage, g, dim3(256), 0, s, act_.back(), st, st ? casc_[st - 1] : rad, (int)W,
+                               (int)H, st == kStages - 1 ? out : casc_[st]);
+    }
     return hipGetLastError() == hipSuccess;
 }
 
@@ -677,6 +700,10 @@ void CnnNet::release_train_acts() {
     if (out_) (void)hipFree(out_);
     dx_ = nullptr;
     out_ = nullptr;
+    for (float*& p : gcas_) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
     tW_ = tH_ = 0;
 }
 
@@ -695,6 +722,8 @@ bool CnnNet::ensure_train(uint32_t W, uint32_t H) {
         dxMax = std::max(dxMax, (size_t)w * h * layer_cin_pad((int)li));
     }
     if (!dalloc(dx_, dxMax) || !dalloc(out_, (size_t)W * H * 3)) return false;
+    for (float*& p : gcas_)
+        if (!dalloc(p, (size_t)W * H * 3)) return false;
     tW_ = W;
     tH_ = H;
     return true;
@@ -702,12 +731,18 @@ bool CnnNet::ensure_train(uint32_t W, uint32_t H) {
 
 bool CnnNet::train_step(const float* rad, const float* alb, const float* nrm, const float* dep, const float* target,
                         uint32_t W, uint32_t H, float lr, float* lossDev, hipStream_t s, bool apply) {
-    if (!training() || !run(rad, alb, nrm, dep, W, H, nullptr, s) || !ensure_train(W, H)) return false;
+    if (!training() || !ensure_train(W, H) || !run(rad, alb, nrm, dep, W, H, out_, s)) return false;
     const std::vector<CnnLayerSpec>& Ls = cnn_layers();
     const int nl = (int)Ls.size();
     const int n = (int)(W * H);
-    hipLaunchKernelGGL(k_kpn_loss_back, dim3((n + 255) / 256), dim3(256), 0, s, act_[nl - 1], rad, target, (int)W,
-                       (int)H, 1.0f / (3.0f * (float)n), dyp_[nl - 1], lossDev);
+    const dim3 g1((n + 255) / 256);
+    hipLaunchKernelGGL(k_kpn_loss, g1, dim3(256), 0, s, out_, target, n, 1.0f / (3.0f * (float)n),
+                       gcas_[kStages - 1], lossDev);
+    for (int st = 0; st < kStages - 1; ++st)
+        if (hipMemsetAsync(gcas_[st], 0, (size_t)n * 12, s) != hipSuccess) return false;
+    for (int st = kStages - 1; st >= 0; --st)
+        hipLaunchKernelGGL(k_kpn_stage_back, g1, dim3(256), 0, s, act_[nl - 1], st, st ? casc_[st - 1] : rad,
+                           gcas_[st], (int)W, (int)H, dyp_[nl - 1], st ? gcas_[st - 1] : (float*)nullptr);
     for (int li = nl - 1; li >= 0; --li) {
         const CnnLayerSpec& L = Ls[li];
         const ConvArgs F = layer_args(li, W, H, feat_, act_, w_, b_);

@@ -1,0 +1,195 @@
+"""Trains the kernel-predicting denoiser on pairs the tracer renders itself, and compares it with
+the bilateral filter against high-spp references.
+
+    python -m sharpray_amd.train_denoiser --scenes 96 --views 4 --size 128 --steps 4000 --out weights.npy
+
+Training scenes are seeded procedural scenes (fBm-displaced spheres of 10k-100k triangles at random
+positions over a ground quad, random GGX materials, log-normal or sun-and-gradient environment
+maps). Each scene is seen from `--views` random cameras, and each view contributes one example:
+the AOVs of a low-spp render (radiance, albedo, normals, depth; the sample count cycles through
+`--noisy-spp`) and the radiance of a high-spp render of the same view. Held-out scenes are
+then rendered at the evaluation size; the learned denoiser at `--eval-spp` samples is compared
+with the bilateral filter at 1x and 4x those samples (its strength chosen by grid search on the
+references, which favours it), in relMSE = mean (x - r)^2 / (r^2 + 0.01).
+"""
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from . import (RNG_COUNTER, Compositor, KpnDenoiser, Quat, Renderer, Scene, Transform, View, scenes)
+
+
+def random_scene(seed, size, spp):
+    rng = np.random.default_rng(seed)
+    s = scenes.SceneSpec(name=f"train_{seed}")
+    s.width = s.height = size
+    s.min_samples = s.max_samples = spp
+    s.bounces = int(rng.integers(3, 7))
+    s.indirect_clamp = 10.0
+    if rng.random() < 0.6:
+        s.textures.append(scenes.lognormal_env(128, 64, float(rng.uniform(0.5, 1.5)), seed + 11))
+    else:
+        sun = rng.normal(size=3)
+        sun[1] = abs(sun[1]) + 0.3
+        sun /= np.linalg.norm(sun)
+        s.textures.append(scenes.procedural_sky(128, 64, peak=float(rng.uniform(10, 60)), sun_dir=tuple(sun),
+                                                sun_width=float(rng.uniform(0.03, 0.1)), seed=seed + 12))
+    s.sky_texture = 0
+    s.sky_yaw = float(rng.uniform(0, 2 * math.pi))
+    gp, guv = scenes.quad(-1.0, 8.0)
+    s.meshes.append(scenes.triangles_build(gp, None, guv))
+    s.mesh_bins.append(8)
+    s.objects.append(scenes.ObjectSpec("ground", scenes._identity(),
+                                       scenes._mat(rng.uniform(0.2, 0.8, 3), float(rng.uniform(0.3, 1.0))), 0, 0))
+    for k in range(int(rng.integers(1, 4))):
+        rings = int(rng.integers(40, 160))
+        pos, uv, _ = scenes.fbm_sphere(rings, 2 * rings, float(rng.uniform(0.02, 0.2)), 4, seed * 7 + k,
+                                       radius=float(rng.uniform(0.3, 0.8)))
+        pos = pos.reshape(-1, 3, 3) + np.array([rng.uniform(-1.5, 1.5), 0.0, rng.uniform(1.5, 4.0)])
+        pos[..., 1] += 0.0 - pos[..., 1].min() - 1.0
+        s.meshes.append(scenes.triangles_build(pos.reshape(-1, 9).astype(np.float32), None, uv, smooth=True))
+        s.mesh_bins.append(8)
+        s.objects.append(scenes.ObjectSpec(f"blob{k}", scenes._identity(),
+                                           scenes._mat(rng.uniform(0.05, 0.95, 3), float(rng.uniform(0.05, 1.0)),
+                                                       float(rng.integers(0, 2))), 0, len(s.meshes) - 1))
+    s.view = scenes._view((float(rng.uniform(-0.8, 0.8)), float(rng.uniform(0.0, 0.8)), -1.5),
+                          float(rng.uniform(0, 15)), float(rng.uniform(45, 70)))
+    return s
+
+
+def random_view(rng):
+    """A camera behind the scene's objects, turned by up to 25 degrees and pitched down by up to 20."""
+    pos = (float(rng.uniform(-1.2, 1.2)), float(rng.uniform(-0.3, 1.0)), float(rng.uniform(-2.5, -0.8)))
+    yaw = Quat.angle_axis(math.radians(rng.uniform(-25, 25)), (0.0, 1.0, 0.0))
+    pitch = Quat.angle_axis(math.radians(rng.uniform(-5, 20)), (1.0, 0.0, 0.0))
+    return View(Transform(pos, yaw * pitch), float(math.radians(rng.uniform(40, 75))), 0.1)
+
+
+def render_pair(spec, noisy_spp, target_spp, seed, scene=None, view=None):
+    """Noisy AOVs and the target radiance of one view (counter RNG: independent seeds)."""
+    if scene is None:
+        scene, view = scenes.instantiate(spec)
+    out = {}
+    for key, spp, sd in (("noisy", noisy_spp, seed), ("target", target_spp, seed + 7777)):
+        r = Renderer(scene, view, spec.width, spec.height, 32, spp, spp, 0.0, spec.bounces, spec.indirect_clamp,
+                     rng_policy=RNG_COUNTER, rng_seed=sd)
+        r.wait()
+        out[key] = r
+        if key == "noisy":
+            H, W = spec.height, spec.width
+            aov = dict(rad=r.radiance.reshape(H, W, 3).copy(), alb=r.albedo.reshape(H, W, 3).copy(),
+                       nrm=r.normals.reshape(H, W, 3).copy(), dep=r.depth.reshape(H, W).copy(),
+                       var=r.variance.copy())
+    target = out["target"].radiance.reshape(spec.height, spec.width, 3).copy()
+    return scene, view, aov, target
+
+
+def rel_mse(x, ref):
+    ref = np.asarray(ref, np.float64)
+    x = np.asarray(x, np.float64).reshape(ref.shape)
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + 0.01)))
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--scenes", type=int, default=96)
+    ap.add_argument("--size", type=int, default=128, help="training render size")
+    ap.add_argument("--crop", type=int, default=0, help="split each training render into crops of this size")
+    ap.add_argument("--views", type=int, default=1, help="cameras per training scene")
+    ap.add_argument("--noisy-spp", default="1", help="comma-separated sample counts of the noisy inputs")
+    ap.add_argument("--target-spp", type=int, default=1024)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--eval-scenes", type=int, default=8)
+    ap.add_argument("--eval-size", type=int, default=256)
+    ap.add_argument("--eval-spp", default="1,4", help="comma-separated input sample counts to evaluate")
+    ap.add_argument("--ref-spp", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=0x5EED0100)
+    ap.add_argument("--init", default=None, help="initial weights (.npy); default He initialisation")
+    ap.add_argument("--out", default=None, help="write the trained weights (.npy, float32)")
+    args = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    trainer = Scene(0)  # a device context that only holds the denoiser and its examples
+    den = KpnDenoiser(trainer)
+    den.load(np.load(args.init) if args.init else KpnDenoiser.random_weights(args.seed))
+    den.train_begin()
+    t0 = time.perf_counter()
+    noisy_spp = [int(v) for v in str(args.noisy_spp).split(",")]
+    vrng = np.random.default_rng(args.seed ^ 0x7E3)
+    n_ex = 0
+    for i in range(args.scenes):
+        spec = random_scene(args.seed + i, args.size, noisy_spp[0])
+        scene, view = scenes.instantiate(spec)
+        for v in range(args.views):
+            spp = noisy_spp[(i * args.views + v) % len(noisy_spp)]
+            _, _, a, target = render_pair(spec, spp, args.target_spp, args.seed + 100000 + 97 * i + v,
+                                          scene, view if v == 0 else random_view(vrng))
+            c = args.crop or args.size
+            for y0 in range(0, args.size - c + 1, c):
+                for x0 in range(0, args.size - c + 1, c):
+                    sl = (slice(y0, y0 + c), slice(x0, x0 + c))
+                    den.train_add(np.ascontiguousarray(a["rad"][sl]), np.ascontiguousarray(a["alb"][sl]),
+                                  np.ascontiguousarray(a["nrm"][sl]), np.ascontiguousarray(a["dep"][sl]),
+                                  np.ascontiguousarray(target[sl]), c, c)
+                    n_ex += 1
+        if (i + 1) % 16 == 0:
+            log(f"[train] {i + 1}/{args.scenes} examples rendered ({time.perf_counter() - t0:.1f} s)")
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    chunk = 250
+    losses = []
+    for k in range(0, args.steps, chunk):
+        n = min(chunk, args.steps - k)
+        lr = args.lr * (0.1 ** (k / max(args.steps, 1)))  # exponential decay to lr / 10
+        losses.append(den.train(n, lr, args.seed + k))
+        log(f"[train] step {k + n}/{args.steps}: mean loss {losses[-1]:.5f} (lr {lr:.2e}, "
+            f"{time.perf_counter() - t0:.1f} s)")
+    t_train = time.perf_counter() - t0
+    weights = den.export()
+    if args.out:
+        np.save(args.out, weights)
+
+    # Held-out evaluation.
+    report = {"train_scenes": args.scenes, "train_examples": n_ex, "train_size": args.size, "noisy_spp": args.noisy_spp,
+              "target_spp": args.target_spp, "steps": args.steps, "data_seconds": round(t_data, 1),
+              "train_seconds": round(t_train, 1), "final_loss": losses[-1] if losses else None,
+              "eval_size": args.eval_size, "ref_spp": args.ref_spp, "eval": {}}
+    for espp in [int(v) for v in str(args.eval_spp).split(",")]:
+        res = []
+        for j in range(args.eval_scenes):
+            seed = args.seed + 50000 + j
+            spec = random_scene(seed, args.eval_size, espp)
+            scene, view, a, ref = render_pair(spec, espp, args.ref_spp, seed)
+            H = W = args.eval_size
+            ev = KpnDenoiser(scene)
+            ev.load(weights)
+            r1 = Renderer(scene, view, W, H, 32, espp, espp, 0.0, spec.bounces, spec.indirect_clamp,
+                          rng_policy=RNG_COUNTER, rng_seed=seed)
+            r1.wait()
+            row = {"noisy": rel_mse(a["rad"], ref), "kpn": rel_mse(ev.denoise(r1), ref)}
+            for mult in (1, 4):
+                spp = espp * mult
+                rb = Renderer(scene, view, W, H, 32, spp, spp, 0.0, spec.bounces, spec.indirect_clamp,
+                              rng_policy=RNG_COUNTER, rng_seed=seed + 31 * mult)
+                rb.wait()
+                best = rel_mse(rb.radiance, ref)
+                # 7x7 circular window (radius 3 px at this size), strength by grid search.
+                for strength in (0.05, 0.2, 1.0, 5.0, 25.0):
+                    comp = Compositor(1.0 / W, strength, 1.75, 0.25, 2.0, 0.125, 0.2)
+                    best = min(best, rel_mse(comp.filter(rb), ref))
+                row[f"bilateral_{mult}x"] = best
+            res.append(row)
+            log(f"[eval {espp} spp] scene {j}: " + ", ".join(f"{k} {v:.4f}" for k, v in row.items()))
+        report["eval"][f"{espp}spp"] = {k: round(float(np.mean([r[k] for r in res])), 5) for k in res[0]}
+    print(json.dumps(report), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -351,7 +351,7 @@ def _kpn_pair(spec_fn, seed):
 
 
 def test_kpn_training_gradient_matches_cpu_restatement():
-    """One backward pass of the hand-written training kernels (loss, softmax-kernel, ReLU, MFMA
+    """One backward pass of the hand-written training kernels (loss, kernel cascade, ReLU, MFMA
     data and weight gradients, pooling / upsampling / concatenation) against the reverse-mode
     restatement, step by step: each layer's output gradient recomputed on the CPU from the GPU's
     own activations and its consumers' GPU gradients, and each layer's weight and bias gradient
@@ -371,19 +371,13 @@ def test_kpn_training_gradient_matches_cpu_restatement():
     acts = {i: den.layer(i).astype(np.float64) for i in range(-1, 19)}
     dys = {i: den.layer(1000 + i).astype(np.float64) for i in range(19)}
     wb = C.split_weights(blob)
-    # Output gradient of the last layer: the softmax-kernel gradient of the loss.
+    # Output gradient of the last layer: the kernel cascade's logit gradient of the loss.
     out_logits = acts[18]
     r = ex["rad"].astype(np.float64)
-    l = out_logits[..., :25]
-    e = np.exp(l - l.max(-1, keepdims=True))
-    ws = e / e.sum(-1, keepdims=True)
-    ys, xs = np.arange(H), np.arange(W)
-    nb = [r[np.clip(ys + k // 5 - 2, 0, H - 1)][:, np.clip(xs + k % 5 - 2, 0, W - 1)] for k in range(25)]
-    out = sum(ws[..., k:k + 1] * nb[k] for k in range(25))
+    out = C.kpn_apply(out_logits, r)
     t = target.astype(np.float64)
     gr = 2.0 * (out - t) / (t ** 2 + 0.01) / out.size
-    dwk = np.stack([(gr * nb[k]).sum(-1) for k in range(25)], -1)
-    dl = C.bf16((ws * (dwk - (ws * dwk).sum(-1, keepdims=True))).astype(np.float32))
+    dl = C.bf16(C.kpn_cascade_back(out_logits, r, gr).astype(np.float32))
     errs = {"dY18": float(np.linalg.norm(dys[18] - dl) / np.linalg.norm(dl))}
     dact = {i: None for i in range(-1, 19)}
 

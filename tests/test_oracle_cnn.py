@@ -45,22 +45,63 @@ def test_pool_and_up():
     assert u.shape == (5, 3, 2) and np.array_equal(u[4, 2], p[2, 1]) and np.array_equal(u[1, 1], p[0, 0])
 
 
-def test_kpn_apply_identity_and_box():
-    rng = np.random.default_rng(1)
-    rad = rng.uniform(0, 3, (6, 7, 3)).astype(np.float32)
-    logits = np.full((6, 7, 32), -30.0, np.float32)
-    logits[..., 12] = 30.0
-    assert np.allclose(C.kpn_apply(logits, rad), rad, rtol=1e-6)
-    box = C.kpn_apply(np.zeros((6, 7, 32), np.float32), rad)
-    ref = np.zeros_like(rad)
-    for y in range(6):
-        for x in range(7):
+def _box_loop(img, d):
+    h, w, _ = img.shape
+    ref = np.zeros_like(img)
+    for y in range(h):
+        for x in range(w):
             acc = 0
             for dy in range(-2, 3):
                 for dx in range(-2, 3):
-                    acc = acc + rad[min(max(y + dy, 0), 5), min(max(x + dx, 0), 6)]
+                    acc = acc + img[min(max(y + d * dy, 0), h - 1), min(max(x + d * dx, 0), w - 1)]
             ref[y, x] = acc / 25
-    assert np.allclose(box, ref, rtol=1e-5)
+    return ref
+
+
+def test_kpn_apply_identity_and_box():
+    """One-hot logits on the centre tap of every stage are the identity; zero logits make each
+    stage an edge-clamped 5x5 box filter with taps 1, 2, 4 and 8 pixels apart, applied in order."""
+    rng = np.random.default_rng(1)
+    rad = rng.uniform(0, 3, (9, 11, 3)).astype(np.float32)
+    logits = np.full((9, 11, 128), -30.0, np.float32)
+    logits[..., [12, 37, 62, 87]] = 30.0
+    assert np.allclose(C.kpn_apply(logits, rad), rad, rtol=1e-6)
+    stages = C.kpn_cascade(np.zeros((9, 11, 128), np.float32), rad)
+    assert len(stages) == 5
+    ref = rad.astype(np.float64)
+    for st in range(4):
+        ref = _box_loop(ref, 1 << st)
+        assert np.allclose(stages[st + 1], ref, rtol=1e-5), st
+    # Only stage 1's kernel shifted: a one-hot tap at (dy, dx) = (-1, 2) moves the image by
+    # (-2, 4) pixels (edge-clamped) and the other stages stay the identity.
+    logits[..., 37] = -30.0
+    logits[..., 25 + 1 * 5 + 4] = 30.0
+    out = C.kpn_apply(logits, rad)
+    ys, xs = np.clip(np.arange(9) - 2, 0, 8), np.clip(np.arange(11) + 4, 0, 10)
+    assert np.allclose(out, rad[ys][:, xs], rtol=1e-6)
+
+
+def test_kpn_cascade_backward_matches_finite_differences():
+    """The cascade's logit gradient against central differences of a float64 weighted sum of its
+    output, for every one of the 100 logits at a few pixels."""
+    rng = np.random.default_rng(4)
+    h, w = 7, 9
+    rad = rng.uniform(0, 2, (h, w, 3))
+    logits = rng.normal(0, 1.5, (h, w, 128))
+    g = rng.normal(size=(h, w, 3))
+    dl = C.kpn_cascade_back(logits, rad, g)
+
+    def f(lg):
+        return float((C.kpn_apply(lg, rad) * g).sum())
+
+    for (y, x) in [(0, 0), (3, 4), (6, 8), (2, 7)]:
+        for c in range(100):
+            eps = 1e-6
+            hi, lo = logits.copy(), logits.copy()
+            hi[y, x, c] += eps
+            lo[y, x, c] -= eps
+            fd = (f(hi) - f(lo)) / (2 * eps)
+            assert abs(fd - dl[y, x, c]) <= 1e-6 * max(1.0, abs(fd)), (y, x, c, fd, dl[y, x, c])
 
 
 def test_weight_count_matches_library():
@@ -105,7 +146,7 @@ def test_backward_matches_finite_differences_in_float64():
         picks += list(k + rng.choice(nw, 3, replace=False)) + [k + nw + int(rng.integers(cout))]
         k += nw + cout
     for i in picks:
-        eps = 1e-6
+        eps = 2e-5  # truncation O(eps^2) stays far below the tolerance; smaller eps loses digits
         hi, lo = blob.copy(), blob.copy()
         hi[i] += eps
         lo[i] -= eps
