@@ -116,14 +116,14 @@ def bench_adjoint(args, spec, scene, view):
 
 
 def bench_cnn(args, spec, scene, view):
-    """The kernel-predicting denoiser on device-resident AOVs of one render (random He-initialised
-    weights: the cost does not depend on their values). value = frames per second."""
+    """The kernel-predicting denoiser (shipped trained weights) on device-resident AOVs of one
+    render. value = frames per second."""
     from sharpray_amd import KpnDenoiser, Renderer
     r = Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
                  spec.variance_threshold, spec.bounces, spec.indirect_clamp)
     r.wait()
     den = KpnDenoiser(scene)
-    den.load(KpnDenoiser.random_weights(0))
+    den.load(KpnDenoiser.pretrained_weights())
     den.benchmark(r, max(1, args.warmup))
     ms = den.benchmark(r, max(1, args.steps))
     flops = KpnDenoiser.flops(spec.width, spec.height)
@@ -140,7 +140,7 @@ def bench_cnn(args, spec, scene, view):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16",
-        "data": "synthetic (seeded procedural scene, random weights)",
+        "data": "synthetic (seeded procedural scene, shipped trained weights)",
         "config": {"workload": "kpn_unet_1080p", "width": spec.width, "height": spec.height,
                    "input_spp": spec.max_samples, "gflop_per_frame": round(flops / 1e9, 2)},
         "roofline": {"kernel": "k_conv3x3 (bf16 MFMA implicit GEMM)", "bound": "mfma", "achieved": round(tflops, 2),

@@ -211,6 +211,31 @@ int sr_denoise_bilateral_device(sr_ctx* ctx, const sr_denoise_params* params, co
                                 const float* d_normals, const float* d_depth, const float* d_variance,
                                 uint32_t width, uint32_t height, float* d_out_rgb);
 
+/* Tonemapper (compose.cs:5). */
+#define SR_TONEMAP_REINHARD 0
+#define SR_TONEMAP_LINEAR_SMOOTH 1
+
+/* Compositor ctor arguments (compose.cs:22-34). */
+typedef struct sr_compose_params {
+    sr_denoise_params denoise;
+    int32_t tonemapper;
+    float exposure;
+    float fog_radius;   /* denoiseFogRadius: Gaussian radius as a fraction of sqrt(w*h) */
+    float fog_strength; /* denoiseFogStrength: neighbour weight scale */
+} sr_compose_params;
+
+/* Compositor.Compose (compose.cs:74-99) of the last render without the overlay: Filter of the
+ * radiance + FilterFog of the fog radiance, Tonemap, Color.ToPixel. Writes w*h*3 bytes (R, G, B). */
+int sr_compose(sr_ctx* ctx, const sr_compose_params* params, uint8_t* out_rgb);
+/* Compositor.Preview (compose.cs:57-65) without the overlay: Tonemap(radiance + fog radiance). */
+int sr_preview(sr_ctx* ctx, int32_t tonemapper, float exposure, uint8_t* out_rgb);
+/* Compositor.FilterFog (compose.cs:206-245) of the last render's fog radiance: w*h*3 floats. */
+int sr_filter_fog(sr_ctx* ctx, float fog_radius, float fog_strength, float* out_rgb);
+/* Image.WriteBmp (image.cs:147-191): 24-bit top-down BMP of w*h*3 RGB bytes. encode writes the file
+ * bytes to out (null: only *size is set); write_bmp stores them at path (Image.Save). Host only. */
+int sr_image_encode_bmp(const uint8_t* rgb, uint32_t width, uint32_t height, uint8_t* out, uint64_t* size);
+int sr_image_write_bmp(const char* path, const uint8_t* rgb, uint32_t width, uint32_t height);
+
 /* ---- Diagnostics --------------------------------------------------------------------------- */
 /* Sample-0 primary hit of every pixel (renderer.cs:156,175; math.cs:1613-1636; scene.cs:602-625):
  * object index, triangle index (-1 for analytic shapes / misses) and world distance (+inf). */

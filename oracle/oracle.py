@@ -65,6 +65,8 @@ _SIGS = {
                                  ctypes.c_int, _F]),
     "oc_filter": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _F, _L, _F,
                           ctypes.c_float]),
+    "oc_filter_fog": (None, [ctypes.c_float, ctypes.c_float, _F, ctypes.c_uint32, ctypes.c_uint32, _F]),
+    "oc_tonemap": (None, [_F, _F, ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_char_p]),
     "oc_sky_eval": (None, [_V, _F, ctypes.c_uint32, _F, _F]),
     "oc_sky_sample": (None, [_V, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _F]),
     "oc_surface_eval": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, _F]),
@@ -251,6 +253,35 @@ def triangles_make(pos, normals=None, uv=None, smooth=False, precision="f32"):
     uvs = _keep(uv, np.float32)
     lib(precision).oc_triangles_make(f(pos), f(nrm), f(uvs), n, 1 if smooth else 0, out.ctypes.data_as(_F))
     return out
+
+
+def filter_fog(radius, strength, fog, width, height):
+    """Compositor.FilterFog (compose.cs:206-245) of a w*h*3 fog radiance image."""
+    out = np.empty((width * height, 3), np.float32)
+    lib().oc_filter_fog(float(radius), float(strength), f(fog), width, height, out.ctypes.data_as(_F))
+    return out
+
+
+def tonemap(main, fog, tonemapper, exposure):
+    """Tonemap (compose.cs:247-266) + ToPixel (math.cs:127-136) of main + fog: uint8 (n, 3)."""
+    main = np.ascontiguousarray(main, np.float32).reshape(-1, 3)
+    fog = np.ascontiguousarray(fog, np.float32).reshape(-1, 3)
+    out = ctypes.create_string_buffer(main.shape[0] * 3)
+    lib().oc_tonemap(f(main), f(fog), main.shape[0], int(tonemapper), float(exposure), out)
+    return np.frombuffer(out.raw, np.uint8).reshape(-1, 3).copy()
+
+
+def encode_bmp(rgb, width, height):
+    """Image.WriteBmp (image.cs:147-191): 24-bit BGR, top-down (negative height), rows padded to 4."""
+    import struct
+    px = np.asarray(rgb, np.uint8).reshape(height, width, 3)
+    stride = (width * 3 + 3) & ~3
+    data = bytearray()
+    data += b"BM" + struct.pack("<III", 14 + 40 + stride * height, 0, 54)
+    data += struct.pack("<IiiHHII", 40, width, -height, 1, 24, 0, stride * height) + bytes(16)
+    for y in range(height):
+        data += px[y, :, ::-1].tobytes() + bytes(stride - width * 3)
+    return bytes(data)
 
 
 def bilateral(params7, radiance, normals, depth, variance, width, height, threads=None, albedo=None,

@@ -2425,6 +2425,65 @@ void oc_filter(const float* params7, const float* radiance, const float* normals
     }
 }
 
+// ---- Compose tail ----------------------------------------------------------------------------
+// Compositor.FilterFog, compose.cs:206-245: circular Gaussian over the fog radiance with the centre
+// weight fixed at 1 and every neighbour weighted strength * exp(-d^2 / (2 r^2)).
+void oc_filter_fog(float radius, float strength, const float* fog, uint32_t W, uint32_t H, float* out) {
+    const real radiusPixels = (real)radius * std::sqrt((real)(W * H));
+    const real inv = (real)1 / (radiusPixels * radiusPixels * (real)2);
+    const int kr = (int)std::ceil(radiusPixels * (real)3);
+    for (int cy = 0; cy < (int)H; ++cy)
+        for (int cx = 0; cx < (int)W; ++cx) {
+            const size_t ci = (size_t)cy * W + cx;
+            real wsum = 1;
+            Color sum = v3(fog[ci * 3], fog[ci * 3 + 1], fog[ci * 3 + 2]);
+            for (int ky = -kr; ky <= kr; ++ky) {
+                const int ky2 = ky * ky;
+                for (int kx = -kr; kx <= kr; ++kx) {
+                    if (kx == 0 && ky == 0) continue;
+                    const real kd2 = (real)(kx * kx + ky2);
+                    if (kd2 > (real)(kr * kr)) continue;
+                    const int nx = cx + kx, ny = cy + ky;
+                    if (nx < 0 || nx >= (int)W || ny < 0 || ny >= (int)H) continue;
+                    const real w = (real)strength * std::exp(-kd2 * inv);
+                    const size_t ni = (size_t)ny * W + nx;
+                    wsum += w;
+                    sum = sum + v3(fog[ni * 3], fog[ni * 3 + 1], fog[ni * 3 + 2]) * w;
+                }
+            }
+            const Color r = sum / wsum;
+            out[ci * 3] = (float)r.x; out[ci * 3 + 1] = (float)r.y; out[ci * 3 + 2] = (float)r.z;
+        }
+}
+
+// Color.ToPixel, math.cs:127-136: Clamp01 (Math.Clamp leaves NaN), the sRGB power approximation,
+// MathF.Max against 0 and the byte conversion (a NaN converts to 0 through the int32 route).
+static uint8_t to_srgb_byte(real c) {
+    c = c < (real)0 ? (real)0 : (c > (real)1 ? (real)1 : c);
+    const real e = nmax((real)1.055f * std::pow(c, (real)0.416666667f) - (real)0.055f, (real)0);
+    const real v = e * (real)255 + (real)0.5f;
+    return v == v ? (uint8_t)(int32_t)v : (uint8_t)0;
+}
+
+// Compositor.Tonemap, compose.cs:247-266 (0 = Reinhard, 1 = LinearSmooth), then ToPixel, for
+// main + fog per pixel (Compose, compose.cs:92; Preview, compose.cs:62).
+void oc_tonemap(const float* main, const float* fog, uint32_t n, int tonemapper, float exposure, uint8_t* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const Color rad = v3(main[i * 3], main[i * 3 + 1], main[i * 3 + 2]) + v3(fog[i * 3], fog[i * 3 + 1], fog[i * 3 + 2]);
+        const Color x = rad * (real)exposure;
+        Color t;
+        if (tonemapper == 0) {
+            t = x / (x + v3(1, 1, 1));
+        } else {
+            const real a = (real)1.8f, b = (real)1.4f, c = (real)0.5f, d = (real)1.5f;
+            t = (x * (a * x + v3(b, b, b))) / (x * (a * x + v3(c, c, c)) + v3(d, d, d));
+        }
+        out[i * 3] = to_srgb_byte(t.x);
+        out[i * 3 + 1] = to_srgb_byte(t.y);
+        out[i * 3 + 2] = to_srgb_byte(t.z);
+    }
+}
+
 // ---- Adjoint (no reference counterpart) -----------------------------------------------------
 // Parameters: per object albedo rgb + roughness (4 floats), then the sky texture's texels (3 per
 // texel, row-major) when the sky is a texture.

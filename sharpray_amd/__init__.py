@@ -7,6 +7,7 @@ reference's blittable layouts (Color/Vec3 = 3 floats, Vec2 = 2, Triangle = 36).
 """
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -574,6 +575,41 @@ class KpnDenoiser:
             parts.append(np.zeros(cout, np.float32))
         return np.concatenate(parts)
 
+    WEIGHTS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "kpn_unet.npz")
+
+    @staticmethod
+    def _bias_mask():
+        parts = []
+        for cin_a, cin_b, cout, *_ in CNN_LAYERS:
+            parts += [np.zeros(cout * (cin_a + cin_b) * 9, bool), np.ones(cout, bool)]
+        return np.concatenate(parts)
+
+    @staticmethod
+    def save_weights(path, weights):
+        """Stores a blob compactly: conv weights as their bf16 bits (the precision the device
+        multiplies with, round to nearest even), biases as float32 (the device adds them in fp32)."""
+        w = np.ascontiguousarray(weights, np.float32)
+        m = KpnDenoiser._bias_mask()
+        if w.size != m.size:
+            raise ValueError(f"expected {m.size} floats, got {w.size}")
+        u = w[~m].view(np.uint32).astype(np.uint64)
+        bits = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+        np.savez(path, wbits=bits, bias=w[m])
+
+    @staticmethod
+    def pretrained_weights(path=None):
+        """The shipped weights, trained on tracer-rendered pairs (1 and 4 spp inputs, 1024 spp
+        targets) by sharpray_amd.train_denoiser."""
+        with np.load(path or KpnDenoiser.WEIGHTS_FILE, allow_pickle=False) as z:
+            bits, bias = z["wbits"], z["bias"]
+        m = KpnDenoiser._bias_mask()
+        if bits.size + bias.size != m.size:
+            raise ValueError("weight file does not match the network topology")
+        w = np.empty(m.size, np.float32)
+        w[~m] = (bits.astype(np.uint32) << 16).view(np.float32)
+        w[m] = bias
+        return w
+
     def load(self, weights):
         w = np.ascontiguousarray(weights, np.float32)
         check(N.lib().sr_cnn_load(self.scene.ctx, fptr(w), w.size), "cnn load")
@@ -633,16 +669,73 @@ class KpnDenoiser:
         return f.value
 
 
+TONEMAP_REINHARD = 0
+TONEMAP_LINEAR_SMOOTH = 1
+
+
+class Image:
+    """Image (image.cs): width x height RGB bytes, row-major, top row first."""
+
+    def __init__(self, width, height, pixels=None):
+        self.width, self.height = int(width), int(height)
+        self.pixels = (np.zeros((self.height, self.width, 3), np.uint8) if pixels is None
+                       else np.ascontiguousarray(pixels, np.uint8).reshape(self.height, self.width, 3))
+
+    def encode_bmp(self):
+        """The bytes Image.WriteBmp (image.cs:147-191) writes."""
+        size = ctypes.c_uint64()
+        px = self.pixels.ctypes.data_as(N.c_uint8_p)
+        check(N.lib().sr_image_encode_bmp(px, self.width, self.height, None, ctypes.byref(size)), "bmp")
+        out = np.empty(size.value, np.uint8)
+        check(N.lib().sr_image_encode_bmp(px, self.width, self.height, out.ctypes.data_as(N.c_uint8_p),
+                                          ctypes.byref(size)), "bmp")
+        return out.tobytes()
+
+    def save(self, path):
+        """Image.Save (image.cs) for .bmp paths."""
+        if not str(path).lower().endswith(".bmp"):
+            raise ValueError("only .bmp output is supported")
+        check(N.lib().sr_image_write_bmp(str(path).encode(), self.pixels.ctypes.data_as(N.c_uint8_p), self.width,
+                                         self.height), "bmp write")
+
+
 class Compositor:
-    """Compositor (compose.cs:7-267): the joint bilateral denoise of Filter (compose.cs:101-204)."""
+    """Compositor (compose.cs:7-267): Filter (compose.cs:101-204), FilterFog (206-245), Tonemap
+    (247-266), Preview and Compose (57-99; the overlay is the C# host's and is not drawn)."""
 
     def __init__(self, denoise_radius, denoise_strength, denoise_strength_max, denoise_luminance_boost,
-                 denoise_luminance_limit, denoise_normal_limit, denoise_depth_limit):
+                 denoise_luminance_limit, denoise_normal_limit, denoise_depth_limit, denoise_fog_radius=0.0,
+                 denoise_fog_strength=0.0, tonemapper=TONEMAP_REINHARD, exposure=1.0):
         p = N.DenoiseParams_()
         p.radius, p.strength, p.strength_max = denoise_radius, denoise_strength, denoise_strength_max
         p.luminance_boost, p.luminance_limit = denoise_luminance_boost, denoise_luminance_limit
         p.normal_limit, p.depth_limit = denoise_normal_limit, denoise_depth_limit
         self.params = p
+        c = N.ComposeParams_()
+        c.denoise = p
+        c.tonemapper, c.exposure = int(tonemapper), float(exposure)
+        c.fog_radius, c.fog_strength = float(denoise_fog_radius), float(denoise_fog_strength)
+        self.compose_params = c
+
+    def filter_fog(self, renderer):
+        """FilterFog (compose.cs:206-245) of the renderer's fog radiance: (w*h, 3) floats."""
+        out = np.empty((renderer.width * renderer.height, 3), np.float32)
+        c = self.compose_params
+        check(N.lib().sr_filter_fog(renderer.scene.ctx, c.fog_radius, c.fog_strength, fptr(out)), "fog filter")
+        return out
+
+    def preview(self, renderer):
+        img = Image(renderer.width, renderer.height)
+        c = self.compose_params
+        check(N.lib().sr_preview(renderer.scene.ctx, c.tonemapper, c.exposure, img.pixels.ctypes.data_as(N.c_uint8_p)),
+              "preview")
+        return img
+
+    def compose(self, renderer):
+        img = Image(renderer.width, renderer.height)
+        check(N.lib().sr_compose(renderer.scene.ctx, ctypes.byref(self.compose_params),
+                                 img.pixels.ctypes.data_as(N.c_uint8_p)), "compose")
+        return img
 
     def as_array(self):
         p = self.params

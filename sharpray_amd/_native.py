@@ -82,6 +82,15 @@ class DenoiseParams_(ctypes.Structure):
     ]
 
 
+class ComposeParams_(ctypes.Structure):
+    _fields_ = [
+        ("denoise", DenoiseParams_), ("tonemapper", ctypes.c_int32), ("exposure", ctypes.c_float),
+        ("fog_radius", ctypes.c_float), ("fog_strength", ctypes.c_float),
+    ]
+
+
+c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
+
 # name -> (restype, argtypes); every entry point declared in include/sharpray.h.
 SIGNATURES = {
     "sr_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]),
@@ -109,6 +118,12 @@ SIGNATURES = {
     "sr_denoise_bilateral_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DenoiseParams_), ctypes.c_void_p,
                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                    ctypes.c_uint32, ctypes.c_void_p]),
+    "sr_compose": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ComposeParams_), c_uint8_p]),
+    "sr_preview": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, c_uint8_p]),
+    "sr_filter_fog": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, c_float_p]),
+    "sr_image_encode_bmp": (ctypes.c_int, [c_uint8_p, ctypes.c_uint32, ctypes.c_uint32, c_uint8_p,
+                                           ctypes.POINTER(ctypes.c_uint64)]),
+    "sr_image_write_bmp": (ctypes.c_int, [ctypes.c_char_p, c_uint8_p, ctypes.c_uint32, ctypes.c_uint32]),
     "sr_trace_primary": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(View_), ctypes.c_uint32, ctypes.c_uint32,
                                         c_int32_p, c_int32_p, c_float_p]),
     "sr_bvh_export": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_int32_p, c_uint32_p, c_uint32_p]),

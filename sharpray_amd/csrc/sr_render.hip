@@ -697,6 +697,65 @@ __global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* ra
     out[ci * 3] = r.x; out[ci * 3 + 1] = r.y; out[ci * 3 + 2] = r.z;
 }
 
+// Compositor.FilterFog (compose.cs:206-245): circular Gaussian of the fog radiance, centre weight 1,
+// neighbours strength * exp(-d^2 / (2 r^2)), same loop order as the reference.
+__global__ void __launch_bounds__(256) k_filter_fog(float radiusInv, float strength, int32_t kr, const float* fog,
+                                                     int32_t W, int32_t H, float* out) {
+    const int32_t cx = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int32_t cy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (cx >= W || cy >= H) return;
+    const int32_t ci = cy * W + cx;
+    float wSum = 1.0f;
+    vec3 rSum = mk3(fog[ci * 3], fog[ci * 3 + 1], fog[ci * 3 + 2]);
+    for (int32_t ky = -kr; ky <= kr; ++ky) {
+        const int32_t ky2 = ky * ky;
+        const int32_t ny = cy + ky;
+        for (int32_t kx = -kr; kx <= kr; ++kx) {
+            if (kx == 0 && ky == 0) continue;
+            const float kd2 = (float)(kx * kx + ky2);
+            if (kd2 > (float)(kr * kr)) continue;
+            const int32_t nx = cx + kx;
+            if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+            const int32_t ni = ny * W + nx;
+            const float w = strength * expf(-kd2 * radiusInv);
+            wSum += w;
+            rSum = rSum + mk3(fog[ni * 3], fog[ni * 3 + 1], fog[ni * 3 + 2]) * w;
+        }
+    }
+    const vec3 r = rSum / wSum;
+    out[ci * 3] = r.x; out[ci * 3 + 1] = r.y; out[ci * 3 + 2] = r.z;
+}
+
+// Color.ToPixel (math.cs:127-136): Clamp01 (Math.Clamp leaves NaN), sRGB power approximation, MathF.Max
+// against 0, byte conversion (NaN converts to 0, as through .NET's int32 route).
+__device__ __forceinline__ uint32_t srgb_byte(float c) {
+    c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+    const float e = net_max(1.055f * powf(c, 0.416666667f) - 0.055f, 0.0f);
+    const float v = e * 255.0f + 0.5f;
+    return v == v ? (uint32_t)(int32_t)v & 0xFFu : 0u;
+}
+
+// Compositor.Tonemap (compose.cs:247-266) of main + fog, then ToPixel: 3 bytes (R, G, B) per pixel.
+// tonemapper 0 = Reinhard, 1 = LinearSmooth.
+__global__ void __launch_bounds__(256) k_tonemap(const float* mainRad, const float* fog, int32_t n, int32_t tonemapper,
+                                                  float exposure, uint8_t* out) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const vec3 rad = mk3(mainRad[i * 3], mainRad[i * 3 + 1], mainRad[i * 3 + 2]) +
+                     mk3(fog[i * 3], fog[i * 3 + 1], fog[i * 3 + 2]);
+    const vec3 x = rad * exposure;
+    vec3 t;
+    if (tonemapper == 0) {
+        t = x / (x + splat3(1.0f));
+    } else {
+        const float a = 1.8f, b = 1.4f, c = 0.5f, d = 1.5f;
+        t = (x * (a * x + splat3(b))) / (x * (a * x + splat3(c)) + splat3(d));
+    }
+    out[i * 3] = (uint8_t)srgb_byte(t.x);
+    out[i * 3 + 1] = (uint8_t)srgb_byte(t.y);
+    out[i * 3 + 2] = (uint8_t)srgb_byte(t.z);
+}
+
 }  // namespace
 
 // =================================================================================================
@@ -1771,6 +1830,142 @@ int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* p, float* out_rgb
 
 int sr_denoise_bilateral_guided(sr_ctx* ctx, const sr_denoise_params* p, float albedo_limit, float* out_rgb) {
     return filter_to_host(ctx, p, true, albedo_limit, out_rgb);
+}
+
+// ---- Compose tail ----------------------------------------------------------------------------------
+namespace {
+int launch_filter_fog(sr_ctx* ctx, float radius, float strength, const float* fog, uint32_t W, uint32_t H,
+                      float* out) {
+    if (!(radius >= 0.0f)) return fail(SR_ERR_INVALID, "fog filter: radius must be >= 0");
+    // FilterFog's per-image constants (compose.cs:211-213).
+    const float radiusPixels = radius * sqrtf((float)(W * H));
+    const float radiusInv = 1.0f / (radiusPixels * radiusPixels * 2.0f);
+    const float krf = ceilf(radiusPixels * 3.0f);
+    if (krf > 4096.0f) return fail(SR_ERR_LIMIT, "fog filter: kernel radius above 4096 pixels");
+    const dim3 grid((W + 15) / 16, (H + 15) / 16);
+    hipLaunchKernelGGL(k_filter_fog, grid, dim3(256), 0, ctx->stream, radiusInv, strength, (int32_t)krf, fog,
+                       (int32_t)W, (int32_t)H, out);
+    HIP_TRY(hipGetLastError());
+    return SR_OK;
+}
+
+// The last render's AOVs, finished, on a device context.
+int compose_ready(sr_ctx* ctx) {
+    if (!ctx->rendering) return fail(SR_ERR_STATE, "no render to compose");
+    int rc = wait_render(ctx);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    return SR_OK;
+}
+
+int tonemap_to_host(sr_ctx* ctx, const float* dMain, const float* dFog, int32_t tonemapper, float exposure,
+                    uint8_t* out_rgb) {
+    if (tonemapper != SR_TONEMAP_REINHARD && tonemapper != SR_TONEMAP_LINEAR_SMOOTH)
+        return fail(SR_ERR_INVALID, "unknown tonemapper");
+    const uint32_t n = ctx->params.width * ctx->params.height;
+    uint8_t* dOut = nullptr;
+    HIP_TRY(hipMalloc((void**)&dOut, (size_t)n * 3));
+    hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, dMain, dFog, (int32_t)n,
+                       tonemapper, exposure, dOut);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out_rgb, dOut, (size_t)n * 3, hipMemcpyDeviceToHost);
+    (void)hipFree(dOut);
+    if (e != hipSuccess) return fail(SR_ERR_HIP, hipGetErrorString(e));
+    return SR_OK;
+}
+}  // namespace
+
+int sr_filter_fog(sr_ctx* ctx, float fog_radius, float fog_strength, float* out_rgb) {
+    if (!ctx || !out_rgb) return fail(SR_ERR_INVALID, "fog filter: null argument");
+    int rc = compose_ready(ctx);
+    if (rc) return rc;
+    const uint32_t W = ctx->params.width, H = ctx->params.height;
+    float* dOut = nullptr;
+    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 12));
+    rc = launch_filter_fog(ctx, fog_radius, fog_strength, ctx->dRadFog.ptr, W, H, dOut);
+    hipError_t e = rc == SR_OK ? hipStreamSynchronize(ctx->stream) : hipSuccess;
+    if (rc == SR_OK && e == hipSuccess) e = hipMemcpy(out_rgb, dOut, (size_t)W * H * 12, hipMemcpyDeviceToHost);
+    (void)hipFree(dOut);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(SR_ERR_HIP, hipGetErrorString(e));
+    return SR_OK;
+}
+
+int sr_preview(sr_ctx* ctx, int32_t tonemapper, float exposure, uint8_t* out_rgb) {
+    if (!ctx || !out_rgb) return fail(SR_ERR_INVALID, "preview: null argument");
+    int rc = compose_ready(ctx);
+    if (rc) return rc;
+    return tonemap_to_host(ctx, ctx->dRad.ptr, ctx->dRadFog.ptr, tonemapper, exposure, out_rgb);
+}
+
+int sr_compose(sr_ctx* ctx, const sr_compose_params* p, uint8_t* out_rgb) {
+    if (!ctx || !p || !out_rgb) return fail(SR_ERR_INVALID, "compose: null argument");
+    int rc = compose_ready(ctx);
+    if (rc) return rc;
+    const uint32_t W = ctx->params.width, H = ctx->params.height;
+    float* dTmp = nullptr;
+    HIP_TRY(hipMalloc((void**)&dTmp, (size_t)W * H * 24));
+    float* dMain = dTmp;
+    float* dFog = dTmp + (size_t)W * H * 3;
+    rc = launch_filter(ctx, &p->denoise, ctx->dRad.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, ctx->dVar.ptr, nullptr,
+                       0.0f, W, H, dMain);
+    if (rc == SR_OK) rc = launch_filter_fog(ctx, p->fog_radius, p->fog_strength, ctx->dRadFog.ptr, W, H, dFog);
+    if (rc == SR_OK) rc = tonemap_to_host(ctx, dMain, dFog, p->tonemapper, p->exposure, out_rgb);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(dTmp);
+    return rc;
+}
+
+int sr_image_encode_bmp(const uint8_t* rgb, uint32_t width, uint32_t height, uint8_t* out, uint64_t* size) {
+    if (!rgb || !size || width == 0 || height == 0) return fail(SR_ERR_INVALID, "bmp: bad arguments");
+    if (width > (1u << 20) || height > (1u << 20)) return fail(SR_ERR_LIMIT, "bmp: image above 2^20 pixels a side");
+    // Image.WriteBmp (image.cs:147-191): 14-byte file header, 40-byte DIB header, 24-bit BGR rows
+    // padded to 4 bytes, top-down (negative height).
+    const uint64_t stride = ((uint64_t)width * 3 + 3) & ~3ull;
+    const uint64_t pixelBytes = stride * height;
+    const uint64_t total = 54 + pixelBytes;
+    if (total > 0xFFFFFFFFull) return fail(SR_ERR_LIMIT, "bmp: file above 4 GiB");
+    *size = total;
+    if (!out) return SR_OK;
+    auto put32 = [](uint8_t* p, uint32_t v) { p[0] = v & 255; p[1] = (v >> 8) & 255; p[2] = (v >> 16) & 255; p[3] = v >> 24; };
+    memset(out, 0, 54);
+    out[0] = 'B';
+    out[1] = 'M';
+    put32(out + 2, (uint32_t)total);
+    put32(out + 10, 54);
+    put32(out + 14, 40);
+    put32(out + 18, width);
+    put32(out + 22, (uint32_t)(-(int32_t)height));
+    out[26] = 1;   // colour planes
+    out[28] = 24;  // bits per pixel
+    put32(out + 34, (uint32_t)pixelBytes);
+    for (uint32_t y = 0; y < height; ++y) {
+        uint8_t* row = out + 54 + stride * y;
+        const uint8_t* src = rgb + (size_t)y * width * 3;
+        for (uint32_t x = 0; x < width; ++x) {
+            row[x * 3] = src[x * 3 + 2];
+            row[x * 3 + 1] = src[x * 3 + 1];
+            row[x * 3 + 2] = src[x * 3];
+        }
+        memset(row + (size_t)width * 3, 0, stride - (uint64_t)width * 3);
+    }
+    return SR_OK;
+}
+
+int sr_image_write_bmp(const char* path, const uint8_t* rgb, uint32_t width, uint32_t height) {
+    if (!path) return fail(SR_ERR_INVALID, "bmp: null path");
+    uint64_t size = 0;
+    int rc = sr_image_encode_bmp(rgb, width, height, nullptr, &size);
+    if (rc) return rc;
+    std::vector<uint8_t> buf(size);
+    rc = sr_image_encode_bmp(rgb, width, height, buf.data(), &size);
+    if (rc) return rc;
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(SR_ERR_INVALID, std::string("bmp: cannot open ") + path);
+    const bool ok = fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    if (fclose(f) != 0 || !ok) return fail(SR_ERR_INVALID, std::string("bmp: write failed: ") + path);
+    return SR_OK;
 }
 
 // ---- Adjoint -------------------------------------------------------------------------------------

@@ -103,6 +103,8 @@ def main(argv=None):
     ap.add_argument("--noisy-spp", default="1", help="comma-separated sample counts of the noisy inputs")
     ap.add_argument("--target-spp", type=int, default=1024)
     ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--data-seconds", type=float, default=0.0,
+                    help="stop rendering training scenes after this many seconds (0: no limit)")
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--eval-scenes", type=int, default=8)
     ap.add_argument("--eval-size", type=int, default=256)
@@ -110,7 +112,7 @@ def main(argv=None):
     ap.add_argument("--ref-spp", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0x5EED0100)
     ap.add_argument("--init", default=None, help="initial weights (.npy); default He initialisation")
-    ap.add_argument("--out", default=None, help="write the trained weights (.npy, float32)")
+    ap.add_argument("--out", default=None, help="write the trained weights (.npy float32, or .npz as shipped)")
     args = ap.parse_args(argv)
 
     def log(msg):
@@ -124,13 +126,23 @@ def main(argv=None):
     noisy_spp = [int(v) for v in str(args.noisy_spp).split(",")]
     vrng = np.random.default_rng(args.seed ^ 0x7E3)
     n_ex = 0
+    t_build = t_render = 0.0
+    n_scenes = 0
     for i in range(args.scenes):
+        if args.data_seconds and time.perf_counter() - t0 > args.data_seconds:
+            log(f"[train] data time limit reached after {i} scenes")
+            break
+        ts = time.perf_counter()
         spec = random_scene(args.seed + i, args.size, noisy_spp[0])
         scene, view = scenes.instantiate(spec)
+        t_build += time.perf_counter() - ts
+        n_scenes += 1
         for v in range(args.views):
             spp = noisy_spp[(i * args.views + v) % len(noisy_spp)]
+            ts = time.perf_counter()
             _, _, a, target = render_pair(spec, spp, args.target_spp, args.seed + 100000 + 97 * i + v,
                                           scene, view if v == 0 else random_view(vrng))
+            t_render += time.perf_counter() - ts
             c = args.crop or args.size
             for y0 in range(0, args.size - c + 1, c):
                 for x0 in range(0, args.size - c + 1, c):
@@ -140,7 +152,8 @@ def main(argv=None):
                                   np.ascontiguousarray(target[sl]), c, c)
                     n_ex += 1
         if (i + 1) % 16 == 0:
-            log(f"[train] {i + 1}/{args.scenes} examples rendered ({time.perf_counter() - t0:.1f} s)")
+            log(f"[train] {i + 1}/{args.scenes} scenes, {n_ex} examples ({time.perf_counter() - t0:.1f} s: "
+                f"scene build {t_build:.1f} s, rendering {t_render:.1f} s)")
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
     chunk = 250
@@ -154,10 +167,13 @@ def main(argv=None):
     t_train = time.perf_counter() - t0
     weights = den.export()
     if args.out:
-        np.save(args.out, weights)
+        if args.out.endswith(".npz"):
+            KpnDenoiser.save_weights(args.out, weights)
+        else:
+            np.save(args.out, weights)
 
     # Held-out evaluation.
-    report = {"train_scenes": args.scenes, "train_examples": n_ex, "train_size": args.size, "noisy_spp": args.noisy_spp,
+    report = {"train_scenes": n_scenes, "train_examples": n_ex, "train_size": args.size, "noisy_spp": args.noisy_spp,
               "target_spp": args.target_spp, "steps": args.steps, "data_seconds": round(t_data, 1),
               "train_seconds": round(t_train, 1), "final_loss": losses[-1] if losses else None,
               "eval_size": args.eval_size, "ref_spp": args.ref_spp, "eval": {}}

@@ -442,3 +442,60 @@ def test_kpn_training_reduces_loss():
     last = den.train(10, 1e-3, 2)
     print(f"kpn training: loss {first:.4f} -> {last:.4f}", flush=True)
     assert last < 0.7 * first
+
+
+@pytest.mark.parametrize("scene_index", [0, 1])
+def test_kpn_shipped_weights_denoise_held_out_scene(scene_index):
+    """The shipped weights on held-out procedural scenes (index 0: sun-and-gradient sky, 1:
+    log-normal sky) at 128^2 and 1 spp, against a 1024 spp reference: the learned denoiser must
+    cut relMSE = mean (x - r)^2 / (r^2 + 0.01) well below the noisy input's and below the best
+    bilateral filter (strength grid) at the same sample count."""
+    from sharpray_amd import RNG_COUNTER, Compositor, KpnDenoiser, train_denoiser as T
+    seed = 0x5EED0100 + 50000 + scene_index
+    spec = T.random_scene(seed, 128, 1)
+    scene, view, a, ref = T.render_pair(spec, 1, 1024, seed)
+    den = KpnDenoiser(scene)
+    den.load(KpnDenoiser.pretrained_weights())
+    from sharpray_amd import Renderer
+    r = Renderer(scene, view, 128, 128, 32, 1, 1, 0.0, spec.bounces, spec.indirect_clamp,
+                 rng_policy=RNG_COUNTER, rng_seed=seed)
+    r.wait()
+    noisy = T.rel_mse(a["rad"], ref)
+    kpn = T.rel_mse(den.denoise(r), ref)
+    bil = min(T.rel_mse(Compositor(1.0 / 128, st, 1.75, 0.25, 2.0, 0.125, 0.2).filter(r), ref)
+              for st in (0.05, 0.2, 1.0, 5.0, 25.0))
+    print(f"kpn shipped weights, scene {scene_index}: noisy {noisy:.4f}, kpn {kpn:.4f}, bilateral {bil:.4f}",
+          flush=True)
+    assert kpn < 0.5 * noisy and kpn < bil
+
+
+def test_compose_tail_parity(oracle_lib):
+    """Compose (compose.cs:74-99): Filter + FilterFog + Tonemap + ToPixel, and Preview, against the
+    oracle on the GPU's own AOVs (a height-fog render, so the fog radiance is non-zero). FilterFog
+    within 1e-5 relative; bytes equal except where a transcendental's last bit moves a value
+    across a rounding boundary (at most 1 step, on under 1 % of the bytes)."""
+    from sharpray_amd import RNG_COUNTER, TONEMAP_LINEAR_SMOOTH, TONEMAP_REINHARD, Compositor, scenes
+    spec = scenes.config_plain_cpu(scale=0.5, spp=4)
+    spec.fog = (0.35, (0.9, 0.85, 0.8), 0.3, 0.6)
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=11)
+    r.wait()
+    W, H = spec.width, spec.height
+    assert float(np.abs(r.radiance_fog).max()) > 0.0
+    for tm, exposure in [(TONEMAP_REINHARD, 1.0), (TONEMAP_LINEAR_SMOOTH, 0.7)]:
+        comp = Compositor(0.01, 0.1, 1.75, 0.25, 2.0, 0.125, 0.2, denoise_fog_radius=0.01,
+                          denoise_fog_strength=0.5, tonemapper=tm, exposure=exposure)
+        fog = comp.filter_fog(r)
+        fog_ref = oracle_lib.filter_fog(0.01, 0.5, r.radiance_fog, W, H)
+        ferr = rel_err(fog, fog_ref, floor=1e-3)
+        main_ref, _ = oracle_lib.bilateral(comp.as_array(), r.radiance, r.normals, r.depth, r.variance, W, H)
+        img = comp.compose(r).pixels.reshape(-1, 3).astype(int)
+        ref = oracle_lib.tonemap(main_ref, fog_ref, tm, exposure).astype(int)
+        d = np.abs(img - ref)
+        prev = comp.preview(r).pixels.reshape(-1, 3).astype(int)
+        pref = oracle_lib.tonemap(r.radiance, r.radiance_fog, tm, exposure).astype(int)
+        dp = np.abs(prev - pref)
+        print(f"compose tm={tm}: fog filter max rel err {ferr.max():.2e}; compose bytes off {(d > 0).mean():.5f} "
+              f"(max {d.max()}), preview bytes off {(dp > 0).mean():.5f} (max {dp.max()})", flush=True)
+        assert ferr.max() <= 1e-5
+        assert d.max() <= 1 and (d > 0).mean() <= 0.01
+        assert dp.max() <= 1 and (dp > 0).mean() <= 0.01
