@@ -320,6 +320,32 @@ int sr_cnn_train_gradient(sr_ctx* ctx, uint32_t example, float* gradient, float*
 /* Floating-point operations of one denoise at width x height (2 per multiply-add). */
 int sr_cnn_flops(uint32_t width, uint32_t height, double* flops);
 
+/* ---- Asset ingest (host only; the reference's ObjLoader, ImageHdr and Image loaders) --------- */
+/* ObjLoader.Load (obj.cs:138-258) with ObjConfig (obj.cs:127-134): parses an OBJ file and the MTL
+ * files it names, and adds one mesh object per o / g / usemtl block to the scene (textures of
+ * map_Kd / map_Pm / map_Pr / map_bump via Image.Load, TGA only). transform null = identity;
+ * material null = Material(White, roughness 1) for objects without an MTL material; smooth_normals
+ * != 0 computes area-weighted vertex normals when the file has no vn. Errors (SR_ERR_INVALID) carry
+ * the reference's exception texts; objects added before an error stay in the scene. */
+int sr_obj_load(sr_ctx* ctx, const char* path, const sr_transform* transform, const sr_material* material,
+                int32_t smooth_normals, uint32_t* objects_added);
+/* ImageHdr.Load (image.cs:307-382) + Texture.FromHdr (texture.cs:154-163): Radiance RGBE .hdr
+ * (flat or RLE scanlines, -Y h +X w) to linear float RGB w*h*3, luminance clamped to
+ * max_luminance. rgb null: only width/height. */
+int sr_hdr_load(const char* path, float max_luminance, float* rgb, uint32_t* width, uint32_t* height);
+int sr_hdr_decode(const uint8_t* data, uint64_t size, float max_luminance, float* rgb, uint32_t* width,
+                  uint32_t* height);
+/* Image.LoadTga (image.cs:217-266): uncompressed 24-bit / 8-bit grey TGA to top-down RGB bytes. */
+int sr_tga_decode(const uint8_t* data, uint64_t size, uint8_t* rgb, uint32_t* width, uint32_t* height);
+/* What the scene holds (for hosts and tests): counts, a texture's texels, a mesh's Triangle
+ * records (36 floats each, as given to sr_mesh_create) and an object's transform, material and
+ * shape. Arrays may be null to query sizes. */
+int sr_scene_counts(sr_ctx* ctx, uint32_t* textures, uint32_t* meshes, uint32_t* objects);
+int sr_texture_read(sr_ctx* ctx, int32_t id, float* rgb, uint32_t* width, uint32_t* height);
+int sr_mesh_read(sr_ctx* ctx, int32_t id, float* triangles, uint32_t* count);
+int sr_scene_object_read(sr_ctx* ctx, int32_t id, sr_transform* trans, sr_material* material, int32_t* shape_kind,
+                         int32_t* mesh);
+
 /* ---- Host data preparation (the C# host computes these in Triangle's constructors) ---------- */
 /* Triangle ctors (math.cs:1503-1531) and Mesh.ComputeSmoothNormals (mesh.cs:99-123): pos n*9,
  * normals n*9 or null, uv n*6 or null (null = the OBJ defaults (0,0),(1,0),(0,1); obj.cs:417-427).

@@ -17,6 +17,7 @@ from ._native import SharpRayError, check, fptr, iptr, uptr
 __all__ = [
     "SharpRayError", "Quat", "Transform", "View", "Texture", "Mesh", "Sphere", "AABox", "Material", "Object",
     "SkyTexture", "SkyProcedural", "Fog", "Scene", "Renderer", "Compositor", "triangles_build", "RNG_REFERENCE",
+    "ObjConfig", "ObjLoader", "load_hdr", "decode_hdr", "decode_tga", "Image",
     "RNG_COUNTER",
 ]
 
@@ -344,15 +345,50 @@ class Scene:
         check(N.lib().sr_scene_set_fog(self._ctx, ctypes.byref(f)), "set_fog")
         self.fog = fog
 
-    def add_object(self, obj):
-        self._guard()
-        m = obj.material
+    def _material_struct(self, m):
         cm = N.Material_()
         cm.albedo[:] = m.albedo.tolist()
         cm.roughness, cm.metallic, cm.transparency, cm.ior = m.roughness, m.metallic, m.transparency, m.ior
         cm.radiance[:] = m.radiance.tolist()
         ids = [self.texture_id(t) for t in m.textures]
         cm.color_texture, cm.roughness_texture, cm.metallic_texture, cm.normal_texture = ids
+        return cm
+
+    # ---- what the scene holds (also objects added natively, e.g. by ObjLoader) ----------------
+    def counts(self):
+        """(textures, meshes, objects)."""
+        t, m, o = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        check(N.lib().sr_scene_counts(self._ctx, ctypes.byref(t), ctypes.byref(m), ctypes.byref(o)), "counts")
+        return t.value, m.value, o.value
+
+    def mesh_triangles(self, mesh):
+        """Triangle records (n, 36) of a mesh: lean (A, B-A, C-A), normals, tangents, uvs."""
+        n = ctypes.c_uint32()
+        check(N.lib().sr_mesh_read(self._ctx, mesh, None, ctypes.byref(n)), "mesh read")
+        out = np.empty((n.value, 36), np.float32)
+        check(N.lib().sr_mesh_read(self._ctx, mesh, fptr(out), ctypes.byref(n)), "mesh read")
+        return out
+
+    def texture_texels(self, tex):
+        w, h = ctypes.c_uint32(), ctypes.c_uint32()
+        check(N.lib().sr_texture_read(self._ctx, tex, None, ctypes.byref(w), ctypes.byref(h)), "texture read")
+        out = np.empty((h.value, w.value, 3), np.float32)
+        check(N.lib().sr_texture_read(self._ctx, tex, fptr(out), ctypes.byref(w), ctypes.byref(h)), "texture read")
+        return out
+
+    def object_record(self, index):
+        """dict(transform, material, shape_kind, mesh) of an object as stored."""
+        t, m = N.Transform_(), N.Material_()
+        kind, mesh = ctypes.c_int32(), ctypes.c_int32()
+        check(N.lib().sr_scene_object_read(self._ctx, index, ctypes.byref(t), ctypes.byref(m), ctypes.byref(kind),
+                                           ctypes.byref(mesh)), "object read")
+        mat = {k: (list(getattr(m, k)) if k in ("albedo", "radiance") else getattr(m, k)) for k, _ in m._fields_}
+        return {"pos": list(t.pos), "rot": list(t.rot), "scale": list(t.scale), "material": mat,
+                "shape_kind": kind.value, "mesh": mesh.value}
+
+    def add_object(self, obj):
+        self._guard()
+        cm = self._material_struct(obj.material)
         params = None
         if isinstance(obj.shape, Mesh):
             kind, mesh = N.SR_SHAPE_MESH, self.mesh_id(obj.shape)
@@ -667,6 +703,70 @@ class KpnDenoiser:
         f = ctypes.c_double()
         check(N.lib().sr_cnn_flops(width, height, ctypes.byref(f)), "cnn flops")
         return f.value
+
+
+class ObjConfig:
+    """ObjConfig (obj.cs:127-134): transform of every object, material for objects without an MTL
+    material (None: Material(White, roughness 1)), smooth normals for files without vn."""
+
+    def __init__(self, transform=None, material=None, smooth_normals=True):
+        self.transform = transform
+        self.material = material
+        self.smooth_normals = bool(smooth_normals)
+
+
+class ObjLoader:
+    """ObjLoader (obj.cs:136-512), natively: OBJ + MTL (+ TGA maps) into a scene."""
+
+    @staticmethod
+    def load(path, scene, config=None):
+        """Adds the file's objects to `scene`; returns how many were added."""
+        config = config or ObjConfig()
+        scene._guard()
+        t = config.transform._c() if config.transform is not None else None
+        m = None
+        if config.material is not None:
+            m = scene._material_struct(config.material)
+        n = ctypes.c_uint32()
+        check(N.lib().sr_obj_load(scene.ctx, str(path).encode(), ctypes.byref(t) if t is not None else None,
+                                  ctypes.byref(m) if m is not None else None, 1 if config.smooth_normals else 0,
+                                  ctypes.byref(n)), "obj load")
+        return n.value
+
+
+def load_hdr(path, max_luminance=None):
+    """Texture.FromHdr(ImageHdr.Load(path), maxLuminance) (image.cs:307-382, texture.cs:154-163):
+    (h, w, 3) float32 texels."""
+    w, h = ctypes.c_uint32(), ctypes.c_uint32()
+    lim = np.float32(np.finfo(np.float32).max if max_luminance is None else max_luminance)
+    check(N.lib().sr_hdr_load(str(path).encode(), lim, None, ctypes.byref(w), ctypes.byref(h)), "hdr load")
+    out = np.empty((h.value, w.value, 3), np.float32)
+    check(N.lib().sr_hdr_load(str(path).encode(), lim, fptr(out), ctypes.byref(w), ctypes.byref(h)), "hdr load")
+    return out
+
+
+def decode_hdr(data, max_luminance=None):
+    """The same decoding from the bytes of a .hdr file."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    w, h = ctypes.c_uint32(), ctypes.c_uint32()
+    lim = np.float32(np.finfo(np.float32).max if max_luminance is None else max_luminance)
+    p = buf.ctypes.data_as(N.c_uint8_p)
+    check(N.lib().sr_hdr_decode(p, buf.size, lim, None, ctypes.byref(w), ctypes.byref(h)), "hdr decode")
+    out = np.empty((h.value, w.value, 3), np.float32)
+    check(N.lib().sr_hdr_decode(p, buf.size, lim, fptr(out), ctypes.byref(w), ctypes.byref(h)), "hdr decode")
+    return out
+
+
+def decode_tga(data):
+    """Image.LoadTga (image.cs:217-266) of the bytes of a .tga file: (h, w, 3) uint8, top row first."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    w, h = ctypes.c_uint32(), ctypes.c_uint32()
+    p = buf.ctypes.data_as(N.c_uint8_p)
+    check(N.lib().sr_tga_decode(p, buf.size, None, ctypes.byref(w), ctypes.byref(h)), "tga decode")
+    out = np.empty((h.value, w.value, 3), np.uint8)
+    check(N.lib().sr_tga_decode(p, buf.size, out.ctypes.data_as(N.c_uint8_p), ctypes.byref(w), ctypes.byref(h)),
+          "tga decode")
+    return out
 
 
 TONEMAP_REINHARD = 0

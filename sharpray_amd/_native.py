@@ -152,6 +152,16 @@ SIGNATURES = {
     "sr_cnn_train_gradient": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, c_float_p, c_float_p]),
     "sr_cnn_flops": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     "sr_triangles_build": (ctypes.c_int, [c_float_p, c_float_p, c_float_p, ctypes.c_uint32, ctypes.c_int32, c_float_p]),
+    "sr_obj_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(Transform_),
+                                   ctypes.POINTER(Material_), ctypes.c_int32, c_uint32_p]),
+    "sr_hdr_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, c_float_p, c_uint32_p, c_uint32_p]),
+    "sr_hdr_decode": (ctypes.c_int, [c_uint8_p, ctypes.c_uint64, ctypes.c_float, c_float_p, c_uint32_p, c_uint32_p]),
+    "sr_tga_decode": (ctypes.c_int, [c_uint8_p, ctypes.c_uint64, c_uint8_p, c_uint32_p, c_uint32_p]),
+    "sr_scene_counts": (ctypes.c_int, [ctypes.c_void_p, c_uint32_p, c_uint32_p, c_uint32_p]),
+    "sr_texture_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_uint32_p, c_uint32_p]),
+    "sr_mesh_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_uint32_p]),
+    "sr_scene_object_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(Transform_),
+                                            ctypes.POINTER(Material_), c_int32_p, c_int32_p]),
 }
 
 _lib = None

@@ -220,7 +220,8 @@ BvhResult build_bvh(const box3* bounds, int n, int bins, float ct, float ci) {
     return b.run();
 }
 
-void build_triangles(const float* pos, const float* nrm, const float* uv, uint32_t n, bool smooth, float* out) {
+void build_triangles(const float* pos, const float* nrm, const float* uv, uint32_t n, bool smooth, float* out,
+                     const uint8_t* hasNrm) {
     auto uvs = [&](uint32_t i, vec2& a, vec2& b, vec2& c) {
         if (uv) {
             a = mk2(uv[i * 6], uv[i * 6 + 1]);
@@ -235,7 +236,9 @@ void build_triangles(const float* pos, const float* nrm, const float* uv, uint32
         uvs(i, a, b, c);
         vec3 nv[3];
         if (nrm) { nv[0] = ld3(nrm + i * 9); nv[1] = ld3(nrm + i * 9 + 3); nv[2] = ld3(nrm + i * 9 + 6); }
-        emit(out + (size_t)i * 36, ld3(pos + i * 9), ld3(pos + i * 9 + 3), ld3(pos + i * 9 + 6), nrm ? nv : nullptr, a, b, c);
+        const bool useN = nrm && (!hasNrm || hasNrm[i]);
+        emit(out + (size_t)i * 36, ld3(pos + i * 9), ld3(pos + i * 9 + 3), ld3(pos + i * 9 + 6), useN ? nv : nullptr, a, b,
+             c);
     }
     if (!smooth || nrm || n == 0) return;
 

@@ -31,7 +31,12 @@ struct BvhResult {
 BvhResult build_bvh(const box3* bounds, int n, int bins, float cost_traverse, float cost_intersect);
 
 // Triangle ctors (math.cs:1503-1531) and Mesh.ComputeSmoothNormals (mesh.cs:99-123).
-void build_triangles(const float* pos, const float* nrm, const float* uv, uint32_t n, bool smooth, float* out36);
+// has_nrm (optional, with nrm): per triangle, 0 = the face-normal ctor despite nrm (obj.cs:430-441).
+void build_triangles(const float* pos, const float* nrm, const float* uv, uint32_t n, bool smooth, float* out36,
+                     const uint8_t* has_nrm = nullptr);
+
+// Records an error message for sr_last_error and returns code (defined with the C ABI).
+int sr_set_error(int code, const char* msg);
 
 // Bounds of a TriangleLean given as (A, B-A, C-A) (math.cs:1353-1359).
 box3 lean_bounds(const float* t9);

@@ -769,6 +769,11 @@ int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+}  // namespace
+
+int sr::sr_set_error(int code, const char* msg) { return fail(code, msg ? msg : ""); }
+
+namespace {
 
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
@@ -2300,6 +2305,45 @@ int sr_cnn_train_export(sr_ctx* ctx, float* weights) {
 int sr_cnn_flops(uint32_t width, uint32_t height, double* flops) {
     if (!flops) return fail(SR_ERR_INVALID, "cnn: null argument");
     *flops = sr::CnnNet::flops(width, height);
+    return SR_OK;
+}
+
+int sr_scene_counts(sr_ctx* ctx, uint32_t* textures, uint32_t* meshes, uint32_t* objects) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    if (textures) *textures = (uint32_t)ctx->textures.size();
+    if (meshes) *meshes = (uint32_t)ctx->meshes.size();
+    if (objects) *objects = (uint32_t)ctx->objects.size();
+    return SR_OK;
+}
+
+int sr_texture_read(sr_ctx* ctx, int32_t id, float* rgb, uint32_t* width, uint32_t* height) {
+    if (!ctx || !width || !height) return fail(SR_ERR_INVALID, "texture read: null argument");
+    if (id < 0 || id >= (int32_t)ctx->textures.size()) return fail(SR_ERR_INVALID, "texture read: bad id");
+    const HostTexture& t = ctx->textures[id];
+    *width = t.w;
+    *height = t.h;
+    if (rgb) memcpy(rgb, t.rgb.data(), t.rgb.size() * 4);
+    return SR_OK;
+}
+
+int sr_mesh_read(sr_ctx* ctx, int32_t id, float* triangles, uint32_t* count) {
+    if (!ctx || !count) return fail(SR_ERR_INVALID, "mesh read: null argument");
+    if (id < 0 || id >= (int32_t)ctx->meshes.size()) return fail(SR_ERR_INVALID, "mesh read: bad id");
+    const HostMesh& m = ctx->meshes[id];
+    *count = (uint32_t)(m.tris.size() / 36);
+    if (triangles) memcpy(triangles, m.tris.data(), m.tris.size() * 4);
+    return SR_OK;
+}
+
+int sr_scene_object_read(sr_ctx* ctx, int32_t id, sr_transform* trans, sr_material* material, int32_t* shape_kind,
+                         int32_t* mesh) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    if (id < 0 || id >= (int32_t)ctx->objects.size()) return fail(SR_ERR_INVALID, "object read: bad id");
+    const HostObject& o = ctx->objects[id];
+    if (trans) *trans = o.trans;
+    if (material) *material = o.mat;
+    if (shape_kind) *shape_kind = o.kind;
+    if (mesh) *mesh = o.mesh;
     return SR_OK;
 }
 
