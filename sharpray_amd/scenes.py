@@ -251,6 +251,30 @@ def rgbe_encode(rgb):
     return out
 
 
+def write_hdr(path, rgbe):
+    """Writes RGBE bytes (h, w, 4) as a Radiance .hdr file with flat (uncompressed) scanlines."""
+    rgbe = np.ascontiguousarray(rgbe, dtype=np.uint8)
+    h, w, _ = rgbe.shape
+    with open(path, "wb") as f:
+        f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {h} +X {w}\n".encode() + rgbe.tobytes())
+
+
+def write_obj(path, pos, uv=None, name="mesh"):
+    """Writes triangles (n, 9) (and per-corner uvs (n, 6), stored with V flipped back) as an OBJ
+    without vn, three vertices per face, every float with 9 significant digits (exact for float32)."""
+    pos = np.asarray(pos, np.float32).reshape(-1, 3)
+    lines = [f"o {name}"]
+    lines += ["v %.9g %.9g %.9g" % tuple(float(c) for c in p) for p in pos]
+    if uv is not None:
+        t = np.asarray(uv, np.float32).reshape(-1, 2)
+        lines += ["vt %.9g %.9g" % (float(a), float(np.float32(1.0) - b)) for a, b in t]
+        lines += [f"f {3 * i + 1}/{3 * i + 1} {3 * i + 2}/{3 * i + 2} {3 * i + 3}/{3 * i + 3}" for i in range(len(pos) // 3)]
+    else:
+        lines += [f"f {3 * i + 1} {3 * i + 2} {3 * i + 3}" for i in range(len(pos) // 3)]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def procedural_sky(w, h, peak=50.0, sun_dir=(0.35, 0.6, 0.45), sun_width=0.05, seed=0):
     """Equirectangular float sky: zenith-to-horizon gradient, dark ground, Gaussian sun lobe."""
     u = (np.arange(w) + 0.5) / w
