@@ -156,6 +156,13 @@ int sr_texture_create(sr_ctx* ctx, const float* rgb, uint32_t width, uint32_t he
  * binned-SAH BVH with sahCostIntersect 0.5 (mesh.cs:41) and `split_bins` bins (reference: 8). */
 int sr_mesh_create(sr_ctx* ctx, const float* triangles, uint32_t count, uint32_t split_bins, int32_t* out_id);
 
+/* Which builder sr_mesh_create uses for the mesh BVH: the host builder (default) or the device
+ * builder, which produces the identical node array and item permutation (bvh.cs:285-476) on the
+ * context's GPU (2..64 bins). No counterpart in the reference. */
+#define SR_BVH_BUILDER_HOST 0
+#define SR_BVH_BUILDER_DEVICE 1
+int sr_set_bvh_builder(sr_ctx* ctx, int32_t builder);
+
 /* ---- Scene (scene.cs:507-602) -------------------------------------------------------------- */
 /* Scene.SetSky(new SkyTexture(texture, multiplier, rotYaw)) (scene.cs:543-551, 389-404). */
 int sr_scene_set_sky_texture(sr_ctx* ctx, int32_t texture, float multiplier, float rot_yaw);

@@ -345,6 +345,12 @@ class Scene:
         check(N.lib().sr_scene_set_fog(self._ctx, ctypes.byref(f)), "set_fog")
         self.fog = fog
 
+    def set_bvh_builder(self, device):
+        """Mesh BVHs of later meshes built on the GPU (True) or the host (False); both give the
+        reference's tree bit for bit."""
+        check(N.lib().sr_set_bvh_builder(self._ctx, N.SR_BVH_BUILDER_DEVICE if device else N.SR_BVH_BUILDER_HOST),
+              "bvh builder")
+
     def _material_struct(self, m):
         cm = N.Material_()
         cm.albedo[:] = m.albedo.tolist()

@@ -17,6 +17,7 @@ SR_LIGHT_CDF = 0
 SR_LIGHT_ALIAS = 1
 SR_SHAPE_MESH, SR_SHAPE_SPHERE, SR_SHAPE_BOX = 0, 1, 2
 SR_COUNTER_COUNT = 15
+SR_BVH_BUILDER_HOST, SR_BVH_BUILDER_DEVICE = 0, 1
 COUNTER_NAMES = [
     "Pixel", "Sample", "SampleBounce", "SampleHit", "SampleMiss", "SampleTerminate", "SampleFogScatter",
     "SampleFogEscape", "SamplePixelConverged", "SamplePixelMaxed", "SceneTrace", "SceneOcclude", "MeshIntersect",
@@ -157,6 +158,7 @@ SIGNATURES = {
     "sr_hdr_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_float, c_float_p, c_uint32_p, c_uint32_p]),
     "sr_hdr_decode": (ctypes.c_int, [c_uint8_p, ctypes.c_uint64, ctypes.c_float, c_float_p, c_uint32_p, c_uint32_p]),
     "sr_tga_decode": (ctypes.c_int, [c_uint8_p, ctypes.c_uint64, c_uint8_p, c_uint32_p, c_uint32_p]),
+    "sr_set_bvh_builder": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "sr_scene_counts": (ctypes.c_int, [ctypes.c_void_p, c_uint32_p, c_uint32_p, c_uint32_p]),
     "sr_texture_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_uint32_p, c_uint32_p]),
     "sr_mesh_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_uint32_p]),

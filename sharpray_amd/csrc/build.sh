@@ -18,6 +18,7 @@ trap 'rm -rf "${tmp}"' EXIT
 "${HIPCC}" "${COMMON[@]}" -x hip --offload-host-only -c "${here}/sr_ingest.cpp" -o "${tmp}/sr_ingest.o" -fno-builtin-powf
 "${HIPCC}" "${COMMON[@]}" "${DEVICE[@]}" -c "${here}/sr_render.hip" -o "${tmp}/sr_render.o"
 "${HIPCC}" "${COMMON[@]}" "${DEVICE[@]}" -c "${here}/sr_cnn.hip" -o "${tmp}/sr_cnn.o"
-"${HIPCC}" -shared --offload-arch="${ARCH}" "${tmp}/sr_host.o" "${tmp}/sr_ingest.o" "${tmp}/sr_render.o" "${tmp}/sr_cnn.o" -o "${out}.tmp"
+"${HIPCC}" "${COMMON[@]}" "${DEVICE[@]}" -c "${here}/sr_bvh.hip" -o "${tmp}/sr_bvh.o"
+"${HIPCC}" -shared --offload-arch="${ARCH}" "${tmp}/sr_host.o" "${tmp}/sr_ingest.o" "${tmp}/sr_render.o" "${tmp}/sr_cnn.o" "${tmp}/sr_bvh.o" -o "${out}.tmp"
 mv "${out}.tmp" "${out}"
 echo "built ${out}"

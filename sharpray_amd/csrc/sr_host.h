@@ -4,6 +4,7 @@
 
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 #include "sr_math.h"
@@ -34,6 +35,12 @@ BvhResult build_bvh(const box3* bounds, int n, int bins, float cost_traverse, fl
 // has_nrm (optional, with nrm): per triangle, 0 = the face-normal ctor despite nrm (obj.cs:430-441).
 void build_triangles(const float* pos, const float* nrm, const float* uv, uint32_t n, bool smooth, float* out36,
                      const uint8_t* has_nrm = nullptr);
+
+// The same build on the current HIP device from n TriangleLean records (9 floats each, device
+// memory), with 2..64 bins: identical nodes and items (defined in sr_bvh.hip). Returns false with
+// err set on a HIP error.
+bool device_build_bvh(const float* dLean, int n, int bins, float cost_traverse, float cost_intersect, BvhResult& out,
+                      hipStream_t stream, std::string& err);
 
 // Records an error message for sr_last_error and returns code (defined with the C ABI).
 int sr_set_error(int code, const char* msg);

@@ -814,6 +814,7 @@ struct DevBuf {
 
 struct sr_ctx {
     int device = 0;
+    int bvhBuilder = SR_BVH_BUILDER_HOST;
     hipStream_t stream = nullptr;
     std::vector<HostTexture> textures;
     std::vector<HostMesh> meshes;
@@ -1171,9 +1172,25 @@ int sr_mesh_create(sr_ctx* ctx, const float* tris, uint32_t count, uint32_t bins
     HostMesh m;
     m.tris.assign(tris, tris + (size_t)count * 36);
     m.bins = bins;
-    std::vector<box3> bb(count);
-    for (uint32_t i = 0; i < count; ++i) bb[i] = lean_bounds(tris + (size_t)i * 36);
-    m.bvh = build_bvh(bb.data(), (int)count, (int)bins, 1.0f, 0.5f);
+    if (ctx->bvhBuilder == SR_BVH_BUILDER_DEVICE && count > 0) {
+        if (bins > 64) return fail(SR_ERR_LIMIT, "mesh: the device BVH builder takes at most 64 bins");
+        HIP_TRY(hipSetDevice(ctx->device));
+        std::vector<float> lean((size_t)count * 9);
+        for (uint32_t i = 0; i < count; ++i) memcpy(&lean[(size_t)i * 9], tris + (size_t)i * 36, 36);
+        float* dLean = nullptr;
+        HIP_TRY(hipMalloc((void**)&dLean, lean.size() * 4));
+        hipError_t e = hipMemcpy(dLean, lean.data(), lean.size() * 4, hipMemcpyHostToDevice);
+        std::string err;
+        const bool ok = e == hipSuccess && device_build_bvh(dLean, (int)count, (int)bins, 1.0f, 0.5f, m.bvh,
+                                                            ctx->stream, err);
+        (void)hipFree(dLean);
+        if (e != hipSuccess) return fail(SR_ERR_HIP, hipGetErrorString(e));
+        if (!ok) return fail(SR_ERR_HIP, err);
+    } else {
+        std::vector<box3> bb(count);
+        for (uint32_t i = 0; i < count; ++i) bb[i] = lean_bounds(tris + (size_t)i * 36);
+        m.bvh = build_bvh(bb.data(), (int)count, (int)bins, 1.0f, 0.5f);
+    }
     if (m.bvh.degenerate) return fail(SR_ERR_BUILD, "mesh BVH: a split produced an empty child (bvh.cs:448)");
     ctx->meshes.push_back(std::move(m));
     *out_id = (int32_t)ctx->meshes.size() - 1;
@@ -2305,6 +2322,16 @@ int sr_cnn_train_export(sr_ctx* ctx, float* weights) {
 int sr_cnn_flops(uint32_t width, uint32_t height, double* flops) {
     if (!flops) return fail(SR_ERR_INVALID, "cnn: null argument");
     *flops = sr::CnnNet::flops(width, height);
+    return SR_OK;
+}
+
+int sr_set_bvh_builder(sr_ctx* ctx, int32_t builder) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    if (builder != SR_BVH_BUILDER_HOST && builder != SR_BVH_BUILDER_DEVICE)
+        return fail(SR_ERR_INVALID, "unknown BVH builder");
+    if (builder == SR_BVH_BUILDER_DEVICE && ctx->device < 0)
+        return fail(SR_ERR_STATE, "host-only context (device -1) has no device builder");
+    ctx->bvhBuilder = builder;
     return SR_OK;
 }
 

@@ -499,3 +499,49 @@ def test_compose_tail_parity(oracle_lib):
         assert ferr.max() <= 1e-5
         assert d.max() <= 1 and (d > 0).mean() <= 0.01
         assert dp.max() <= 1 and (dp > 0).mean() <= 0.01
+
+
+def _fbm_sphere_tris(rings, seed):
+    from sharpray_amd import scenes
+    pos, uv, _ = scenes.fbm_sphere(rings, 2 * rings, 0.15, 6, seed)
+    return pos.reshape(-1, 9).astype(np.float32)
+
+
+@pytest.mark.parametrize("case", ["ico3_8", "ico5_16", "clusters_32", "fbm_64", "fbm1m_8", "tiny_2"])
+def test_device_bvh_builder_bit_exact(case):
+    """The device builder (level-wise large nodes, per-wave subtrees, depth-first renumbering)
+    against the host builder, which matches the reference's bvh.cs order: node arrays (bounds,
+    child, count) and item permutations bit-identical."""
+    import time
+    from sharpray_amd import Mesh, Scene, scenes, triangles_build
+    rng = np.random.default_rng(7)
+    if case == "ico3_8":
+        tris, bins = scenes.icosphere(3, seed=3), 8
+    elif case == "ico5_16":
+        tris, bins = scenes.icosphere(5, seed=5), 16
+    elif case == "clusters_32":  # two far clusters plus duplicates: exact ties in centroids
+        a = rng.normal(0, 0.1, (3000, 9)).astype(np.float32)
+        b = (rng.normal(0, 0.1, (3000, 9)) + 5).astype(np.float32)
+        tris, bins = np.concatenate([a, b, a[:500]]), 32
+    elif case == "fbm_64":
+        tris, bins = _fbm_sphere_tris(96, 11), 64
+    elif case == "fbm1m_8":
+        tris, bins = _fbm_sphere_tris(512, 1), 8
+    else:
+        tris, bins = rng.normal(0, 1, (3, 9)).astype(np.float32), 2
+    recs = triangles_build(tris, None, None)
+    out = {}
+    for dev in (False, True):
+        sc = Scene(0)
+        sc.set_bvh_builder(dev)
+        m = Mesh(recs, split_bins=bins)
+        t0 = time.perf_counter()
+        mid = sc.mesh_id(m)
+        dt = time.perf_counter() - t0
+        out[dev] = sc.bvh(mid) + (dt,)
+    (hn, hi, ht), (dn, di, dtm) = out[False], out[True]
+    print(f"device BVH {case}: {len(tris)} triangles, {len(hn)} nodes, host {ht * 1e3:.1f} ms, "
+          f"device {dtm * 1e3:.1f} ms", flush=True)
+    assert hn.shape == dn.shape
+    assert np.array_equal(hn.view(np.uint32), dn.view(np.uint32))
+    assert np.array_equal(hi, di)
