@@ -164,6 +164,8 @@ def main():
     ap.add_argument("--light-sampler", choices=["cdf", "alias"], default="cdf",
                     help="environment light sampling: the reference's row/column CDFs or the alias table")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--tail-segment", type=int, default=None,
+                    help="segment from which paths finish in one fused launch (default: the library's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (torch clone), to calibrate HBM counters")
     args = ap.parse_args()
@@ -197,6 +199,8 @@ def main():
         f"{spec.max_samples} spp ({time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
     scene, view = scenes.instantiate(spec)
+    if args.tail_segment is not None:
+        scene.set_tail_segment(args.tail_segment)
     scene.build()
     log(f"[bench] BVH build + upload {time.perf_counter() - t0:.1f} s")
     if args.workload == "adjoint":
@@ -220,7 +224,7 @@ def main():
         r.wait()
         log(f"[bench] warmup {i} done")
     # Timed region: K full frames back to back, host-synchronised on both sides.
-    walls, gpus, ext_ms, ext_launch = [], [], 0.0, 0
+    walls, gpus, ext_ms, ext_launch, tail_ms = [], [], 0.0, 0, 0.0
     for i in range(args.steps):
         t0 = time.perf_counter()
         r = frame(kernel_timing=True)
@@ -228,22 +232,28 @@ def main():
         walls.append(time.perf_counter() - t0)
         gpus.append(r.gpu_seconds())
         kt = r.kernel_times()
-        ext_ms += kt["trace"][0] + kt["trace_tail"][0]
-        ext_launch += kt["trace"][1] + kt["trace_tail"][1]
+        ext_ms += kt["trace"][0]
+        ext_launch += kt["trace"][1]
+        tail_ms += kt["trace_tail"][0]
         log(f"[bench] step {i}: {walls[-1] * 1e3:.1f} ms wall, {gpus[-1] * 1e3:.1f} ms GPU, "
             f"trace {kt['trace'][0]:.1f} ms / {kt['trace'][1]} launches, shade {kt['shade'][0]:.1f} ms, "
-            f"trace tail {kt['trace_tail'][0]:.1f} ms, other {kt['other'][0]:.1f} ms")
+            f"fused tail {kt['trace_tail'][0]:.1f} ms / {kt['trace_tail'][1]} launches, "
+            f"other {kt['other'][0]:.1f} ms")
     samples = spec.width * spec.height * spec.max_samples
     ms_per_step = 1e3 * sum(walls) / len(walls)
     msps = samples / (sum(walls) / len(walls)) / 1e6
 
     # Traversal counters of one (untimed) frame give the algorithmic bytes of the trace kernel
-    # (its node and triangle tests include the shadow rays').
+    # (its node and triangle tests include the shadow rays'): all traversal minus the fused tail
+    # launch's share.
     r = frame(counters=True)
     r.wait()
     c = r.counters()
-    ext_bytes = (BYTES_PER_BOX * c["BvhIntersectNode"] + BYTES_PER_TRI * c["BvhIntersectShape"] +
-                 BYTES_PER_RAY * c["SceneTrace"] + BYTES_PER_SHADOW * c["SceneOcclude"])
+    fz = r.counters_fused()
+    ext_bytes = (BYTES_PER_BOX * (c["BvhIntersectNode"] - fz["BvhIntersectNode"]) +
+                 BYTES_PER_TRI * (c["BvhIntersectShape"] - fz["BvhIntersectShape"]) +
+                 BYTES_PER_RAY * (c["SceneTrace"] - fz["SceneTrace"]) +
+                 BYTES_PER_SHADOW * (c["SceneOcclude"] - fz["SceneOcclude"]))
     launches_per_frame = ext_launch / args.steps
     avg_launch_s = ext_ms / ext_launch / 1e3
     bytes_per_launch = ext_bytes / launches_per_frame
@@ -253,7 +263,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
-    log(f"[bench] counters: {c}")
+    log(f"[bench] counters: {c}; fused tail share: {fz}")
 
     cpu = None
     if not args.no_cpu:
@@ -283,6 +293,7 @@ def main():
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
                      "traffic": traffic, "avg_launch_us": round(avg_launch_s * 1e6, 2),
                      "algorithmic_bytes_per_launch": round(bytes_per_launch)},
+        "fused_tail_ms_per_step": round(tail_ms / args.steps, 3),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -261,6 +261,15 @@ int sr_bvh_export(sr_ctx* ctx, int32_t mesh, float* nodes, int32_t* items, uint3
 #define SR_COUNTER_COUNT 15
 int sr_counters_enable(sr_ctx* ctx, int32_t enable);
 int sr_counters_read(sr_ctx* ctx, int64_t* out);
+/* The share of BvhIntersectNode, BvhIntersectShape, SceneTrace and SceneOcclude (out[0..3]) done by
+ * the fused launch that finishes paths from the tail segment on (sr_set_tail_segment); the rest
+ * was done by the per-segment traversal launches. No counterpart in the reference. */
+int sr_counters_read_fused(sr_ctx* ctx, int64_t* out);
+/* Later renders trace segments 0..segment-1 as separate wavefront launches and finish every path
+ * from `segment` on in one fused launch (default 3; a value above the bounce count disables it;
+ * results are identical either way). The environment variable SHARPRAY_TAIL_SEGMENT sets the
+ * default of new contexts. No counterpart in the reference. */
+int sr_set_tail_segment(sr_ctx* ctx, uint32_t segment);
 /* Kernel timing of the next renders: with timing enabled, an event is recorded on the render
  * stream after every launch; sr_profile_read sums the spans per kernel kind of the last render:
  * ms[4] / launches[4] for [0] trace (closest hit of a segment's rays plus the previous segment's

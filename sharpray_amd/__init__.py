@@ -345,6 +345,10 @@ class Scene:
         check(N.lib().sr_scene_set_fog(self._ctx, ctypes.byref(f)), "set_fog")
         self.fog = fog
 
+    def set_tail_segment(self, segment):
+        """Renders finish every path from this segment on in one fused launch (results unchanged)."""
+        check(N.lib().sr_set_tail_segment(self._ctx, int(segment)), "tail segment")
+
     def set_bvh_builder(self, device):
         """Mesh BVHs of later meshes built on the GPU (True) or the host (False); both give the
         reference's tree bit for bit."""
@@ -513,8 +517,9 @@ class Renderer:
 
     def kernel_times(self):
         """{kind: (total ms, launches)} of this render: "trace" (closest hit of a segment's rays plus
-        the previous segment's shadow rays), "shade", "trace_tail" (the same trace kernel on a
-        sample's last shadow rays) and "other" (camera rays, accumulation)."""
+        the previous segment's shadow rays), "shade", "trace_tail" (the fused launch that finishes
+        the paths from the tail segment on, or the trace kernel on a sample's last shadow rays when
+        there is none) and "other" (camera rays, accumulation)."""
         ms = (ctypes.c_double * 4)()
         n = (ctypes.c_uint32 * 4)()
         check(N.lib().sr_profile_read(self.scene.ctx, ms, n), "kernel times")
@@ -524,6 +529,12 @@ class Renderer:
         out = (ctypes.c_int64 * N.SR_COUNTER_COUNT)()
         check(N.lib().sr_counters_read(self.scene.ctx, out), "counters")
         return dict(zip(N.COUNTER_NAMES, list(out)))
+
+    def counters_fused(self):
+        """The fused tail launch's share of the traversal counters."""
+        out = (ctypes.c_int64 * 4)()
+        check(N.lib().sr_counters_read_fused(self.scene.ctx, out), "counters")
+        return dict(zip(["BvhIntersectNode", "BvhIntersectShape", "SceneTrace", "SceneOcclude"], list(out)))
 
 
 class Adjoint:

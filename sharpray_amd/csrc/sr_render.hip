@@ -16,6 +16,7 @@
 // a wave-aligned grid-stride loop (one shared atomic work cursor measured slower).
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -81,6 +82,10 @@ struct ViewD {
     float nearPlane, tanFov, aspect;
 };
 
+// Render counters beyond the reference's SR_COUNTER_COUNT: the fused tail launch's share of
+// BvhIntersectNode, BvhIntersectShape, SceneTrace and SceneOcclude.
+enum : int { kTailNode = SR_COUNTER_COUNT, kTailShape, kTailTrace, kTailOcclude, kCtrWords };
+
 // Queue counters (uint32 each).
 enum : int {
     kRayCount0 = 0, kRayCount1 = 1, kShadowCount0 = 2, kShadowCount1 = 3, kCounterWords = 4
@@ -96,7 +101,7 @@ struct Frame {
     PixAcc* acc;
     uint32_t* queue[2];
     uint32_t* counts;   // kCounterWords words
-    unsigned long long* ctr;  // SR_COUNTER_COUNT counters or null
+    unsigned long long* ctr;  // kCtrWords counters or null
     float* outRad;
     float* outRadFog;
     float* outNormal;
@@ -245,14 +250,140 @@ __global__ void k_start(Frame F) {
     if (live) F.queue[0][slot] = q;
 }
 
-// One path segment (scene.cs:641-737). Direct sky light becomes a deferred shadow ray whose
-// contribution the next trace launch adds before the path's next segment, which keeps the
-// reference's summation order.
+// Counts of one shade pass, flushed into the render counters once per thread.
+struct ShadeCount {
+    uint32_t hit = 0, miss = 0, term = 0, fog = 0, esc = 0, occ = 0;
+};
+
+// One path segment (scene.cs:641-737) of path q at the hit h of `ray`: emission, the NEE shadow
+// ray (returned in sr, not traced here), Russian roulette and the scatter. On return `cont` says
+// whether the path continues with the ray (no, nd); p and rng are updated in place.
+template <class R>
+SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p, R& rng, const HitRec& h,
+                       const ray3& ray, bool& cont, bool& shadow, ShadowRec& sr, vec3& no, vec3& nd, ShadeCount& c) {
+    cont = false;
+    shadow = false;
+    const uint32_t seg = p.seg;
+    const bool primary = seg == 0;
+    const bool last = seg == F.bounces;
+    const bool hit = h.obj >= 0;
+    float sd = 0.0f;
+    no = ray.o;
+    nd = ray.d;
+    if (S.fog.enabled && medium_scatter(S.fog, ray, 0.0f, hit ? h.t : kInf, rng, sd)) {
+        ++c.fog;
+        p.energy = p.energy * S.fog.color;
+        const vec3 pos = at(ray, sd);
+        vec3 ld;
+        float lpdf;
+        if (sky_light_rand(S, rng, ld, lpdf) && lpdf > 0.0f) {
+            const float fogPdf = hg_pdf(dot3(ray.d, ld), S.fog.anisotropy);
+            sr.o = pos; sr.d = ld;
+            sr.skyRad = sky_radiance(S, ld);
+            sr.f = splat3(fogPdf);
+            sr.w = power_heuristic(lpdf, fogPdf);
+            sr.pdf = lpdf;
+            sr.energy = p.energy;
+            sr.path = q | (primary ? 0x80000000u : 0u);
+            shadow = true;
+        }
+        bool term = false;
+        if (seg >= 3) {
+            const float survive = net_min(1.0f, max_comp(p.energy));
+            if (next_float(rng) >= survive) { term = true; ++c.term; }
+            else p.energy = p.energy / survive;
+        }
+        if (!term) {
+            const vec3 dir = hg_scatter(ray.d, S.fog.anisotropy, rng);
+            p.flags |= 1u;
+            p.scatterPdf = hg_pdf(dot3(ray.d, dir), S.fog.anisotropy);
+            no = pos;
+            nd = dir;
+            cont = !last;
+        }
+    } else if (hit) {
+        ++c.hit;
+        if (S.fog.enabled) ++c.esc;
+        SurfaceD s;
+        make_surface(S, ray, h, s);
+        const float dist = h.t;
+        const vec3 view = -ray.d;
+        const float bias = net_max(1e-4f, dist * 1e-4f);
+        const vec3 hitPos = at(ray, dist - bias);
+        if (p.flags & 2u) p.energy = p.energy * beer(p.medium, dist);
+        p.rad = p.rad + s.radiance * p.energy;
+        if (primary) {
+            PixAcc* a = &F.acc[q];
+            a->normalSum = a->normalSum + s.normal;
+            a->normalFallback = s.normal;
+            a->normalCount += 1;
+            a->depthSum += dist;
+            a->depthCount += 1;
+            a->albedoSum = a->albedoSum + s.albedo;
+            if (!a->hasUv) { a->hasUv = 1; a->uv = s.uv; }
+        }
+        // SampleSkyDirect (scene.cs:774-794): the sides are tested before the shadow ray,
+        // which only skips rays whose contribution is black either way.
+        vec3 ld;
+        float lpdf;
+        if (sky_light_rand(S, rng, ld, lpdf) && lpdf > 0.0f) {
+            const vec3 nn = dot3(s.normal, view) >= 0.0f ? s.normal : -s.normal;
+            const vec3 ng = dot3(s.normalGeo, view) >= 0.0f ? s.normalGeo : -s.normalGeo;
+            if (dot3(nn, ld) > 0.0f && dot3(ng, ld) > 0.0f) {
+                sr.o = hitPos; sr.d = ld;
+                sr.skyRad = sky_radiance(S, ld);
+                sr.f = opaque_eval(s, view, ld);
+                sr.w = power_heuristic(lpdf, opaque_pdf(s, view, ld, spec_prob(s, view)));
+                sr.pdf = lpdf;
+                sr.energy = p.energy;
+                sr.path = q;
+                shadow = true;
+            }
+        }
+        bool term = false;
+        if (seg >= 3) {
+            const float survive = net_min(1.0f, max_comp(p.energy));
+            if (next_float(rng) >= survive) { term = true; ++c.term; }
+            else p.energy = p.energy / survive;
+        }
+        if (!term) {
+            const ScatterD sc = surface_scatter(s, ray.d, view, rng);
+            p.energy = p.energy * sc.weight;
+            p.energy = clamp_lum(p.energy, F.clamp);
+            const bool enter = dot3(sc.dir, s.normalGeo) < 0.0f;
+            if (sc.refracted) {
+                if (enter) { p.flags |= 2u; p.medium = s.albedo; }
+                else p.flags &= ~2u;
+            } else if (!enter) {
+                p.flags &= ~2u;
+            }
+            if (sc.hasPdf) { p.flags |= 1u; p.scatterPdf = sc.pdf; }
+            else p.flags &= ~1u;
+            no = at(ray, dist) + sc.dir * bias;
+            nd = sc.dir;
+            cont = !last;
+        }
+    } else {
+        ++c.miss;
+        if (S.fog.enabled) ++c.esc;
+        float mis = 1.0f;
+        if (p.flags & 1u) {
+            float lp = 0.0f;
+            if (!sky_light_pdf(S, ray.d, lp)) lp = 0.0f;
+            mis = power_heuristic(p.scatterPdf, lp);
+        }
+        p.rad = p.rad + sky_radiance(S, ray.d) * mis * p.energy;
+    }
+}
+
+// One path segment for every queued path (scene.cs:641-737). Direct sky light becomes a deferred
+// shadow ray whose contribution the next trace launch adds before the path's next segment, which
+// keeps the reference's summation order.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) {
     const uint32_t n = F.counts[cur];
     const uint32_t* Q = F.queue[cur];
-    uint32_t cHit = 0, cMiss = 0, cTerm = 0, cFog = 0, cEsc = 0, cOcc = 0;
+    ShadeCount c;
     const uint32_t stride = gridDim.x * blockDim.x;
     // Grid-stride over the queue with a wave-uniform base, so every lane of a wave runs the same
     // number of iterations (the wave-level appends need the whole wave).
@@ -270,140 +401,33 @@ __global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) 
             h.t = ho.x; h.u = ho.y; h.v = ho.z; h.obj = __float_as_int(ho.w); h.slot = F.hitSlot[q];
             R rng = rng_load<R>(p);
             const ray3 ray = mk_ray(mk3(ro.x, ro.y, ro.z), mk3(rd.x, rd.y, rd.z));
-            const uint32_t seg = p.seg;
-            const bool primary = seg == 0;
-            const bool last = seg == F.bounces;
-            const bool hit = h.obj >= 0;
-            float sd = 0.0f;
-            vec3 no = ray.o, nd = ray.d;
-            if (S.fog.enabled && medium_scatter(S.fog, ray, 0.0f, hit ? h.t : kInf, rng, sd)) {
-                ++cFog;
-                p.energy = p.energy * S.fog.color;
-                const vec3 pos = at(ray, sd);
-                vec3 ld;
-                float lpdf;
-                if (sky_light_rand(S, rng, ld, lpdf) && lpdf > 0.0f) {
-                    const float fogPdf = hg_pdf(dot3(ray.d, ld), S.fog.anisotropy);
-                    sr.o = pos; sr.d = ld;
-                    sr.skyRad = sky_radiance(S, ld);
-                    sr.f = splat3(fogPdf);
-                    sr.w = power_heuristic(lpdf, fogPdf);
-                    sr.pdf = lpdf;
-                    sr.energy = p.energy;
-                    sr.path = q | (primary ? 0x80000000u : 0u);
-                    shadow = true;
-                }
-                bool term = false;
-                if (seg >= 3) {
-                    const float survive = net_min(1.0f, max_comp(p.energy));
-                    if (next_float(rng) >= survive) { term = true; ++cTerm; }
-                    else p.energy = p.energy / survive;
-                }
-                if (!term) {
-                    const vec3 dir = hg_scatter(ray.d, S.fog.anisotropy, rng);
-                    p.flags |= 1u;
-                    p.scatterPdf = hg_pdf(dot3(ray.d, dir), S.fog.anisotropy);
-                    no = pos;
-                    nd = dir;
-                    cont = !last;
-                }
-            } else if (hit) {
-                ++cHit;
-                if (S.fog.enabled) ++cEsc;
-                SurfaceD s;
-                make_surface(S, ray, h, s);
-                const float dist = h.t;
-                const vec3 view = -ray.d;
-                const float bias = net_max(1e-4f, dist * 1e-4f);
-                const vec3 hitPos = at(ray, dist - bias);
-                if (p.flags & 2u) p.energy = p.energy * beer(p.medium, dist);
-                p.rad = p.rad + s.radiance * p.energy;
-                if (primary) {
-                    PixAcc* a = &F.acc[q];
-                    a->normalSum = a->normalSum + s.normal;
-                    a->normalFallback = s.normal;
-                    a->normalCount += 1;
-                    a->depthSum += dist;
-                    a->depthCount += 1;
-                    a->albedoSum = a->albedoSum + s.albedo;
-                    if (!a->hasUv) { a->hasUv = 1; a->uv = s.uv; }
-                }
-                // SampleSkyDirect (scene.cs:774-794): the sides are tested before the shadow ray,
-                // which only skips rays whose contribution is black either way.
-                vec3 ld;
-                float lpdf;
-                if (sky_light_rand(S, rng, ld, lpdf) && lpdf > 0.0f) {
-                    const vec3 nn = dot3(s.normal, view) >= 0.0f ? s.normal : -s.normal;
-                    const vec3 ng = dot3(s.normalGeo, view) >= 0.0f ? s.normalGeo : -s.normalGeo;
-                    if (dot3(nn, ld) > 0.0f && dot3(ng, ld) > 0.0f) {
-                        sr.o = hitPos; sr.d = ld;
-                        sr.skyRad = sky_radiance(S, ld);
-                        sr.f = opaque_eval(s, view, ld);
-                        sr.w = power_heuristic(lpdf, opaque_pdf(s, view, ld, spec_prob(s, view)));
-                        sr.pdf = lpdf;
-                        sr.energy = p.energy;
-                        sr.path = q;
-                        shadow = true;
-                    }
-                }
-                bool term = false;
-                if (seg >= 3) {
-                    const float survive = net_min(1.0f, max_comp(p.energy));
-                    if (next_float(rng) >= survive) { term = true; ++cTerm; }
-                    else p.energy = p.energy / survive;
-                }
-                if (!term) {
-                    const ScatterD sc = surface_scatter(s, ray.d, view, rng);
-                    p.energy = p.energy * sc.weight;
-                    p.energy = clamp_lum(p.energy, F.clamp);
-                    const bool enter = dot3(sc.dir, s.normalGeo) < 0.0f;
-                    if (sc.refracted) {
-                        if (enter) { p.flags |= 2u; p.medium = s.albedo; }
-                        else p.flags &= ~2u;
-                    } else if (!enter) {
-                        p.flags &= ~2u;
-                    }
-                    if (sc.hasPdf) { p.flags |= 1u; p.scatterPdf = sc.pdf; }
-                    else p.flags &= ~1u;
-                    no = at(ray, dist) + sc.dir * bias;
-                    nd = sc.dir;
-                    cont = !last;
-                }
-            } else {
-                ++cMiss;
-                if (S.fog.enabled) ++cEsc;
-                float mis = 1.0f;
-                if (p.flags & 1u) {
-                    float lp = 0.0f;
-                    if (!sky_light_pdf(S, ray.d, lp)) lp = 0.0f;
-                    mis = power_heuristic(p.scatterPdf, lp);
-                }
-                p.rad = p.rad + sky_radiance(S, ray.d) * mis * p.energy;
-            }
+            vec3 no, nd;
+            shade_path<R>(F, S, q, p, rng, h, ray, cont, shadow, sr, no, nd, c);
             if (cont) {
-                p.seg = seg + 1;
+                p.seg = p.seg + 1;
                 F.rayO[q] = make_float4(no.x, no.y, no.z, 0.0f);
                 F.rayD[q] = make_float4(nd.x, nd.y, nd.z, 0.0f);
             }
             rng_store(p, rng);
             F.paths[q] = p;
         }
-        if (shadow) ++cOcc;
+        if (shadow) ++c.occ;
         const uint32_t ns = wave_append(&F.counts[kShadowCount0 + cur], shadow);
         if (shadow) F.shadows[cur][ns] = sr;
         const uint32_t nc = wave_append(&F.counts[cur ^ 1], cont);
         if (cont) F.queue[cur ^ 1][nc] = q;
     }
     if (F.ctr) {
-        bump(F, 3, cHit); bump(F, 4, cMiss); bump(F, 5, cTerm); bump(F, 6, cFog); bump(F, 7, cEsc);
-        bump(F, 11, cOcc);
+        bump(F, 3, c.hit); bump(F, 4, c.miss); bump(F, 5, c.term); bump(F, 6, c.fog); bump(F, 7, c.esc);
+        bump(F, 11, c.occ);
     }
 }
 
-// ShadowTransmittance (scene.cs:576-600) for one deferred shadow ray, then its direct-light
-// contribution ((((sky * T) * f) * w) / pdf) * energy (scene.cs:793, 809, 659-661, 699).
+// ShadowTransmittance (scene.cs:576-600) of one shadow ray: 0 or 1 from an any-hit traversal in
+// scenes without transparent objects, else the reference's closest-hit walk through transparent
+// surfaces.
 template <bool COUNT, bool TRANSP>
-SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, Tally& tl, uint32_t& extra) {
+SR_DEV vec3 shadow_transmittance(const DevScene& S, const ShadowRec& sr, Tally& tl, uint32_t& extra) {
     ray3 ray = mk_ray(sr.o, sr.d);
     vec3 tr = splat3(1.0f);
     if (!TRANSP) {
@@ -425,13 +449,85 @@ SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, 
             if (!(max_comp(tr) > 1e-4f)) break;
         }
     }
-    if (max_comp(tr) > 0.0f) {
-        if (S.fog.enabled) tr = tr * expf(-medium_optical_depth(S.fog, mk_ray(sr.o, sr.d), 0.0f, kInf));
-        const vec3 c = sr.skyRad * tr * sr.f * sr.w / sr.pdf * sr.energy;
-        const uint32_t q = sr.path & 0x7fffffffu;
-        PathRec* p = &F.paths[q];
-        if (sr.path & 0x80000000u) p->radFog = p->radFog + c;
-        else p->rad = p->rad + c;
+    return tr;
+}
+
+// The shadow ray's direct-light contribution ((((sky * T) * f) * w) / pdf) * energy
+// (scene.cs:793, 809, 659-661, 699), added to the path's radiance or fog radiance.
+SR_DEV void add_direct(const DevScene& S, const ShadowRec& sr, vec3 tr, vec3& rad, vec3& radFog) {
+    if (!(max_comp(tr) > 0.0f)) return;
+    if (S.fog.enabled) tr = tr * expf(-medium_optical_depth(S.fog, mk_ray(sr.o, sr.d), 0.0f, kInf));
+    const vec3 c = sr.skyRad * tr * sr.f * sr.w / sr.pdf * sr.energy;
+    if (sr.path & 0x80000000u) radFog = radFog + c;
+    else rad = rad + c;
+}
+
+template <bool COUNT, bool TRANSP>
+SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, Tally& tl, uint32_t& extra) {
+    const vec3 tr = shadow_transmittance<COUNT, TRANSP>(S, sr, tl, extra);
+    if (!(max_comp(tr) > 0.0f)) return;
+    PathRec* p = &F.paths[sr.path & 0x7fffffffu];
+    vec3 rad = p->rad, radFog = p->radFog;
+    add_direct(S, sr, tr, rad, radFog);
+    if (sr.path & 0x80000000u) p->radFog = radFog;
+    else p->rad = rad;
+}
+
+// Sums a per-lane count over the wave and adds it to a render counter with one atomic; every lane
+// of the wave calls it.
+SR_DEV void bump_sum(const Frame& F, int idx, uint32_t v) {
+    if (F.ctr == nullptr) return;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane_id() == 0 && v) atomicAdd(&F.ctr[idx], (unsigned long long)v);
+}
+
+// The remaining segments of every queued path in one launch: each lane runs its path's
+// Scene.Sample loop (scene.cs:641-737) to the end, shading, tracing the shadow ray and adding its
+// light at once (the order the wavefront launches keep), then tracing the next closest hit. Late
+// segments hold few paths, and one launch per segment would mostly wait on each launch's slowest
+// ray.
+template <class R, bool COUNT, bool TRANSP>
+__global__ void __launch_bounds__(kBlock) k_tail(Frame F, DevScene S, int cur) {
+    const uint32_t n = F.counts[cur];
+    const uint32_t* Q = F.queue[cur];
+    ShadeCount c;
+    Tally tl = {0, 0, 0};
+    uint32_t extra = 0, traces = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t q = Q[i];
+        PathRec p = F.paths[q];
+        R rng = rng_load<R>(p);
+        const float4 ho = F.hitA[q];
+        const float4 ro = F.rayO[q], rd = F.rayD[q];
+        HitRec h;
+        h.t = ho.x; h.u = ho.y; h.v = ho.z; h.obj = __float_as_int(ho.w); h.slot = F.hitSlot[q];
+        ray3 ray = mk_ray(mk3(ro.x, ro.y, ro.z), mk3(rd.x, rd.y, rd.z));
+        for (;;) {
+            bool cont, shadow;
+            ShadowRec sr;
+            vec3 no, nd;
+            shade_path<R>(F, S, q, p, rng, h, ray, cont, shadow, sr, no, nd, c);
+            if (shadow) {
+                ++c.occ;
+                add_direct(S, sr, shadow_transmittance<COUNT, TRANSP>(S, sr, tl, extra), p.rad, p.radFog);
+            }
+            if (!cont) break;
+            p.seg = p.seg + 1;
+            ray = mk_ray(no, nd);
+            h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
+            scene_closest<COUNT>(S, ray, h, tl, sr_lds + threadIdx.x);
+            ++traces;
+        }
+        rng_store(p, rng);
+        F.paths[q] = p;
+    }
+    if (F.ctr) {
+        bump_sum(F, 3, c.hit); bump_sum(F, 4, c.miss); bump_sum(F, 5, c.term); bump_sum(F, 6, c.fog);
+        bump_sum(F, 7, c.esc); bump_sum(F, 11, c.occ + extra); bump_sum(F, 2, traces);
+        // This launch's traversal work is kept apart (kTailNode..) so the trace kernel's own share
+        // stays measurable; sr_counters_read adds it back.
+        bump_sum(F, kTailTrace, traces); bump_sum(F, kTailOcclude, c.occ + extra);
+        bump_sum(F, kTailNode, tl.box); bump_sum(F, kTailShape, tl.shape); bump_sum(F, 12, tl.mesh);
     }
 }
 
@@ -479,6 +575,15 @@ __global__ void __launch_bounds__(kBlock) k_trace(Frame F, DevScene S, int cur) 
     }
     flush_tally<COUNT>(F, tl);
     if (F.ctr && extra) atomicAdd(&F.ctr[11], (unsigned long long)extra);
+}
+
+// After the fused tail launch: both ray queues are spent. k_next appends the next sample's camera
+// rays to queue 0, which the tail launch read but did not empty.
+__global__ void k_queue_reset(Frame F) {
+    if (threadIdx.x == 0) {
+        F.counts[kRayCount0] = 0;
+        F.counts[kRayCount1] = 0;
+    }
 }
 
 // Renderer.RelativeStdErr (renderer.cs:232-237).
@@ -815,6 +920,9 @@ struct DevBuf {
 struct sr_ctx {
     int device = 0;
     int bvhBuilder = SR_BVH_BUILDER_HOST;
+    // Segment from which a render finishes every path in one fused launch (k_tail); beyond the
+    // bounce count it never does. SHARPRAY_TAIL_SEGMENT overrides the default.
+    uint32_t tailSegment = 3;
     hipStream_t stream = nullptr;
     std::vector<HostTexture> textures;
     std::vector<HostMesh> meshes;
@@ -1137,6 +1245,7 @@ int sr_create(int32_t device, sr_ctx** out) {
     if (e != hipSuccess) { delete c; return fail(SR_ERR_HIP, hipGetErrorString(e)); }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->gridCUs = prop.multiProcessorCount;
+    if (const char* t = getenv("SHARPRAY_TAIL_SEGMENT")) c->tailSegment = (uint32_t)strtoul(t, nullptr, 10);
     *out = c;
     return SR_OK;
 }
@@ -1464,7 +1573,7 @@ int alloc_frame(sr_ctx* c, uint32_t maxSlots) {
         (rc = ensure(c->dHitSlot, maxSlots)) || (rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dShadows, (size_t)maxSlots * 2)) ||
         (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxSlots)) || (rc = ensure(c->dQueue1, maxSlots)) ||
         (rc = ensure(c->dCounts, kCounterWords)) ||
-        (rc = ensure(c->dCtr, SR_COUNTER_COUNT)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
+        (rc = ensure(c->dCtr, kCtrWords)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
         (rc = ensure(c->dNormal, px * 3)) || (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) || (rc = ensure(c->dAlbedo, px * 3)) ||
         (rc = ensure(c->dVar, px)) || (rc = ensure(c->dSamples, px)))
         return rc;
@@ -1497,18 +1606,28 @@ int run_band(sr_ctx* c, const Frame& F) {
     hipLaunchKernelGGL(k_start<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
     mark(c, 3);
     auto trace = c->dev.anyTransparent ? k_trace<COUNT, true> : k_trace<COUNT, false>;
+    auto tail = c->dev.anyTransparent ? k_tail<R, COUNT, true> : k_tail<R, COUNT, false>;
     for (uint32_t smp = 0; smp < F.maxS; ++smp) {
         int cur = 0;
+        bool fused = false;
         for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
             hipLaunchKernelGGL(trace, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
             mark(c, 0);
+            if (seg == c->tailSegment) {  // the rest of every path in one launch
+                hipLaunchKernelGGL(tail, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
+                mark(c, 2);
+                hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, F);
+                fused = true;
+                break;
+            }
             hipLaunchKernelGGL(k_shade<R>, dim3(g), dim3(kBlock), 0, s, F, c->dev, cur);
             mark(c, 1);
             cur ^= 1;
         }
-        // The last segment's shadow rays (its ray queue is empty).
-        hipLaunchKernelGGL(trace, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
-        mark(c, 2);
+        if (!fused) {  // the last segment's shadow rays (its ray queue is empty)
+            hipLaunchKernelGGL(trace, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
+            mark(c, 2);
+        }
         hipLaunchKernelGGL(k_next<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
         mark(c, 3);
         c->iterations++;
@@ -1593,7 +1712,7 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
     if (rc) return rc;
     if (!ctx->evStart) HIP_TRY(hipEventCreate(&ctx->evStart));
     if (!ctx->evEnd) HIP_TRY(hipEventCreate(&ctx->evEnd));
-    if (ctx->countersOn) HIP_TRY(hipMemsetAsync(ctx->dCtr.ptr, 0, SR_COUNTER_COUNT * sizeof(unsigned long long), ctx->stream));
+    if (ctx->countersOn) HIP_TRY(hipMemsetAsync(ctx->dCtr.ptr, 0, kCtrWords * sizeof(unsigned long long), ctx->stream));
     ctx->profUsed = 0;
     ctx->iterations = 0;
     ctx->view = make_view(view, p->width, p->height);
@@ -1718,10 +1837,32 @@ int sr_counters_read(sr_ctx* ctx, int64_t* out) {
     if (!ctx->rendering || !ctx->countersOn) return fail(SR_ERR_STATE, "counters were not collected");
     int rc = wait_render(ctx);
     if (rc) return rc;
-    unsigned long long v[SR_COUNTER_COUNT];
+    unsigned long long v[kCtrWords];
     HIP_TRY(hipMemcpy(v, ctx->dCtr.ptr, sizeof(v), hipMemcpyDeviceToHost));
     for (int i = 0; i < SR_COUNTER_COUNT; ++i) out[i] = (int64_t)v[i];
-    // Host-known counters: Pixel, Sample (per pixel: its sample count, capped at maxSamples).
+    out[10] += (int64_t)v[kTailTrace];
+    out[13] += (int64_t)v[kTailNode];
+    out[14] += (int64_t)v[kTailShape];
+    return SR_OK;
+}
+
+int sr_counters_read_fused(sr_ctx* ctx, int64_t* out) {
+    if (!ctx || !out) return fail(SR_ERR_INVALID, "null argument");
+    if (!ctx->rendering || !ctx->countersOn) return fail(SR_ERR_STATE, "counters were not collected");
+    int rc = wait_render(ctx);
+    if (rc) return rc;
+    unsigned long long v[kCtrWords];
+    HIP_TRY(hipMemcpy(v, ctx->dCtr.ptr, sizeof(v), hipMemcpyDeviceToHost));
+    out[0] = (int64_t)v[kTailNode];
+    out[1] = (int64_t)v[kTailShape];
+    out[2] = (int64_t)v[kTailTrace];
+    out[3] = (int64_t)v[kTailOcclude];
+    return SR_OK;
+}
+
+int sr_set_tail_segment(sr_ctx* ctx, uint32_t segment) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    ctx->tailSegment = segment;
     return SR_OK;
 }
 

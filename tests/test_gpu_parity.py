@@ -545,3 +545,23 @@ def test_device_bvh_builder_bit_exact(case):
     assert hn.shape == dn.shape
     assert np.array_equal(hn.view(np.uint32), dn.view(np.uint32))
     assert np.array_equal(hi, di)
+
+
+@pytest.mark.parametrize("tail", [0, 1, 2, 5])
+def test_fused_tail_launch_matches_wavefront(tail):
+    """Finishing paths in one fused launch from segment `tail` on gives bit-identical AOVs and
+    counters to the all-wavefront render (tail disabled), for odd and even switch segments and two
+    samples in a row (the second sample starts from the queues the first left)."""
+    from sharpray_amd import RNG_COUNTER, Renderer, scenes
+    spec = scenes.config_plain_cpu(scale=0.25, spp=3)
+    out = {}
+    for t in (99, tail):
+        scene, view = scenes.instantiate(spec)
+        scene.set_tail_segment(t)
+        r = Renderer(scene, view, spec.width, spec.height, 32, 3, 3, 0.0, spec.bounces, spec.indirect_clamp,
+                     rng_policy=RNG_COUNTER, rng_seed=5, counters=True)
+        r.wait()
+        out[t] = (r.radiance.copy(), r.radiance_fog.copy(), r.depth.copy(), r.counters())
+    a, b = out[99], out[tail]
+    assert np.array_equal(bits(a[0]), bits(b[0])) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    assert a[3] == b[3], (a[3], b[3])
