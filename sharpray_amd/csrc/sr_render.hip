@@ -537,7 +537,12 @@ __global__ void __launch_bounds__(kBlock) k_tail(Frame F, DevScene S, int cur) {
 // summation order), and the short tail launches of late segments share the GPU with the shadow
 // rays instead of running alone. Each wave takes only one kind of item.
 template <bool COUNT, bool TRANSP>
-__global__ void __launch_bounds__(kBlock) k_trace(Frame F, DevScene S, int cur) {
+// Register budget of the trace kernel: at least 7 waves per SIMD (72 VGPRs, a few spilled) beat
+// the unconstrained 85 VGPRs / 5 waves by 4 % of trace time on the 1080p / 1M-triangle frame.
+#ifndef SR_TRACE_MIN_WAVES
+#define SR_TRACE_MIN_WAVES 7
+#endif
+__global__ void __launch_bounds__(kBlock, SR_TRACE_MIN_WAVES) k_trace(Frame F, DevScene S, int cur) {
     // The two queues this segment's shade fills start empty; nothing else touches them meanwhile.
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         F.counts[cur ^ 1] = 0;
