@@ -379,8 +379,11 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
 // One path segment for every queued path (scene.cs:641-737). Direct sky light becomes a deferred
 // shadow ray whose contribution the next trace launch adds before the path's next segment, which
 // keeps the reference's summation order.
+#ifndef SR_SHADE_MIN_WAVES
+#define SR_SHADE_MIN_WAVES 1
+#endif
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_shade(Frame F, DevScene S, int cur) {
+__global__ void __launch_bounds__(kBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, DevScene S, int cur) {
     const uint32_t n = F.counts[cur];
     const uint32_t* Q = F.queue[cur];
     ShadeCount c;
@@ -486,8 +489,11 @@ SR_DEV void bump_sum(const Frame& F, int idx, uint32_t v) {
 // light at once (the order the wavefront launches keep), then tracing the next closest hit. Late
 // segments hold few paths, and one launch per segment would mostly wait on each launch's slowest
 // ray.
+#ifndef SR_TAIL_MIN_WAVES
+#define SR_TAIL_MIN_WAVES 1
+#endif
 template <class R, bool COUNT, bool TRANSP>
-__global__ void __launch_bounds__(kBlock) k_tail(Frame F, DevScene S, int cur) {
+__global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, DevScene S, int cur) {
     const uint32_t n = F.counts[cur];
     const uint32_t* Q = F.queue[cur];
     ShadeCount c;
