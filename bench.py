@@ -149,6 +149,100 @@ def bench_cnn(args, spec, scene, view):
     print(json.dumps(line), flush=True)
 
 
+def bench_denoise(args, spec, scene, view):
+    """Equal-quality / equal-time comparison at 1080p on the 1M-triangle scene: the plain tracer's
+    relMSE = mean (x - r)^2 / (r^2 + 0.01) against a high-spp render (counter RNG, other seed) as a
+    function of its samples, the learned denoiser (shipped weights) and the bilateral filter (7x7
+    window, strength chosen by grid search against the reference, which favours it; on the per-pixel
+    sample mean and on the Radiance AOV) on 1 and 4 spp renders, and the time the plain tracer needs to reach each denoiser's relMSE (log-log
+    interpolation of its relMSE-time curve). value = that time for the learned denoiser at 1 spp
+    divided by the denoiser's own time (render + denoise)."""
+    import math
+    import numpy as np
+    from sharpray_amd import RNG_COUNTER, Compositor, KpnDenoiser, Renderer
+    W, H = spec.width, spec.height
+
+    def render(spp, seed):
+        t0 = time.perf_counter()
+        r = Renderer(scene, view, W, H, 32, spp, spp, 0.0, spec.bounces, spec.indirect_clamp,
+                     rng_policy=RNG_COUNTER, rng_seed=seed)
+        r.wait()
+        return r, time.perf_counter() - t0
+
+    render(1, 3)  # warm-up
+    t0 = time.perf_counter()
+    ref = render(args.ref_spp, 999)[0].sample_mean.reshape(H, W, 3).astype(np.float64)
+    log(f"[bench] reference {args.ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+
+    def rel(x):
+        x = np.asarray(x, np.float64).reshape(H, W, 3)
+        return float(np.mean((x - ref) ** 2 / (ref ** 2 + 0.01)))
+
+    # The plain tracer's per-pixel sample mean (its Radiance AOV divides pixels at the sample cap by
+    # M + 1, like the reference; relmse_radiance_aov is that image's error).
+    plain = []
+    for spp in (1, 2, 4, 8, 16, 32, 64):
+        r, t = render(spp, 7)
+        plain.append({"spp": spp, "ms": round(1e3 * t, 2), "relmse": round(rel(r.sample_mean), 6),
+                      "relmse_radiance_aov": round(rel(r.radiance), 6)})
+        log(f"[bench] plain {spp} spp: {1e3 * t:.1f} ms, relMSE {plain[-1]['relmse']:.5f}")
+
+    def matched_ms(e):
+        """Plain-tracer time at relMSE e (log-log interpolation; 1/spp extrapolation past the ends)."""
+        pts = [(math.log(p["ms"]), math.log(p["relmse"])) for p in plain]
+        for (t0_, e0), (t1_, e1) in zip(pts, pts[1:]):
+            if e1 <= math.log(e) <= e0:
+                f = (math.log(e) - e0) / (e1 - e0) if e1 != e0 else 0.0
+                return math.exp(t0_ + f * (t1_ - t0_))
+        tl, el = pts[-1] if math.log(e) < pts[-1][1] else pts[0]
+        return math.exp(tl + (el - math.log(e)))
+
+    den = KpnDenoiser(scene)
+    den.load(KpnDenoiser.pretrained_weights())
+    rows = {}
+    for spp in (1, 4):
+        r, t = render(spp, 7)
+        kpn = den.denoise(r)
+        ms_den = den.benchmark(r, 10)
+        rows[f"kpn_{spp}spp"] = {"relmse": round(rel(kpn), 6), "ms": round(1e3 * t + ms_den, 2),
+                                 "denoise_ms": round(ms_den, 3)}
+        for tag, mean in (("", True), ("_radiance_aov", False)):
+            best = None
+            for strength in (0.05, 0.2, 1.0, 5.0, 25.0):
+                comp = Compositor(0.000694, strength, 1.75, 0.25, 2.0, 0.125, 0.2)
+                tf = time.perf_counter()
+                out = comp.filter(r, sample_mean=mean)
+                tf = time.perf_counter() - tf
+                e = rel(out)
+                if best is None or e < best[0]:
+                    best = (e, tf, strength)
+            rows[f"bilateral_{spp}spp{tag}"] = {"relmse": round(best[0], 6), "ms": round(1e3 * (t + best[1]), 2),
+                                                "strength": best[2]}
+    for k, v in rows.items():
+        v["plain_ms_at_same_relmse"] = round(matched_ms(v["relmse"]), 2)
+        v["speedup_vs_plain"] = round(v["plain_ms_at_same_relmse"] / v["ms"], 3)
+        log(f"[bench] {k}: {v}")
+    line = {
+        "metric": "relMSE-matched time: learned denoiser vs plain @1080p/1M tris",
+        "value": rows["kpn_1spp"]["speedup_vs_plain"],
+        "unit": "x (plain-tracer time to the same relMSE / render + denoise time)",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": rows["kpn_1spp"]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (seeded procedural scene, shipped trained weights)",
+        "config": {"workload": spec.name, "width": W, "height": H, "bounces": spec.bounces,
+                   "triangles": spec.triangle_count, "reference_spp": args.ref_spp, "relmse_eps": 0.01},
+        "plain": plain,
+        "denoisers": rows,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,10 +250,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="image scale (1.0 = 1920x1080)")
     ap.add_argument("--spp", type=int, default=None, help="samples per pixel (default 64; 4 for --workload fog)")
-    ap.add_argument("--workload", choices=["plain", "fog", "adjoint", "cnn"], default="plain",
+    ap.add_argument("--ref-spp", type=int, default=2048, help="reference samples for --workload denoise")
+    ap.add_argument("--workload", choices=["plain", "fog", "adjoint", "cnn", "denoise"], default="plain",
                     help="plain: the 1080p / 1M-triangle forward frame; fog: the same scene inside a 128^3 fp16 "
                          "density grid (delta tracking); adjoint: forward + backward at 512x512, 16 spp, 100k triangles; "
-                         "cnn: the U-Net denoiser on a 1080p 4 spp render")
+                         "cnn: the U-Net denoiser on a 1080p 4 spp render; denoise: relMSE-matched time of the learned "
+                         "denoiser and the bilateral filter against the plain tracer at 1080p")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--light-sampler", choices=["cdf", "alias"], default="cdf",
                     help="environment light sampling: the reference's row/column CDFs or the alias table")
@@ -191,7 +287,7 @@ def main():
         spec = scenes.config_fog(scale=args.scale, spp=args.spp or 4)
     elif args.workload == "adjoint":
         spec = scenes.config_adjoint(scale=args.scale, spp=args.spp or 16)
-    elif args.workload == "cnn":
+    elif args.workload in ("cnn", "denoise"):
         spec = scenes.config_plain_gpu(scale=args.scale, spp=args.spp or 4)
     else:
         spec = scenes.config_plain_gpu(scale=args.scale, spp=args.spp or 64)
@@ -208,6 +304,9 @@ def main():
         return
     if args.workload == "cnn":
         bench_cnn(args, spec, scene, view)
+        return
+    if args.workload == "denoise":
+        bench_denoise(args, spec, scene, view)
         return
 
     from sharpray_amd import LIGHT_ALIAS, LIGHT_CDF, Renderer

@@ -213,6 +213,10 @@ int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* params, float* ou
  * weight is also multiplied by exp(-|albedo_c - albedo_n|^2 / (2 albedo_limit^2)), applied after
  * the depth term (albedo_limit > 0). No counterpart in the reference. */
 int sr_denoise_bilateral_guided(sr_ctx* ctx, const sr_denoise_params* params, float albedo_limit, float* out_rgb);
+/* The same filter on the per-pixel sample mean Radiance * Samples / min(Samples, max_samples)
+ * instead of the Radiance AOV (the reference divides capped pixels by M + 1, renderer.cs:205-217).
+ * No counterpart in the reference. */
+int sr_denoise_bilateral_mean(sr_ctx* ctx, const sr_denoise_params* params, float* out_rgb);
 /* Same filter on caller-provided device arrays (radiance/normals: w*h*3, depth/variance: w*h). */
 int sr_denoise_bilateral_device(sr_ctx* ctx, const sr_denoise_params* params, const float* d_radiance,
                                 const float* d_normals, const float* d_depth, const float* d_variance,
@@ -307,10 +311,13 @@ int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad);
  * the order of sharpray_amd.cnn_layers(); concatenated inputs list the upsampled channels first. */
 int sr_cnn_weight_count(uint64_t* count);
 int sr_cnn_load(sr_ctx* ctx, const float* weights, uint64_t count);
-/* Denoises the last render (radiance, albedo, normal, depth AOVs) into out_rgb (host, w*h*3). */
+/* Denoises the last render into out_rgb (host, w*h*3). The network sees the per-pixel sample mean,
+ * Radiance * Samples / min(Samples, max_samples): pixels that ran to the sample cap hold M samples
+ * divided by M + 1 (renderer.cs:205-217), a bias of 1/(M+1) that no kernel of convex weights could
+ * undo. Albedo, normal and depth AOVs are the guides. */
 int sr_cnn_denoise(sr_ctx* ctx, float* out_rgb);
-/* Same on caller device arrays (rad/alb/nrm: w*h*3, dep: w*h), out device w*h*3; asynchronous on
- * the context's stream. */
+/* Same on caller device arrays (rad: the per-pixel sample mean, alb/nrm: w*h*3, dep: w*h), out
+ * device w*h*3; asynchronous on the context's stream. */
 int sr_cnn_denoise_device(sr_ctx* ctx, const float* d_radiance, const float* d_albedo, const float* d_normals,
                           const float* d_depth, uint32_t width, uint32_t height, float* d_out);
 /* Activations of a layer after the last denoise (layer -1: the input features) as floats,

@@ -507,6 +507,15 @@ class Renderer:
     uv = property(lambda self: self._read()["uv"])
     depth = property(lambda self: self._read()["depth"])
     samples = property(lambda self: self._read()["samples"])
+
+    @property
+    def sample_mean(self):
+        """Radiance * Samples / min(Samples, max_samples): the per-pixel mean of the samples taken
+        (the reference divides pixels that reach the cap by M + 1, renderer.cs:205-217)."""
+        c = self.samples.astype(np.float32)
+        m = np.float32(self.params.max_samples)
+        f = np.where(c > m, c / m, np.float32(1.0)).astype(np.float32)
+        return (self.radiance * f[:, None]).astype(np.float32)
     variance = property(lambda self: self._read()["variance"])
     albedo = property(lambda self: self._read()["albedo"])  # primary-hit albedo guide (no reference counterpart)
 
@@ -859,11 +868,18 @@ class Compositor:
         return np.array([p.radius, p.strength, p.strength_max, p.luminance_boost, p.luminance_limit, p.normal_limit,
                          p.depth_limit], dtype=np.float32)
 
-    def filter(self, renderer, albedo_limit=None):
+    def filter(self, renderer, albedo_limit=None, sample_mean=False):
         """Filter (compose.cs:101-204) of the renderer's surface radiance; with albedo_limit the
-        albedo AOV joins normal and depth as a guide (no reference counterpart)."""
+        albedo AOV joins normal and depth as a guide; with sample_mean the filter sees
+        Radiance * Samples / min(Samples, max_samples), the unbiased per-pixel mean (neither has a
+        reference counterpart)."""
         out = np.empty((renderer.width * renderer.height, 3), np.float32)
-        if albedo_limit is None:
+        if sample_mean:
+            if albedo_limit is not None:
+                raise ValueError("sample_mean and albedo_limit cannot be combined")
+            check(N.lib().sr_denoise_bilateral_mean(renderer.scene.ctx, ctypes.byref(self.params), fptr(out)),
+                  "denoise")
+        elif albedo_limit is None:
             check(N.lib().sr_denoise_bilateral(renderer.scene.ctx, ctypes.byref(self.params), fptr(out)), "denoise")
         else:
             check(N.lib().sr_denoise_bilateral_guided(renderer.scene.ctx, ctypes.byref(self.params),

@@ -116,6 +116,7 @@ SIGNATURES = {
     "sr_denoise_bilateral": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DenoiseParams_), c_float_p]),
     "sr_denoise_bilateral_guided": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DenoiseParams_), ctypes.c_float,
                                                    c_float_p]),
+    "sr_denoise_bilateral_mean": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DenoiseParams_), c_float_p]),
     "sr_denoise_bilateral_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(DenoiseParams_), ctypes.c_void_p,
                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                    ctypes.c_uint32, ctypes.c_void_p]),

@@ -813,6 +813,19 @@ __global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* ra
     out[ci * 3] = r.x; out[ci * 3 + 1] = r.y; out[ci * 3 + 2] = r.z;
 }
 
+// The per-pixel sample mean of a render: Radiance * Samples / min(Samples, maxSamples). A pixel that
+// ran to the sample cap holds the sum of M samples divided by M + 1 (renderer.cs:205-217); one that
+// converged early holds the true mean. No reference counterpart (denoiser input).
+__global__ void k_sample_mean(const float* rad, const uint32_t* samples, uint32_t maxS, int32_t n, float* out) {
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = samples[i];
+    const float f = c > maxS ? (float)c / (float)maxS : 1.0f;
+    out[i * 3] = rad[i * 3] * f;
+    out[i * 3 + 1] = rad[i * 3 + 1] * f;
+    out[i * 3 + 2] = rad[i * 3 + 2] * f;
+}
+
 // Compositor.FilterFog (compose.cs:206-245): circular Gaussian of the fog radiance, centre weight 1,
 // neighbours strength * exp(-d^2 / (2 r^2)), same loop order as the reference.
 __global__ void __launch_bounds__(256) k_filter_fog(float radiusInv, float strength, int32_t kr, const float* fog,
@@ -1973,7 +1986,8 @@ int launch_filter(sr_ctx* ctx, const sr_denoise_params* p, const float* rad, con
 }
 
 // Filters the last render's AOVs into host memory.
-int filter_to_host(sr_ctx* ctx, const sr_denoise_params* p, bool guided, float albedoLimit, float* out_rgb) {
+int filter_to_host(sr_ctx* ctx, const sr_denoise_params* p, bool guided, float albedoLimit, float* out_rgb,
+                   bool sampleMean = false) {
     if (!ctx || !p || !out_rgb) return fail(SR_ERR_INVALID, "denoise: null argument");
     if (!ctx->rendering) return fail(SR_ERR_STATE, "no render to denoise");
     int rc0 = wait_render(ctx);
@@ -1981,8 +1995,15 @@ int filter_to_host(sr_ctx* ctx, const sr_denoise_params* p, bool guided, float a
     HIP_TRY(hipSetDevice(ctx->device));
     const uint32_t W = ctx->params.width, H = ctx->params.height;
     float* dOut = nullptr;
-    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 12));
-    int rc = launch_filter(ctx, p, ctx->dRad.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, ctx->dVar.ptr,
+    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * (sampleMean ? 24 : 12)));
+    const float* rad = ctx->dRad.ptr;
+    if (sampleMean) {
+        float* dMean = dOut + (size_t)W * H * 3;
+        hipLaunchKernelGGL(k_sample_mean, dim3((W * H + 255) / 256), dim3(256), 0, ctx->stream, ctx->dRad.ptr,
+                           ctx->dSamples.ptr, ctx->params.max_samples, (int32_t)(W * H), dMean);
+        rad = dMean;
+    }
+    int rc = launch_filter(ctx, p, rad, ctx->dNormal.ptr, ctx->dDepth.ptr, ctx->dVar.ptr,
                            guided ? ctx->dAlbedo.ptr : nullptr, albedoLimit, W, H, dOut);
     hipError_t e = rc == SR_OK ? hipStreamSynchronize(ctx->stream) : hipSuccess;
     if (rc == SR_OK && e == hipSuccess) e = hipMemcpy(out_rgb, dOut, (size_t)W * H * 12, hipMemcpyDeviceToHost);
@@ -2004,6 +2025,10 @@ int sr_denoise_bilateral(sr_ctx* ctx, const sr_denoise_params* p, float* out_rgb
 
 int sr_denoise_bilateral_guided(sr_ctx* ctx, const sr_denoise_params* p, float albedo_limit, float* out_rgb) {
     return filter_to_host(ctx, p, true, albedo_limit, out_rgb);
+}
+
+int sr_denoise_bilateral_mean(sr_ctx* ctx, const sr_denoise_params* p, float* out_rgb) {
+    return filter_to_host(ctx, p, false, 0.0f, out_rgb, true);
 }
 
 // ---- Compose tail ----------------------------------------------------------------------------------
@@ -2323,8 +2348,11 @@ int sr_cnn_denoise(sr_ctx* ctx, float* out_rgb) {
     HIP_TRY(hipSetDevice(ctx->device));
     const uint32_t W = ctx->params.width, H = ctx->params.height;
     float* dOut = nullptr;
-    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 12));
-    rc = sr_cnn_denoise_device(ctx, ctx->dRad.ptr, ctx->dAlbedo.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, W, H, dOut);
+    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 24));
+    float* dMean = dOut + (size_t)W * H * 3;
+    hipLaunchKernelGGL(k_sample_mean, dim3((W * H + 255) / 256), dim3(256), 0, ctx->stream, ctx->dRad.ptr,
+                       ctx->dSamples.ptr, ctx->params.max_samples, (int32_t)(W * H), dMean);
+    rc = sr_cnn_denoise_device(ctx, dMean, ctx->dAlbedo.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, W, H, dOut);
     hipError_t e = rc == SR_OK ? hipStreamSynchronize(ctx->stream) : hipSuccess;
     if (rc == SR_OK && e == hipSuccess) e = hipMemcpy(out_rgb, dOut, (size_t)W * H * 12, hipMemcpyDeviceToHost);
     (void)hipFree(dOut);
@@ -2352,16 +2380,24 @@ int sr_cnn_benchmark(sr_ctx* ctx, uint32_t iterations, double* ms_per_run) {
     HIP_TRY(hipSetDevice(ctx->device));
     const uint32_t W = ctx->params.width, H = ctx->params.height;
     float* dOut = nullptr;
-    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 12));
+    HIP_TRY(hipMalloc((void**)&dOut, (size_t)W * H * 24));
+    float* dMean = dOut + (size_t)W * H * 3;
+    const dim3 gm((W * H + 255) / 256);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t e = hipEventCreate(&e0);
     if (e == hipSuccess) e = hipEventCreate(&e1);
-    // One untimed run sizes the activations; the timed runs are back to back on the stream.
-    if (e == hipSuccess)
-        rc = sr_cnn_denoise_device(ctx, ctx->dRad.ptr, ctx->dAlbedo.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, W, H, dOut);
+    // One untimed run sizes the activations; the timed runs (sample mean + network) are back to back.
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_sample_mean, gm, dim3(256), 0, ctx->stream, ctx->dRad.ptr, ctx->dSamples.ptr,
+                           ctx->params.max_samples, (int32_t)(W * H), dMean);
+        rc = sr_cnn_denoise_device(ctx, dMean, ctx->dAlbedo.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, W, H, dOut);
+    }
     if (e == hipSuccess && rc == SR_OK) e = hipEventRecord(e0, ctx->stream);
-    for (uint32_t i = 0; e == hipSuccess && rc == SR_OK && i < iterations; ++i)
-        rc = sr_cnn_denoise_device(ctx, ctx->dRad.ptr, ctx->dAlbedo.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, W, H, dOut);
+    for (uint32_t i = 0; e == hipSuccess && rc == SR_OK && i < iterations; ++i) {
+        hipLaunchKernelGGL(k_sample_mean, gm, dim3(256), 0, ctx->stream, ctx->dRad.ptr, ctx->dSamples.ptr,
+                           ctx->params.max_samples, (int32_t)(W * H), dMean);
+        rc = sr_cnn_denoise_device(ctx, dMean, ctx->dAlbedo.ptr, ctx->dNormal.ptr, ctx->dDepth.ptr, W, H, dOut);
+    }
     if (e == hipSuccess && rc == SR_OK) e = hipEventRecord(e1, ctx->stream);
     if (e == hipSuccess && rc == SR_OK) e = hipEventSynchronize(e1);
     float ms = 0.0f;

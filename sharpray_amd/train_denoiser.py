@@ -7,7 +7,8 @@ Training scenes are seeded procedural scenes (fBm-displaced spheres of 10k-100k 
 positions over a ground quad, random GGX materials, log-normal or sun-and-gradient environment
 maps). Each scene is seen from `--views` random cameras, and each view contributes one example:
 the AOVs of a low-spp render (radiance, albedo, normals, depth; the sample count cycles through
-`--noisy-spp`) and the radiance of a high-spp render of the same view. Held-out scenes are
+`--noisy-spp`) and the radiance of a high-spp render of the same view, both as per-pixel sample
+means (the renderer, like the reference, divides pixels that reach the sample cap by M + 1). Held-out scenes are
 then rendered at the evaluation size; the learned denoiser at `--eval-spp` samples is compared
 with the bilateral filter at 1x and 4x those samples (its strength chosen by grid search on the
 references, which favours it), in relMSE = mean (x - r)^2 / (r^2 + 0.01).
@@ -81,10 +82,11 @@ def render_pair(spec, noisy_spp, target_spp, seed, scene=None, view=None):
         out[key] = r
         if key == "noisy":
             H, W = spec.height, spec.width
-            aov = dict(rad=r.radiance.reshape(H, W, 3).copy(), alb=r.albedo.reshape(H, W, 3).copy(),
+            aov = dict(rad=r.sample_mean.reshape(H, W, 3).copy(), raw=r.radiance.reshape(H, W, 3).copy(),
+                       alb=r.albedo.reshape(H, W, 3).copy(),
                        nrm=r.normals.reshape(H, W, 3).copy(), dep=r.depth.reshape(H, W).copy(),
                        var=r.variance.copy())
-    target = out["target"].radiance.reshape(spec.height, spec.width, 3).copy()
+    target = out["target"].sample_mean.reshape(spec.height, spec.width, 3).copy()
     return scene, view, aov, target
 
 
@@ -189,18 +191,21 @@ def main(argv=None):
             r1 = Renderer(scene, view, W, H, 32, espp, espp, 0.0, spec.bounces, spec.indirect_clamp,
                           rng_policy=RNG_COUNTER, rng_seed=seed)
             r1.wait()
-            row = {"noisy": rel_mse(a["rad"], ref), "kpn": rel_mse(ev.denoise(r1), ref)}
+            row = {"noisy": rel_mse(a["rad"], ref), "noisy_raw": rel_mse(a["raw"], ref),
+                   "kpn": rel_mse(ev.denoise(r1), ref)}
             for mult in (1, 4):
                 spp = espp * mult
                 rb = Renderer(scene, view, W, H, 32, spp, spp, 0.0, spec.bounces, spec.indirect_clamp,
                               rng_policy=RNG_COUNTER, rng_seed=seed + 31 * mult)
                 rb.wait()
-                best = rel_mse(rb.radiance, ref)
-                # 7x7 circular window (radius 3 px at this size), strength by grid search.
-                for strength in (0.05, 0.2, 1.0, 5.0, 25.0):
-                    comp = Compositor(1.0 / W, strength, 1.75, 0.25, 2.0, 0.125, 0.2)
-                    best = min(best, rel_mse(comp.filter(rb), ref))
-                row[f"bilateral_{mult}x"] = best
+                # 7x7 circular window (radius 3 px at this size), strength by grid search; on the
+                # reference's Radiance AOV and on the per-pixel sample mean.
+                for tag, mean in (("", True), ("_raw", False)):
+                    best = rel_mse(rb.sample_mean if mean else rb.radiance, ref)
+                    for strength in (0.05, 0.2, 1.0, 5.0, 25.0):
+                        comp = Compositor(1.0 / W, strength, 1.75, 0.25, 2.0, 0.125, 0.2)
+                        best = min(best, rel_mse(comp.filter(rb, sample_mean=mean), ref))
+                    row[f"bilateral_{mult}x{tag}"] = best
             res.append(row)
             log(f"[eval {espp} spp] scene {j}: " + ", ".join(f"{k} {v:.4f}" for k, v in row.items()))
         report["eval"][f"{espp}spp"] = {k: round(float(np.mean([r[k] for r in res])), 5) for k in res[0]}

@@ -313,7 +313,7 @@ def test_kpn_denoiser_layers_match_cpu_restatement():
     den.load(blob)
     out = den.denoise(r)
     H, W = spec.height, spec.width
-    rad = r.radiance.reshape(H, W, 3)
+    rad = r.sample_mean.reshape(H, W, 3)  # the network's input: Radiance * Samples / min(Samples, spp)
     alb, nrm, dep = r.albedo.reshape(H, W, 3), r.normals.reshape(H, W, 3), r.depth.reshape(H, W)
     feat = den.layer(-1)
     ref_feat = C.features(rad, alb, nrm, dep)
@@ -342,12 +342,12 @@ def _kpn_pair(spec_fn, seed):
     scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=seed)
     r.wait()
     H, W = spec.height, spec.width
-    noisy = dict(rad=r.radiance.reshape(H, W, 3).copy(), alb=r.albedo.reshape(H, W, 3).copy(),
+    noisy = dict(rad=r.sample_mean.reshape(H, W, 3).copy(), alb=r.albedo.reshape(H, W, 3).copy(),
                  nrm=r.normals.reshape(H, W, 3).copy(), dep=r.depth.reshape(H, W).copy())
     spec.min_samples = spec.max_samples = 64
     _, _, rt = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=seed + 1000)
     rt.wait()
-    return scene, noisy, rt.radiance.reshape(H, W, 3).copy(), W, H
+    return scene, noisy, rt.sample_mean.reshape(H, W, 3).copy(), W, H
 
 
 def test_kpn_training_gradient_matches_cpu_restatement():
