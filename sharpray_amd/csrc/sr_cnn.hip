@@ -93,13 +93,12 @@ SR_CNN_DEV uint4 fetch8(const ConvArgs& A, int c, int y, int x) {
 // A 4-wave workgroup computes a 16-wide, 4*MT-tall output tile: wave w owns rows w*MT .. w*MT+MT-1,
 // one 16-pixel MFMA M tile per row. Per 32-channel K chunk the (4*MT+2) x 18 input halo is staged
 // in LDS; per tap each wave loads its NT weight fragments once and reuses them for its MT rows.
-// With 32 output channels and MT = 8 the result is transposed through LDS so that every lane
-// stores whole 16-byte groups of a pixel's channels.
+// The result is transposed through LDS so that every lane stores whole 16-byte groups of a pixel's
+// channels.
 template <int NT, int MT>
 __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs A) {
     constexpr int kRows = 4 * MT;
     constexpr int kHaloPx = kHW * (kRows + 2);
-    constexpr bool kStageOut = NT == 2 && MT >= 4;
     __shared__ uint4 tile[kHaloPx * 4];  // halo pixels x 32 channels of bf16
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx0 = blockIdx.x * kTW, ty0 = blockIdx.y * kRows;
@@ -134,47 +133,34 @@ __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs A) {
             }
         }
     }
-    if (kStageOut) {
-        __syncthreads();  // every wave is done reading the halo
-        uint16_t* ob = (uint16_t*)tile;  // [kRows * 16 pixels][32 channels]
-        #pragma unroll
+    // Epilogue, one 32-channel group at a time: bias, ReLU and bf16 rounding into LDS as
+    // [pixel][32], then 16-byte stores of each pixel's group (channels past cout are written as 0).
+    uint16_t* ob = (uint16_t*)tile;  // [kRows * 16 pixels][32 channels]
+#pragma unroll
+    for (int g = 0; g < (NT + 1) / 2; ++g) {
+        __syncthreads();  // the halo (or the previous group) is no longer read
+#pragma unroll
         for (int mt = 0; mt < MT; ++mt)
-            #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int nt = 2 * g + h;
                 const int co = nt * 16 + (lane & 15);
-                const float bias = co < A.cout ? A.bias[co] : 0.0f;
-                #pragma unroll
+                const bool live = nt < NT && co < A.cout;
+                const float bias = live ? A.bias[co] : 0.0f;
+#pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    float v = acc[mt][nt][r] + bias;
+                    float v = nt < NT ? acc[mt][nt < NT ? nt : 0][r] + bias : 0.0f;
                     if (A.relu) v = v > 0.0f ? v : 0.0f;
-                    ob[((wave * MT + mt) * 16 + 4 * (lane >> 4) + r) * 32 + co] = co < A.cout ? f2bf(v) : (uint16_t)0;
+                    ob[((wave * MT + mt) * 16 + 4 * (lane >> 4) + r) * 32 + h * 16 + (lane & 15)] =
+                        live ? f2bf(v) : (uint16_t)0;
                 }
             }
         __syncthreads();
         for (int idx = threadIdx.x; idx < kRows * 16 * 4; idx += 256) {
             const int pl = idx >> 2, part = idx & 3;
             const int y = ty0 + pl / 16, x = tx0 + pl % 16;
-            if (y < A.H && x < A.W) ((uint4*)(A.out + ((size_t)y * A.W + x) * 32))[part] = tile[idx];
-        }
-        return;
-    }
-    #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const int y = ty0 + wave * MT + mt;
-        if (y >= A.H) continue;
-        #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            const int co = nt * 16 + (lane & 15);
-            if (co >= A.cout) continue;
-            const float bias = A.bias[co];
-            #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int x = tx0 + 4 * (lane >> 4) + r;
-                if (x >= A.W) continue;
-                float v = acc[mt][nt][r] + bias;
-                if (A.relu) v = v > 0.0f ? v : 0.0f;
-                A.out[((size_t)y * A.W + x) * A.coutPad + co] = f2bf(v);
-            }
+            if (y < A.H && x < A.W)
+                ((uint4*)(A.out + ((size_t)y * A.W + x) * A.coutPad + 32 * g))[part] = tile[idx];
         }
     }
 }
