@@ -55,6 +55,10 @@ SR_DEV uint32_t pack_node(float childBits, float countBits) {
 
 // Closest hit in one mesh BVH (bvh.cs:165-216 + mesh.cs:51-62). Nearer child first, ties to
 // child B; prune when the entry distance is >= the best hit; accept strictly closer hits.
+// While-while form: interior steps repeat until the lane holds a leaf (or its stack runs dry),
+// then one leaf step, so a wave runs its box tests and its triangle tests as separate batches
+// instead of paying for both whenever any lane sits on a leaf. Each lane visits the same node
+// sequence as the plain loop over (pop, test) (same hits, same counters).
 template <bool COUNT>
 SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, float& bestT,
                          float& bu, float& bv, int32_t& bslot, Tally& tl, uint2* lds) {
@@ -67,13 +71,13 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
     uint32_t cur = pack_node(root.z, root.w);
     float curT = 0.0f;
-    for (;;) {
-        bool next = false;
-        if (curT < bestT) {
-            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
-            if (cnt == 0) {
+    bool live = true;
+    while (live) {
+        while ((cur >> 24) == 0) {
+            bool next = false;
+            if (curT < bestT) {
                 if (COUNT) tl.box += 2;
-                const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)child) * 2;
+                const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
                 const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
                 float tA = 0.0f, tB = 0.0f;
                 const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= bestT;
@@ -91,30 +95,36 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
                 } else if (hB) {
                     cur = pB; curT = tB; next = true;
                 }
-            } else {
-                if (COUNT) tl.shape += cnt;
-                const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
-                for (uint32_t i = 0; i < cnt; ++i, T += 9) {
-                    float t, u, v;
-                    if (tri_hit(T, r, t, u, v) && t < bestT) {
-                        bestT = t; bu = u; bv = v;
-                        bslot = (int32_t)(child + i);
-                        found = true;
-                    }
+            }
+            if (!next) {
+                if (sp == 0) { live = false; break; }
+                --sp;
+                st.get(sp, cur, curT);
+            }
+        }
+        if (!live) break;
+        if (curT < bestT) {
+            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+            if (COUNT) tl.shape += cnt;
+            const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
+            for (uint32_t i = 0; i < cnt; ++i, T += 9) {
+                float t, u, v;
+                if (tri_hit(T, r, t, u, v) && t < bestT) {
+                    bestT = t; bu = u; bv = v;
+                    bslot = (int32_t)(child + i);
+                    found = true;
                 }
             }
         }
-        if (!next) {
-            if (sp == 0) break;
-            --sp;
-            st.get(sp, cur, curT);
-        }
+        if (sp == 0) break;
+        --sp;
+        st.get(sp, cur, curT);
     }
     return found;
 }
 
 // Whether any primitive of a mesh is hit (the same node visits as the closest-hit traversal up
-// to its first accepted hit, which is where the existence of a hit is decided).
+// to its first accepted hit, which is where the existence of a hit is decided); while-while form.
 template <bool COUNT>
 SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, Tally& tl, uint2* lds) {
     const float4* N = S.nodes;
@@ -123,12 +133,11 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
     int sp = 0;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
     uint32_t cur = pack_node(root.z, root.w);
+    float unused;
     for (;;) {
-        bool next = false;
-        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
-        if (cnt == 0) {
+        while ((cur >> 24) == 0) {
             if (COUNT) tl.box += 2;
-            const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)child) * 2;
+            const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
             const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
             float tA = 0.0f, tB = 0.0f;
             const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
@@ -138,25 +147,24 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
                 const bool aNear = tA < tB;
                 st.put(sp++, aNear ? pB : pA, 0.0f);
                 cur = aNear ? pA : pB;
-                next = true;
             } else if (hA) {
-                cur = pA; next = true;
+                cur = pA;
             } else if (hB) {
-                cur = pB; next = true;
-            }
-        } else {
-            const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
-            for (uint32_t i = 0; i < cnt; ++i, T += 9) {
-                if (COUNT) tl.shape += 1;
-                float t, u, v;
-                if (tri_hit(T, r, t, u, v)) return true;
+                cur = pB;
+            } else {
+                if (sp == 0) return false;
+                st.get(--sp, cur, unused);
             }
         }
-        if (!next) {
-            if (sp == 0) return false;
-            float unused;
-            st.get(--sp, cur, unused);
+        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+        const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
+        for (uint32_t i = 0; i < cnt; ++i, T += 9) {
+            if (COUNT) tl.shape += 1;
+            float t, u, v;
+            if (tri_hit(T, r, t, u, v)) return true;
         }
+        if (sp == 0) return false;
+        st.get(--sp, cur, unused);
     }
 }
 
@@ -196,7 +204,9 @@ SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, Ta
     return ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
 }
 
-// Scene BVH over objects (costI 10, 32 bins; scene.cs:528-533), two-level closest hit.
+// Scene BVH over objects (costI 10, 32 bins; scene.cs:528-533), two-level closest hit, in the
+// while-while form of the mesh traversal: a wave descends the object BVH until every lane holds
+// an object leaf, so the lanes enter their mesh traversals together.
 template <bool COUNT>
 SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& tl, uint2* lds) {
     if (S.sceneNodes == 0) return false;
@@ -209,13 +219,13 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
     const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
     uint32_t cur = pack_node(root.z, root.w);
     float curT = 0.0f;
-    for (;;) {
-        bool next = false;
-        if (curT < best) {
-            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
-            if (cnt == 0) {
+    bool live = true;
+    while (live) {
+        while ((cur >> 24) == 0) {
+            bool next = false;
+            if (curT < best) {
                 if (COUNT) tl.box += 2;
-                const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)child) * 2;
+                const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
                 const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
                 float tA = 0.0f, tB = 0.0f;
                 const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= best;
@@ -234,26 +244,33 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
                 } else if (hB) {
                     cur = pB; curT = tB; next = true;
                 }
-            } else {
-                if (COUNT) tl.shape += cnt;
-                for (uint32_t i = 0; i < cnt; ++i) {
-                    const int32_t oi = S.sceneItems[child + i];
-                    float d, u, v;
-                    int32_t slot;
-                    if (object_closest<COUNT>(S, S.objects[oi], r, d, u, v, slot, tl, lds) && d < best) {
-                        best = d;
-                        out.t = d; out.u = u; out.v = v; out.obj = oi; out.slot = slot;
-                        found = true;
-                    }
+            }
+            if (!next) {
+                if (sp == 0) { live = false; break; }
+                --sp;
+                cur = stk[sp];
+                curT = stt[sp];
+            }
+        }
+        if (!live) break;
+        if (curT < best) {
+            const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+            if (COUNT) tl.shape += cnt;
+            for (uint32_t i = 0; i < cnt; ++i) {
+                const int32_t oi = S.sceneItems[child + i];
+                float d, u, v;
+                int32_t slot;
+                if (object_closest<COUNT>(S, S.objects[oi], r, d, u, v, slot, tl, lds) && d < best) {
+                    best = d;
+                    out.t = d; out.u = u; out.v = v; out.obj = oi; out.slot = slot;
+                    found = true;
                 }
             }
         }
-        if (!next) {
-            if (sp == 0) break;
-            --sp;
-            cur = stk[sp];
-            curT = stt[sp];
-        }
+        if (sp == 0) break;
+        --sp;
+        cur = stk[sp];
+        curT = stt[sp];
     }
     return found;
 }
@@ -269,11 +286,9 @@ SR_DEV bool scene_any(const DevScene& S, const ray3& r, Tally& tl, uint2* lds) {
     const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
     uint32_t cur = pack_node(root.z, root.w);
     for (;;) {
-        bool next = false;
-        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
-        if (cnt == 0) {
+        while ((cur >> 24) == 0) {
             if (COUNT) tl.box += 2;
-            const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)child) * 2;
+            const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
             const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
             float tA = 0.0f, tB = 0.0f;
             const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
@@ -283,25 +298,24 @@ SR_DEV bool scene_any(const DevScene& S, const ray3& r, Tally& tl, uint2* lds) {
                 const bool aNear = tA < tB;
                 stk[sp++] = aNear ? pB : pA;
                 cur = aNear ? pA : pB;
-                next = true;
             } else if (hA) {
-                cur = pA; next = true;
+                cur = pA;
             } else if (hB) {
-                cur = pB; next = true;
-            }
-        } else {
-            for (uint32_t i = 0; i < cnt; ++i) {
-                if (COUNT) tl.shape += 1;
-                if (object_any<COUNT>(S, S.objects[S.sceneItems[child + i]], r, tl, lds)) return true;
+                cur = pB;
+            } else {
+                if (sp == 0) return false;
+                cur = stk[--sp];
             }
         }
-        if (!next) {
-            if (sp == 0) return false;
-            cur = stk[--sp];
+        const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            if (COUNT) tl.shape += 1;
+            if (object_any<COUNT>(S, S.objects[S.sceneItems[child + i]], r, tl, lds)) return true;
         }
+        if (sp == 0) return false;
+        cur = stk[--sp];
     }
 }
-
 // Scene.Trace (scene.cs:602-625) from a hit record: world-space ShapeHit, material sampling,
 // normal mapping and tangent re-orthogonalisation.
 SR_DEV void make_surface(const DevScene& S, const ray3& r, const HitRec& h, SurfaceD& s) {
