@@ -38,6 +38,11 @@ using namespace sr;
 namespace {
 
 constexpr int kBlock = kTravBlock;
+#ifndef SR_TRACE_BLOCK
+#define SR_TRACE_BLOCK 64
+#endif
+constexpr int kTraceBlock = SR_TRACE_BLOCK;  // threads per block of k_trace (one wave)
+static_assert(kLdsDepth == 0 || kTraceBlock == kTravBlock, "LDS stack columns assume kTravBlock lanes");
 // Per-lane LDS columns of the mesh traversal stack (kTravLdsBytes per block).
 extern __shared__ uint2 sr_lds[];
 
@@ -85,6 +90,7 @@ struct ViewD {
 // Render counters beyond the reference's SR_COUNTER_COUNT: the fused tail launch's share of
 // BvhIntersectNode, BvhIntersectShape, SceneTrace and SceneOcclude.
 enum : int { kTailNode = SR_COUNTER_COUNT, kTailShape, kTailTrace, kTailOcclude, kCtrWords };
+constexpr int kCtrSpread = 16;  // copies of the counter words (a power of two)
 
 // Queue counters (uint32 each).
 enum : int {
@@ -203,21 +209,30 @@ SR_DEV void begin_sample(const Frame& F, uint32_t q, uint32_t x, uint32_t y, uin
     rng_store(p, rng);
 }
 
+// The render counters are kept in kCtrSpread copies, picked by block index, so the many short
+// blocks of a launch do not all queue their atomics on one address; reading sums the copies.
+SR_DEV unsigned long long* ctr_at(const Frame& F, int idx) {
+    return F.ctr + (size_t)(blockIdx.x & (kCtrSpread - 1)) * kCtrWords + idx;
+}
+// Sums a per-lane count over the wave and adds it to a render counter with one atomic; every lane
+// of the wave calls it.
+SR_DEV void bump_sum(const Frame& F, int idx, uint32_t v) {
+    if (F.ctr == nullptr) return;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane_id() == 0 && v) atomicAdd(ctr_at(F, idx), (unsigned long long)v);
+}
 template <bool COUNT>
 SR_DEV void flush_tally(const Frame& F, const Tally& tl) {
     if (!COUNT || F.ctr == nullptr) return;
-    if (tl.box) atomicAdd(&F.ctr[13], (unsigned long long)tl.box);
-    if (tl.shape) atomicAdd(&F.ctr[14], (unsigned long long)tl.shape);
-    if (tl.mesh) atomicAdd(&F.ctr[12], (unsigned long long)tl.mesh);
-}
-SR_DEV void bump(const Frame& F, int idx, uint32_t n) {
-    if (F.ctr != nullptr && n) atomicAdd(&F.ctr[idx], (unsigned long long)n);
+    bump_sum(F, 13, tl.box);
+    bump_sum(F, 14, tl.shape);
+    bump_sum(F, 12, tl.mesh);
 }
 // Counts the lanes of the wave where pred holds with one atomic; every lane of the wave calls it.
 SR_DEV void bump_wave(const Frame& F, int idx, bool pred) {
     if (F.ctr == nullptr) return;
     const unsigned long long mask = __ballot(pred);
-    if (mask && (int)lane_id() == __ffsll(mask) - 1) atomicAdd(&F.ctr[idx], (unsigned long long)__popcll(mask));
+    if (mask && (int)lane_id() == __ffsll(mask) - 1) atomicAdd(ctr_at(F, idx), (unsigned long long)__popcll(mask));
 }
 
 // ---- Kernels ------------------------------------------------------------------------------------
@@ -421,8 +436,8 @@ __global__ void __launch_bounds__(kBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, D
         if (cont) F.queue[cur ^ 1][nc] = q;
     }
     if (F.ctr) {
-        bump(F, 3, c.hit); bump(F, 4, c.miss); bump(F, 5, c.term); bump(F, 6, c.fog); bump(F, 7, c.esc);
-        bump(F, 11, c.occ);
+        bump_sum(F, 3, c.hit); bump_sum(F, 4, c.miss); bump_sum(F, 5, c.term); bump_sum(F, 6, c.fog);
+        bump_sum(F, 7, c.esc); bump_sum(F, 11, c.occ);
     }
 }
 
@@ -474,14 +489,6 @@ SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, 
     add_direct(S, sr, tr, rad, radFog);
     if (sr.path & 0x80000000u) p->radFog = radFog;
     else p->rad = rad;
-}
-
-// Sums a per-lane count over the wave and adds it to a render counter with one atomic; every lane
-// of the wave calls it.
-SR_DEV void bump_sum(const Frame& F, int idx, uint32_t v) {
-    if (F.ctr == nullptr) return;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane_id() == 0 && v) atomicAdd(&F.ctr[idx], (unsigned long long)v);
 }
 
 // The remaining segments of every queued path in one launch: each lane runs its path's
@@ -548,7 +555,7 @@ template <bool COUNT, bool TRANSP>
 #ifndef SR_TRACE_MIN_WAVES
 #define SR_TRACE_MIN_WAVES 7
 #endif
-__global__ void __launch_bounds__(kBlock, SR_TRACE_MIN_WAVES) k_trace(Frame F, DevScene S, int cur) {
+__global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame F, DevScene S, int cur) {
     // The two queues this segment's shade fills start empty; nothing else touches them meanwhile.
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         F.counts[cur ^ 1] = 0;
@@ -561,8 +568,8 @@ __global__ void __launch_bounds__(kBlock, SR_TRACE_MIN_WAVES) k_trace(Frame F, D
     const uint32_t* Q = F.queue[cur];
     const ShadowRec* SR = F.shadows[cur ^ 1];
     if (COUNT && blockIdx.x == 0 && threadIdx.x == 0 && F.ctr) {
-        atomicAdd(&F.ctr[2], (unsigned long long)nE);
-        atomicAdd(&F.ctr[10], (unsigned long long)nE);
+        atomicAdd(ctr_at(F, 2), (unsigned long long)nE);
+        atomicAdd(ctr_at(F, 10), (unsigned long long)nE);
     }
     Tally tl = {0, 0, 0};
     uint32_t extra = 0;
@@ -585,7 +592,7 @@ __global__ void __launch_bounds__(kBlock, SR_TRACE_MIN_WAVES) k_trace(Frame F, D
         }
     }
     flush_tally<COUNT>(F, tl);
-    if (F.ctr && extra) atomicAdd(&F.ctr[11], (unsigned long long)extra);
+    bump_sum(F, 11, extra);
 }
 
 // After the fused tail launch: both ray queues are spent. k_next appends the next sample's camera
@@ -1597,7 +1604,7 @@ int alloc_frame(sr_ctx* c, uint32_t maxSlots) {
         (rc = ensure(c->dHitSlot, maxSlots)) || (rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dShadows, (size_t)maxSlots * 2)) ||
         (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxSlots)) || (rc = ensure(c->dQueue1, maxSlots)) ||
         (rc = ensure(c->dCounts, kCounterWords)) ||
-        (rc = ensure(c->dCtr, kCtrWords)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
+        (rc = ensure(c->dCtr, (size_t)kCtrWords * kCtrSpread)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
         (rc = ensure(c->dNormal, px * 3)) || (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) || (rc = ensure(c->dAlbedo, px * 3)) ||
         (rc = ensure(c->dVar, px)) || (rc = ensure(c->dSamples, px)))
         return rc;
@@ -1625,6 +1632,16 @@ int run_band(sr_ctx* c, const Frame& F) {
     hipStream_t s = c->stream;
     const int slotBlocks = (int)((F.nSlots + kBlock - 1) / kBlock);
     const int g = c->gridCUs * 8;
+    // Traversal and shade launches get one block per queue chunk of the largest possible queue
+    // (blocks past the live count exit at once): the dispatcher then hands chunks to whichever CU
+    // frees up, where a grid-stride loop over a resident-sized grid fixes each wave's share in
+    // advance and the launch waits for the unluckiest wave (trace 207 -> 188 ms, shade 85.5 ->
+    // 82.8 ms per 1080p frame; one-wave trace blocks, 188 -> 179 ms, free a CU slot per finished
+    // wave). A trace launch holds at most nSlots rays (padded to whole waves) plus nSlots shadow
+    // rays; a shade launch at most nSlots rays.
+    const int gTrace = (int)((2u * F.nSlots + 64u + kTraceBlock - 1) / kTraceBlock);
+    const int gShade = (int)((F.nSlots + kBlock - 1) / kBlock);
+    const int gTail = g;
     mark(c, 3);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, F);
     hipLaunchKernelGGL(k_start<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
@@ -1635,21 +1652,21 @@ int run_band(sr_ctx* c, const Frame& F) {
         int cur = 0;
         bool fused = false;
         for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
-            hipLaunchKernelGGL(trace, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
+            hipLaunchKernelGGL(trace, dim3(gTrace), dim3(kTraceBlock), kTravLdsBytes, s, F, c->dev, cur);
             mark(c, 0);
             if (seg == c->tailSegment) {  // the rest of every path in one launch
-                hipLaunchKernelGGL(tail, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
+                hipLaunchKernelGGL(tail, dim3(gTail), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
                 mark(c, 2);
                 hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, F);
                 fused = true;
                 break;
             }
-            hipLaunchKernelGGL(k_shade<R>, dim3(g), dim3(kBlock), 0, s, F, c->dev, cur);
+            hipLaunchKernelGGL(k_shade<R>, dim3(gShade), dim3(kBlock), 0, s, F, c->dev, cur);
             mark(c, 1);
             cur ^= 1;
         }
         if (!fused) {  // the last segment's shadow rays (its ray queue is empty)
-            hipLaunchKernelGGL(trace, dim3(g), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
+            hipLaunchKernelGGL(trace, dim3(gTrace), dim3(kTraceBlock), kTravLdsBytes, s, F, c->dev, cur);
             mark(c, 2);
         }
         hipLaunchKernelGGL(k_next<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
@@ -1736,7 +1753,8 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
     if (rc) return rc;
     if (!ctx->evStart) HIP_TRY(hipEventCreate(&ctx->evStart));
     if (!ctx->evEnd) HIP_TRY(hipEventCreate(&ctx->evEnd));
-    if (ctx->countersOn) HIP_TRY(hipMemsetAsync(ctx->dCtr.ptr, 0, kCtrWords * sizeof(unsigned long long), ctx->stream));
+    if (ctx->countersOn)
+        HIP_TRY(hipMemsetAsync(ctx->dCtr.ptr, 0, (size_t)kCtrWords * kCtrSpread * sizeof(unsigned long long), ctx->stream));
     ctx->profUsed = 0;
     ctx->iterations = 0;
     ctx->view = make_view(view, p->width, p->height);
@@ -1856,13 +1874,27 @@ int sr_counters_enable(sr_ctx* ctx, int32_t enable) {
     return SR_OK;
 }
 
+namespace {
+// The counter words of the last render, summed over their kCtrSpread copies.
+int read_counters(sr_ctx* ctx, unsigned long long* v) {
+    std::vector<unsigned long long> all((size_t)kCtrWords * kCtrSpread);
+    HIP_TRY(hipMemcpy(all.data(), ctx->dCtr.ptr, all.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int i = 0; i < kCtrWords; ++i) {
+        v[i] = 0;
+        for (int k = 0; k < kCtrSpread; ++k) v[i] += all[(size_t)k * kCtrWords + i];
+    }
+    return SR_OK;
+}
+}  // namespace
+
 int sr_counters_read(sr_ctx* ctx, int64_t* out) {
     if (!ctx || !out) return fail(SR_ERR_INVALID, "null argument");
     if (!ctx->rendering || !ctx->countersOn) return fail(SR_ERR_STATE, "counters were not collected");
     int rc = wait_render(ctx);
     if (rc) return rc;
     unsigned long long v[kCtrWords];
-    HIP_TRY(hipMemcpy(v, ctx->dCtr.ptr, sizeof(v), hipMemcpyDeviceToHost));
+    int rc2 = read_counters(ctx, v);
+    if (rc2) return rc2;
     for (int i = 0; i < SR_COUNTER_COUNT; ++i) out[i] = (int64_t)v[i];
     out[10] += (int64_t)v[kTailTrace];
     out[13] += (int64_t)v[kTailNode];
@@ -1876,7 +1908,8 @@ int sr_counters_read_fused(sr_ctx* ctx, int64_t* out) {
     int rc = wait_render(ctx);
     if (rc) return rc;
     unsigned long long v[kCtrWords];
-    HIP_TRY(hipMemcpy(v, ctx->dCtr.ptr, sizeof(v), hipMemcpyDeviceToHost));
+    int rc2 = read_counters(ctx, v);
+    if (rc2) return rc2;
     out[0] = (int64_t)v[kTailNode];
     out[1] = (int64_t)v[kTailShape];
     out[2] = (int64_t)v[kTailTrace];
