@@ -269,6 +269,15 @@ int sr_counters_read(sr_ctx* ctx, int64_t* out);
  * the fused launch that finishes paths from the tail segment on (sr_set_tail_segment); the rest
  * was done by the per-segment traversal launches. No counterpart in the reference. */
 int sr_counters_read_fused(sr_ctx* ctx, int64_t* out);
+/* The reference-only share of BvhIntersectNode, BvhIntersectShape, MeshIntersect and SceneOcclude
+ * (out[0..3]) in the per-segment traversal launches: traversal the reference does that a render
+ * without counters skips. The reference traces the direct-light shadow ray before testing whether
+ * the light lies behind the shading or the geometric normal (scene.cs:774-783), and walks every
+ * shadow ray with its closest-hit traversal (scene.cs:576-600), where an any-hit walk decides
+ * occlusion at the first hit. While counters are enabled the tracer traces those rays and walks
+ * shadow rays the reference's way, so sr_counters_read follows the reference's taxonomy; the work
+ * of a render without counters is the totals minus this share. No counterpart in the reference. */
+int sr_counters_read_reference_only(sr_ctx* ctx, int64_t* out);
 /* Later renders trace segments 0..segment-1 as separate wavefront launches and finish every path
  * from `segment` on in one fused launch (default 3; a value above the bounce count disables it;
  * results are identical either way). The environment variable SHARPRAY_TAIL_SEGMENT sets the

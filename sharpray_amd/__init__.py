@@ -545,6 +545,15 @@ class Renderer:
         check(N.lib().sr_counters_read_fused(self.scene.ctx, out), "counters")
         return dict(zip(["BvhIntersectNode", "BvhIntersectShape", "SceneTrace", "SceneOcclude"], list(out)))
 
+    def counters_reference_only(self):
+        """The per-segment traversal launches' share of the counters that only the reference's way of
+        tracing shadow rays incurs (rays whose light lies behind the surface, scene.cs:774-783, and
+        closest-hit instead of any-hit walks, scene.cs:576-600): traced only while counting, so the
+        work of a render without counters excludes this share."""
+        out = (ctypes.c_int64 * 4)()
+        check(N.lib().sr_counters_read_reference_only(self.scene.ctx, out), "counters")
+        return dict(zip(["BvhIntersectNode", "BvhIntersectShape", "MeshIntersect", "SceneOcclude"], list(out)))
+
 
 class Adjoint:
     """Adjoint rendering of a built scene (no reference counterpart): parameters are each object's

@@ -160,6 +160,7 @@ SIGNATURES = {
     "sr_hdr_decode": (ctypes.c_int, [c_uint8_p, ctypes.c_uint64, ctypes.c_float, c_float_p, c_uint32_p, c_uint32_p]),
     "sr_tga_decode": (ctypes.c_int, [c_uint8_p, ctypes.c_uint64, c_uint8_p, c_uint32_p, c_uint32_p]),
     "sr_counters_read_fused": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "sr_counters_read_reference_only": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "sr_set_tail_segment": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "sr_set_bvh_builder": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "sr_scene_counts": (ctypes.c_int, [ctypes.c_void_p, c_uint32_p, c_uint32_p, c_uint32_p]),

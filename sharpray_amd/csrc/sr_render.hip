@@ -89,7 +89,17 @@ struct ViewD {
 
 // Render counters beyond the reference's SR_COUNTER_COUNT: the fused tail launch's share of
 // BvhIntersectNode, BvhIntersectShape, SceneTrace and SceneOcclude.
-enum : int { kTailNode = SR_COUNTER_COUNT, kTailShape, kTailTrace, kTailOcclude, kCtrWords };
+// Then the reference-only share of BvhIntersectNode, BvhIntersectShape, MeshIntersect and
+// SceneOcclude in the per-segment launches: traversal the reference does but a render without
+// counters skips (side-culled shadow rays, see kCountOnly, and the nodes the reference's
+// closest-hit walk of a shadow ray visits beyond those of the any-hit walk).
+enum : int { kTailNode = SR_COUNTER_COUNT, kTailShape, kTailTrace, kTailOcclude, kRefNode, kRefShape, kRefMesh,
+             kRefOcclude, kCtrWords };
+// ShadowRec::path flags. kCountOnly marks a shadow ray whose light lies behind the shading or the
+// geometric normal: its contribution is black, so the tracer skips it, but the reference traces
+// it before its side tests (scene.cs:774-783) and counts that traversal. While counters are on,
+// such rays are traced for their counts only.
+constexpr uint32_t kToFog = 0x80000000u, kCountOnly = 0x40000000u, kPathMask = 0x3fffffffu;
 constexpr int kCtrSpread = 16;  // copies of the counter words (a power of two)
 
 // Queue counters (uint32 each).
@@ -221,13 +231,6 @@ SR_DEV void bump_sum(const Frame& F, int idx, uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     if (lane_id() == 0 && v) atomicAdd(ctr_at(F, idx), (unsigned long long)v);
 }
-template <bool COUNT>
-SR_DEV void flush_tally(const Frame& F, const Tally& tl) {
-    if (!COUNT || F.ctr == nullptr) return;
-    bump_sum(F, 13, tl.box);
-    bump_sum(F, 14, tl.shape);
-    bump_sum(F, 12, tl.mesh);
-}
 // Counts the lanes of the wave where pred holds with one atomic; every lane of the wave calls it.
 SR_DEV void bump_wave(const Frame& F, int idx, bool pred) {
     if (F.ctr == nullptr) return;
@@ -267,7 +270,7 @@ __global__ void k_start(Frame F) {
 
 // Counts of one shade pass, flushed into the render counters once per thread.
 struct ShadeCount {
-    uint32_t hit = 0, miss = 0, term = 0, fog = 0, esc = 0, occ = 0;
+    uint32_t hit = 0, miss = 0, term = 0, fog = 0, esc = 0, occ = 0, occCulled = 0;
 };
 
 // One path segment (scene.cs:641-737) of path q at the hit h of `ray`: emission, the NEE shadow
@@ -299,7 +302,7 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
             sr.w = power_heuristic(lpdf, fogPdf);
             sr.pdf = lpdf;
             sr.energy = p.energy;
-            sr.path = q | (primary ? 0x80000000u : 0u);
+            sr.path = q | (primary ? kToFog : 0u);
             shadow = true;
         }
         bool term = false;
@@ -352,6 +355,10 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
                 sr.pdf = lpdf;
                 sr.energy = p.energy;
                 sr.path = q;
+                shadow = true;
+            } else if (F.ctr) {
+                sr.o = hitPos; sr.d = ld;
+                sr.path = q | kCountOnly;
                 shadow = true;
             }
         }
@@ -429,7 +436,10 @@ __global__ void __launch_bounds__(kBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, D
             rng_store(p, rng);
             F.paths[q] = p;
         }
-        if (shadow) ++c.occ;
+        if (shadow) {
+            if (sr.path & kCountOnly) ++c.occCulled;
+            else ++c.occ;
+        }
         const uint32_t ns = wave_append(&F.counts[kShadowCount0 + cur], shadow);
         if (shadow) F.shadows[cur][ns] = sr;
         const uint32_t nc = wave_append(&F.counts[cur ^ 1], cont);
@@ -437,18 +447,19 @@ __global__ void __launch_bounds__(kBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, D
     }
     if (F.ctr) {
         bump_sum(F, 3, c.hit); bump_sum(F, 4, c.miss); bump_sum(F, 5, c.term); bump_sum(F, 6, c.fog);
-        bump_sum(F, 7, c.esc); bump_sum(F, 11, c.occ);
+        bump_sum(F, 7, c.esc); bump_sum(F, 11, c.occ); bump_sum(F, kRefOcclude, c.occCulled);
     }
 }
 
-// ShadowTransmittance (scene.cs:576-600) of one shadow ray: 0 or 1 from an any-hit traversal in
-// scenes without transparent objects, else the reference's closest-hit walk through transparent
-// surfaces.
+// ShadowTransmittance (scene.cs:576-600) of one shadow ray: the reference's closest-hit walk
+// through transparent surfaces, or, in scenes without transparent objects, 0 or 1 from an any-hit
+// traversal, which decides occlusion at the first hit the closest-hit walk would find. While
+// counting (COUNT), the closest-hit walk runs in every scene so the counters follow the reference.
 template <bool COUNT, bool TRANSP>
 SR_DEV vec3 shadow_transmittance(const DevScene& S, const ShadowRec& sr, Tally& tl, uint32_t& extra) {
     ray3 ray = mk_ray(sr.o, sr.d);
     vec3 tr = splat3(1.0f);
-    if (!TRANSP) {
+    if (!TRANSP && !COUNT) {
         if (scene_any<COUNT>(S, ray, tl, sr_lds + threadIdx.x)) tr = splat3(0.0f);
     } else {
         for (;;) {
@@ -476,18 +487,18 @@ SR_DEV void add_direct(const DevScene& S, const ShadowRec& sr, vec3 tr, vec3& ra
     if (!(max_comp(tr) > 0.0f)) return;
     if (S.fog.enabled) tr = tr * expf(-medium_optical_depth(S.fog, mk_ray(sr.o, sr.d), 0.0f, kInf));
     const vec3 c = sr.skyRad * tr * sr.f * sr.w / sr.pdf * sr.energy;
-    if (sr.path & 0x80000000u) radFog = radFog + c;
+    if (sr.path & kToFog) radFog = radFog + c;
     else rad = rad + c;
 }
 
 template <bool COUNT, bool TRANSP>
 SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, Tally& tl, uint32_t& extra) {
     const vec3 tr = shadow_transmittance<COUNT, TRANSP>(S, sr, tl, extra);
-    if (!(max_comp(tr) > 0.0f)) return;
-    PathRec* p = &F.paths[sr.path & 0x7fffffffu];
+    if ((sr.path & kCountOnly) || !(max_comp(tr) > 0.0f)) return;
+    PathRec* p = &F.paths[sr.path & kPathMask];
     vec3 rad = p->rad, radFog = p->radFog;
     add_direct(S, sr, tr, rad, radFog);
-    if (sr.path & 0x80000000u) p->radFog = radFog;
+    if (sr.path & kToFog) p->radFog = radFog;
     else p->rad = rad;
 }
 
@@ -520,9 +531,10 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
             ShadowRec sr;
             vec3 no, nd;
             shade_path<R>(F, S, q, p, rng, h, ray, cont, shadow, sr, no, nd, c);
-            if (shadow) {
+            if (shadow) {  // side-culled rays (counters only) count with this launch's traversal
                 ++c.occ;
-                add_direct(S, sr, shadow_transmittance<COUNT, TRANSP>(S, sr, tl, extra), p.rad, p.radFog);
+                const vec3 tr = shadow_transmittance<COUNT, TRANSP>(S, sr, tl, extra);
+                if (!(sr.path & kCountOnly)) add_direct(S, sr, tr, p.rad, p.radFog);
             }
             if (!cont) break;
             p.seg = p.seg + 1;
@@ -571,8 +583,11 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
         atomicAdd(ctr_at(F, 2), (unsigned long long)nE);
         atomicAdd(ctr_at(F, 10), (unsigned long long)nE);
     }
-    Tally tl = {0, 0, 0};
-    uint32_t extra = 0;
+    // tl: closest-hit rays; tlShadow: shadow rays as the reference walks them; tlAny: the any-hit
+    // walks a render without counters does instead (counting, opaque scenes); tlRef: side-culled
+    // shadow rays (counting only).
+    Tally tl = {0, 0, 0}, tlShadow = {0, 0, 0}, tlAny = {0, 0, 0}, tlRef = {0, 0, 0};
+    uint32_t extra = 0, extraRef = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < total; base += stride) {
         const uint32_t i = base + lane_id();
@@ -588,10 +603,28 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
                 F.hitSlot[q] = h.slot;
             }
         } else if (i - nE64 < nS) {
-            shadow_item<COUNT, TRANSP>(F, S, SR[i - nE64], tl, extra);
+            const ShadowRec& rec = SR[i - nE64];
+            if (rec.path & kCountOnly) {
+                shadow_item<COUNT, TRANSP>(F, S, rec, tlRef, extraRef);
+            } else {
+                shadow_item<COUNT, TRANSP>(F, S, rec, tlShadow, extra);
+                if (COUNT && !TRANSP) (void)scene_any<COUNT>(S, mk_ray(rec.o, rec.d), tlAny, sr_lds + threadIdx.x);
+            }
         }
     }
-    flush_tally<COUNT>(F, tl);
+    if (COUNT && F.ctr) {
+        // Reported totals follow the reference; the reference-only words hold what a render
+        // without counters does not traverse (per ray the closest-hit walk visits at least the
+        // any-hit walk's nodes).
+        const Tally work = TRANSP ? tlShadow : tlAny;
+        bump_sum(F, 13, tl.box + work.box);
+        bump_sum(F, 14, tl.shape + work.shape);
+        bump_sum(F, 12, tl.mesh + work.mesh);
+        bump_sum(F, kRefNode, tlRef.box + tlShadow.box - work.box);
+        bump_sum(F, kRefShape, tlRef.shape + tlShadow.shape - work.shape);
+        bump_sum(F, kRefMesh, tlRef.mesh + tlShadow.mesh - work.mesh);
+        bump_sum(F, kRefOcclude, extraRef);
+    }
     bump_sum(F, 11, extra);
 }
 
@@ -1899,6 +1932,25 @@ int sr_counters_read(sr_ctx* ctx, int64_t* out) {
     out[10] += (int64_t)v[kTailTrace];
     out[13] += (int64_t)v[kTailNode];
     out[14] += (int64_t)v[kTailShape];
+    out[11] += (int64_t)v[kRefOcclude];
+    out[12] += (int64_t)v[kRefMesh];
+    out[13] += (int64_t)v[kRefNode];
+    out[14] += (int64_t)v[kRefShape];
+    return SR_OK;
+}
+
+int sr_counters_read_reference_only(sr_ctx* ctx, int64_t* out) {
+    if (!ctx || !out) return fail(SR_ERR_INVALID, "null argument");
+    if (!ctx->rendering || !ctx->countersOn) return fail(SR_ERR_STATE, "counters were not collected");
+    int rc = wait_render(ctx);
+    if (rc) return rc;
+    unsigned long long v[kCtrWords];
+    int rc2 = read_counters(ctx, v);
+    if (rc2) return rc2;
+    out[0] = (int64_t)v[kRefNode];
+    out[1] = (int64_t)v[kRefShape];
+    out[2] = (int64_t)v[kRefMesh];
+    out[3] = (int64_t)v[kRefOcclude];
     return SR_OK;
 }
 
