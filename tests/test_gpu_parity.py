@@ -86,29 +86,51 @@ def test_radiance_parity_plain_cpu(plain_cpu, oracle_lib):
     assert np.all(rel_err(r.variance[ok, None], ref["variance"][ok, None]) <= 1e-2)
 
 
-def test_render_counters_match_oracle(plain_cpu, oracle_lib):
+def _counter_case(case):
+    from sharpray_amd import RNG_COUNTER, scenes
+    if case == "plain":
+        return scenes.config_plain_cpu(), {}, {}
+    ctr = (dict(rng_policy=RNG_COUNTER, rng_seed=77), dict(rng_policy=1, rng_seed=77))
+    if case == "height_fog":
+        spec = scenes.config_plain_cpu(scale=0.5, spp=8)
+        spec.fog = (0.35, (0.9, 0.85, 0.8), 0.3, 0.6)
+        return (spec,) + ctr
+    spec = scenes.config_analytic(scale=1.0, spp=8)
+    for o in spec.objects[1:6]:
+        o.material = o.material.copy()
+        o.material[4] = 0.0   # metallic
+        o.material[5] = 0.85  # transparency
+        o.material[6] = 1.45  # ior
+    return (spec,) + ctr
+
+
+@pytest.mark.parametrize("case", ["plain", "height_fog", "transparent"])
+def test_render_counters_match_oracle(case, oracle_lib):
     """Counters.Type render-path counters (counters.cs:24-79) of the GPU render against the oracle's
-    on the same scene and xorwow streams. Sample and pixel counts are exact; the per-ray counters
-    (bounces, hits, traces, node and shape tests) differ only through the few divergent paths
-    (see the radiance bar) and must agree to 0.2 % (measured: 1e-6). Shadow rays count as the
-    reference traces them: also toward a light behind the surface (scene.cs:774-783), and with the
-    closest-hit walk (scene.cs:576-600); skipping either undercounts SceneOcclude by ~40 % or the
-    node tests by ~1.4 % on this scene."""
+    on the same scene and random streams. Sample and pixel counts are exact; the per-ray counters
+    (bounces, hits, fog events, traces, node and shape tests) differ only through the few divergent
+    paths (see the radiance bar) and must agree to 0.2 % (measured on the plain scene: 1e-6). Shadow
+    rays count as the reference traces them: also toward a light behind the surface
+    (scene.cs:774-783), and with the closest-hit walk (scene.cs:576-600); skipping either
+    undercounts SceneOcclude by ~40 % or the node tests by ~1.4 % on the plain scene."""
     from sharpray_amd import scenes
-    spec = plain_cpu
-    scene, view, r = scenes.render(spec, counters=True)
+    spec, kw, okw = _counter_case(case)
+    scene, view, r = scenes.render(spec, counters=True, **kw)
     r.wait()
     g = r.counters()
-    o = oracle_lib.OracleScene(spec).render(counters=True)["counters"]
-    print("gpu", g, "\noracle", o, flush=True)
+    o = oracle_lib.OracleScene(spec).render(counters=True, **okw)["counters"]
+    print(case, "gpu", g, "\noracle", o, flush=True)
     for k in ("Pixel", "Sample", "SamplePixelConverged", "SamplePixelMaxed"):
         assert g[k] == o[k], k
     assert g["Pixel"] == spec.width * spec.height
-    for k in ("SampleBounce", "SampleHit", "SampleMiss", "SampleTerminate", "SceneTrace", "SceneOcclude",
-              "MeshIntersect", "BvhIntersectNode", "BvhIntersectShape"):
-        assert o[k] > 0 or k == "SampleTerminate", k
+    for k in ("SampleBounce", "SampleHit", "SampleMiss", "SampleTerminate", "SampleFogScatter", "SampleFogEscape",
+              "SceneTrace", "SceneOcclude", "MeshIntersect", "BvhIntersectNode", "BvhIntersectShape"):
         assert abs(g[k] - o[k]) <= 0.002 * max(o[k], 1000), (k, g[k], o[k])
-    # Every traced segment is one SceneTrace; every hit or miss closes one.
+    for k in ("SampleBounce", "SampleHit", "SceneOcclude", "BvhIntersectNode"):
+        assert o[k] > 0, k
+    if case == "height_fog":
+        assert o["SampleFogScatter"] > 0
+    # Every traced segment is one SceneTrace.
     assert g["SceneTrace"] == g["SampleBounce"]
     assert g["BvhIntersectNode"] % 2 == 0  # two child boxes per interior visit (bvh.cs:188-189)
 
