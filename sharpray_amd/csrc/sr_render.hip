@@ -598,7 +598,8 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
                 const ray3 r = mk_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z));
                 HitRec h;
                 h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
-                scene_closest<COUNT>(S, r, h, tl, sr_lds + threadIdx.x);
+                // Speculative mesh traversal outside counting renders: same hit, wider waves.
+                scene_closest<COUNT, !COUNT>(S, r, h, tl, sr_lds + threadIdx.x);
                 F.hitA[q] = make_float4(h.t, h.u, h.v, __int_as_float(h.obj));
                 F.hitSlot[q] = h.slot;
             }

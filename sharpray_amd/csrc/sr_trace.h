@@ -168,9 +168,147 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
     }
 }
 
+// Speculative form of mesh_closest (Aila and Laine's speculative while-while): a lane that reaches
+// a leaf postpones it and keeps descending while other lanes of its wave still look for one, so the
+// wave's triangle tests run with more lanes busy. Leaves are still intersected in the order the
+// plain traversal reaches them, with the same best distance at each, so the hit is the same (a
+// leaf the plain traversal would have pruned holds no strictly closer hit); the extra box tests
+// come from descending before a postponed leaf has lowered the best distance, which is why
+// counting renders use mesh_closest itself.
+SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, float& bestT,
+                              float& bu, float& bv, int32_t& bslot, uint2* lds) {
+    const float4* N = S.nodes;
+    TravStack st;
+    st.col = lds;
+    int sp = 0;
+    bestT = kInf;
+    bool found = false;
+    const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    float curT = 0.0f;
+    bool live = true;   // cur is a node still to visit
+    uint32_t leaf = 0;  // postponed leaf (count << 24 | first item; 0 = none)
+    float leafT = 0.0f;
+    for (;;) {
+        while (live) {
+            if ((cur >> 24) != 0) {
+                if (leaf != 0) break;  // a second leaf: intersect the postponed one first
+                leaf = cur;
+                leafT = curT;
+                if (sp == 0) { live = false; break; }
+                --sp;
+                st.get(sp, cur, curT);
+            } else {
+                bool next = false;
+                if (curT < bestT) {
+                    const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
+                    const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+                    float tA = 0.0f, tB = 0.0f;
+                    const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= bestT;
+                    const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= bestT;
+                    const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+                    if (hA && hB) {
+                        const bool aNear = tA < tB;
+                        st.put(sp, aNear ? pB : pA, aNear ? tB : tA);
+                        ++sp;
+                        cur = aNear ? pA : pB;
+                        curT = aNear ? tA : tB;
+                        next = true;
+                    } else if (hA) {
+                        cur = pA; curT = tA; next = true;
+                    } else if (hB) {
+                        cur = pB; curT = tB; next = true;
+                    }
+                }
+                if (!next) {
+                    if (sp == 0) { live = false; break; }
+                    --sp;
+                    st.get(sp, cur, curT);
+                }
+            }
+            // Every lane still descending holds a leaf: intersect them together.
+            if (__ballot(leaf != 0) == __ballot(1)) break;
+        }
+        if (leaf != 0) {
+            if (leafT < bestT) {
+                const uint32_t cnt = leaf >> 24, child = leaf & kMaxChild;
+                const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
+                for (uint32_t i = 0; i < cnt; ++i, T += 9) {
+                    float t, u, v;
+                    if (tri_hit(T, r, t, u, v) && t < bestT) {
+                        bestT = t; bu = u; bv = v;
+                        bslot = (int32_t)(child + i);
+                        found = true;
+                    }
+                }
+            }
+            leaf = 0;
+        } else if (!live) {
+            break;
+        }
+    }
+    return found;
+}
+
+// Speculative form of mesh_any, used outside counting renders: lanes holding a leaf keep
+// descending while other lanes of the wave look for one. Whether a hit exists does not depend on
+// the order of the leaves.
+SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, uint2* lds) {
+    const float4* N = S.nodes;
+    TravStack st;
+    st.col = lds;
+    int sp = 0;
+    const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
+    uint32_t cur = pack_node(root.z, root.w);
+    float unused;
+    bool live = true;
+    uint32_t leaf = 0;
+    for (;;) {
+        while (live) {
+            if ((cur >> 24) != 0) {
+                if (leaf != 0) break;
+                leaf = cur;
+                if (sp == 0) { live = false; break; }
+                st.get(--sp, cur, unused);
+            } else {
+                const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
+                const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
+                float tA = 0.0f, tB = 0.0f;
+                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
+                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
+                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+                if (hA && hB) {
+                    const bool aNear = tA < tB;
+                    st.put(sp++, aNear ? pB : pA, 0.0f);
+                    cur = aNear ? pA : pB;
+                } else if (hA) {
+                    cur = pA;
+                } else if (hB) {
+                    cur = pB;
+                } else {
+                    if (sp == 0) { live = false; break; }
+                    st.get(--sp, cur, unused);
+                }
+            }
+            if (__ballot(leaf != 0) == __ballot(1)) break;
+        }
+        if (leaf != 0) {
+            const uint32_t cnt = leaf >> 24, child = leaf & kMaxChild;
+            const float* T = S.tris + (size_t)(triBase + (int32_t)child) * 9;
+            for (uint32_t i = 0; i < cnt; ++i, T += 9) {
+                float t, u, v;
+                if (tri_hit(T, r, t, u, v)) return true;
+            }
+            leaf = 0;
+        } else if (!live) {
+            return false;
+        }
+    }
+}
+
 // Object.Intersect (scene.cs:227 -> math.cs:67-79): distance in world units only; the hit
 // record keeps (u, v, slot) so shading can rebuild the full ShapeHit.
-template <bool COUNT>
+template <bool COUNT, bool SPEC = false>
 SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r, float& dist, float& u, float& v,
                            int32_t& slot, Tally& tl, uint2* lds) {
     float scale;
@@ -181,7 +319,8 @@ SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r
     bool hit;
     if (ob.kind == 0) {
         if (COUNT) tl.mesh += 1;
-        hit = ob.nodeBase >= 0 && mesh_closest<COUNT>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, tl, lds);
+        if (SPEC && !COUNT) hit = ob.nodeBase >= 0 && mesh_closest_spec(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, lds);
+        else hit = ob.nodeBase >= 0 && mesh_closest<COUNT>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, tl, lds);
     } else {
         ShapeHitD h;
         hit = ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
@@ -198,6 +337,7 @@ SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, Ta
     const ray3 lr = to_object(ob, r, scale);
     if (ob.kind == 0) {
         if (COUNT) tl.mesh += 1;
+        if (!COUNT) return ob.nodeBase >= 0 && mesh_any_spec(S, ob.nodeBase, ob.triBase, lr, lds);
         return ob.nodeBase >= 0 && mesh_any<COUNT>(S, ob.nodeBase, ob.triBase, lr, tl, lds);
     }
     ShapeHitD h;
@@ -207,7 +347,7 @@ SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, Ta
 // Scene BVH over objects (costI 10, 32 bins; scene.cs:528-533), two-level closest hit, in the
 // while-while form of the mesh traversal: a wave descends the object BVH until every lane holds
 // an object leaf, so the lanes enter their mesh traversals together.
-template <bool COUNT>
+template <bool COUNT, bool SPEC = false>
 SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& tl, uint2* lds) {
     if (S.sceneNodes == 0) return false;
     const float4* N = S.nodes;
@@ -260,7 +400,7 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
                 const int32_t oi = S.sceneItems[child + i];
                 float d, u, v;
                 int32_t slot;
-                if (object_closest<COUNT>(S, S.objects[oi], r, d, u, v, slot, tl, lds) && d < best) {
+                if (object_closest<COUNT, SPEC>(S, S.objects[oi], r, d, u, v, slot, tl, lds) && d < best) {
                     best = d;
                     out.t = d; out.u = u; out.v = v; out.obj = oi; out.slot = slot;
                     found = true;
