@@ -42,9 +42,10 @@ constexpr int kBlock = kTravBlock;
 #define SR_TRACE_BLOCK 64
 #endif
 constexpr int kTraceBlock = SR_TRACE_BLOCK;  // threads per block of k_trace (one wave)
-static_assert(kLdsDepth == 0 || kTraceBlock == kTravBlock, "LDS stack columns assume kTravBlock lanes");
-// Per-lane LDS columns of the mesh traversal stack (kTravLdsBytes per block).
+// Per-lane LDS columns of the mesh traversal stack (trav_lds_bytes(threads) per block).
 extern __shared__ uint2 sr_lds[];
+// This lane's column of its wave's LDS stack rows.
+SR_DEV uint2* lds_column() { return sr_lds + (threadIdx.x >> 6) * (kLdsDepth * 64) + (threadIdx.x & 63); }
 
 // ---- Frame state --------------------------------------------------------------------------------
 struct alignas(16) PathRec {  // 96 B per path: everything shade keeps between segments
@@ -460,12 +461,12 @@ SR_DEV vec3 shadow_transmittance(const DevScene& S, const ShadowRec& sr, Tally& 
     ray3 ray = mk_ray(sr.o, sr.d);
     vec3 tr = splat3(1.0f);
     if (!TRANSP && !COUNT) {
-        if (scene_any<COUNT>(S, ray, tl, sr_lds + threadIdx.x)) tr = splat3(0.0f);
+        if (scene_any<COUNT>(S, ray, tl, lds_column())) tr = splat3(0.0f);
     } else {
         for (;;) {
             HitRec h;
             h.obj = -1;
-            if (!scene_closest<COUNT>(S, ray, h, tl, sr_lds + threadIdx.x)) break;
+            if (!scene_closest<COUNT>(S, ray, h, tl, lds_column())) break;
             const DevObject& ob = S.objects[h.obj];
             if (ob.transparency <= 0.0f) { tr = splat3(0.0f); break; }
             SurfaceD s;
@@ -540,7 +541,7 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
             p.seg = p.seg + 1;
             ray = mk_ray(no, nd);
             h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
-            scene_closest<COUNT>(S, ray, h, tl, sr_lds + threadIdx.x);
+            scene_closest<COUNT>(S, ray, h, tl, lds_column());
             ++traces;
         }
         rng_store(p, rng);
@@ -599,7 +600,7 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
                 HitRec h;
                 h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
                 // Speculative mesh traversal outside counting renders: same hit, wider waves.
-                scene_closest<COUNT, !COUNT>(S, r, h, tl, sr_lds + threadIdx.x);
+                scene_closest<COUNT, !COUNT>(S, r, h, tl, lds_column());
                 F.hitA[q] = make_float4(h.t, h.u, h.v, __int_as_float(h.obj));
                 F.hitSlot[q] = h.slot;
             }
@@ -609,7 +610,7 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
                 shadow_item<COUNT, TRANSP>(F, S, rec, tlRef, extraRef);
             } else {
                 shadow_item<COUNT, TRANSP>(F, S, rec, tlShadow, extra);
-                if (COUNT && !TRANSP) (void)scene_any<COUNT>(S, mk_ray(rec.o, rec.d), tlAny, sr_lds + threadIdx.x);
+                if (COUNT && !TRANSP) (void)scene_any<COUNT>(S, mk_ray(rec.o, rec.d), tlAny, lds_column());
             }
         }
     }
@@ -752,7 +753,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(AdjointArgs A, DevScene S) {
             rng.draw = 0;
             vec3 o, d;
             camera_ray(A.view, A.W, A.H, x, y, rng, o, d);
-            adjoint_path(S, mk_ray(o, d), rng, A.bounces, A.clampLum, delta, gm, A.gEnv, sr_lds + threadIdx.x);
+            adjoint_path(S, mk_ray(o, d), rng, A.bounces, A.clampLum, delta, gm, A.gEnv, lds_column());
         }
     }
     if (gm.gTotal != 0.0f) atomicAdd(&glTotal, gm.gTotal);
@@ -774,7 +775,7 @@ __global__ void k_primary(Frame F, DevScene S, int32_t* obj, int32_t* tri, float
     h.t = kInf; h.obj = -1; h.slot = -1;
     Tally tl = {0, 0, 0};
     const size_t idx = (size_t)y * F.W + x;
-    if (scene_closest<false>(S, mk_ray(o, d), h, tl, sr_lds + threadIdx.x)) {
+    if (scene_closest<false>(S, mk_ray(o, d), h, tl, lds_column())) {
         const DevObject& ob = S.objects[h.obj];
         obj[idx] = h.obj;
         tri[idx] = ob.kind == 0 ? S.slotTri[ob.triBase + h.slot] : -1;
@@ -1686,10 +1687,10 @@ int run_band(sr_ctx* c, const Frame& F) {
         int cur = 0;
         bool fused = false;
         for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
-            hipLaunchKernelGGL(trace, dim3(gTrace), dim3(kTraceBlock), kTravLdsBytes, s, F, c->dev, cur);
+            hipLaunchKernelGGL(trace, dim3(gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
             mark(c, 0);
             if (seg == c->tailSegment) {  // the rest of every path in one launch
-                hipLaunchKernelGGL(tail, dim3(gTail), dim3(kBlock), kTravLdsBytes, s, F, c->dev, cur);
+                hipLaunchKernelGGL(tail, dim3(gTail), dim3(kBlock), trav_lds_bytes(kBlock), s, F, c->dev, cur);
                 mark(c, 2);
                 hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, F);
                 fused = true;
@@ -1700,7 +1701,7 @@ int run_band(sr_ctx* c, const Frame& F) {
             cur ^= 1;
         }
         if (!fused) {  // the last segment's shadow rays (its ray queue is empty)
-            hipLaunchKernelGGL(trace, dim3(gTrace), dim3(kTraceBlock), kTravLdsBytes, s, F, c->dev, cur);
+            hipLaunchKernelGGL(trace, dim3(gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
             mark(c, 2);
         }
         hipLaunchKernelGGL(k_next<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
@@ -2000,7 +2001,7 @@ int sr_trace_primary(sr_ctx* ctx, const sr_view* view, uint32_t w, uint32_t h, i
     Frame F = frame_for(ctx, 0, h);
     ctx->params = saved;
     F.view = make_view(view, w, h);
-    hipLaunchKernelGGL(k_primary, dim3((F.nSlots + kBlock - 1) / kBlock), dim3(kBlock), kTravLdsBytes, ctx->stream, F, ctx->dev, dObj,
+    hipLaunchKernelGGL(k_primary, dim3((F.nSlots + kBlock - 1) / kBlock), dim3(kBlock), trav_lds_bytes(kBlock), ctx->stream, F, ctx->dev, dObj,
                        dTri, dDist);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -2372,7 +2373,7 @@ int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad) {
     const uint64_t total = (uint64_t)px * p.max_samples;
     const uint64_t blocks = (total + kBlock - 1) / kBlock;
     if (blocks > 0x7fffffffu) return fail(SR_ERR_LIMIT, "adjoint backward: too many paths");
-    hipLaunchKernelGGL(k_adjoint, dim3((uint32_t)blocks), dim3(kBlock), kTravLdsBytes, s, A, ctx->dev);
+    hipLaunchKernelGGL(k_adjoint, dim3((uint32_t)blocks), dim3(kBlock), trav_lds_bytes(kBlock), s, A, ctx->dev);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(grad, ctx->dAdjGrad.ptr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
     float gTotal = 0.0f;

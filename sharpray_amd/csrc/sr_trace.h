@@ -20,25 +20,28 @@ struct Tally {
     uint32_t box, shape, mesh;
 };
 
+// Eight LDS entries per lane: 4 KB per one-wave trace block, so 7 waves per SIMD still fit the
+// CU's 160 KB (16 entries cost occupancy; 6 to 10 measure the same).
 #ifndef SR_LDS_DEPTH
-#define SR_LDS_DEPTH 0
+#define SR_LDS_DEPTH 8
 #endif
 constexpr int kTravBlock = 256;        // threads per block of the traversal kernels
 constexpr int kLdsDepth = SR_LDS_DEPTH;  // stack entries per lane kept in LDS
 
-// Mesh traversal stack: the first kLdsDepth entries of each lane live in LDS (one column per
-// lane, so a wave's accesses are bank-conflict free); deeper entries spill to private memory.
+// Mesh traversal stack: the first kLdsDepth entries of each lane live in LDS (each wave owns
+// kLdsDepth rows of 64 entries, one column per lane, so a wave's accesses are bank-conflict free);
+// deeper entries spill to private memory.
 struct TravStack {
-    uint2* col;  // this lane's LDS column (stride kTravBlock entries)
+    uint2* col;  // this lane's LDS column (stride 64 entries)
     uint32_t node[kStack - kLdsDepth];
     float t[kStack - kLdsDepth];
     SR_DEV void put(int i, uint32_t n, float tt) {
-        if (kLdsDepth > 0 && i < kLdsDepth) col[i * kTravBlock] = make_uint2(n, __float_as_uint(tt));
+        if (kLdsDepth > 0 && i < kLdsDepth) col[i * 64] = make_uint2(n, __float_as_uint(tt));
         else { node[i - kLdsDepth] = n; t[i - kLdsDepth] = tt; }
     }
     SR_DEV void get(int i, uint32_t& n, float& tt) const {
         if (kLdsDepth > 0 && i < kLdsDepth) {
-            const uint2 e = col[i * kTravBlock];
+            const uint2 e = col[i * 64];
             n = e.x;
             tt = __uint_as_float(e.y);
         } else {
@@ -47,7 +50,8 @@ struct TravStack {
         }
     }
 };
-constexpr size_t kTravLdsBytes = (size_t)kLdsDepth * kTravBlock * sizeof(uint2);
+// Dynamic LDS of a traversal launch with `threads` threads per block.
+constexpr size_t trav_lds_bytes(int threads) { return (size_t)kLdsDepth * (size_t)threads * sizeof(uint2); }
 
 SR_DEV uint32_t pack_node(float childBits, float countBits) {
     return ((uint32_t)__float_as_int(countBits) << 24) | (uint32_t)__float_as_int(childBits);

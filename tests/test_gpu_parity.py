@@ -614,3 +614,21 @@ def test_fused_tail_launch_matches_wavefront(tail):
     a, b = out[99], out[tail]
     assert np.array_equal(bits(a[0]), bits(b[0])) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
     assert a[3] == b[3], (a[3], b[3])
+
+
+def test_speculative_traversal_matches_counting_render():
+    """Renders without counters traverse mesh BVHs speculatively (postponed leaves); counting renders
+    use the plain order. Both must give bit-identical AOVs on the 1M-triangle scene."""
+    from sharpray_amd import Renderer, scenes
+    spec = scenes.config_plain_gpu(scale=0.25, spp=4)
+    scene, view = scenes.instantiate(spec)
+    scene.build()
+    out = {}
+    for counting in (False, True):
+        r = Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
+                     spec.variance_threshold, spec.bounces, spec.indirect_clamp, counters=counting)
+        r.wait()
+        out[counting] = (r.radiance.copy(), r.radiance_fog.copy(), r.depth.copy(), r.normals.copy())
+    print(f"plain_gpu 1/4 scale {spec.width}x{spec.height}, {spec.max_samples} spp", flush=True)
+    for a, b in zip(out[False], out[True]):
+        assert np.array_equal(bits(a), bits(b))
