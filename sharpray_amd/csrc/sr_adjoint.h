@@ -225,7 +225,7 @@ SR_DEV void adjoint_path(const DevScene& S, ray3 ray, CounterRng& rng, uint32_t 
     for (uint32_t i = 0; i <= bounces; ++i) {
         HitRec h;
         h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
-        const bool hit = scene_closest<false>(S, ray, h, tl, lds);
+        const bool hit = scene_closest<false, false, 0>(S, ray, h, tl, lds);
         if (!hit) {
             float mis = 1.0f;
             const vec3 skyD = sky_radiance(S, ray.d);
@@ -271,7 +271,7 @@ SR_DEV void adjoint_path(const DevScene& S, ray3 ray, CounterRng& rng, uint32_t 
         if (sky_light_rand(S, rng, ld, lpdf, &ltex) && lpdf > 0.0f) {
             const vec3 nn = dot3(s.normal, view) >= 0.0f ? s.normal : -s.normal;
             const vec3 ng = dot3(s.normalGeo, view) >= 0.0f ? s.normalGeo : -s.normalGeo;
-            if (dot3(nn, ld) > 0.0f && dot3(ng, ld) > 0.0f && !scene_any<false>(S, mk_ray(hitPos, ld), tl, lds)) {
+            if (dot3(nn, ld) > 0.0f && dot3(ng, ld) > 0.0f && !scene_any<false, 0>(S, mk_ray(hitPos, ld), tl, lds)) {
                 const vec3 skyL = sky_radiance(S, ld);
                 const vec3 refl = opaque_eval(s, view, ld);
                 const float sp = spec_prob(s, view);

@@ -28,25 +28,26 @@ struct Tally {
 constexpr int kTravBlock = 256;        // threads per block of the traversal kernels
 constexpr int kLdsDepth = SR_LDS_DEPTH;  // stack entries per lane kept in LDS
 
-// Mesh traversal stack: the first kLdsDepth entries of each lane live in LDS (each wave owns
-// kLdsDepth rows of 64 entries, one column per lane, so a wave's accesses are bank-conflict free);
-// deeper entries spill to private memory.
+// Mesh traversal stack: the first LD entries of each lane live in LDS (each wave owns LD rows of
+// 64 entries, one column per lane, so a wave's accesses are bank-conflict free); deeper entries
+// spill to private memory.
+template <int LD>
 struct TravStack {
     uint2* col;  // this lane's LDS column (stride 64 entries)
-    uint32_t node[kStack - kLdsDepth];
-    float t[kStack - kLdsDepth];
+    uint32_t node[kStack - LD];
+    float t[kStack - LD];
     SR_DEV void put(int i, uint32_t n, float tt) {
-        if (kLdsDepth > 0 && i < kLdsDepth) col[i * 64] = make_uint2(n, __float_as_uint(tt));
-        else { node[i - kLdsDepth] = n; t[i - kLdsDepth] = tt; }
+        if (LD > 0 && i < LD) col[i * 64] = make_uint2(n, __float_as_uint(tt));
+        else { node[i - LD] = n; t[i - LD] = tt; }
     }
     SR_DEV void get(int i, uint32_t& n, float& tt) const {
-        if (kLdsDepth > 0 && i < kLdsDepth) {
+        if (LD > 0 && i < LD) {
             const uint2 e = col[i * 64];
             n = e.x;
             tt = __uint_as_float(e.y);
         } else {
-            n = node[i - kLdsDepth];
-            tt = t[i - kLdsDepth];
+            n = node[i - LD];
+            tt = t[i - LD];
         }
     }
 };
@@ -63,11 +64,11 @@ SR_DEV uint32_t pack_node(float childBits, float countBits) {
 // then one leaf step, so a wave runs its box tests and its triangle tests as separate batches
 // instead of paying for both whenever any lane sits on a leaf. Each lane visits the same node
 // sequence as the plain loop over (pop, test) (same hits, same counters).
-template <bool COUNT>
+template <bool COUNT, int LD>
 SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, float& bestT,
                          float& bu, float& bv, int32_t& bslot, Tally& tl, uint2* lds) {
     const float4* N = S.nodes;
-    TravStack st;
+    TravStack<LD> st;
     st.col = lds;
     int sp = 0;
     bestT = kInf;
@@ -129,10 +130,10 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
 
 // Whether any primitive of a mesh is hit (the same node visits as the closest-hit traversal up
 // to its first accepted hit, which is where the existence of a hit is decided); while-while form.
-template <bool COUNT>
+template <bool COUNT, int LD>
 SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, Tally& tl, uint2* lds) {
     const float4* N = S.nodes;
-    TravStack st;
+    TravStack<LD> st;
     st.col = lds;
     int sp = 0;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
@@ -179,10 +180,11 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
 // leaf the plain traversal would have pruned holds no strictly closer hit); the extra box tests
 // come from descending before a postponed leaf has lowered the best distance, which is why
 // counting renders use mesh_closest itself.
+template <int LD>
 SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, float& bestT,
                               float& bu, float& bv, int32_t& bslot, uint2* lds) {
     const float4* N = S.nodes;
-    TravStack st;
+    TravStack<LD> st;
     st.col = lds;
     int sp = 0;
     bestT = kInf;
@@ -257,9 +259,10 @@ SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBa
 // Speculative form of mesh_any, used outside counting renders: lanes holding a leaf keep
 // descending while other lanes of the wave look for one. Whether a hit exists does not depend on
 // the order of the leaves.
+template <int LD>
 SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, uint2* lds) {
     const float4* N = S.nodes;
-    TravStack st;
+    TravStack<LD> st;
     st.col = lds;
     int sp = 0;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
@@ -312,7 +315,7 @@ SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, 
 
 // Object.Intersect (scene.cs:227 -> math.cs:67-79): distance in world units only; the hit
 // record keeps (u, v, slot) so shading can rebuild the full ShapeHit.
-template <bool COUNT, bool SPEC = false>
+template <bool COUNT, bool SPEC, int LD>
 SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r, float& dist, float& u, float& v,
                            int32_t& slot, Tally& tl, uint2* lds) {
     float scale;
@@ -323,8 +326,8 @@ SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r
     bool hit;
     if (ob.kind == 0) {
         if (COUNT) tl.mesh += 1;
-        if (SPEC && !COUNT) hit = ob.nodeBase >= 0 && mesh_closest_spec(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, lds);
-        else hit = ob.nodeBase >= 0 && mesh_closest<COUNT>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, tl, lds);
+        if (SPEC && !COUNT) hit = ob.nodeBase >= 0 && mesh_closest_spec<LD>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, lds);
+        else hit = ob.nodeBase >= 0 && mesh_closest<COUNT, LD>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, tl, lds);
     } else {
         ShapeHitD h;
         hit = ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
@@ -335,14 +338,14 @@ SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r
     return true;
 }
 
-template <bool COUNT>
+template <bool COUNT, int LD>
 SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, Tally& tl, uint2* lds) {
     float scale;
     const ray3 lr = to_object(ob, r, scale);
     if (ob.kind == 0) {
         if (COUNT) tl.mesh += 1;
-        if (!COUNT) return ob.nodeBase >= 0 && mesh_any_spec(S, ob.nodeBase, ob.triBase, lr, lds);
-        return ob.nodeBase >= 0 && mesh_any<COUNT>(S, ob.nodeBase, ob.triBase, lr, tl, lds);
+        if (!COUNT) return ob.nodeBase >= 0 && mesh_any_spec<LD>(S, ob.nodeBase, ob.triBase, lr, lds);
+        return ob.nodeBase >= 0 && mesh_any<COUNT, LD>(S, ob.nodeBase, ob.triBase, lr, tl, lds);
     }
     ShapeHitD h;
     return ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
@@ -351,7 +354,8 @@ SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, Ta
 // Scene BVH over objects (costI 10, 32 bins; scene.cs:528-533), two-level closest hit, in the
 // while-while form of the mesh traversal: a wave descends the object BVH until every lane holds
 // an object leaf, so the lanes enter their mesh traversals together.
-template <bool COUNT, bool SPEC = false>
+// LD: mesh-stack entries per lane kept in the caller's LDS rows (0: all in private memory).
+template <bool COUNT, bool SPEC = false, int LD = kLdsDepth>
 SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& tl, uint2* lds) {
     if (S.sceneNodes == 0) return false;
     const float4* N = S.nodes;
@@ -404,7 +408,7 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
                 const int32_t oi = S.sceneItems[child + i];
                 float d, u, v;
                 int32_t slot;
-                if (object_closest<COUNT, SPEC>(S, S.objects[oi], r, d, u, v, slot, tl, lds) && d < best) {
+                if (object_closest<COUNT, SPEC, LD>(S, S.objects[oi], r, d, u, v, slot, tl, lds) && d < best) {
                     best = d;
                     out.t = d; out.u = u; out.v = v; out.obj = oi; out.slot = slot;
                     found = true;
@@ -421,7 +425,7 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
 
 // Whether a ray reaches no surface at all: for scenes without transparent objects this decides
 // ShadowTransmittance (scene.cs:576-600) exactly as its closest-hit trace does.
-template <bool COUNT>
+template <bool COUNT, int LD = kLdsDepth>
 SR_DEV bool scene_any(const DevScene& S, const ray3& r, Tally& tl, uint2* lds) {
     if (S.sceneNodes == 0) return false;
     const float4* N = S.nodes;
@@ -454,7 +458,7 @@ SR_DEV bool scene_any(const DevScene& S, const ray3& r, Tally& tl, uint2* lds) {
         const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
         for (uint32_t i = 0; i < cnt; ++i) {
             if (COUNT) tl.shape += 1;
-            if (object_any<COUNT>(S, S.objects[S.sceneItems[child + i]], r, tl, lds)) return true;
+            if (object_any<COUNT, LD>(S, S.objects[S.sceneItems[child + i]], r, tl, lds)) return true;
         }
         if (sp == 0) return false;
         cur = stk[--sp];
