@@ -186,18 +186,32 @@ __global__ void k_features(const float* rad, const float* alb, const float* nrm,
 
 // One stage of the kernel cascade: out(p) = sum_k w_k(p) in(clamp(p + d (dx_k, dy_k))), w = softmax of
 // the stage's 25 logits, 5 x 5 taps spaced d = 2^stage pixels, edges clamped.
-__global__ void k_kpn_stage(const uint16_t* logits, int stage, const float* in, int W, int H, float* out) {
+// The stage's 25 logits sit at halves [25 STAGE, 25 STAGE + 25) of the pixel's 256-byte row; they
+// are read as the four 16-byte words from half 25 STAGE rounded down to a multiple of 8.
+template <int STAGE>
+__global__ void k_kpn_stage(const uint16_t* logits, const float* in, int W, int H, float* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= W * H) return;
-    const int x = i % W, y = i / W, d = 1 << stage;
-    const uint16_t* lg = logits + (size_t)i * kLogitStride + stage * kTaps;
+    constexpr int kA = (STAGE * kTaps) & ~7, kO = STAGE * kTaps - kA;
+    static_assert(kO + kTaps <= 32 && kA + 32 <= kLogitStride, "four words hold the stage's logits");
+    const int x = i % W, y = i / W, d = 1 << STAGE;
+    const uint4* src = (const uint4*)(logits + (size_t)i * kLogitStride + kA);
+    uint32_t wds[16];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint4 v = src[j];
+        wds[4 * j] = v.x; wds[4 * j + 1] = v.y; wds[4 * j + 2] = v.z; wds[4 * j + 3] = v.w;
+    }
     float l[kTaps];
     float m = -INFINITY;
+#pragma unroll
     for (int k = 0; k < kTaps; ++k) {
-        l[k] = bf2f(lg[k]);
+        const uint32_t wv = wds[(kO + k) >> 1];
+        l[k] = bf2f((uint16_t)(((kO + k) & 1) ? (wv >> 16) : (wv & 0xffffu)));
         m = fmaxf(m, l[k]);
     }
     float s = 0.0f, r = 0.0f, g = 0.0f, b = 0.0f;
+#pragma unroll
     for (int k = 0; k < kTaps; ++k) {
         const float e = expf(l[k] - m);
         const int yy = min(max(y + d * (k / 5 - 2), 0), H - 1), xx = min(max(x + d * (k % 5 - 2), 0), W - 1);
@@ -599,9 +613,26 @@ ConvArgs layer_args(int li, uint32_t W, uint32_t H, const uint16_t* feat, const 
 #ifndef SR_CONV_SMALL_MT
 #define SR_CONV_SMALL_MT 4
 #endif
-// Rows per wave for layers of at most 48 output channels (16x16 tiles: 2.34 ms per 1080p frame
-// against 2.40 ms with 8 rows per wave and 2.75 ms with 2).
-constexpr int kSmallMT = SR_CONV_SMALL_MT;
+constexpr int kSmallMT = SR_CONV_SMALL_MT;  // rows per wave of the 32-channel (2-tile) layers
+// Rows per wave by output-channel tiles, measured per layer at 1080p (more rows reuse each weight
+// fragment more often, more tiles hold more accumulators): 2 tiles 4 rows (8 rows: 94 -> 113 us on
+// L0), 3 tiles 8 rows (4 rows: L14 143 vs 104 us), 4 tiles 4 rows (2 rows: L5 43 vs 30 us), 6 and 8
+// tiles 2 rows (4 rows: L10 52 vs 64 us), the 7-tile head 4 rows (2 rows: 410 vs 345 us).
+#ifndef SR_HEAD_MT
+#define SR_HEAD_MT 4
+#endif
+#ifndef SR_MT3
+#define SR_MT3 8
+#endif
+#ifndef SR_MT4
+#define SR_MT4 4
+#endif
+#ifndef SR_MT6
+#define SR_MT6 2
+#endif
+#ifndef SR_MT8
+#define SR_MT8 2
+#endif
 
 template <int NT, int MT>
 void launch_conv_t(const ConvArgs& A, hipStream_t s) {
@@ -613,12 +644,12 @@ void launch_conv_t(const ConvArgs& A, hipStream_t s) {
 bool launch_conv(const ConvArgs& A, hipStream_t s) {
     switch ((A.cout + 15) / 16) {
         case 2: launch_conv_t<2, kSmallMT>(A, s); break;
-        case 3: launch_conv_t<3, kSmallMT>(A, s); break;
-        case 4: launch_conv_t<4, 2>(A, s); break;
+        case 3: launch_conv_t<3, SR_MT3>(A, s); break;
+        case 4: launch_conv_t<4, SR_MT4>(A, s); break;
         case 5: launch_conv_t<5, 2>(A, s); break;
-        case 6: launch_conv_t<6, 2>(A, s); break;
-        case 7: launch_conv_t<7, 2>(A, s); break;
-        case 8: launch_conv_t<8, 2>(A, s); break;
+        case 6: launch_conv_t<6, SR_MT6>(A, s); break;
+        case 7: launch_conv_t<7, SR_HEAD_MT>(A, s); break;
+        case 8: launch_conv_t<8, SR_MT8>(A, s); break;
         case 10: launch_conv_t<10, 2>(A, s); break;
         case 14: launch_conv_t<14, 2>(A, s); break;
         default: return false;
@@ -637,9 +668,12 @@ bool CnnNet::run(const float* rad, const float* alb, const float* nrm, const flo
         if (!launch_conv(layer_args((int)li, W, H, feat_, act_, w_, b_), s)) return false;
     if (out) {
         const dim3 g((n + 255) / 256);
+        static_assert(kStages == 4, "one k_kpn_stage instance per stage");
+        void (*const stage[kStages])(const uint16_t*, const float*, int, int, float*) = {
+            k_kpn_stage<0>, k_kpn_stage<1>, k_kpn_stage<2>, k_kpn_stage<3>};
         for (int st = 0; st < kStages; ++st)
-            hipLaunchKernelGGL(k_kpn_stage, g, dim3(256), 0, s, act_.back(), st, st ? casc_[st - 1] : rad, (int)W,
-                               (int)H, st == kStages - 1 ? out : casc_[st]);
+            hipLaunchKernelGGL(stage[st], g, dim3(256), 0, s, act_.back(), st ? casc_[st - 1] : rad, (int)W, (int)H,
+                               st == kStages - 1 ? out : casc_[st]);
     }
     return hipGetLastError() == hipSuccess;
 }
