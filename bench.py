@@ -323,7 +323,7 @@ def main():
         r.wait()
         log(f"[bench] warmup {i} done")
     # Timed region: K full frames back to back, host-synchronised on both sides.
-    walls, gpus, ext_ms, ext_launch, tail_ms = [], [], 0.0, 0, 0.0
+    walls, gpus, ext_ms, ext_launch, tail_ms, busy_ms = [], [], 0.0, 0, 0.0, 0.0
     for i in range(args.steps):
         t0 = time.perf_counter()
         r = frame(kernel_timing=True)
@@ -331,13 +331,16 @@ def main():
         walls.append(time.perf_counter() - t0)
         gpus.append(r.gpu_seconds())
         kt = r.kernel_times()
+        kb = r.kernel_busy()
         ext_ms += kt["trace"][0]
         ext_launch += kt["trace"][1]
         tail_ms += kt["trace_tail"][0]
+        busy_ms += kb["trace"]
         log(f"[bench] step {i}: {walls[-1] * 1e3:.1f} ms wall, {gpus[-1] * 1e3:.1f} ms GPU, "
             f"trace {kt['trace'][0]:.1f} ms / {kt['trace'][1]} launches, shade {kt['shade'][0]:.1f} ms, "
             f"fused tail {kt['trace_tail'][0]:.1f} ms / {kt['trace_tail'][1]} launches, "
-            f"other {kt['other'][0]:.1f} ms")
+            f"other {kt['other'][0]:.1f} ms (launch spans summed over the concurrent slices); "
+            f"trace busy {kb['trace']:.1f} ms")
     samples = spec.width * spec.height * spec.max_samples
     ms_per_step = 1e3 * sum(walls) / len(walls)
     msps = samples / (sum(walls) / len(walls)) / 1e6
@@ -356,10 +359,14 @@ def main():
                  BYTES_PER_TRI * (c["BvhIntersectShape"] - fz["BvhIntersectShape"] - sc["BvhIntersectShape"]) +
                  BYTES_PER_RAY * (c["SceneTrace"] - fz["SceneTrace"]) +
                  BYTES_PER_SHADOW * (c["SceneOcclude"] - fz["SceneOcclude"] - sc["SceneOcclude"]))
+    # With concurrent row slices (sr_set_slices; one by default) trace launches overlap each other
+    # and other kernels: achieved = the frame's algorithmic trace bytes / the time at least one trace
+    # launch ran (union of the launch spans); with one slice that is the launches' summed duration.
     launches_per_frame = ext_launch / args.steps
     avg_launch_s = ext_ms / ext_launch / 1e3
+    busy_launch_s = busy_ms / ext_launch / 1e3
     bytes_per_launch = ext_bytes / launches_per_frame
-    achieved = bytes_per_launch / avg_launch_s / 1e9
+    achieved = bytes_per_launch / busy_launch_s / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "trace_hbm_traffic.json")
     if os.path.exists(tpath):
@@ -393,7 +400,9 @@ def main():
         "gpu_ms_per_step": round(1e3 * sum(gpus) / len(gpus), 3),
         "roofline": {"kernel": "k_trace (closest hit + deferred shadow rays)", "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4),
-                     "traffic": traffic, "avg_launch_us": round(avg_launch_s * 1e6, 2),
+                     "traffic": traffic, "busy_us_per_launch": round(busy_launch_s * 1e6, 2),
+                     "avg_launch_us": round(avg_launch_s * 1e6, 2), "launches_per_frame": round(launches_per_frame),
+                     "trace_busy_ms_per_frame": round(busy_ms / args.steps, 3),
                      "algorithmic_bytes_per_launch": round(bytes_per_launch)},
         "fused_tail_ms_per_step": round(tail_ms / args.steps, 3),
         "cpu_baseline": cpu,

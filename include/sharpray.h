@@ -283,13 +283,24 @@ int sr_counters_read_reference_only(sr_ctx* ctx, int64_t* out);
  * results are identical either way). The environment variable SHARPRAY_TAIL_SEGMENT sets the
  * default of new contexts. No counterpart in the reference. */
 int sr_set_tail_segment(sr_ctx* ctx, uint32_t segment);
-/* Kernel timing of the next renders: with timing enabled, an event is recorded on the render
- * stream after every launch; sr_profile_read sums the spans per kernel kind of the last render:
+/* Later renders run every band as `slices` concurrent row slices (whole 8-row tile rows), each with
+ * its own queues, path records and HIP stream, so the launch tails of one slice overlap the work of
+ * the others (default 1, one wavefront per band; 2 measured 5 % faster on the 1080p frame; results
+ * are identical either way). The
+ * environment variable SHARPRAY_SLICES sets the default of new contexts. No counterpart in the
+ * reference (its worker threads claim 32x32 blocks, renderer.cs:122-151). */
+int sr_set_slices(sr_ctx* ctx, uint32_t slices);
+/* Kernel timing of the next renders: with timing enabled, an event is recorded on the slice's
+ * stream after every launch; sr_profile_read sums the spans per kernel kind of the last render
+ * over all slices (concurrent slices overlap, so sums can exceed the frame time):
  * ms[4] / launches[4] for [0] trace (closest hit of a segment's rays plus the previous segment's
  * shadow rays), [1] shade, [2] trace of a sample's last shadow rays, [3] raygen + accumulate.
  * No counterpart in the reference (its TimeScope covers whole phases, counters.cs:4-20). */
 int sr_profile_enable(sr_ctx* ctx, int32_t enable);
 int sr_profile_read(sr_ctx* ctx, double* ms, uint32_t* launches);
+/* ms[4]: per kernel kind of the last render, the time during which at least one launch of that
+ * kind was running on any slice (the union of the launch spans). No counterpart in the reference. */
+int sr_profile_busy(sr_ctx* ctx, double* ms);
 /* Seconds of GPU time of the last render (HIP events around the whole frame). */
 int sr_render_gpu_seconds(sr_ctx* ctx, double* out_seconds);
 

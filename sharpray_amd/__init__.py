@@ -349,6 +349,11 @@ class Scene:
         """Renders finish every path from this segment on in one fused launch (results unchanged)."""
         check(N.lib().sr_set_tail_segment(self._ctx, int(segment)), "tail segment")
 
+    def set_slices(self, slices):
+        """Renders run each band as this many concurrent row slices on their own streams (results
+        unchanged)."""
+        check(N.lib().sr_set_slices(self._ctx, int(slices)), "slices")
+
     def set_bvh_builder(self, device):
         """Mesh BVHs of later meshes built on the GPU (True) or the host (False); both give the
         reference's tree bit for bit."""
@@ -533,6 +538,13 @@ class Renderer:
         n = (ctypes.c_uint32 * 4)()
         check(N.lib().sr_profile_read(self.scene.ctx, ms, n), "kernel times")
         return {k: (ms[i], n[i]) for i, k in enumerate(["trace", "shade", "trace_tail", "other"])}
+
+    def kernel_busy(self):
+        """{kind: ms} of this render during which at least one launch of that kind ran on any of
+        the concurrent slices (kinds as in kernel_times)."""
+        ms = (ctypes.c_double * 4)()
+        check(N.lib().sr_profile_busy(self.scene.ctx, ms), "kernel busy")
+        return {k: ms[i] for i, k in enumerate(["trace", "shade", "trace_tail", "other"])}
 
     def counters(self):
         out = (ctypes.c_int64 * N.SR_COUNTER_COUNT)()

@@ -1085,11 +1085,24 @@ struct sr_ctx {
     uint32_t iterations = 0;
     hipEvent_t evStart = nullptr, evEnd = nullptr;
     int gridCUs = 256;
-    // Kernel timing: one event after every launch of a frame; kernel i spans events i-1..i.
+    // Concurrent row slices of every band: each slice has its own queues, path records and
+    // stream, so one slice's launch tails and fused tail run beside the others' work (one 1080p
+    // frame: 278 ms as one slice, 263 ms as 2 or 4, 325 ms as 6). The default keeps one slice, so
+    // every kernel launch has the GPU to itself and its timing measures that kernel alone.
+    // SHARPRAY_SLICES or sr_set_slices override it.
+    uint32_t slices = 1;
+    uint32_t sliceCount = 1;                // slices of the current render
+    uint32_t sliceRows = 0, sliceSlots = 0;  // rows (multiple of 8) and slots per slice
+    std::vector<hipStream_t> sliceStreams;   // slices 1.. (slice 0 runs on `stream`)
+    // Kernel timing per slice: one event after every launch; kernel i spans events i-1..i of its
+    // slice.
+    struct ProfLane {
+        std::vector<hipEvent_t> ev;
+        std::vector<int> kind;  // 0 extend, 1 shade, 2 shadow, 3 other
+        size_t used = 0;
+    };
     bool profileOn = false;
-    std::vector<hipEvent_t> profEvents;
-    std::vector<int> profKinds;  // 0 extend, 1 shade, 2 shadow, 3 other
-    size_t profUsed = 0;
+    std::vector<ProfLane> prof;
 
     void join() {
         if (worker.joinable()) worker.join();
@@ -1097,7 +1110,9 @@ struct sr_ctx {
 
     ~sr_ctx() {
         join();
-        for (auto e : profEvents) (void)hipEventDestroy(e);
+        for (ProfLane& l : prof)
+            for (auto e : l.ev) (void)hipEventDestroy(e);
+        for (hipStream_t st : sliceStreams) (void)hipStreamDestroy(st);
         if (hostCount) (void)hipHostFree(hostCount);
         if (evStart) (void)hipEventDestroy(evStart);
         if (evEnd) (void)hipEventDestroy(evEnd);
@@ -1342,6 +1357,7 @@ int sr_create(int32_t device, sr_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->gridCUs = prop.multiProcessorCount;
     if (const char* t = getenv("SHARPRAY_TAIL_SEGMENT")) c->tailSegment = (uint32_t)strtoul(t, nullptr, 10);
+    if (const char* t = getenv("SHARPRAY_SLICES")) c->slices = std::max(1u, (uint32_t)strtoul(t, nullptr, 10));
     *out = c;
     return SR_OK;
 }
@@ -1623,20 +1639,22 @@ ViewD make_view(const sr_view* v, uint32_t W, uint32_t H) {
     return d;
 }
 
-Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows) {
+// The frame state of rows [y0, y0 + rows) run as slice `slice` (its own part of every buffer).
+Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     Frame F;
     memset(&F, 0, sizeof(F));
-    F.rayO = c->dRayO.ptr;
-    F.rayD = c->dRayD.ptr;
-    F.hitA = c->dHitA.ptr;
-    F.hitSlot = c->dHitSlot.ptr;
-    F.paths = c->dPaths.ptr;
-    F.shadows[0] = c->dShadows.ptr;
-    F.shadows[1] = c->dShadows.ptr + c->dShadows.n / 2;
-    F.acc = c->dAcc.ptr;
-    F.queue[0] = c->dQueue0.ptr;
-    F.queue[1] = c->dQueue1.ptr;
-    F.counts = c->dCounts.ptr;
+    const size_t o = (size_t)slice * c->sliceSlots;
+    F.rayO = c->dRayO.ptr + o;
+    F.rayD = c->dRayD.ptr + o;
+    F.hitA = c->dHitA.ptr + o;
+    F.hitSlot = c->dHitSlot.ptr + o;
+    F.paths = c->dPaths.ptr + o;
+    F.shadows[0] = c->dShadows.ptr + 2 * o;
+    F.shadows[1] = c->dShadows.ptr + 2 * o + c->sliceSlots;
+    F.acc = c->dAcc.ptr + o;
+    F.queue[0] = c->dQueue0.ptr + o;
+    F.queue[1] = c->dQueue1.ptr + o;
+    F.counts = c->dCounts.ptr + (size_t)slice * kCounterWords;
     F.ctr = c->countersOn ? c->dCtr.ptr : nullptr;
     F.outRad = c->dRad.ptr;
     F.outRadFog = c->dRadFog.ptr;
@@ -1662,13 +1680,16 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows) {
     return F;
 }
 
-int alloc_frame(sr_ctx* c, uint32_t maxSlots) {
+// Frame buffers for `slices` slices of `sliceSlots` slots each.
+int alloc_frame(sr_ctx* c, uint32_t sliceSlots, uint32_t slices) {
     const size_t px = (size_t)c->params.width * c->params.height;
+    const size_t maxSlots = (size_t)sliceSlots * slices;
+    c->sliceSlots = sliceSlots;
     int rc;
     if ((rc = ensure(c->dRayO, maxSlots)) || (rc = ensure(c->dRayD, maxSlots)) || (rc = ensure(c->dHitA, maxSlots)) ||
-        (rc = ensure(c->dHitSlot, maxSlots)) || (rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dShadows, (size_t)maxSlots * 2)) ||
+        (rc = ensure(c->dHitSlot, maxSlots)) || (rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dShadows, maxSlots * 2)) ||
         (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxSlots)) || (rc = ensure(c->dQueue1, maxSlots)) ||
-        (rc = ensure(c->dCounts, kCounterWords)) ||
+        (rc = ensure(c->dCounts, (size_t)kCounterWords * slices)) ||
         (rc = ensure(c->dCtr, (size_t)kCtrWords * kCtrSpread)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
         (rc = ensure(c->dNormal, px * 3)) || (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) || (rc = ensure(c->dAlbedo, px * 3)) ||
         (rc = ensure(c->dVar, px)) || (rc = ensure(c->dSamples, px)))
@@ -1677,85 +1698,113 @@ int alloc_frame(sr_ctx* c, uint32_t maxSlots) {
     return SR_OK;
 }
 
-void mark(sr_ctx* c, int kind) {
+
+hipStream_t slice_stream(const sr_ctx* c, uint32_t slice) { return slice == 0 ? c->stream : c->sliceStreams[slice - 1]; }
+
+void mark(sr_ctx* c, uint32_t slice, int kind) {
     if (!c->profileOn) return;
-    if (c->profUsed == c->profEvents.size()) {
+    sr_ctx::ProfLane& l = c->prof[slice];
+    if (l.used == l.ev.size()) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return;
-        c->profEvents.push_back(e);
-        c->profKinds.push_back(kind);
+        l.ev.push_back(e);
+        l.kind.push_back(kind);
     }
-    c->profKinds[c->profUsed] = kind;
-    (void)hipEventRecord(c->profEvents[c->profUsed++], c->stream);
+    l.kind[l.used] = kind;
+    (void)hipEventRecord(l.ev[l.used++], slice_stream(c, slice));
 }
 
-// Enqueues one band: sample 0's camera rays, then per sample (bounces + 1) segments of extend,
-// shade and shadow followed by the accumulate / next-camera-ray pass. Queues that run empty
-// (converged pixels, terminated paths) make the remaining launches exit at once.
+// One slice of a band: its frame state, stream and launch shapes. Traversal and shade launches
+// get one block per queue chunk of the largest possible queue (blocks past the live count exit at
+// once): the dispatcher then hands chunks to whichever CU frees up, where a grid-stride loop over
+// a resident-sized grid fixes each wave's share in advance and the launch waits for the unluckiest
+// wave (trace 207 -> 188 ms, shade 85.5 -> 82.8 ms per 1080p frame; one-wave trace blocks,
+// 188 -> 179 ms, free a CU slot per finished wave). A trace launch holds at most nSlots rays
+// (padded to whole waves) plus nSlots shadow rays; a shade launch at most nSlots rays.
+struct SliceRun {
+    Frame F;
+    uint32_t slice;
+    hipStream_t s;
+    int slotBlocks, gTrace, gShade, gTail;
+};
+
+SliceRun slice_run(sr_ctx* c, const Frame& F, uint32_t slice) {
+    SliceRun r;
+    r.F = F;
+    r.slice = slice;
+    r.s = slice_stream(c, slice);
+    r.slotBlocks = (int)((F.nSlots + kBlock - 1) / kBlock);
+    r.gTrace = (int)((2u * F.nSlots + 64u + kTraceBlock - 1) / kTraceBlock);
+    r.gShade = (int)((F.nSlots + kShadeBlock - 1) / kShadeBlock);
+    r.gTail = c->gridCUs * 8;
+    return r;
+}
+
+// Sample 0's camera rays of a slice.
+template <class R>
+void enqueue_start(sr_ctx* c, const SliceRun& r) {
+    mark(c, r.slice, 3);
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, r.s, r.F);
+    hipLaunchKernelGGL(k_start<R>, dim3(r.slotBlocks), dim3(kBlock), 0, r.s, r.F);
+    mark(c, r.slice, 3);
+}
+
+// One sample of a slice: (bounces + 1) segments of extend, shade and shadow (the segments from the
+// tail segment on in one fused launch), then the accumulate / next-camera-ray pass. Queues that run
+// empty (converged pixels, terminated paths) make the remaining launches exit at once.
 template <class R, bool COUNT>
-int run_band(sr_ctx* c, const Frame& F) {
-    hipStream_t s = c->stream;
-    const int slotBlocks = (int)((F.nSlots + kBlock - 1) / kBlock);
-    const int g = c->gridCUs * 8;
-    // Traversal and shade launches get one block per queue chunk of the largest possible queue
-    // (blocks past the live count exit at once): the dispatcher then hands chunks to whichever CU
-    // frees up, where a grid-stride loop over a resident-sized grid fixes each wave's share in
-    // advance and the launch waits for the unluckiest wave (trace 207 -> 188 ms, shade 85.5 ->
-    // 82.8 ms per 1080p frame; one-wave trace blocks, 188 -> 179 ms, free a CU slot per finished
-    // wave). A trace launch holds at most nSlots rays (padded to whole waves) plus nSlots shadow
-    // rays; a shade launch at most nSlots rays.
-    const int gTrace = (int)((2u * F.nSlots + 64u + kTraceBlock - 1) / kTraceBlock);
-    const int gShade = (int)((F.nSlots + kShadeBlock - 1) / kShadeBlock);
-    const int gTail = g;
-    mark(c, 3);
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, F);
-    hipLaunchKernelGGL(k_start<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
-    mark(c, 3);
+void enqueue_sample(sr_ctx* c, const SliceRun& r) {
+    const Frame& F = r.F;
+    const hipStream_t s = r.s;
     auto trace = c->dev.anyTransparent ? k_trace<COUNT, true> : k_trace<COUNT, false>;
     auto tail = c->dev.anyTransparent ? k_tail<R, COUNT, true> : k_tail<R, COUNT, false>;
-    for (uint32_t smp = 0; smp < F.maxS; ++smp) {
-        int cur = 0;
-        bool fused = false;
-        for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
-            hipLaunchKernelGGL(trace, dim3(gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
-            mark(c, 0);
-            if (seg == c->tailSegment) {  // the rest of every path in one launch
-                hipLaunchKernelGGL(tail, dim3(gTail), dim3(kBlock), trav_lds_bytes(kBlock), s, F, c->dev, cur);
-                mark(c, 2);
-                hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, F);
-                fused = true;
-                break;
-            }
-            hipLaunchKernelGGL(k_shade<R>, dim3(gShade), dim3(kShadeBlock), 0, s, F, c->dev, cur);
-            mark(c, 1);
-            cur ^= 1;
+    int cur = 0;
+    bool fused = false;
+    for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
+        hipLaunchKernelGGL(trace, dim3(r.gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
+        mark(c, r.slice, 0);
+        if (seg == c->tailSegment) {  // the rest of every path in one launch
+            hipLaunchKernelGGL(tail, dim3(r.gTail), dim3(kBlock), trav_lds_bytes(kBlock), s, F, c->dev, cur);
+            mark(c, r.slice, 2);
+            hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, F);
+            fused = true;
+            break;
         }
-        if (!fused) {  // the last segment's shadow rays (its ray queue is empty)
-            hipLaunchKernelGGL(trace, dim3(gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
-            mark(c, 2);
-        }
-        hipLaunchKernelGGL(k_next<R>, dim3(slotBlocks), dim3(kBlock), 0, s, F);
-        mark(c, 3);
-        c->iterations++;
+        hipLaunchKernelGGL(k_shade<R>, dim3(r.gShade), dim3(kShadeBlock), 0, s, F, c->dev, cur);
+        mark(c, r.slice, 1);
+        cur ^= 1;
     }
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s));
-    return SR_OK;
+    if (!fused) {  // the last segment's shadow rays (its ray queue is empty)
+        hipLaunchKernelGGL(trace, dim3(r.gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
+        mark(c, r.slice, 2);
+    }
+    hipLaunchKernelGGL(k_next<R>, dim3(r.slotBlocks), dim3(kBlock), 0, s, F);
+    mark(c, r.slice, 3);
 }
 
+// Every band as concurrent row slices, one stream each. The host enqueues sample by sample across
+// the slices, so every stream has work queued from the start; the band is done when all are.
 template <class R, bool COUNT>
 int run_frame(sr_ctx* c) {
-    const sr_render_params& p = c->params;
     for (size_t b = 0; b < c->bandEnd.size(); ++b) {
         const uint32_t y0 = b == 0 ? 0 : c->bandEnd[b - 1];
-        const Frame F = frame_for(c, y0, c->bandEnd[b] - y0);
-        int rc = run_band<R, COUNT>(c, F);
-        if (rc) return rc;
+        std::vector<SliceRun> runs;
+        for (uint32_t j = 0; j < c->sliceCount; ++j) {
+            const uint32_t sy = y0 + j * c->sliceRows;
+            if (sy >= c->bandEnd[b]) break;
+            runs.push_back(slice_run(c, frame_for(c, sy, std::min(c->sliceRows, c->bandEnd[b] - sy), j), j));
+        }
+        for (const SliceRun& r : runs) enqueue_start<R>(c, r);
+        for (uint32_t smp = 0; smp < c->params.max_samples; ++smp) {
+            for (const SliceRun& r : runs) enqueue_sample<R, COUNT>(c, r);
+            c->iterations++;
+        }
+        HIP_TRY(hipGetLastError());
+        for (const SliceRun& r : runs) HIP_TRY(hipStreamSynchronize(r.s));
         std::lock_guard<std::mutex> lk(c->mu);
         c->bandsDone = b + 1;
         c->cv.notify_all();
     }
-    (void)p;
     return SR_OK;
 }
 
@@ -1766,10 +1815,17 @@ void drive(sr_ctx* c) {
         rc = SR_ERR_HIP;
         msg = "render worker: could not start";
     } else {
+        for (uint32_t j = 1; j < c->sliceCount && rc == SR_OK; ++j)
+            if (hipStreamWaitEvent(slice_stream(c, j), c->evStart, 0) != hipSuccess) {
+                rc = SR_ERR_HIP;
+                msg = "render worker: could not order the slice streams";
+            }
         const bool counter = c->params.rng_policy == SR_RNG_COUNTER;
-        if (counter) rc = c->countersOn ? run_frame<CounterRng, true>(c) : run_frame<CounterRng, false>(c);
-        else rc = c->countersOn ? run_frame<XorwowRng, true>(c) : run_frame<XorwowRng, false>(c);
-        if (rc) msg = g_err;
+        if (rc == SR_OK) {
+            if (counter) rc = c->countersOn ? run_frame<CounterRng, true>(c) : run_frame<CounterRng, false>(c);
+            else rc = c->countersOn ? run_frame<XorwowRng, true>(c) : run_frame<XorwowRng, false>(c);
+            if (rc) msg = g_err;
+        }
         hipError_t e = hipEventRecord(c->evEnd, c->stream);
         if (e == hipSuccess) e = hipEventSynchronize(c->evEnd);
         if (!rc && e != hipSuccess) { rc = SR_ERR_HIP; msg = hipGetErrorString(e); }
@@ -1813,14 +1869,24 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
     const uint32_t bands = std::min(ctx->params.tick_bands, (p->height + 7) / 8);
     const uint32_t rowsPer = ((p->height + bands - 1) / bands + 7) / 8 * 8;
     const uint32_t tilesX = (p->width + 7) / 8;
-    const uint32_t maxSlots = tilesX * ((rowsPer + 7) / 8) * 64;
-    int rc = alloc_frame(ctx, maxSlots);
+    // Each band runs as up to `slices` concurrent slices of whole 8-row tile rows.
+    const uint32_t tileRows = rowsPer / 8;
+    ctx->sliceCount = std::max(1u, std::min(ctx->slices, tileRows));
+    ctx->sliceRows = (tileRows + ctx->sliceCount - 1) / ctx->sliceCount * 8;
+    ctx->sliceCount = (rowsPer + ctx->sliceRows - 1) / ctx->sliceRows;
+    while (ctx->sliceStreams.size() + 1 < ctx->sliceCount) {
+        hipStream_t st;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        ctx->sliceStreams.push_back(st);
+    }
+    int rc = alloc_frame(ctx, tilesX * (ctx->sliceRows / 8) * 64, ctx->sliceCount);
     if (rc) return rc;
     if (!ctx->evStart) HIP_TRY(hipEventCreate(&ctx->evStart));
     if (!ctx->evEnd) HIP_TRY(hipEventCreate(&ctx->evEnd));
     if (ctx->countersOn)
         HIP_TRY(hipMemsetAsync(ctx->dCtr.ptr, 0, (size_t)kCtrWords * kCtrSpread * sizeof(unsigned long long), ctx->stream));
-    ctx->profUsed = 0;
+    ctx->prof.resize(std::max<size_t>(ctx->prof.size(), ctx->sliceCount));
+    for (sr_ctx::ProfLane& l : ctx->prof) l.used = 0;
     ctx->iterations = 0;
     ctx->view = make_view(view, p->width, p->height);
     ctx->bandEnd.clear();
@@ -1923,13 +1989,59 @@ int sr_profile_read(sr_ctx* ctx, double* ms, uint32_t* launches) {
     int rc = wait_render(ctx);
     if (rc) return rc;
     for (int k = 0; k < 4; ++k) { ms[k] = 0.0; launches[k] = 0; }
-    for (size_t i = 1; i < ctx->profUsed; ++i) {
-        float t = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&t, ctx->profEvents[i - 1], ctx->profEvents[i]));
-        const int k = ctx->profKinds[i];
-        ms[k] += t;
-        launches[k] += 1;
+    for (uint32_t j = 0; j < ctx->sliceCount; ++j) {
+        const sr_ctx::ProfLane& l = ctx->prof[j];
+        for (size_t i = 1; i < l.used; ++i) {
+            float t = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&t, l.ev[i - 1], l.ev[i]));
+            const int k = l.kind[i];
+            ms[k] += t;
+            launches[k] += 1;
+        }
     }
+    return SR_OK;
+}
+
+int sr_profile_busy(sr_ctx* ctx, double* ms) {
+    if (!ctx || !ms) return fail(SR_ERR_INVALID, "null argument");
+    if (!ctx->rendering || !ctx->profileOn) return fail(SR_ERR_STATE, "kernel timing was not enabled");
+    int rc = wait_render(ctx);
+    if (rc) return rc;
+    // Each launch spans [t(i-1), t(i)] on its slice's clock (times from the render's start event);
+    // per kind, the union of those spans over all slices.
+    std::vector<std::pair<float, float>> iv[4];
+    for (uint32_t j = 0; j < ctx->sliceCount; ++j) {
+        const sr_ctx::ProfLane& l = ctx->prof[j];
+        float prev = 0.0f;
+        for (size_t i = 0; i < l.used; ++i) {
+            float t = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&t, ctx->evStart, l.ev[i]));
+            if (i > 0) iv[l.kind[i]].push_back({prev, t});
+            prev = t;
+        }
+    }
+    for (int k = 0; k < 4; ++k) {
+        std::sort(iv[k].begin(), iv[k].end());
+        double total = 0.0;
+        float a = 0.0f, b = -1.0f;
+        for (const auto& x : iv[k]) {
+            if (x.first > b) {
+                if (b > a) total += b - a;
+                a = x.first;
+                b = x.second;
+            } else if (x.second > b) {
+                b = x.second;
+            }
+        }
+        if (b > a) total += b - a;
+        ms[k] = total;
+    }
+    return SR_OK;
+}
+
+int sr_set_slices(sr_ctx* ctx, uint32_t slices) {
+    if (!ctx || slices == 0) return fail(SR_ERR_INVALID, "set_slices: need a context and at least one slice");
+    ctx->slices = slices;
     return SR_OK;
 }
 

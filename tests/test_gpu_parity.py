@@ -632,3 +632,28 @@ def test_speculative_traversal_matches_counting_render():
     print(f"plain_gpu 1/4 scale {spec.width}x{spec.height}, {spec.max_samples} spp", flush=True)
     for a, b in zip(out[False], out[True]):
         assert np.array_equal(bits(a), bits(b))
+
+
+@pytest.mark.parametrize("slices", [2, 3, 5])
+def test_concurrent_slices_match_single_wavefront(slices):
+    """A band run as concurrent row slices (own queues and streams) gives bit-identical AOVs and
+    counters to the band run as one wavefront, including slice counts that do not divide the tile
+    rows and a height that is not a multiple of 8."""
+    from sharpray_amd import RNG_COUNTER, Renderer, scenes
+    spec = scenes.config_plain_gpu(scale=0.25, spp=4)
+    spec.height = 268
+    scene, view = scenes.instantiate(spec)
+    scene.build()
+    out = {}
+    for k in (1, slices):
+        scene.set_slices(k)
+        r = Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
+                     spec.variance_threshold, spec.bounces, spec.indirect_clamp, counters=True,
+                     rng_policy=RNG_COUNTER, rng_seed=3)
+        r.wait()
+        out[k] = (r.radiance.copy(), r.radiance_fog.copy(), r.depth.copy(), r.normals.copy(), r.samples.copy(),
+                  r.counters())
+    print(f"plain_gpu {spec.width}x{spec.height}, {spec.max_samples} spp, {slices} slices", flush=True)
+    for a, b in zip(out[1][:5], out[slices][:5]):
+        assert np.array_equal(bits(a), bits(b)) if a.dtype == np.float32 else np.array_equal(a, b)
+    assert out[1][5] == out[slices][5]
