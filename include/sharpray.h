@@ -285,7 +285,7 @@ int sr_counters_read_reference_only(sr_ctx* ctx, int64_t* out);
 int sr_set_tail_segment(sr_ctx* ctx, uint32_t segment);
 /* Later renders run every band as `slices` concurrent row slices (whole 8-row tile rows), each with
  * its own queues, path records and HIP stream, so the launch tails of one slice overlap the work of
- * the others (default 1, one wavefront per band; 2 measured 5 % faster on the 1080p frame; results
+ * the others (default 1, one wavefront per band; 2 measured 6 % faster on the 1080p frame; results
  * are identical either way). The
  * environment variable SHARPRAY_SLICES sets the default of new contexts. No counterpart in the
  * reference (its worker threads claim 32x32 blocks, renderer.cs:122-151). */

@@ -1121,7 +1121,7 @@ struct sr_ctx {
     int gridCUs = 256;
     // Concurrent row slices of every band: each slice has its own queues, path records and
     // stream, so one slice's launch tails and fused tail run beside the others' work (one 1080p
-    // frame: 278 ms as one slice, 263 ms as 2 or 4, 325 ms as 6). The default keeps one slice, so
+    // frame: 252 ms as one slice, 237 ms as 2). The default keeps one slice, so
     // every kernel launch has the GPU to itself and its timing measures that kernel alone.
     // SHARPRAY_SLICES or sr_set_slices override it.
     uint32_t slices = 1;
