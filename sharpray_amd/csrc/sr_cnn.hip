@@ -165,6 +165,74 @@ __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs A) {
     }
 }
 
+// Wide layers split by output channels: wave w of the 4-wave workgroup computes the whole
+// 16 x MT tile for output channels [32 w, 32 w + 32) (two MFMA N tiles), so each wave loads only its
+// own weight fragments and the block's pixels are staged once. Used for the 100-logit head
+// (128 padded channels).
+template <int MT>
+__global__ void __launch_bounds__(256) k_conv3x3_split(ConvArgs A) {
+    constexpr int kHaloPx = kHW * (MT + 2);
+    constexpr int kHaloWords = kHaloPx * 4;
+    constexpr int kOutWords = 4 * MT * 16 * 4;  // 4 waves x MT*16 pixels x 32 channels of bf16
+    __shared__ uint4 smem[kHaloWords > kOutWords ? kHaloWords : kOutWords];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tx0 = blockIdx.x * kTW, ty0 = blockIdx.y * MT;
+    f32x4 acc[MT][2];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[mt][0] = acc[mt][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int c0 = 0; c0 < A.cinPad; c0 += 32) {
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < kHaloWords; idx += 256) {
+            const int p = idx >> 2, part = idx & 3;
+            smem[idx] = fetch8(A, c0 + part * 8, ty0 - 1 + p / kHW, tx0 - 1 + p % kHW);
+        }
+        __syncthreads();
+        const uint16_t* wc = A.w + c0 + (lane >> 4) * 8;
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
+            const uint16_t* wt = wc + (size_t)tap * A.coutPad * A.cinPad;
+            bf16x8 bfr[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                bfr[h] = __builtin_bit_cast(
+                    bf16x8, *(const uint4*)(wt + (size_t)((2 * wave + h) * 16 + (lane & 15)) * A.cinPad));
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int p = (mt + dy) * kHW + (lane & 15) + dx;
+                const bf16x8 af = __builtin_bit_cast(bf16x8, smem[p * 4 + (lane >> 4)]);
+                acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[0], acc[mt][0], 0, 0, 0);
+                acc[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[1], acc[mt][1], 0, 0, 0);
+            }
+        }
+    }
+    // Epilogue: bias, ReLU and bf16 rounding into the wave's LDS region as [pixel][32 channels],
+    // then 16-byte stores of each pixel's 32-channel group (channels past cout are written as 0).
+    __syncthreads();  // the halo is no longer read
+    uint16_t* ob = (uint16_t*)smem + (size_t)wave * MT * 16 * 32;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int co = 32 * wave + 16 * h + (lane & 15);
+        const bool live = co < A.cout;
+        const float bias = live ? A.bias[co] : 0.0f;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float v = acc[mt][h][r] + bias;
+                if (A.relu) v = v > 0.0f ? v : 0.0f;
+                ob[(mt * 16 + 4 * (lane >> 4) + r) * 32 + h * 16 + (lane & 15)] = live ? f2bf(v) : (uint16_t)0;
+            }
+    }
+    __syncthreads();
+    const uint4* ow = smem + (size_t)wave * MT * 16 * 4;
+    for (int idx = lane; idx < MT * 16 * 4; idx += 64) {
+        const int pl = idx >> 2, part = idx & 3;
+        const int y = ty0 + pl / 16, x = tx0 + pl % 16;
+        if (y < A.H && x < A.W) ((uint4*)(A.out + ((size_t)y * A.W + x) * A.coutPad + 32 * wave))[part] = ow[idx];
+    }
+}
+
 // Input features: log(1 + radiance), albedo, normal, 1 / (1 + depth) (0 without a surface).
 __global__ void k_features(const float* rad, const float* alb, const float* nrm, const float* dep, int W, int H,
                            uint16_t* out) {
@@ -621,6 +689,14 @@ constexpr int kSmallMT = SR_CONV_SMALL_MT;  // rows per wave of the 32-channel (
 #ifndef SR_HEAD_MT
 #define SR_HEAD_MT 4
 #endif
+// The head runs as k_conv3x3_split with 4 rows (305 us at 1080p; 8 rows 318 us, the 7-tile
+// k_conv3x3 with 4 rows per wave 349 us).
+#ifndef SR_HEAD_SPLIT
+#define SR_HEAD_SPLIT 1
+#endif
+#ifndef SR_HEAD_SPLIT_MT
+#define SR_HEAD_SPLIT_MT 4
+#endif
 #ifndef SR_MT3
 #define SR_MT3 8
 #endif
@@ -648,7 +724,14 @@ bool launch_conv(const ConvArgs& A, hipStream_t s) {
         case 4: launch_conv_t<4, SR_MT4>(A, s); break;
         case 5: launch_conv_t<5, 2>(A, s); break;
         case 6: launch_conv_t<6, SR_MT6>(A, s); break;
-        case 7: launch_conv_t<7, SR_HEAD_MT>(A, s); break;
+        case 7:
+            if (SR_HEAD_SPLIT && A.coutPad == 128) {
+                const dim3 grid((A.W + kTW - 1) / kTW, (A.H + SR_HEAD_SPLIT_MT - 1) / SR_HEAD_SPLIT_MT);
+                hipLaunchKernelGGL((k_conv3x3_split<SR_HEAD_SPLIT_MT>), grid, dim3(256), 0, s, A);
+            } else {
+                launch_conv_t<7, SR_HEAD_MT>(A, s);
+            }
+            break;
         case 8: launch_conv_t<8, SR_MT8>(A, s); break;
         case 10: launch_conv_t<10, 2>(A, s); break;
         case 14: launch_conv_t<14, 2>(A, s); break;
