@@ -165,15 +165,14 @@ __global__ void __launch_bounds__(256) k_conv3x3(ConvArgs A) {
     }
 }
 
-// Wide layers split by output channels: wave w of the 4-wave workgroup computes the whole
-// 16 x MT tile for output channels [32 w, 32 w + 32) (two MFMA N tiles), so each wave loads only its
-// own weight fragments and the block's pixels are staged once. Used for the 100-logit head
-// (128 padded channels).
-template <int MT>
-__global__ void __launch_bounds__(256) k_conv3x3_split(ConvArgs A) {
+// Layers split by output channels: wave w of the NW-wave workgroup (NW = coutPad / 32) computes
+// the whole 16 x MT tile for output channels [32 w, 32 w + 32) (two MFMA N tiles), so each wave
+// loads only its own weight fragments and the block's pixels are staged once.
+template <int MT, int NW>
+__global__ void __launch_bounds__(NW * 64) k_conv3x3_split(ConvArgs A) {
     constexpr int kHaloPx = kHW * (MT + 2);
     constexpr int kHaloWords = kHaloPx * 4;
-    constexpr int kOutWords = 4 * MT * 16 * 4;  // 4 waves x MT*16 pixels x 32 channels of bf16
+    constexpr int kOutWords = NW * MT * 16 * 4;  // NW waves x MT*16 pixels x 32 channels of bf16
     __shared__ uint4 smem[kHaloWords > kOutWords ? kHaloWords : kOutWords];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx0 = blockIdx.x * kTW, ty0 = blockIdx.y * MT;
@@ -182,7 +181,7 @@ __global__ void __launch_bounds__(256) k_conv3x3_split(ConvArgs A) {
     for (int mt = 0; mt < MT; ++mt) acc[mt][0] = acc[mt][1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int c0 = 0; c0 < A.cinPad; c0 += 32) {
         __syncthreads();
-        for (int idx = threadIdx.x; idx < kHaloWords; idx += 256) {
+        for (int idx = threadIdx.x; idx < kHaloWords; idx += NW * 64) {
             const int p = idx >> 2, part = idx & 3;
             smem[idx] = fetch8(A, c0 + part * 8, ty0 - 1 + p / kHW, tx0 - 1 + p % kHW);
         }
@@ -689,13 +688,31 @@ constexpr int kSmallMT = SR_CONV_SMALL_MT;  // rows per wave of the 32-channel (
 #ifndef SR_HEAD_MT
 #define SR_HEAD_MT 4
 #endif
-// The head runs as k_conv3x3_split with 4 rows (305 us at 1080p; 8 rows 318 us, the 7-tile
-// k_conv3x3 with 4 rows per wave 349 us).
+// Layers of 64 to 128 output channels run as k_conv3x3_split (each wave owns 32 output channels
+// of the block's tile), measured per layer at 1080p against k_conv3x3: the 100-logit head with 4
+// rows 295 vs 349 us (8 rows 318 us), 128 channels with 4 rows L9 22 vs 45 us, 96 channels with 4
+// rows L10 42 vs 53 us (8 rows 48 us), 64 channels with 8 rows L5 22 vs 30 us (4 rows 28 us). The
+// 48-channel layers stay on k_conv3x3 (split into 2 waves with 16 padded channels: L14 140 vs 105 us).
 #ifndef SR_HEAD_SPLIT
 #define SR_HEAD_SPLIT 1
 #endif
 #ifndef SR_HEAD_SPLIT_MT
 #define SR_HEAD_SPLIT_MT 4
+#endif
+#ifndef SR_SPLIT128
+#define SR_SPLIT128 1
+#endif
+#ifndef SR_SPLIT96
+#define SR_SPLIT96 1
+#endif
+#ifndef SR_SPLIT64
+#define SR_SPLIT64 1
+#endif
+#ifndef SR_SPLIT96_MT
+#define SR_SPLIT96_MT 4
+#endif
+#ifndef SR_SPLIT64_MT
+#define SR_SPLIT64_MT 8
 #endif
 #ifndef SR_MT3
 #define SR_MT3 8
@@ -710,6 +727,12 @@ constexpr int kSmallMT = SR_CONV_SMALL_MT;  // rows per wave of the 32-channel (
 #define SR_MT8 2
 #endif
 
+template <int MT, int NW>
+void launch_split(const ConvArgs& A, hipStream_t s) {
+    const dim3 grid((A.W + kTW - 1) / kTW, (A.H + MT - 1) / MT);
+    hipLaunchKernelGGL((k_conv3x3_split<MT, NW>), grid, dim3(NW * 64), 0, s, A);
+}
+
 template <int NT, int MT>
 void launch_conv_t(const ConvArgs& A, hipStream_t s) {
     const dim3 grid((A.W + kTW - 1) / kTW, (A.H + 4 * MT - 1) / (4 * MT));
@@ -721,18 +744,23 @@ bool launch_conv(const ConvArgs& A, hipStream_t s) {
     switch ((A.cout + 15) / 16) {
         case 2: launch_conv_t<2, kSmallMT>(A, s); break;
         case 3: launch_conv_t<3, SR_MT3>(A, s); break;
-        case 4: launch_conv_t<4, SR_MT4>(A, s); break;
-        case 5: launch_conv_t<5, 2>(A, s); break;
-        case 6: launch_conv_t<6, SR_MT6>(A, s); break;
-        case 7:
-            if (SR_HEAD_SPLIT && A.coutPad == 128) {
-                const dim3 grid((A.W + kTW - 1) / kTW, (A.H + SR_HEAD_SPLIT_MT - 1) / SR_HEAD_SPLIT_MT);
-                hipLaunchKernelGGL((k_conv3x3_split<SR_HEAD_SPLIT_MT>), grid, dim3(256), 0, s, A);
-            } else {
-                launch_conv_t<7, SR_HEAD_MT>(A, s);
-            }
+        case 4:
+            if (SR_SPLIT64 && A.coutPad == 64) launch_split<SR_SPLIT64_MT, 2>(A, s);
+            else launch_conv_t<4, SR_MT4>(A, s);
             break;
-        case 8: launch_conv_t<8, SR_MT8>(A, s); break;
+        case 5: launch_conv_t<5, 2>(A, s); break;
+        case 6:
+            if (SR_SPLIT96 && A.coutPad == 96) launch_split<SR_SPLIT96_MT, 3>(A, s);
+            else launch_conv_t<6, SR_MT6>(A, s);
+            break;
+        case 7:
+            if (SR_HEAD_SPLIT && A.coutPad == 128) launch_split<SR_HEAD_SPLIT_MT, 4>(A, s);
+            else launch_conv_t<7, SR_HEAD_MT>(A, s);
+            break;
+        case 8:
+            if (SR_SPLIT128 && A.coutPad == 128) launch_split<SR_HEAD_SPLIT_MT, 4>(A, s);
+            else launch_conv_t<8, SR_MT8>(A, s);
+            break;
         case 10: launch_conv_t<10, 2>(A, s); break;
         case 14: launch_conv_t<14, 2>(A, s); break;
         default: return false;
