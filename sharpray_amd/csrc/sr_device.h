@@ -50,6 +50,7 @@ struct DevSky {
     quat rot, rotInv;
     int32_t w, h;
     int32_t cdfOff;  // marginal CDF (h+1 floats) followed by h conditional CDFs (w+1 floats each)
+    int32_t guideOff;  // -1, or the CDFs' bucket bounds (uint32 bits in the same pool, see cdf_sample_guided)
     int32_t sampler;  // 0: row/column CDFs, 1: alias table (DevScene::alias)
     float total, pdfScale;
     vec3 top, middle, bottom, sunDir, sunRad;
@@ -412,6 +413,25 @@ SR_DEV uint32_t cdf_sample(const float* cdf, uint32_t n, float x) {
     }
     return lo < n - 1 ? lo : n - 1;
 }
+// Bucket bounds of a CDF table: with m buckets (a power of two), entry j < m is the search's
+// unclamped answer for x = j / m and entry m is n. The marginal CDF has kGuideMarg buckets, each
+// conditional CDF kGuideCond.
+constexpr uint32_t kGuideMarg = 1024, kGuideCond = 512;
+// Cdf1.Sample with the search confined to the bounds of x's bucket j = floor(x m): x lies in
+// [j / m, (j + 1) / m), the predicate cdf[i + 1] <= x falls from true to false once along i (the
+// table does not decrease), so its first false index lies between the bucket's bounds and the
+// confined search returns the index of the full one in about log2(n / m) steps instead of log2(n).
+SR_DEV uint32_t cdf_sample_guided(const float* cdf, uint32_t n, float x, const uint32_t* g, uint32_t m) {
+    const float s = x * (float)m;
+    const uint32_t j = s >= 0.0f ? (s < (float)m ? (uint32_t)s : m - 1) : 0u;
+    uint32_t lo = g[j], hi = g[j + 1];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cdf[mid + 1] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < n - 1 ? lo : n - 1;
+}
 template <class R>
 SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf, uint32_t* texelOut = nullptr) {
     const DevSky& k = S.sky;
@@ -432,9 +452,17 @@ SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf, uin
             pdf = keep ? e.z : e.w;
         } else {
             const float* marg = S.cdf + k.cdfOff;
-            y = cdf_sample(marg, (uint32_t)k.h, c.y);
-            const float* cond = marg + (k.h + 1) + (size_t)y * (k.w + 1);
-            x = cdf_sample(cond, (uint32_t)k.w, c.x);
+            if (k.guideOff >= 0) {
+                const uint32_t* G = (const uint32_t*)(S.cdf + k.guideOff);
+                y = cdf_sample_guided(marg, (uint32_t)k.h, c.y, G, kGuideMarg);
+                const float* cond = marg + (k.h + 1) + (size_t)y * (k.w + 1);
+                x = cdf_sample_guided(cond, (uint32_t)k.w, c.x, G + (kGuideMarg + 1) + (size_t)y * (kGuideCond + 1),
+                                      kGuideCond);
+            } else {
+                y = cdf_sample(marg, (uint32_t)k.h, c.y);
+                const float* cond = marg + (k.h + 1) + (size_t)y * (k.w + 1);
+                x = cdf_sample(cond, (uint32_t)k.w, c.x);
+            }
             const DevTexture t = S.textures[k.tex];
             pdf = luminance(tex_get(S, t, y * (uint32_t)k.w + x)) * k.pdfScale;
         }

@@ -1308,6 +1308,41 @@ void build_sky(sr_ctx* ctx, DevSky& K, std::vector<float>& cdf, std::vector<floa
         }
         K.total = build_cdf1(rowW.data(), H, cdf.data());
         K.pdfScale = (float)(W * H) / (K.total * 2.0f * kPi * kPi);
+        // Bucket bounds of every CDF (cdf_sample_guided), kept only when no table decreases (weights
+        // are non-negative unless a texture holds negative values). SHARPRAY_CDF_GUIDE=0 leaves
+        // them out (the full searches pick the same texels).
+        const char* guideEnv = getenv("SHARPRAY_CDF_GUIDE");
+        bool monotone = !(guideEnv && strcmp(guideEnv, "0") == 0);
+        for (size_t i = 1; i < cdf.size() && monotone; ++i) {
+            const bool rowStart = i > (size_t)H && (i - (size_t)(H + 1)) % (size_t)(W + 1) == 0;
+            if (i != (size_t)(H + 1) && !rowStart && !(cdf[i] >= cdf[i - 1])) monotone = false;
+        }
+        K.guideOff = -1;
+        if (monotone) {
+            auto first_false = [](const float* c, uint32_t n, float x) {
+                uint32_t lo = 0, hi = n;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (c[mid + 1] <= x) lo = mid + 1;
+                    else hi = mid;
+                }
+                return lo;
+            };
+            auto put_bounds = [&](const float* c, uint32_t n, uint32_t m, std::vector<float>& out) {
+                for (uint32_t j = 0; j < m; ++j) {
+                    const uint32_t b = first_false(c, n, (float)j / (float)m);
+                    out.push_back(__builtin_bit_cast(float, b));
+                }
+                out.push_back(__builtin_bit_cast(float, n));
+            };
+            std::vector<float> guide;
+            guide.reserve((size_t)(kGuideMarg + 1) + (size_t)H * (kGuideCond + 1));
+            put_bounds(cdf.data(), (uint32_t)H, kGuideMarg, guide);
+            for (int y = 0; y < H; ++y)
+                put_bounds(cdf.data() + (H + 1) + (size_t)y * (W + 1), (uint32_t)W, kGuideCond, guide);
+            K.guideOff = (int32_t)cdf.size();
+            cdf.insert(cdf.end(), guide.begin(), guide.end());
+        }
         // Alias table over the same weights; each entry carries the pdf LightDirRand returns for
         // either outcome (luminance * pdfScale, scene.cs:414).
         std::vector<float> prob(n);

@@ -659,6 +659,29 @@ def test_concurrent_slices_match_single_wavefront(slices):
     assert out[1][5] == out[slices][5]
 
 
+def test_cdf_guide_tables_bit_identical(monkeypatch):
+    """Environment-light sampling confined to the CDFs' bucket bounds picks the texels of the full
+    binary searches: a textured-sky render with and without the tables (SHARPRAY_CDF_GUIDE=0 when
+    the scene is built) gives bit-identical AOVs and counters."""
+    from sharpray_amd import Renderer, scenes
+    spec = scenes.config_plain_gpu(scale=0.25, spp=4)
+    assert spec.sky_texture >= 0
+    out = []
+    for guide in ("1", "0"):
+        monkeypatch.setenv("SHARPRAY_CDF_GUIDE", guide)
+        scene, view = scenes.instantiate(spec)
+        scene.build()
+        r = Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
+                     spec.variance_threshold, spec.bounces, spec.indirect_clamp, counters=True)
+        r.wait()
+        out.append((r.radiance.copy(), r.radiance_fog.copy(), r.depth.copy(), r.normals.copy(), r.samples.copy(),
+                    r.counters()))
+    print(f"plain_gpu {spec.width}x{spec.height}, {spec.max_samples} spp, guide tables on / off", flush=True)
+    for a, b in zip(out[0][:5], out[1][:5]):
+        assert np.array_equal(bits(a), bits(b)) if a.dtype == np.float32 else np.array_equal(a, b)
+    assert out[0][5] == out[1][5]
+
+
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (13, 9), (9, 17)])
 def test_odd_image_sizes_match_oracle(size, oracle_lib):
     """Images smaller than one 8x8 tile or with partial tiles on both edges: every pixel is rendered
