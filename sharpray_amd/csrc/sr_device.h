@@ -416,7 +416,10 @@ SR_DEV uint32_t cdf_sample(const float* cdf, uint32_t n, float x) {
 // Bucket bounds of a CDF table: with m buckets (a power of two), entry j < m is the search's
 // unclamped answer for x = j / m and entry m is n. The marginal CDF has kGuideMarg buckets, each
 // conditional CDF kGuideCond.
-constexpr uint32_t kGuideMarg = 1024, kGuideCond = 512;
+#ifndef SR_GUIDE_COND
+#define SR_GUIDE_COND 512
+#endif
+constexpr uint32_t kGuideMarg = 1024, kGuideCond = SR_GUIDE_COND;
 // Cdf1.Sample with the search confined to the bounds of x's bucket j = floor(x m): x lies in
 // [j / m, (j + 1) / m), the predicate cdf[i + 1] <= x falls from true to false once along i (the
 // table does not decrease), so its first false index lies between the bucket's bounds and the
