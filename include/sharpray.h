@@ -100,13 +100,14 @@ typedef struct sr_view {
     float near_plane;
 } sr_view;
 
-/* Renderer ctor arguments (renderer.cs:40-51). block_size is kept for API parity; the GPU
- * renders sample-major over whole bands of rows and reports progress in those blocks.
+/* Renderer ctor arguments (renderer.cs:40-51). block_size is the unit of sr_render_tick's progress;
+ * the GPU renders sample-major over whole bands of rows and counts a block as completed once all
+ * its pixels are final.
  * rng_policy: SR_RNG_REFERENCE reproduces the reference's per-pixel stateful xorwow stream
  * (renderer.cs:156); SR_RNG_COUNTER draws from a counter-based hash of (seed, pixel, sample,
  * draw index) so one path's divergence cannot shift another sample's draws (no reference
- * counterpart). tick_bands: number of row bands the frame is split into (>= 1); sr_render_tick
- * returns after each band. light_sampler picks how a texture sky's light directions are drawn:
+ * counterpart). tick_bands: number of row bands the frame is rendered in, one after the other
+ * (>= 1); each band's blocks complete when the band does. light_sampler picks how a texture sky's light directions are drawn:
  * SR_LIGHT_CDF walks the reference's row/column CDFs (Cdf2, cdf.cs:62-98); SR_LIGHT_ALIAS draws
  * the same discrete distribution from a Walker alias table with one 16-byte load (no reference
  * counterpart; same two uniforms, same pdf). */
@@ -188,8 +189,13 @@ int sr_scene_build(sr_ctx* ctx);
 /* Renderer ctor (renderer.cs:40-96): builds the scene if needed, allocates the seven AOVs and
  * starts rendering asynchronously. */
 int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* params);
-/* Renderer.Tick (renderer.cs:98-105): blocks until the next band completes, returns
- * (completed blocks, total blocks); returns (total, total) without blocking once done. */
+/* Renderer.Tick (renderer.cs:98-105): returns (completed blocks, total blocks) in the ctor's
+ * block_size x block_size blocks; a block counts as completed once every one of its pixels has its
+ * final AOVs. Like the reference's semaphore (one release per completed block) each call consumes
+ * one completion: it returns at once while completions are unconsumed and otherwise blocks until
+ * the next block completes; (total, total) without blocking once all are done. The GPU renders
+ * sample-major over whole bands (tick_bands), so blocks complete in bursts: at the end of each
+ * band with fixed sample counts, and as pixels converge with adaptive sampling. */
 int sr_render_tick(sr_ctx* ctx, uint32_t* step, uint32_t* total);
 /* Copies the AOVs of renderer.cs:14-20 to host arrays (any may be null): Radiance and
  * RadianceFog (w*h*3), Normals (w*h*3, zero when no hit), Uv (w*h*2), Depth (w*h, +inf when no
@@ -257,6 +263,20 @@ int sr_trace_primary(sr_ctx* ctx, const sr_view* view, uint32_t width, uint32_t 
  * permutation (bvh.cs:53). Pass null arrays to query node_count / item_count only. */
 int sr_bvh_export(sr_ctx* ctx, int32_t mesh, float* nodes, int32_t* items, uint32_t* node_count,
                   uint32_t* item_count);
+/* Bvh.GetStats (bvh.cs:79-125, the BvhStats record bvh.cs:5-14) of a mesh's BVH (mesh >= 0) or the
+ * scene's (mesh = -1, after sr_scene_build), as Mesh.Describe (mesh.cs:87-96) and the scene dump
+ * (program.cs:139-144) print them. All zero for an empty BVH. */
+typedef struct sr_bvh_stats {
+    int32_t node_count;
+    int32_t depth_max;
+    int32_t leaf_count;
+    int32_t leaf_size_min;
+    int32_t leaf_size_max;
+    float leaf_size_avg;
+    float leaf_depth_avg;
+    float sah_cost;
+} sr_bvh_stats;
+int sr_bvh_stats_get(sr_ctx* ctx, int32_t mesh, sr_bvh_stats* out);
 /* Render-path counters in Counters.Type order subset (counters.cs:24-79):
  * [0] Pixel [1] Sample [2] SampleBounce [3] SampleHit [4] SampleMiss [5] SampleTerminate
  * [6] SampleFogScatter [7] SampleFogEscape [8] SamplePixelConverged [9] SamplePixelMaxed
@@ -265,6 +285,40 @@ int sr_bvh_export(sr_ctx* ctx, int32_t mesh, float* nodes, int32_t* items, uint3
 #define SR_COUNTER_COUNT 15
 int sr_counters_enable(sr_ctx* ctx, int32_t enable);
 int sr_counters_read(sr_ctx* ctx, int64_t* out);
+/* The whole Counters.Type taxonomy (counters.cs:24-79), in the reference's order, accumulated over
+ * the context's work since creation or the last sr_counters_reset, the way the reference's one
+ * Counters object accumulates over a program run. Collected while sr_counters_enable is on:
+ *  - mesh creation: MeshTriangle, TimeMeshBvhBuild (mesh.cs:38-45); scene build: SceneObject,
+ *    SceneBvhNodes, SceneBvhDepth, TimeSceneBvhBuild (scene.cs:528-540); sr_obj_load: TimeObjLoad
+ *    (obj.cs:147);
+ *  - every finished render: Worker (one per concurrent row slice, the GPU analogue of a worker
+ *    thread), Block, the render-path counters of sr_counters_read, SampleVarianceMax,
+ *    SampleCountMin, SampleCountMax (renderer.cs:223-226) and TimeRender (sr_render_begin to the
+ *    last AOV written; program.cs:177 times the same span around Tick);
+ *  - every bilateral / fog filter: FilterSample, DenoiseEarlyOut, DenoiseWeightMax, DenoiseMaxLum,
+ *    DenoiseMaxLumBoost, DenoiseReject* (compose.cs:101-245); sr_compose: TimeCompose.
+ * Sum counters start at 0; the max-reduced ones (SampleVarianceMax, SampleCountMax, Denoise*Max*)
+ * start at INT64_MIN and SampleCountMin at INT64_MAX, the reference's "unset" values
+ * (counters.cs:221-226); float ones hold (long)(value * 1e4) (counters.cs:214-219). Entries the
+ * library has no part in (Overlay*, Rt*, BvhOverlap*, MeshIntersectAny, TimeSetup, TimeTotal) stay
+ * unset for the host's own Counters to fill. Waits for a running render. count: entries to copy
+ * (at most SR_COUNTER_TYPE_COUNT). */
+enum sr_counter_type {
+    SR_CT_WORKER, SR_CT_BLOCK, SR_CT_PIXEL, SR_CT_SAMPLE, SR_CT_SAMPLE_BOUNCE, SR_CT_SAMPLE_HIT, SR_CT_SAMPLE_MISS,
+    SR_CT_SAMPLE_TERMINATE, SR_CT_SAMPLE_FOG_SCATTER, SR_CT_SAMPLE_FOG_ESCAPE, SR_CT_SAMPLE_VARIANCE_MAX,
+    SR_CT_SAMPLE_PIXEL_CONVERGED, SR_CT_SAMPLE_PIXEL_MAXED, SR_CT_SAMPLE_COUNT_MIN, SR_CT_SAMPLE_COUNT_MAX,
+    SR_CT_SCENE_OBJECT, SR_CT_SCENE_BVH_NODES, SR_CT_SCENE_BVH_DEPTH, SR_CT_SCENE_TRACE, SR_CT_SCENE_OCCLUDE,
+    SR_CT_MESH_TRIANGLE, SR_CT_MESH_INTERSECT, SR_CT_MESH_INTERSECT_ANY, SR_CT_BVH_INTERSECT_NODE,
+    SR_CT_BVH_INTERSECT_SHAPE, SR_CT_BVH_OVERLAP_NODE, SR_CT_BVH_OVERLAP_SHAPE, SR_CT_FILTER_SAMPLE,
+    SR_CT_DENOISE_EARLY_OUT, SR_CT_DENOISE_WEIGHT_MAX, SR_CT_DENOISE_MAX_LUM, SR_CT_DENOISE_MAX_LUM_BOOST,
+    SR_CT_DENOISE_REJECT_MISSING, SR_CT_DENOISE_REJECT_NORMAL, SR_CT_DENOISE_REJECT_DEPTH, SR_CT_DENOISE_REJECT_LUM,
+    SR_CT_OVERLAY_LINE, SR_CT_OVERLAY_TEXT, SR_CT_RT_PEAK_WORKING_SET, SR_CT_RT_GC_ALLOCATED_BYTES,
+    SR_CT_RT_GC_CURRENT_BYTES, SR_CT_RT_GC_GEN0, SR_CT_RT_GC_GEN1, SR_CT_RT_GC_GEN2, SR_CT_TIME_SETUP,
+    SR_CT_TIME_OBJ_LOAD, SR_CT_TIME_MESH_BVH_BUILD, SR_CT_TIME_SCENE_BVH_BUILD, SR_CT_TIME_RENDER,
+    SR_CT_TIME_COMPOSE, SR_CT_TIME_TOTAL, SR_COUNTER_TYPE_COUNT
+};
+int sr_counters_read_all(sr_ctx* ctx, int64_t* out, uint32_t count);
+int sr_counters_reset(sr_ctx* ctx);
 /* The share of BvhIntersectNode, BvhIntersectShape, SceneTrace and SceneOcclude (out[0..3]) done by
  * the fused launch that finishes paths from the tail segment on (sr_set_tail_segment); the rest
  * was done by the per-segment traversal launches. No counterpart in the reference. */

@@ -12,7 +12,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 COUNTERS = ["Pixel", "Sample", "SampleBounce", "SampleHit", "SampleMiss", "SampleTerminate", "SampleFogScatter",
             "SampleFogEscape", "SamplePixelConverged", "SamplePixelMaxed", "SceneTrace", "SceneOcclude",
-            "MeshIntersect", "BvhIntersectNode", "BvhIntersectShape", "FilterSample", "DenoiseEarlyOut"]
+            "MeshIntersect", "BvhIntersectNode", "BvhIntersectShape", "FilterSample", "DenoiseEarlyOut",
+            "SampleCountMin", "SampleCountMax", "SampleVarianceMax", "DenoiseRejectMissing", "DenoiseRejectNormal",
+            "DenoiseRejectDepth", "DenoiseRejectLum", "DenoiseWeightMax", "DenoiseMaxLum", "DenoiseMaxLumBoost"]
+# Initial values of the min- and max-reduced counters (counters.cs:221-226): such a counter still at
+# its initial value was never bumped.
+COUNTER_UNSET = {"SampleCountMin": 2 ** 63 - 1, "SampleCountMax": -2 ** 63, "SampleVarianceMax": -2 ** 63,
+                 "DenoiseWeightMax": -2 ** 63, "DenoiseMaxLum": -2 ** 63, "DenoiseMaxLumBoost": -2 ** 63}
 
 _F = ctypes.POINTER(ctypes.c_float)
 _I = ctypes.POINTER(ctypes.c_int32)
@@ -67,6 +73,13 @@ _SIGS = {
                           ctypes.c_float]),
     "oc_filter_fog": (None, [ctypes.c_float, ctypes.c_float, _F, ctypes.c_uint32, ctypes.c_uint32, _F]),
     "oc_tonemap": (None, [_F, _F, ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_char_p]),
+    "oc_scene_trace": (None, [_V, _F, ctypes.c_uint32, _F]),
+    "oc_shadow_transmittance": (None, [_V, _F, ctypes.c_uint32, _F]),
+    "oc_refract": (None, [_F, _F, ctypes.c_float, ctypes.c_uint32, _F]),
+    "oc_russian_roulette": (None, [_F, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _F]),
+    "oc_clamp_luminance": (None, [_F, ctypes.c_float, ctypes.c_uint32, _F]),
+    "oc_beer": (None, [_F, ctypes.c_float, _F]),
+    "oc_relative_std_err": (ctypes.c_float, [ctypes.c_float, ctypes.c_float, ctypes.c_uint32]),
     "oc_sky_eval": (None, [_V, _F, ctypes.c_uint32, _F, _F]),
     "oc_sky_sample": (None, [_V, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _F]),
     "oc_surface_eval": (None, [_F, _F, _F, _F, _F, ctypes.c_uint32, _F]),
@@ -177,6 +190,22 @@ class OracleScene:
         self.L.oc_mesh_intersect(self.h, mesh, f(rays), n, idx.ctypes.data_as(_I), t.ctypes.data_as(_F))
         return idx, t
 
+    def trace(self, rays):
+        """Scene.Trace of rays (n, 6): dict of object, prim, dist, normal, normal_geo, tangent, uv,
+        albedo, roughness, metallic (object -1 on a miss)."""
+        rays = _keep(rays, np.float32).reshape(-1, 6)
+        out = np.empty((rays.shape[0], 20), np.float32)
+        self.L.oc_scene_trace(self.h, f(rays), rays.shape[0], out.ctypes.data_as(_F))
+        return dict(object=out[:, 0].astype(np.int32), prim=out[:, 1].astype(np.int32), dist=out[:, 2],
+                    normal=out[:, 3:6], normal_geo=out[:, 6:9], tangent=out[:, 9:13], uv=out[:, 13:15],
+                    albedo=out[:, 15:18], roughness=out[:, 18], metallic=out[:, 19])
+
+    def shadow_transmittance(self, rays):
+        rays = _keep(rays, np.float32).reshape(-1, 6)
+        out = np.empty((rays.shape[0], 3), np.float32)
+        self.L.oc_shadow_transmittance(self.h, f(rays), rays.shape[0], out.ctypes.data_as(_F))
+        return out
+
     def bvh_stats(self, mesh=None):
         level, mid = (0, 0) if mesh is None else (1, mesh)
         ints = np.zeros(7, np.int32)
@@ -286,9 +315,46 @@ def encode_bmp(rgb, width, height):
 
 def bilateral(params7, radiance, normals, depth, variance, width, height, threads=None, albedo=None,
               albedo_limit=0.0):
+    """Compositor.Filter of a whole image: (filtered (n, 3), the filter's counters by name)."""
     out = np.empty((width * height, 3), np.float32)
-    samples = np.zeros(1, np.int64)
+    ctr = np.zeros(len(COUNTERS), np.int64)
     lib().oc_filter(f(params7), f(radiance), f(normals), f(depth), f(variance), width, height,
-                    threads or os.cpu_count() or 1, out.ctypes.data_as(_F), samples.ctypes.data_as(_L), f(albedo),
+                    threads or os.cpu_count() or 1, out.ctypes.data_as(_F), ctr.ctypes.data_as(_L), f(albedo),
                     float(albedo_limit))
-    return out, int(samples[0])
+    names = ("FilterSample", "DenoiseEarlyOut", "DenoiseRejectMissing", "DenoiseRejectNormal", "DenoiseRejectDepth",
+             "DenoiseRejectLum", "DenoiseWeightMax", "DenoiseMaxLum", "DenoiseMaxLumBoost")
+    full = dict(zip(COUNTERS, ctr.tolist()))
+    return out, {k: full[k] for k in names}
+
+
+def refract(inc, normal, eta):
+    """Brdf.Refract of (n, 3) directions: (dirs (n, 3), refracted (n,) bool)."""
+    inc = _keep(inc, np.float32).reshape(-1, 3)
+    nrm = np.broadcast_to(_keep(normal, np.float32).reshape(-1, 3), inc.shape).copy()
+    out = np.empty((inc.shape[0], 4), np.float32)
+    lib().oc_refract(f(inc), f(nrm), float(eta), inc.shape[0], out.ctypes.data_as(_F))
+    return out[:, :3], out[:, 3] > 0
+
+
+def russian_roulette(energy, bounce, n, a=1, b=2):
+    """Scene.RussianRoulette over n draws of Rng(a, b): (energies (n, 3), terminated (n,) bool)."""
+    out = np.empty((n, 4), np.float32)
+    lib().oc_russian_roulette(f(np.asarray(energy, np.float32)), bounce, a, b, n, out.ctypes.data_as(_F))
+    return out[:, :3], out[:, 3] > 0
+
+
+def clamp_luminance(colors, mx):
+    c = _keep(colors, np.float32).reshape(-1, 3)
+    out = np.empty_like(c)
+    lib().oc_clamp_luminance(f(c), float(mx), c.shape[0], out.ctypes.data_as(_F))
+    return out
+
+
+def beer(albedo, d):
+    out = np.empty(3, np.float32)
+    lib().oc_beer(f(np.asarray(albedo, np.float32)), float(d), out.ctypes.data_as(_F))
+    return out
+
+
+def relative_std_err(total, total_sqr, n, precision="f32"):
+    return float(lib(precision).oc_relative_std_err(float(total), float(total_sqr), int(n)))

@@ -410,9 +410,36 @@ enum CounterType {
     C_Pixel = 0, C_Sample, C_SampleBounce, C_SampleHit, C_SampleMiss, C_SampleTerminate,
     C_SampleFogScatter, C_SampleFogEscape, C_SamplePixelConverged, C_SamplePixelMaxed,
     C_SceneTrace, C_SceneOcclude, C_MeshIntersect, C_BvhIntersectNode, C_BvhIntersectShape,
-    C_FilterSample, C_DenoiseEarlyOut, C_Count
+    C_FilterSample, C_DenoiseEarlyOut,
+    // Min / max-reduced (counters.cs:224-233): kept as their reference values, (long)(v * 1e4) for
+    // the float ones; initial values long.MaxValue (min) and long.MinValue (max).
+    C_SampleCountMin, C_SampleCountMax, C_SampleVarianceMax,
+    C_DenoiseRejectMissing, C_DenoiseRejectNormal, C_DenoiseRejectDepth, C_DenoiseRejectLum,
+    C_DenoiseWeightMax, C_DenoiseMaxLum, C_DenoiseMaxLumBoost, C_Count
 };
 struct Counters { int64_t v[C_Count]; };
+inline bool counter_is_min(int k) { return k == C_SampleCountMin; }
+inline bool counter_is_max(int k) {
+    return k == C_SampleCountMax || k == C_SampleVarianceMax || k == C_DenoiseWeightMax || k == C_DenoiseMaxLum ||
+           k == C_DenoiseMaxLumBoost;
+}
+inline void counters_init(Counters& c) {
+    for (int k = 0; k != C_Count; ++k) c.v[k] = counter_is_min(k) ? INT64_MAX : (counter_is_max(k) ? INT64_MIN : 0);
+}
+inline void counters_merge(int64_t* into, const Counters& c) {
+    for (int k = 0; k != C_Count; ++k) {
+        if (counter_is_min(k)) into[k] = std::min(into[k], c.v[k]);
+        else if (counter_is_max(k)) into[k] = std::max(into[k], c.v[k]);
+        else into[k] += c.v[k];
+    }
+}
+// Counters.BumpMax(float) (counters.cs:214-219): (long)(value * 1e4f), truncating; NaN and values
+// out of the long range give long.MinValue, as x64's conversion does.
+inline int64_t decimal_scaled(float v) {
+    const float s = v * 1e4f;
+    if (!(s == s) || s >= 9.2233720e18f || s < -9.2233720e18f) return INT64_MIN;
+    return (int64_t)s;
+}
 
 // ---- Generic binned-SAH BVH (bvh.cs:16-477) ----------------------------------------------------
 struct BvhNode { AABox bounds; int32_t child; int32_t shapeCount; };
@@ -1912,7 +1939,12 @@ PixelOut render_pixel(const Scene& sc, const View& view, const RenderParams& p, 
     o.depth = depthCount > 0 ? depthSum / (real)depthCount : kInf;
     o.albedo = depthCount > 0 ? albedoSum / (real)depthCount : kBlack;
     o.samples = sampleCount;
-    if (ctr) ctr->v[sampleCount < p.maxSamples ? C_SamplePixelConverged : C_SamplePixelMaxed]++;
+    if (ctr) {
+        ctr->v[sampleCount < p.maxSamples ? C_SamplePixelConverged : C_SamplePixelMaxed]++;
+        ctr->v[C_SampleCountMin] = std::min(ctr->v[C_SampleCountMin], (int64_t)sampleCount);
+        ctr->v[C_SampleCountMax] = std::max(ctr->v[C_SampleCountMax], (int64_t)sampleCount);
+        ctr->v[C_SampleVarianceMax] = std::max(ctr->v[C_SampleVarianceMax], decimal_scaled((float)o.variance));
+    }
     return o;
 }
 
@@ -1941,6 +1973,11 @@ Color filter_pixel(const Denoise& dn, const float* radiance, const float* normal
         if (ctr) ctr->v[C_DenoiseEarlyOut]++;
         return cr;
     }
+    if (ctr) {  // compose.cs:134-136
+        ctr->v[C_DenoiseWeightMax] = std::max(ctr->v[C_DenoiseWeightMax], decimal_scaled((float)dw));
+        ctr->v[C_DenoiseMaxLum] = std::max(ctr->v[C_DenoiseMaxLum], decimal_scaled((float)lum));
+        ctr->v[C_DenoiseMaxLumBoost] = std::max(ctr->v[C_DenoiseMaxLumBoost], decimal_scaled((float)lb));
+    }
     real wSum = 1.0f;
     Color rSum = cr;
     for (int ky = -kr; ky <= kr; ++ky) {
@@ -1957,15 +1994,20 @@ Color filter_pixel(const Denoise& dn, const float* radiance, const float* normal
             real nd = depth[ni];
             bool hasNN = mag_sqr(nn) > 0.0f;
             bool hasND = !std::isinf(nd);
-            if (hasCN != hasNN || hasCD != hasND) continue;
+            if (hasCN != hasNN || hasCD != hasND) {
+                if (ctr) ctr->v[C_DenoiseRejectMissing]++;
+                continue;
+            }
             real w = dw * std::exp(-kd2 * radiusPixelsInv);
             if (hasCN && w > 0.0f) {
                 V3 delta = cn - nn;
                 w *= std::exp(-mag_sqr(delta) * dn.normalLimitInv);
+                if (ctr && w < 1e-4f) ctr->v[C_DenoiseRejectNormal]++;
             }
             if (hasCD && w > 0.0f) {
                 real dd = cd - nd;
                 w *= std::exp(-(dd * dd) * dn.depthLimitInv);
+                if (ctr && w < 1e-4f) ctr->v[C_DenoiseRejectDepth]++;
             }
             // Albedo guide (an extension): after the depth term, on surface pixels only.
             if (dn.albedo && hasCN && w > 0.0f) {
@@ -1976,6 +2018,7 @@ Color filter_pixel(const Denoise& dn, const float* radiance, const float* normal
             if (w > 0.0f) {
                 real ld = nmax(0.0f, luminance(nr) - lum);
                 w *= std::exp(-(ld * ld) * dn.lumLimitInv);
+                if (ctr && w < 1e-4f) ctr->v[C_DenoiseRejectLum]++;
             }
             wSum += w;
             rSum = rSum + nr * w;
@@ -2346,7 +2389,7 @@ int oc_render(void* s, const float* view9, uint32_t W, uint32_t H, uint32_t minS
     uint32_t bx = (W + bs - 1) / bs, by = (rows + bs - 1) / bs, total = bx * by;
     std::atomic<uint32_t> next{0};
     std::vector<Counters> ctrs(std::max(threads, 1));
-    for (auto& c : ctrs) std::memset(&c, 0, sizeof(c));
+    for (auto& c : ctrs) counters_init(c);
     auto worker = [&](int wi) {
         Counters* ctr = counters ? &ctrs[wi] : nullptr;
         while (true) {
@@ -2378,11 +2421,10 @@ int oc_render(void* s, const float* view9, uint32_t W, uint32_t H, uint32_t minS
     worker(0);
     for (auto& t : pool) t.join();
     if (counters) {
-        for (int k = 0; k != C_Count; ++k) {
-            int64_t sum = 0;
-            for (auto& c : ctrs) sum += c.v[k];
-            counters[k] = sum;
-        }
+        Counters init;
+        counters_init(init);
+        std::memcpy(counters, init.v, sizeof(init.v));
+        for (auto& c : ctrs) counters_merge(counters, c);
     }
     return 0;
 }
@@ -2390,7 +2432,7 @@ int oc_render(void* s, const float* view9, uint32_t W, uint32_t H, uint32_t minS
 // Compositor.Filter over a whole image. params: radius strength strengthMax lumBoost lumLimit
 // normalLimit depthLimit (the Compositor ctor arguments, compose.cs:22-34).
 void oc_filter(const float* params7, const float* radiance, const float* normals, const float* depth,
-               const float* variance, uint32_t W, uint32_t H, int threads, float* out, int64_t* filterSamples,
+               const float* variance, uint32_t W, uint32_t H, int threads, float* out, int64_t* filterCounters,
                const float* albedo, float albedoLimit) {
     Denoise dn;
     dn.albedo = albedo;
@@ -2402,7 +2444,7 @@ void oc_filter(const float* params7, const float* radiance, const float* normals
     std::atomic<uint32_t> next{0};
     int nt = std::max(threads, 1);
     std::vector<Counters> ctrs(nt);
-    for (auto& c : ctrs) std::memset(&c, 0, sizeof(c));
+    for (auto& c : ctrs) counters_init(c);
     auto worker = [&](int wi) {
         while (true) {
             uint32_t y = next.fetch_add(1);
@@ -2418,10 +2460,11 @@ void oc_filter(const float* params7, const float* radiance, const float* normals
     for (int i = 1; i < nt; ++i) pool.emplace_back(worker, i);
     worker(0);
     for (auto& t : pool) t.join();
-    if (filterSamples) {
-        int64_t sum = 0;
-        for (auto& c : ctrs) sum += c.v[C_FilterSample];
-        *filterSamples = sum;
+    if (filterCounters) {  // Counters of the whole image in C_Count layout
+        Counters init;
+        counters_init(init);
+        std::memcpy(filterCounters, init.v, sizeof(init.v));
+        for (auto& c : ctrs) counters_merge(filterCounters, c);
     }
 }
 
@@ -2636,6 +2679,75 @@ void oc_surface_scatter(const float* material, const float* n3, const float* ng3
         o[3] = (float)sc.weight.x; o[4] = (float)sc.weight.y; o[5] = (float)sc.weight.z;
         o[6] = sc.hasPdf ? (float)sc.pdf : -1.0f; o[7] = sc.refracted ? 1.0f : 0.0f;
     }
+}
+// Scene.Trace (scene.cs:602-625) of arbitrary rays (6 floats: origin, direction). Per ray 20
+// floats: object (-1 on a miss), primitive, distance, shading normal 3, geometric normal 3, tangent
+// 4, uv 2, albedo 3, roughness, metallic.
+void oc_scene_trace(void* s, const float* rays6, uint32_t n, float* out20) {
+    const Scene* sc = (const Scene*)s;
+    for (uint32_t i = 0; i != n; ++i) {
+        const float* r = rays6 + (size_t)i * 6;
+        float* o = out20 + (size_t)i * 20;
+        Ray ray = make_ray(v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]));
+        Surface surf;
+        real dist;
+        int obj = -1, prim = -1;
+        for (int k = 0; k < 20; ++k) o[k] = 0.0f;
+        if (!sc->trace(ray, nullptr, surf, dist, &obj, &prim)) { o[0] = -1.0f; continue; }
+        o[0] = (float)obj; o[1] = (float)prim; o[2] = (float)dist;
+        o[3] = (float)surf.normal.x; o[4] = (float)surf.normal.y; o[5] = (float)surf.normal.z;
+        o[6] = (float)surf.normalGeo.x; o[7] = (float)surf.normalGeo.y; o[8] = (float)surf.normalGeo.z;
+        o[9] = (float)surf.tangent.x; o[10] = (float)surf.tangent.y; o[11] = (float)surf.tangent.z;
+        o[12] = (float)surf.tangent.w; o[13] = (float)surf.uv.x; o[14] = (float)surf.uv.y;
+        o[15] = (float)surf.albedo.x; o[16] = (float)surf.albedo.y; o[17] = (float)surf.albedo.z;
+        o[18] = (float)surf.roughness; o[19] = (float)surf.metallic;
+    }
+}
+// ShadowTransmittance (scene.cs:576-600) of rays: 3 floats each.
+void oc_shadow_transmittance(void* s, const float* rays6, uint32_t n, float* out3) {
+    const Scene* sc = (const Scene*)s;
+    for (uint32_t i = 0; i != n; ++i) {
+        const float* r = rays6 + (size_t)i * 6;
+        Color t = sc->shadow_transmittance(make_ray(v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5])), nullptr);
+        out3[i * 3] = (float)t.x; out3[i * 3 + 1] = (float)t.y; out3[i * 3 + 2] = (float)t.z;
+    }
+}
+// Brdf.Refract (brdf.cs:76-92): per item (dir 3, 1 if refracted else 0).
+void oc_refract(const float* inc3, const float* n3, float eta, uint32_t n, float* out4) {
+    for (uint32_t i = 0; i != n; ++i) {
+        V3 d;
+        const bool ok = refract(ld3(inc3 + i * 3), ld3(n3 + i * 3), (real)eta, d);
+        out4[i * 4] = ok ? (float)d.x : 0.0f; out4[i * 4 + 1] = ok ? (float)d.y : 0.0f;
+        out4[i * 4 + 2] = ok ? (float)d.z : 0.0f; out4[i * 4 + 3] = ok ? 1.0f : 0.0f;
+    }
+}
+// Scene.RussianRoulette (scene.cs:744-756) on n draws of the Rng(a, b) stream: per draw the energy
+// after the decision (3) and 1 if terminated.
+void oc_russian_roulette(const float* energy3, uint32_t bounce, uint32_t a, uint32_t b, uint32_t n, float* out4) {
+    Rng rng;
+    rng.seed_xy(a, b);
+    for (uint32_t i = 0; i != n; ++i) {
+        Color e = ld3(energy3);
+        const bool term = Scene::russian_roulette(e, bounce, rng, nullptr);
+        out4[i * 4] = (float)e.x; out4[i * 4 + 1] = (float)e.y; out4[i * 4 + 2] = (float)e.z;
+        out4[i * 4 + 3] = term ? 1.0f : 0.0f;
+    }
+}
+// Color.ClampLuminance (math.cs:120-124).
+void oc_clamp_luminance(const float* c3, float mx, uint32_t n, float* out3) {
+    for (uint32_t i = 0; i != n; ++i) {
+        Color c = clamp_luminance(ld3(c3 + i * 3), (real)mx);
+        out3[i * 3] = (float)c.x; out3[i * 3 + 1] = (float)c.y; out3[i * 3 + 2] = (float)c.z;
+    }
+}
+// Brdf.BeerLawTransmittance (brdf.cs:87-92).
+void oc_beer(const float* albedo3, float d, float* out3) {
+    Color c = beer_law(ld3(albedo3), (real)d);
+    out3[0] = (float)c.x; out3[1] = (float)c.y; out3[2] = (float)c.z;
+}
+// Renderer.RelativeStdErr (renderer.cs:232-237).
+float oc_relative_std_err(float sum, float sumSqr, uint32_t n) {
+    return (float)relative_std_err((real)sum, (real)sumSqr, n);
 }
 // Height fog (scene.cs:438-505): optical depth for segments [start,end) of rays.
 void oc_fog_optical_depth(const float* fog6, const float* rays6, const float* se2, uint32_t n, float* out) {

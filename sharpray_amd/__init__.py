@@ -440,6 +440,33 @@ class Scene:
         check(N.lib().sr_bvh_export(self._ctx, mid, fptr(nodes), iptr(items), None, None), "bvh")
         return nodes, items
 
+    def bvh_stats(self, mesh=None):
+        """Bvh.GetStats (bvh.cs:79-125) of a mesh's BVH or (mesh None) of the built scene's: dict with
+        the BvhStats record's fields (bvh.cs:5-14)."""
+        mid = -1 if mesh is None else (mesh if isinstance(mesh, int) else self.mesh_id(mesh))
+        st = N.BvhStats_()
+        check(N.lib().sr_bvh_stats_get(self._ctx, mid, ctypes.byref(st)), "bvh stats")
+        return {"NodeCount": st.node_count, "DepthMax": st.depth_max, "LeafCount": st.leaf_count,
+                "LeafSizeMin": st.leaf_size_min, "LeafSizeMax": st.leaf_size_max, "LeafSizeAvg": st.leaf_size_avg,
+                "LeafDepthAvg": st.leaf_depth_avg, "SahCost": st.sah_cost}
+
+    def counters_enable(self, enable=True):
+        """Collect counters from now on (mesh and scene builds, renders, filters)."""
+        check(N.lib().sr_counters_enable(self._ctx, 1 if enable else 0), "counters")
+
+    def counters_all(self, include_unset=False):
+        """The context's Counters.Type book (counters.cs:24-79) by name; entries still at their
+        reference initial value are left out unless include_unset."""
+        out = (ctypes.c_int64 * N.SR_COUNTER_TYPE_COUNT)()
+        check(N.lib().sr_counters_read_all(self._ctx, out, N.SR_COUNTER_TYPE_COUNT), "counters")
+        d = dict(zip(N.COUNTER_TYPE_NAMES, list(out)))
+        if include_unset:
+            return d
+        return {k: v for k, v in d.items() if v != N.COUNTER_TYPE_UNSET.get(k, 0)}
+
+    def counters_reset(self):
+        check(N.lib().sr_counters_reset(self._ctx), "counters")
+
     def trace_primary(self, view, width, height):
         """Sample-0 primary hits: (object, triangle, dist) arrays of shape (height, width)."""
         obj = np.empty(width * height, dtype=np.int32)

@@ -24,6 +24,23 @@ COUNTER_NAMES = [
     "BvhIntersectNode", "BvhIntersectShape",
 ]
 
+# Counters.Type (counters.cs:24-79) in order: the layout of sr_counters_read_all.
+COUNTER_TYPE_NAMES = [
+    "Worker", "Block", "Pixel", "Sample", "SampleBounce", "SampleHit", "SampleMiss", "SampleTerminate",
+    "SampleFogScatter", "SampleFogEscape", "SampleVarianceMax", "SamplePixelConverged", "SamplePixelMaxed",
+    "SampleCountMin", "SampleCountMax", "SceneObject", "SceneBvhNodes", "SceneBvhDepth", "SceneTrace",
+    "SceneOcclude", "MeshTriangle", "MeshIntersect", "MeshIntersectAny", "BvhIntersectNode", "BvhIntersectShape",
+    "BvhOverlapNode", "BvhOverlapShape", "FilterSample", "DenoiseEarlyOut", "DenoiseWeightMax", "DenoiseMaxLum",
+    "DenoiseMaxLumBoost", "DenoiseRejectMissing", "DenoiseRejectNormal", "DenoiseRejectDepth", "DenoiseRejectLum",
+    "OverlayLine", "OverlayText", "RtPeakWorkingSet", "RtGcAllocatedBytes", "RtGcCurrentBytes", "RtGcGen0",
+    "RtGcGen1", "RtGcGen2", "TimeSetup", "TimeObjLoad", "TimeMeshBvhBuild", "TimeSceneBvhBuild", "TimeRender",
+    "TimeCompose", "TimeTotal",
+]
+SR_COUNTER_TYPE_COUNT = len(COUNTER_TYPE_NAMES)
+# Reductions other than the sum (counters.cs:228-236), with the reference's initial ("unset") values.
+COUNTER_TYPE_UNSET = {"SampleVarianceMax": -2 ** 63, "SampleCountMin": 2 ** 63 - 1, "SampleCountMax": -2 ** 63,
+                      "DenoiseWeightMax": -2 ** 63, "DenoiseMaxLum": -2 ** 63, "DenoiseMaxLumBoost": -2 ** 63}
+
 c_float_p = ctypes.POINTER(ctypes.c_float)
 c_int32_p = ctypes.POINTER(ctypes.c_int32)
 c_uint32_p = ctypes.POINTER(ctypes.c_uint32)
@@ -90,6 +107,14 @@ class ComposeParams_(ctypes.Structure):
     ]
 
 
+class BvhStats_(ctypes.Structure):
+    _fields_ = [
+        ("node_count", ctypes.c_int32), ("depth_max", ctypes.c_int32), ("leaf_count", ctypes.c_int32),
+        ("leaf_size_min", ctypes.c_int32), ("leaf_size_max", ctypes.c_int32), ("leaf_size_avg", ctypes.c_float),
+        ("leaf_depth_avg", ctypes.c_float), ("sah_cost", ctypes.c_float),
+    ]
+
+
 c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
 
 # name -> (restype, argtypes); every entry point declared in include/sharpray.h.
@@ -131,6 +156,9 @@ SIGNATURES = {
     "sr_bvh_export": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, c_float_p, c_int32_p, c_uint32_p, c_uint32_p]),
     "sr_counters_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "sr_counters_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "sr_counters_read_all": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.c_uint32]),
+    "sr_counters_reset": (ctypes.c_int, [ctypes.c_void_p]),
+    "sr_bvh_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(BvhStats_)]),
     "sr_profile_enable": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "sr_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), c_uint32_p]),
     "sr_profile_busy": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),

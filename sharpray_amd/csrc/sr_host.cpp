@@ -345,4 +345,42 @@ float build_cdf1(const float* w, int n, float* cdf) {
     return total;
 }
 
+
+namespace {
+// GetStatsNode (bvh.cs:96-124): leaves give their own record; interior nodes merge A then B
+// (NodeStats.Merge, bvh.cs:38-48), add themselves and their traversal cost.
+BvhStatsHost stats_node(const BvhResult& b, int idx, int depth, float rootSa, float ct, float ci) {
+    const RefNode& nd = b.nodes[idx];
+    const float sa = box_area(nd.bounds);
+    const float ratio = rootSa > 0.0f ? sa / rootSa : 0.0f;
+    BvhStatsHost r;
+    if (nd.count > 0) {
+        r.nodeCount = 1;
+        r.depthMax = depth;
+        r.leafCount = 1;
+        r.leafMin = r.leafMax = r.shapeCount = nd.count;
+        r.depthWeighted = depth * nd.count;
+        r.sah = ci * ratio * (float)nd.count;
+        return r;
+    }
+    const BvhStatsHost a = stats_node(b, nd.child, depth + 1, rootSa, ct, ci);
+    const BvhStatsHost c = stats_node(b, nd.child + 1, depth + 1, rootSa, ct, ci);
+    r.nodeCount = a.nodeCount + c.nodeCount + 1;
+    r.leafCount = a.leafCount + c.leafCount;
+    r.leafMin = std::min(a.leafMin, c.leafMin);
+    r.leafMax = std::max(a.leafMax, c.leafMax);
+    r.shapeCount = a.shapeCount + c.shapeCount;
+    r.depthMax = std::max(a.depthMax, c.depthMax);
+    r.depthWeighted = a.depthWeighted + c.depthWeighted;
+    r.sah = a.sah + c.sah;
+    r.sah += ct * ratio;
+    return r;
+}
+}  // namespace
+
+BvhStatsHost bvh_stats(const BvhResult& b, float cost_traverse, float cost_intersect) {
+    if (b.used == 0) return BvhStatsHost();
+    return stats_node(b, 0, 0, box_area(b.nodes[0].bounds), cost_traverse, cost_intersect);
+}
+
 }  // namespace sr

@@ -9,6 +9,8 @@
 
 #include "sr_math.h"
 
+struct sr_ctx;
+
 namespace sr {
 
 // One BVH node exactly as bvh.cs:26-31 lays it out (AABox 24 B, Child, ShapeCount): 32 bytes.
@@ -31,6 +33,14 @@ struct BvhResult {
 // partition and depth-first pair allocation, so nodes and items match the reference bit for bit.
 BvhResult build_bvh(const box3* bounds, int n, int bins, float cost_traverse, float cost_intersect);
 
+// Bvh.GetStats (bvh.cs:79-125) of a built BVH: the reference's recursion and merge order, so the
+// float SAH cost and averages round as the reference's do.
+struct BvhStatsHost {
+    int32_t nodeCount = 0, depthMax = 0, leafCount = 0, leafMin = 0, leafMax = 0, shapeCount = 0, depthWeighted = 0;
+    float sah = 0.0f;
+};
+BvhStatsHost bvh_stats(const BvhResult& b, float cost_traverse, float cost_intersect);
+
 // Triangle ctors (math.cs:1503-1531) and Mesh.ComputeSmoothNormals (mesh.cs:99-123).
 // has_nrm (optional, with nrm): per triangle, 0 = the face-normal ctor despite nrm (obj.cs:430-441).
 void build_triangles(const float* pos, const float* nrm, const float* uv, uint32_t n, bool smooth, float* out36,
@@ -44,6 +54,8 @@ bool device_build_bvh(const float* dLean, int n, int bins, float cost_traverse, 
 
 // Records an error message for sr_last_error and returns code (defined with the C ABI).
 int sr_set_error(int code, const char* msg);
+// Counters.Bump(entry, micros) on a context that collects counters (defined with the C ABI).
+void sr_count_time(::sr_ctx* ctx, int entry, int64_t micros);
 
 // Bounds of a TriangleLean given as (A, B-A, C-A) (math.cs:1353-1359).
 box3 lean_bounds(const float* t9);

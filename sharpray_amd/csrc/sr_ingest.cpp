@@ -16,6 +16,7 @@
 #include <sys/stat.h>
 
 #include <cerrno>
+#include <chrono>
 #include <map>
 #include <string>
 #include <vector>
@@ -349,10 +350,13 @@ void decode_hdr(const uint8_t* data, size_t size, std::vector<uint8_t>& rgbe, ui
             }
         } else {
             if (hd[0] == 1 && hd[1] == 1 && hd[2] == 1) raise("ImageHdr: Old RLE format not supported");
+            // The header is the scanline's first pixel; the reference's copy into a scanline of
+            // fewer than 4 bytes throws (image.cs:371), i.e. for width 0.
+            if (w == 0) raise("ImageHdr: flat scanline shorter than its first pixel (width 0)");
             memcpy(line.data(), hd, 4);
             for (size_t k = 4; k < line.size(); ++k) line[k] = r.byte("ImageHdr");
         }
-        memcpy(rgbe.data() + (size_t)y * w * 4, line.data(), line.size());
+        if (!line.empty()) memcpy(rgbe.data() + (size_t)y * w * 4, line.data(), line.size());
     }
 }
 
@@ -628,7 +632,11 @@ int sr_obj_load(sr_ctx* ctx, const char* path, const sr_transform* transform, co
         t.scale[0] = t.scale[1] = t.scale[2] = 1.0f;
     }
     try {
+        const auto t0 = std::chrono::steady_clock::now();  // TimeObjLoad (obj.cs:147)
         const uint32_t n = parse_obj(ctx, path, t, material, smooth_normals != 0);
+        sr::sr_count_time(ctx, SR_CT_TIME_OBJ_LOAD,
+                          (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+                              std::chrono::steady_clock::now() - t0).count());
         if (objects_added) *objects_added = n;
         return SR_OK;
     } catch (const IngestError& e) {

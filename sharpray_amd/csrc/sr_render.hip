@@ -20,6 +20,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <condition_variable>
 #include <mutex>
 #include <string>
@@ -96,18 +97,23 @@ struct ViewD {
 // SceneOcclude in the per-segment launches: traversal the reference does but a render without
 // counters skips (side-culled shadow rays, see kCountOnly, and the nodes the reference's
 // closest-hit walk of a shadow ray visits beyond those of the any-hit walk).
+// Then three max-reduced words (atomicMax over unsigned keys, 0 = unset): SampleCountMin as the
+// complement of the count, SampleCountMax, and SampleVarianceMax as a biased (long)(v * 1e4)
+// (renderer.cs:224-226, counters.cs:195-219).
 enum : int { kTailNode = SR_COUNTER_COUNT, kTailShape, kTailTrace, kTailOcclude, kRefNode, kRefShape, kRefMesh,
-             kRefOcclude, kCtrWords };
+             kRefOcclude, kCountMinKey, kCountMaxKey, kVarMaxKey, kCtrWords };
 // ShadowRec::path flags. kCountOnly marks a shadow ray whose light lies behind the shading or the
 // geometric normal: its contribution is black, so the tracer skips it, but the reference traces
 // it before its side tests (scene.cs:774-783) and counts that traversal. While counters are on,
 // such rays are traced for their counts only.
 constexpr uint32_t kToFog = 0x80000000u, kCountOnly = 0x40000000u, kPathMask = 0x3fffffffu;
 constexpr int kCtrSpread = 16;  // copies of the counter words (a power of two)
+constexpr int kProgRing = 4;    // samples an adaptive render's driver runs ahead of its progress reads
 
-// Queue counters (uint32 each).
+// Queue counters (uint32 each), per slice. kOverflow collects the queues an append found full (bit
+// 0: ray queues, bit 1: shadow queues); the append is then dropped and the host fails the render.
 enum : int {
-    kRayCount0 = 0, kRayCount1 = 1, kShadowCount0 = 2, kShadowCount1 = 3, kCounterWords = 4
+    kRayCount0 = 0, kRayCount1 = 1, kShadowCount0 = 2, kShadowCount1 = 3, kOverflow = 4, kCounterWords = 5
 };
 
 struct Frame {
@@ -121,6 +127,11 @@ struct Frame {
     uint32_t* queue[2];
     uint32_t* counts;   // kCounterWords words
     unsigned long long* ctr;  // kCtrWords counters or null
+    // Renderer.Tick's unit (renderer.cs:98-122): finished pixels per bs x bs block of the whole
+    // image (bx blocks per row), and done[0], the count of blocks whose pixels are all final.
+    uint32_t* blockPixels;
+    uint32_t* blocksDone;
+    uint32_t bs, bx;
     float* outRad;
     float* outRadFog;
     float* outNormal;
@@ -144,6 +155,18 @@ SR_DEV bool slot_pixel(const Frame& F, uint32_t q, uint32_t& x, uint32_t& y) {
 }
 
 SR_DEV uint32_t lane_id() { return __lane_id(); }
+
+// Queue appends are bounded by the queue's capacity (nSlots entries per queue and per shadow
+// buffer): an append that finds the queue full is dropped and flagged in kOverflow, which the host
+// turns into an error after the band instead of a write past the buffer.
+SR_DEV bool queue_room(const Frame& F, uint32_t slot, uint32_t bit) {
+    if (slot < F.nSlots) return true;
+    atomicOr(&F.counts[kOverflow], bit);
+    return false;
+}
+// A queue length read back for a launch, clamped to the capacity (only an overflowed queue's count
+// exceeds it, and that render fails).
+SR_DEV uint32_t queue_len(const Frame& F, int word) { return min(F.counts[word], F.nSlots); }
 
 
 // Appends `pred` lanes of the whole block (NW waves) to a queue with one atomic per block; slots
@@ -248,6 +271,25 @@ SR_DEV void bump_sum(const Frame& F, int idx, uint32_t v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     if (lane_id() == 0 && v) atomicAdd(ctr_at(F, idx), (unsigned long long)v);
 }
+// Raises a max-reduced counter word to the wave's largest key (0 = nothing to add); every lane of
+// the wave calls it.
+SR_DEV void bump_max(const Frame& F, int idx, unsigned long long key) {
+    if (F.ctr == nullptr) return;
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long other = __shfl_xor(key, o);
+        key = other > key ? other : key;
+    }
+    if (lane_id() == 0 && key) atomicMax(ctr_at(F, idx), key);
+}
+// Counters.BumpMax(float) (counters.cs:214-219) keeps (long)(value * 1e4) in a signed word; this is
+// that value as an unsigned key whose order matches the signed order (NaN adds nothing).
+SR_DEV unsigned long long decimal_key(float v) {
+    // NaN and values outside the long range convert to long.MinValue on x64 (the reduction's
+    // initial value, i.e. no change).
+    const float s = v * 1e4f;
+    if (!(s == s) || s >= 9.2233720e18f || s < -9.2233720e18f) return 0ull;
+    return (unsigned long long)(long long)s ^ 0x8000000000000000ull;
+}
 // Counts the lanes of the wave where pred holds with one atomic; every lane of the wave calls it.
 SR_DEV void bump_wave(const Frame& F, int idx, bool pred) {
     if (F.ctr == nullptr) return;
@@ -283,7 +325,7 @@ __global__ void __launch_bounds__(kBlock) k_start(Frame F) {
     bump_wave(F, 0, live);
     bump_wave(F, 1, live);
     const uint32_t slot = block_append<kBlock / 64>(&F.counts[kRayCount0], live, sh);
-    if (live) F.queue[0][slot] = q;
+    if (live && queue_room(F, slot, 1u)) F.queue[0][slot] = q;
 }
 
 // Counts of one shade pass, flushed into the render counters once per thread.
@@ -426,7 +468,7 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
 // registers: 3 waves per SIMD instead of 2).
 template <class R, bool FOG>
 __global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, DevScene S, int cur) {
-    const uint32_t n = F.counts[cur];
+    const uint32_t n = queue_len(F, cur);
     const uint32_t* Q = F.queue[cur];
     ShadeCount c;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -492,9 +534,9 @@ __global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame
         // Block-aggregated appends: one atomic per block and queue (per-wave atomics on the two
         // counters cost 66 vs 54 ms of shade per 1080p frame).
         const uint32_t ns = block_append<kShadeBlock / 64>(&F.counts[kShadowCount0 + cur], shadow, shApp);
-        if (shadow) F.shadows[cur][ns] = sr;
+        if (shadow && queue_room(F, ns, 2u)) F.shadows[cur][ns] = sr;
         const uint32_t nc = block_append<kShadeBlock / 64>(&F.counts[cur ^ 1], cont, shApp);
-        if (cont) F.queue[cur ^ 1][nc] = q;
+        if (cont && queue_room(F, nc, 1u)) F.queue[cur ^ 1][nc] = q;
         __syncthreads();  // sQ and sHits serve the block's next chunk
     }
     if (F.ctr) {
@@ -564,7 +606,7 @@ SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, 
 #endif
 template <class R, bool COUNT, bool TRANSP>
 __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, DevScene S, int cur) {
-    const uint32_t n = F.counts[cur];
+    const uint32_t n = queue_len(F, cur);
     const uint32_t* Q = F.queue[cur];
     ShadeCount c;
     Tally tl = {0, 0, 0};
@@ -625,8 +667,8 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
         F.counts[cur ^ 1] = 0;
         F.counts[kShadowCount0 + cur] = 0;
     }
-    const uint32_t nE = F.counts[cur];
-    const uint32_t nS = F.counts[kShadowCount0 + (cur ^ 1)];
+    const uint32_t nE = queue_len(F, cur);
+    const uint32_t nS = queue_len(F, kShadowCount0 + (cur ^ 1));
     const uint32_t nE64 = (nE + 63u) & ~63u;
     const uint32_t total = nE64 + nS;
     const uint32_t* Q = F.queue[cur];
@@ -710,6 +752,7 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
         F.counts[kShadowCount1] = 0;
     }
     bool again = false, converged = false, maxed = false;
+    unsigned long long countMinKey = 0, countMaxKey = 0, varKey = 0;
     // Fields are read and written one by one: a sample touches the accumulator's first 48 B and the
     // path's radiance and random state; the rest is read once, when the pixel finishes.
     PixAcc* A = F.acc + q;
@@ -772,15 +815,25 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
             F.outVar[o] = var;
             converged = count < F.maxS;
             maxed = !converged;
+            countMinKey = ~(unsigned long long)count;
+            countMaxKey = count;
+            varKey = decimal_key(var);
+            // The pixel's block (renderer.cs:125-134) is complete once all its pixels are final.
+            const uint32_t bxi = x / F.bs, byi = y / F.bs;
+            const uint32_t bw = min(F.bs, F.W - bxi * F.bs), bh = min(F.bs, F.H - byi * F.bs);
+            if (atomicAdd(&F.blockPixels[byi * F.bx + bxi], 1u) + 1u == bw * bh) atomicAdd(F.blocksDone, 1u);
         }
     }
     bump_wave(F, 1, again);
     bump_wave(F, 8, converged);
     bump_wave(F, 9, maxed);
+    bump_max(F, kCountMinKey, countMinKey);
+    bump_max(F, kCountMaxKey, countMaxKey);
+    bump_max(F, kVarMaxKey, varKey);
     // One atomic per block: one per wave on the single queue counter serialised the launch
     // (26.1 -> 14.0 ms per 1080p frame).
     const uint32_t slot = block_append<kNextBlock / 64>(&F.counts[kRayCount0], again, sh);
-    if (again) F.queue[0][slot] = q;
+    if (again && queue_room(F, slot, 1u)) F.queue[0][slot] = q;
 }
 
 // Primary-hit diagnostic: sample 0 of every pixel of a band.
@@ -852,6 +905,22 @@ __global__ void k_primary(Frame F, DevScene S, int32_t* obj, int32_t* tri, float
 }
 
 // Compositor.Filter (compose.cs:101-204): joint bilateral filter over the circular window.
+// Device counters of the filter kernels (compose.cs:101-245): sums, then max keys (decimal_key).
+enum : int { kFltSample, kFltEarlyOut, kFltRejectMissing, kFltRejectNormal, kFltRejectDepth, kFltRejectLum,
+             kFltWeightMax, kFltMaxLum, kFltMaxLumBoost, kFilterWords };
+// Adds a per-lane count to a filter counter with one atomic per wave; every lane calls it.
+SR_DEV void flt_sum(unsigned long long* fc, int idx, uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane_id() == 0 && v) atomicAdd(fc + idx, (unsigned long long)v);
+}
+SR_DEV void flt_max(unsigned long long* fc, int idx, unsigned long long key) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long other = __shfl_xor(key, o);
+        key = other > key ? other : key;
+    }
+    if (lane_id() == 0 && key) atomicMax(fc + idx, key);
+}
+
 struct FilterParams {
     float strength, strengthMax, lumBoost, lumLimitInv, normalLimitInv, depthLimitInv, radiusInv;
     float albedoLimitInv;  // ALB only: exp(-|albedo delta|^2 * albedoLimitInv) on surface pixels
@@ -859,13 +928,12 @@ struct FilterParams {
 };
 // Compositor.Filter (compose.cs:101-204), one pixel per lane. ALB adds an albedo guide term (no
 // reference counterpart) after the depth term.
-template <bool ALB>
-__global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* rad, const float* nrm, const float* dep,
-                                                 const float* var, const float* alb, int32_t W, int32_t H,
-                                                 float* out) {
-    const int32_t cx = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int32_t cy = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (cx >= W || cy >= H) return;
+// One pixel of Compositor.Filter. n (COUNT): FilterSample, DenoiseEarlyOut, DenoiseRejectMissing,
+// -Normal, -Depth, -Lum; keys: DenoiseWeightMax, DenoiseMaxLum, DenoiseMaxLumBoost.
+template <bool ALB, bool COUNT>
+SR_DEV void filter_pixel(const FilterParams& fp, const float* rad, const float* nrm, const float* dep,
+                         const float* var, const float* alb, int32_t W, int32_t H, float* out, int32_t cx,
+                         int32_t cy, uint32_t* n, unsigned long long* keys) {
     const int32_t ci = cy * W + cx;
     const vec3 cr = mk3(rad[ci * 3], rad[ci * 3 + 1], rad[ci * 3 + 2]);
     const vec3 cn = mk3(nrm[ci * 3], nrm[ci * 3 + 1], nrm[ci * 3 + 2]);
@@ -878,7 +946,13 @@ __global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* ra
     const float dw = net_min(var[ci] * lb * fp.strength, fp.strengthMax);
     if (dw < 1e-4f) {
         out[ci * 3] = cr.x; out[ci * 3 + 1] = cr.y; out[ci * 3 + 2] = cr.z;
+        if (COUNT) n[1] += 1;
         return;
+    }
+    if (COUNT) {
+        keys[0] = decimal_key(dw);
+        keys[1] = decimal_key(lum);
+        keys[2] = decimal_key(lb);
     }
     float wSum = 1.0f;
     vec3 rSum = cr;
@@ -898,12 +972,19 @@ __global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* ra
             const float nd = dep[ni];
             const bool hasNN = len_sqr(nn) > 0.0f;
             const bool hasND = !isinf(nd);
-            if (hasCN != hasNN || hasCD != hasND) continue;
+            if (hasCN != hasNN || hasCD != hasND) {
+                if (COUNT) n[2] += 1;
+                continue;
+            }
             float w = dw * expf(-kd2 * fp.radiusInv);
-            if (hasCN && w > 0.0f) w *= expf(-len_sqr(cn - nn) * fp.normalLimitInv);
+            if (hasCN && w > 0.0f) {
+                w *= expf(-len_sqr(cn - nn) * fp.normalLimitInv);
+                if (COUNT && w < 1e-4f) n[3] += 1;
+            }
             if (hasCD && w > 0.0f) {
                 const float dd = cd - nd;
                 w *= expf(-(dd * dd) * fp.depthLimitInv);
+                if (COUNT && w < 1e-4f) n[4] += 1;
             }
             if (ALB && hasCN && w > 0.0f) {
                 const vec3 na = mk3(alb[ni * 3], alb[ni * 3 + 1], alb[ni * 3 + 2]);
@@ -912,13 +993,34 @@ __global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* ra
             if (w > 0.0f) {
                 const float ld = net_max(0.0f, luminance(nr) - lum);
                 w *= expf(-(ld * ld) * fp.lumLimitInv);
+                if (COUNT && w < 1e-4f) n[5] += 1;
             }
             wSum += w;
             rSum = rSum + nr * w;
+            if (COUNT) n[0] += 1;
         }
     }
     const vec3 r = rSum / wSum;
     out[ci * 3] = r.x; out[ci * 3 + 1] = r.y; out[ci * 3 + 2] = r.z;
+}
+
+// COUNT bumps the reference's filter counters (fc, kFilterWords words) the way compose.cs:130-199
+// does.
+template <bool ALB, bool COUNT>
+__global__ void __launch_bounds__(256) k_filter(FilterParams fp, const float* rad, const float* nrm, const float* dep,
+                                                 const float* var, const float* alb, int32_t W, int32_t H,
+                                                 float* out, unsigned long long* fc) {
+    const int32_t cx = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int32_t cy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (COUNT) {
+        uint32_t n[6] = {0, 0, 0, 0, 0, 0};
+        unsigned long long keys[3] = {0, 0, 0};
+        if (cx < W && cy < H) filter_pixel<ALB, true>(fp, rad, nrm, dep, var, alb, W, H, out, cx, cy, n, keys);
+        for (int k = 0; k < 6; ++k) flt_sum(fc, kFltSample + k, n[k]);
+        for (int k = 0; k < 3; ++k) flt_max(fc, kFltWeightMax + k, keys[k]);
+    } else if (cx < W && cy < H) {
+        filter_pixel<ALB, false>(fp, rad, nrm, dep, var, alb, W, H, out, cx, cy, nullptr, nullptr);
+    }
 }
 
 // The per-pixel sample mean of a render: Radiance * Samples / min(Samples, maxSamples). A pixel that
@@ -936,12 +1038,10 @@ __global__ void k_sample_mean(const float* rad, const uint32_t* samples, uint32_
 
 // Compositor.FilterFog (compose.cs:206-245): circular Gaussian of the fog radiance, centre weight 1,
 // neighbours strength * exp(-d^2 / (2 r^2)), same loop order as the reference.
-__global__ void __launch_bounds__(256) k_filter_fog(float radiusInv, float strength, int32_t kr, const float* fog,
-                                                     int32_t W, int32_t H, float* out) {
-    const int32_t cx = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int32_t cy = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (cx >= W || cy >= H) return;
+SR_DEV uint32_t filter_fog_pixel(float radiusInv, float strength, int32_t kr, const float* fog, int32_t W, int32_t H,
+                                 float* out, int32_t cx, int32_t cy) {
     const int32_t ci = cy * W + cx;
+    uint32_t taps = 0;  // neighbours weighted: FilterSample (compose.cs:240)
     float wSum = 1.0f;
     vec3 rSum = mk3(fog[ci * 3], fog[ci * 3 + 1], fog[ci * 3 + 2]);
     for (int32_t ky = -kr; ky <= kr; ++ky) {
@@ -957,10 +1057,21 @@ __global__ void __launch_bounds__(256) k_filter_fog(float radiusInv, float stren
             const float w = strength * expf(-kd2 * radiusInv);
             wSum += w;
             rSum = rSum + mk3(fog[ni * 3], fog[ni * 3 + 1], fog[ni * 3 + 2]) * w;
+            ++taps;
         }
     }
     const vec3 r = rSum / wSum;
     out[ci * 3] = r.x; out[ci * 3 + 1] = r.y; out[ci * 3 + 2] = r.z;
+    return taps;
+}
+// fc (optional, counting renders): the FilterSample filter counter.
+__global__ void __launch_bounds__(256) k_filter_fog(float radiusInv, float strength, int32_t kr, const float* fog,
+                                                     int32_t W, int32_t H, float* out, unsigned long long* fc) {
+    const int32_t cx = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int32_t cy = blockIdx.y * 16 + (threadIdx.x >> 4);
+    uint32_t taps = 0;
+    if (cx < W && cy < H) taps = filter_fog_pixel(radiusInv, strength, kr, fog, W, H, out, cx, cy);
+    if (fc) flt_sum(fc, kFltSample, taps);
 }
 
 // Color.ToPixel (math.cs:127-136): Clamp01 (Math.Clamp leaves NaN), sRGB power approximation, MathF.Max
@@ -1015,7 +1126,10 @@ namespace {
 #define HIP_TRY(expr)                                                                        \
     do {                                                                                     \
         hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess) return fail(SR_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+        if (e_ != hipSuccess) {                                                              \
+            (void)hipGetLastError(); /* a later launch check must not see this error */          \
+            return fail(SR_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));            \
+        }                                                                                    \
     } while (0)
 
 struct HostTexture {
@@ -1102,19 +1216,28 @@ struct sr_ctx {
     DevBuf<unsigned long long> dCtr;
     DevBuf<float> dRad, dRadFog, dNormal, dUv, dDepth, dVar, dAlbedo;
     DevBuf<uint32_t> dSamples;
-    uint32_t* hostCount = nullptr;  // pinned: the live-ray count read back between chunks
+    // Progress snapshots of adaptive renders: after each sample, every slice copies its queue-0
+    // count and the finished-block count into pinned memory and records an event; the driver reads
+    // a snapshot kProgRing samples later (no stall), stops enqueueing a band once no pixel of it is
+    // active and publishes the finished blocks to Tick.
+    DevBuf<uint32_t> dBlockPixels, dProgress;
+    uint32_t* progHost = nullptr;
+    size_t progHostWords = 0;
+    std::vector<hipEvent_t> progEv;
     bool rendering = false;
     bool countersOn = false;
     sr_render_params params{};
     ViewD view{};
     // Background driver (the analogue of the reference's worker threads, renderer.cs:88-122):
-    // enqueues the iterations band by band; Tick waits on bandsDone.
+    // enqueues the iterations band by band; Tick waits on blocksDone, the count of bs x bs blocks
+    // whose pixels are all final (renderer.cs:98-122).
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv;
     std::vector<uint32_t> bandEnd;  // last row (exclusive) per band
-    size_t bandsDone = 0;
-    size_t bandsReported = 0;
+    uint32_t blocksTotal = 0;
+    uint32_t blocksDone = 0;     // published by the driver
+    uint32_t ticksConsumed = 0;  // one Tick return per finished block, like the reference's semaphore
     bool workerDone = true;
     int workerStatus = SR_OK;
     std::string workerError;
@@ -1139,6 +1262,13 @@ struct sr_ctx {
     };
     bool profileOn = false;
     std::vector<ProfLane> prof;
+    // Counters.Type book (counters.cs:24-79) over the context's work; guarded by mu (the render
+    // driver folds into it from its thread).
+    int64_t book[SR_COUNTER_TYPE_COUNT];
+    // Filter counters on the device (kFilterWords words), folded into the book at sync points.
+    DevBuf<unsigned long long> dFilterCtr;
+    bool filterCtrPending = false;
+    std::chrono::steady_clock::time_point renderStart;
 
     void join() {
         if (worker.joinable()) worker.join();
@@ -1149,7 +1279,9 @@ struct sr_ctx {
         for (ProfLane& l : prof)
             for (auto e : l.ev) (void)hipEventDestroy(e);
         for (hipStream_t st : sliceStreams) (void)hipStreamDestroy(st);
-        if (hostCount) (void)hipHostFree(hostCount);
+        if (progHost) (void)hipHostFree(progHost);
+        for (auto e : progEv) (void)hipEventDestroy(e);
+        dBlockPixels.release(); dProgress.release(); dFilterCtr.release();
         if (evStart) (void)hipEventDestroy(evStart);
         if (evEnd) (void)hipEventDestroy(evEnd);
         dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release(); dAlias.release(); dFogGrid.release();
@@ -1192,6 +1324,56 @@ int ensure(DevBuf<T>& b, size_t n) {
     b.n = n;
     return SR_OK;
 }
+
+// ---- Counters.Type book ---------------------------------------------------------------------------
+// Reductions of counters.cs:228-236: max for SampleVarianceMax, SampleCountMax and the Denoise*Max*
+// entries, min for SampleCountMin, sum for the rest; initial values counters.cs:221-226.
+int book_reduction(int k) {  // 0 sum, 1 min, 2 max
+    switch (k) {
+        case SR_CT_SAMPLE_COUNT_MIN: return 1;
+        case SR_CT_SAMPLE_VARIANCE_MAX:
+        case SR_CT_SAMPLE_COUNT_MAX:
+        case SR_CT_DENOISE_WEIGHT_MAX:
+        case SR_CT_DENOISE_MAX_LUM:
+        case SR_CT_DENOISE_MAX_LUM_BOOST: return 2;
+        default: return 0;
+    }
+}
+void book_reset(sr_ctx* c) {
+    for (int k = 0; k < SR_COUNTER_TYPE_COUNT; ++k) {
+        const int r = book_reduction(k);
+        c->book[k] = r == 1 ? INT64_MAX : (r == 2 ? INT64_MIN : 0);
+    }
+}
+// Counters.Bump / BumpMin / BumpMax by the entry's reduction; the caller holds c->mu.
+void book_bump(sr_ctx* c, int k, int64_t v) {
+    const int r = book_reduction(k);
+    if (r == 1) c->book[k] = std::min(c->book[k], v);
+    else if (r == 2) c->book[k] = std::max(c->book[k], v);
+    else c->book[k] += v;
+}
+void book_bump_locked(sr_ctx* c, int k, int64_t v) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    book_bump(c, k, v);
+}
+int64_t micros_since(std::chrono::steady_clock::time_point t0) {
+    return (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0).count();
+}
+// An unsigned max key of the device counters (biased (long)(v * 1e4)) back to the signed value;
+// 0 = never bumped.
+bool decimal_from_key(unsigned long long key, int64_t& v) {
+    if (!key) return false;
+    v = (int64_t)(key ^ 0x8000000000000000ull);
+    return true;
+}
+
+}  // namespace
+
+void sr::sr_count_time(sr_ctx* ctx, int entry, int64_t micros) {
+    if (ctx && ctx->countersOn) book_bump_locked(ctx, entry, micros);
+}
+
+namespace {
 
 quat mkq(const float* r) { quat q; q.x = r[0]; q.y = r[1]; q.z = r[2]; q.w = r[3]; return q; }
 
@@ -1414,6 +1596,7 @@ int sr_create(int32_t device, sr_ctx** out) {
     if (device == -1) {  // host-only context: scene preparation and BVH export, no rendering
         sr_ctx* c = new sr_ctx();
         c->device = -1;
+        book_reset(c);
         *out = c;
         return SR_OK;
     }
@@ -1423,6 +1606,7 @@ int sr_create(int32_t device, sr_ctx** out) {
     HIP_TRY(hipSetDevice(device));
     sr_ctx* c = new sr_ctx();
     c->device = device;
+    book_reset(c);
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(SR_ERR_HIP, hipGetErrorString(e)); }
     hipDeviceProp_t prop;
@@ -1464,6 +1648,7 @@ int sr_mesh_create(sr_ctx* ctx, const float* tris, uint32_t count, uint32_t bins
     HostMesh m;
     m.tris.assign(tris, tris + (size_t)count * 36);
     m.bins = bins;
+    const auto t0 = std::chrono::steady_clock::now();  // TimeMeshBvhBuild (mesh.cs:38-42)
     if (ctx->bvhBuilder == SR_BVH_BUILDER_DEVICE && count > 0) {
         if (bins > 64) return fail(SR_ERR_LIMIT, "mesh: the device BVH builder takes at most 64 bins");
         HIP_TRY(hipSetDevice(ctx->device));
@@ -1484,6 +1669,11 @@ int sr_mesh_create(sr_ctx* ctx, const float* tris, uint32_t count, uint32_t bins
         m.bvh = build_bvh(bb.data(), (int)count, (int)bins, 1.0f, 0.5f);
     }
     if (m.bvh.degenerate) return fail(SR_ERR_BUILD, "mesh BVH: a split produced an empty child (bvh.cs:448)");
+    if (ctx->countersOn) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        book_bump(ctx, SR_CT_TIME_MESH_BVH_BUILD, micros_since(t0));
+        book_bump(ctx, SR_CT_MESH_TRIANGLE, (int64_t)count);
+    }
     ctx->meshes.push_back(std::move(m));
     *out_id = (int32_t)ctx->meshes.size() - 1;
     return SR_OK;
@@ -1587,8 +1777,18 @@ int sr_scene_build(sr_ctx* ctx) {
         }
         ob[i] = object_world_bounds(lb, o.trans);
     }
+    const auto t0 = std::chrono::steady_clock::now();
     ctx->sceneBvh = build_bvh(ob.data(), (int)ob.size(), 32, 1.0f, 10.0f);
     if (ctx->sceneBvh.degenerate) return fail(SR_ERR_BUILD, "scene BVH: a split produced an empty child");
+    if (ctx->countersOn) {  // scene.cs:528-540
+        const int64_t us = micros_since(t0);
+        const BvhStatsHost st = bvh_stats(ctx->sceneBvh, 1.0f, 10.0f);
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        book_bump(ctx, SR_CT_TIME_SCENE_BVH_BUILD, us);
+        book_bump(ctx, SR_CT_SCENE_OBJECT, (int64_t)ctx->objects.size());
+        book_bump(ctx, SR_CT_SCENE_BVH_NODES, st.nodeCount);
+        book_bump(ctx, SR_CT_SCENE_BVH_DEPTH, st.depthMax);
+    }
 
     // Node pool, triangles in item order, attributes in triangle order.
     std::vector<float4> nodes;
@@ -1748,6 +1948,10 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     F.threshold = c->params.variance_threshold;
     F.clamp = c->params.indirect_clamp;
     F.view = c->view;
+    F.blockPixels = c->dBlockPixels.ptr;
+    F.blocksDone = c->dProgress.ptr;
+    F.bs = std::max(1u, c->params.block_size);
+    F.bx = (F.W + F.bs - 1) / F.bs;
     return F;
 }
 
@@ -1763,9 +1967,23 @@ int alloc_frame(sr_ctx* c, uint32_t sliceSlots, uint32_t slices) {
         (rc = ensure(c->dCounts, (size_t)kCounterWords * slices)) ||
         (rc = ensure(c->dCtr, (size_t)kCtrWords * kCtrSpread)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
         (rc = ensure(c->dNormal, px * 3)) || (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) || (rc = ensure(c->dAlbedo, px * 3)) ||
-        (rc = ensure(c->dVar, px)) || (rc = ensure(c->dSamples, px)))
+        (rc = ensure(c->dVar, px)) || (rc = ensure(c->dSamples, px)) ||
+        (rc = ensure(c->dBlockPixels, c->blocksTotal)) || (rc = ensure(c->dProgress, 1)))
         return rc;
-    if (!c->hostCount) HIP_TRY(hipHostMalloc((void**)&c->hostCount, sizeof(uint32_t), hipHostMallocDefault));
+    // Pinned progress snapshots: kProgRing samples x slices x (queue-0 count, finished blocks).
+    const size_t words = (size_t)kProgRing * slices * 2;
+    if (c->progHostWords < words) {
+        if (c->progHost) HIP_TRY(hipHostFree(c->progHost));
+        c->progHost = nullptr;
+        c->progHostWords = 0;
+        HIP_TRY(hipHostMalloc((void**)&c->progHost, words * sizeof(uint32_t), hipHostMallocDefault));
+        c->progHostWords = words;
+    }
+    while (c->progEv.size() < (size_t)kProgRing * slices) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->progEv.push_back(e);
+    }
     return SR_OK;
 }
 
@@ -1854,10 +2072,26 @@ void enqueue_sample(sr_ctx* c, const SliceRun& r) {
     mark(c, r.slice, 3);
 }
 
+int fold_render_counters(sr_ctx* c, std::string& msg);
+
+// Publishes the finished-block count for Tick.
+void publish_blocks(sr_ctx* c, uint32_t done) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (done > c->blocksDone) {
+        c->blocksDone = std::min(done, c->blocksTotal);
+        c->cv.notify_all();
+    }
+}
+
 // Every band as concurrent row slices, one stream each. The host enqueues sample by sample across
-// the slices, so every stream has work queued from the start; the band is done when all are.
+// the slices, so every stream has work queued from the start; the band is done when all are. In
+// adaptive renders (renderer.cs:205-211) each sample ends with a progress snapshot per slice; the
+// driver reads the snapshot kProgRing samples behind the last enqueued one and stops enqueueing
+// the band once no pixel of it is active (the reference's workers stop per pixel; the GPU launches
+// after that point would only find empty queues).
 template <class R, bool COUNT>
 int run_frame(sr_ctx* c) {
+    const bool adaptive = c->params.min_samples < c->params.max_samples && c->params.variance_threshold > 0.0f;
     for (size_t b = 0; b < c->bandEnd.size(); ++b) {
         const uint32_t y0 = b == 0 ? 0 : c->bandEnd[b - 1];
         std::vector<SliceRun> runs;
@@ -1866,16 +2100,56 @@ int run_frame(sr_ctx* c) {
             if (sy >= c->bandEnd[b]) break;
             runs.push_back(slice_run(c, frame_for(c, sy, std::min(c->sliceRows, c->bandEnd[b] - sy), j), j));
         }
+        const size_t nr = runs.size();
         for (const SliceRun& r : runs) enqueue_start<R>(c, r);
-        for (uint32_t smp = 0; smp < c->params.max_samples; ++smp) {
+        std::vector<uint32_t> pending;  // ring slots of snapshots not read yet, oldest first
+        bool bandIdle = false;
+        // Reads the oldest snapshot: its finished blocks go to Tick, and an all-empty camera-ray
+        // queue means every pixel of the band is final.
+        auto consume = [&]() -> int {
+            const uint32_t slot = pending.front();
+            pending.erase(pending.begin());
+            uint32_t live = 0, done = 0;
+            for (size_t j = 0; j < nr; ++j) {
+                HIP_TRY(hipEventSynchronize(c->progEv[slot * nr + j]));
+                live += c->progHost[(slot * nr + j) * 2];
+                done = std::max(done, c->progHost[(slot * nr + j) * 2 + 1]);
+            }
+            publish_blocks(c, done);
+            if (live == 0) bandIdle = true;
+            return SR_OK;
+        };
+        for (uint32_t smp = 0; smp < c->params.max_samples && !bandIdle; ++smp) {
             for (const SliceRun& r : runs) enqueue_sample<R, COUNT>(c, r);
             c->iterations++;
+            if (!adaptive) continue;
+            if (pending.size() == (size_t)kProgRing) {
+                int rc = consume();
+                if (rc) return rc;
+            }
+            const uint32_t slot = smp % kProgRing;
+            for (size_t j = 0; j < nr; ++j) {
+                uint32_t* hp = c->progHost + (slot * nr + j) * 2;
+                HIP_TRY(hipMemcpyAsync(hp, runs[j].F.counts + kRayCount0, 4, hipMemcpyDeviceToHost, runs[j].s));
+                HIP_TRY(hipMemcpyAsync(hp + 1, c->dProgress.ptr, 4, hipMemcpyDeviceToHost, runs[j].s));
+                HIP_TRY(hipEventRecord(c->progEv[slot * nr + j], runs[j].s));
+            }
+            pending.push_back(slot);
         }
         HIP_TRY(hipGetLastError());
         for (const SliceRun& r : runs) HIP_TRY(hipStreamSynchronize(r.s));
-        std::lock_guard<std::mutex> lk(c->mu);
-        c->bandsDone = b + 1;
-        c->cv.notify_all();
+        uint32_t overflow = 0;
+        for (const SliceRun& r : runs) {
+            uint32_t ov = 0;
+            HIP_TRY(hipMemcpy(&ov, r.F.counts + kOverflow, 4, hipMemcpyDeviceToHost));
+            overflow |= ov;
+        }
+        if (overflow)
+            return fail(SR_ERR_LIMIT, std::string("render: a ") + ((overflow & 1u) ? "ray" : "shadow-ray") +
+                                          " queue reached its capacity; the render was stopped");
+        uint32_t done = 0;
+        HIP_TRY(hipMemcpy(&done, c->dProgress.ptr, 4, hipMemcpyDeviceToHost));
+        publish_blocks(c, done);
     }
     return SR_OK;
 }
@@ -1901,12 +2175,18 @@ void drive(sr_ctx* c) {
         hipError_t e = hipEventRecord(c->evEnd, c->stream);
         if (e == hipSuccess) e = hipEventSynchronize(c->evEnd);
         if (!rc && e != hipSuccess) { rc = SR_ERR_HIP; msg = hipGetErrorString(e); }
+        if (!rc && c->blocksDone != c->blocksTotal) {
+            rc = SR_ERR_STATE;
+            msg = "render: " + std::to_string(c->blocksDone) + " of " + std::to_string(c->blocksTotal) +
+                  " blocks reported finished";
+        }
+        if (!rc && c->countersOn) rc = fold_render_counters(c, msg);
     }
     std::lock_guard<std::mutex> lk(c->mu);
     c->workerStatus = rc;
     c->workerError = msg;
     c->workerDone = true;
-    c->bandsDone = c->bandEnd.size();
+    c->blocksDone = c->blocksTotal;
     c->cv.notify_all();
 }
 
@@ -1930,6 +2210,9 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
     if (p->light_sampler > 1) return fail(SR_ERR_INVALID, "render: unknown light sampler");
     if (p->width > 32768 || p->height > 32768) return fail(SR_ERR_LIMIT, "render: image larger than 32768");
     ctx->join();  // a previous render finishes first
+    // From here on a failure leaves no render: read and tick report SR_ERR_STATE instead of using
+    // the previous render's buffers with this call's sizes.
+    ctx->rendering = false;
     HIP_TRY(hipSetDevice(ctx->device));
     if (!ctx->built) {
         int rc = sr_scene_build(ctx);
@@ -1951,8 +2234,11 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
         HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         ctx->sliceStreams.push_back(st);
     }
+    ctx->blocksTotal = ((p->width + p->block_size - 1) / p->block_size) * ((p->height + p->block_size - 1) / p->block_size);
     int rc = alloc_frame(ctx, tilesX * (ctx->sliceRows / 8) * 64, ctx->sliceCount);
     if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(ctx->dBlockPixels.ptr, 0, (size_t)ctx->blocksTotal * 4, ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->dProgress.ptr, 0, 4, ctx->stream));
     if (!ctx->evStart) HIP_TRY(hipEventCreate(&ctx->evStart));
     if (!ctx->evEnd) HIP_TRY(hipEventCreate(&ctx->evEnd));
     if (ctx->countersOn)
@@ -1963,8 +2249,9 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
     ctx->view = make_view(view, p->width, p->height);
     ctx->bandEnd.clear();
     for (uint32_t y0 = 0; y0 < p->height; y0 += rowsPer) ctx->bandEnd.push_back(std::min(p->height, y0 + rowsPer));
-    ctx->bandsDone = 0;
-    ctx->bandsReported = 0;
+    ctx->blocksDone = 0;
+    ctx->ticksConsumed = 0;
+    ctx->renderStart = std::chrono::steady_clock::now();
     ctx->workerDone = false;
     ctx->workerStatus = SR_OK;
     ctx->workerError.clear();
@@ -1976,20 +2263,17 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
 int sr_render_tick(sr_ctx* ctx, uint32_t* step, uint32_t* total) {
     if (!ctx || !step || !total) return fail(SR_ERR_INVALID, "tick: null argument");
     if (!ctx->rendering) return fail(SR_ERR_STATE, "no render in progress");
-    const sr_render_params& p = ctx->params;
-    const uint32_t bx = (p.width + p.block_size - 1) / p.block_size;
-    const uint32_t by = (p.height + p.block_size - 1) / p.block_size;
-    *total = bx * by;
-    size_t done;
-    {
-        std::unique_lock<std::mutex> lk(ctx->mu);
-        ctx->cv.wait(lk, [&] { return ctx->workerDone || ctx->bandsDone > ctx->bandsReported || ctx->bandsReported == ctx->bandEnd.size(); });
-        if (ctx->workerDone && ctx->workerStatus != SR_OK) return fail(ctx->workerStatus, "render: " + ctx->workerError);
-        ctx->bandsReported = ctx->bandsDone;
-        done = ctx->bandsDone;
+    std::unique_lock<std::mutex> lk(ctx->mu);
+    *total = ctx->blocksTotal;
+    if (ctx->blocksDone >= ctx->blocksTotal && ctx->workerDone) {
+        if (ctx->workerStatus != SR_OK) return fail(ctx->workerStatus, "render: " + ctx->workerError);
+        *step = ctx->blocksTotal;  // renderer.cs:100-101
+        return SR_OK;
     }
-    const uint32_t rowsDone = done ? ctx->bandEnd[done - 1] : 0;
-    *step = rowsDone >= p.height ? *total : bx * (rowsDone / p.block_size);
+    ctx->cv.wait(lk, [&] { return ctx->workerDone || ctx->ticksConsumed < ctx->blocksDone; });
+    if (ctx->workerDone && ctx->workerStatus != SR_OK) return fail(ctx->workerStatus, "render: " + ctx->workerError);
+    if (ctx->ticksConsumed < ctx->blocksDone) ctx->ticksConsumed++;
+    *step = ctx->blocksDone;
     return SR_OK;
 }
 
@@ -2129,9 +2413,75 @@ int read_counters(sr_ctx* ctx, unsigned long long* v) {
     std::vector<unsigned long long> all((size_t)kCtrWords * kCtrSpread);
     HIP_TRY(hipMemcpy(all.data(), ctx->dCtr.ptr, all.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     for (int i = 0; i < kCtrWords; ++i) {
+        const bool isMax = i == kCountMinKey || i == kCountMaxKey || i == kVarMaxKey;
         v[i] = 0;
-        for (int k = 0; k < kCtrSpread; ++k) v[i] += all[(size_t)k * kCtrWords + i];
+        for (int k = 0; k < kCtrSpread; ++k) {
+            const unsigned long long x = all[(size_t)k * kCtrWords + i];
+            v[i] = isMax ? std::max(v[i], x) : v[i] + x;
+        }
     }
+    return SR_OK;
+}
+
+// The reference-taxonomy render counters (SR_COUNTER_COUNT words) from the device words: the fused
+// tail launch's traversal and the reference-only share are added back.
+void render_counters(const unsigned long long* v, int64_t* out) {
+    for (int i = 0; i < SR_COUNTER_COUNT; ++i) out[i] = (int64_t)v[i];
+    out[10] += (int64_t)v[kTailTrace];
+    out[13] += (int64_t)v[kTailNode];
+    out[14] += (int64_t)v[kTailShape];
+    out[11] += (int64_t)v[kRefOcclude];
+    out[12] += (int64_t)v[kRefMesh];
+    out[13] += (int64_t)v[kRefNode];
+    out[14] += (int64_t)v[kRefShape];
+}
+
+// After a counting render (driver thread): the render's counters into the book.
+int fold_render_counters(sr_ctx* c, std::string& msg) {
+    unsigned long long v[kCtrWords];
+    if (read_counters(c, v) != SR_OK) {
+        msg = g_err;
+        return SR_ERR_HIP;
+    }
+    int64_t r[SR_COUNTER_COUNT];
+    render_counters(v, r);
+    // Position of each SR_COUNTER_COUNT word in Counters.Type.
+    static const int kMap[SR_COUNTER_COUNT] = {
+        SR_CT_PIXEL, SR_CT_SAMPLE, SR_CT_SAMPLE_BOUNCE, SR_CT_SAMPLE_HIT, SR_CT_SAMPLE_MISS, SR_CT_SAMPLE_TERMINATE,
+        SR_CT_SAMPLE_FOG_SCATTER, SR_CT_SAMPLE_FOG_ESCAPE, SR_CT_SAMPLE_PIXEL_CONVERGED, SR_CT_SAMPLE_PIXEL_MAXED,
+        SR_CT_SCENE_TRACE, SR_CT_SCENE_OCCLUDE, SR_CT_MESH_INTERSECT, SR_CT_BVH_INTERSECT_NODE,
+        SR_CT_BVH_INTERSECT_SHAPE};
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (int i = 0; i < SR_COUNTER_COUNT; ++i) book_bump(c, kMap[i], r[i]);
+    book_bump(c, SR_CT_WORKER, (int64_t)c->sliceCount);
+    book_bump(c, SR_CT_BLOCK, (int64_t)c->blocksTotal);
+    if (v[kCountMinKey]) book_bump(c, SR_CT_SAMPLE_COUNT_MIN, (int64_t)(~v[kCountMinKey]));
+    if (v[kCountMaxKey]) book_bump(c, SR_CT_SAMPLE_COUNT_MAX, (int64_t)v[kCountMaxKey]);
+    int64_t var;
+    if (decimal_from_key(v[kVarMaxKey], var)) book_bump(c, SR_CT_SAMPLE_VARIANCE_MAX, var);
+    book_bump(c, SR_CT_TIME_RENDER, micros_since(c->renderStart));
+    return SR_OK;
+}
+
+// The filter kernels' device counters into the book (waits for the context's stream).
+int fold_filter_counters(sr_ctx* c) {
+    if (!c->filterCtrPending) return SR_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unsigned long long v[kFilterWords];
+    HIP_TRY(hipMemcpy(v, c->dFilterCtr.ptr, sizeof(v), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemset(c->dFilterCtr.ptr, 0, sizeof(v)));
+    c->filterCtrPending = false;
+    std::lock_guard<std::mutex> lk(c->mu);
+    book_bump(c, SR_CT_FILTER_SAMPLE, (int64_t)v[kFltSample]);
+    book_bump(c, SR_CT_DENOISE_EARLY_OUT, (int64_t)v[kFltEarlyOut]);
+    book_bump(c, SR_CT_DENOISE_REJECT_MISSING, (int64_t)v[kFltRejectMissing]);
+    book_bump(c, SR_CT_DENOISE_REJECT_NORMAL, (int64_t)v[kFltRejectNormal]);
+    book_bump(c, SR_CT_DENOISE_REJECT_DEPTH, (int64_t)v[kFltRejectDepth]);
+    book_bump(c, SR_CT_DENOISE_REJECT_LUM, (int64_t)v[kFltRejectLum]);
+    int64_t x;
+    if (decimal_from_key(v[kFltWeightMax], x)) book_bump(c, SR_CT_DENOISE_WEIGHT_MAX, x);
+    if (decimal_from_key(v[kFltMaxLum], x)) book_bump(c, SR_CT_DENOISE_MAX_LUM, x);
+    if (decimal_from_key(v[kFltMaxLumBoost], x)) book_bump(c, SR_CT_DENOISE_MAX_LUM_BOOST, x);
     return SR_OK;
 }
 }  // namespace
@@ -2144,14 +2494,66 @@ int sr_counters_read(sr_ctx* ctx, int64_t* out) {
     unsigned long long v[kCtrWords];
     int rc2 = read_counters(ctx, v);
     if (rc2) return rc2;
-    for (int i = 0; i < SR_COUNTER_COUNT; ++i) out[i] = (int64_t)v[i];
-    out[10] += (int64_t)v[kTailTrace];
-    out[13] += (int64_t)v[kTailNode];
-    out[14] += (int64_t)v[kTailShape];
-    out[11] += (int64_t)v[kRefOcclude];
-    out[12] += (int64_t)v[kRefMesh];
-    out[13] += (int64_t)v[kRefNode];
-    out[14] += (int64_t)v[kRefShape];
+    render_counters(v, out);
+    return SR_OK;
+}
+
+int sr_counters_read_all(sr_ctx* ctx, int64_t* out, uint32_t count) {
+    if (!ctx || !out) return fail(SR_ERR_INVALID, "null argument");
+    if (count > SR_COUNTER_TYPE_COUNT) return fail(SR_ERR_INVALID, "counters: count above SR_COUNTER_TYPE_COUNT");
+    if (ctx->rendering) {
+        int rc = wait_render(ctx);
+        if (rc) return rc;
+    }
+    if (ctx->device >= 0) {
+        HIP_TRY(hipSetDevice(ctx->device));
+        int rc = fold_filter_counters(ctx);
+        if (rc) return rc;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (uint32_t i = 0; i < count; ++i) out[i] = ctx->book[i];
+    return SR_OK;
+}
+
+int sr_counters_reset(sr_ctx* ctx) {
+    if (!ctx) return fail(SR_ERR_INVALID, "ctx is null");
+    if (ctx->rendering) {
+        int rc = wait_render(ctx);
+        if (rc) return rc;
+    }
+    if (ctx->device >= 0) {
+        HIP_TRY(hipSetDevice(ctx->device));
+        int rc = fold_filter_counters(ctx);
+        if (rc) return rc;
+    }
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    book_reset(ctx);
+    return SR_OK;
+}
+
+int sr_bvh_stats_get(sr_ctx* ctx, int32_t mesh, sr_bvh_stats* out) {
+    if (!ctx || !out) return fail(SR_ERR_INVALID, "bvh stats: null argument");
+    const BvhResult* b;
+    float ci;
+    if (mesh < 0) {
+        if (!ctx->built) return fail(SR_ERR_STATE, "Scene not built");
+        b = &ctx->sceneBvh;
+        ci = 10.0f;  // scene.cs:530
+    } else {
+        if ((size_t)mesh >= ctx->meshes.size()) return fail(SR_ERR_INVALID, "bvh stats: no such mesh");
+        b = &ctx->meshes[mesh].bvh;
+        ci = 0.5f;  // mesh.cs:40
+    }
+    const BvhStatsHost st = bvh_stats(*b, 1.0f, ci);
+    out->node_count = st.nodeCount;
+    out->depth_max = st.depthMax;
+    out->leaf_count = st.leafCount;
+    out->leaf_size_min = st.leafMin;
+    out->leaf_size_max = st.leafMax;
+    // bvh.cs:90-91
+    out->leaf_size_avg = st.leafCount > 0 ? (float)st.shapeCount / (float)st.leafCount : 0.0f;
+    out->leaf_depth_avg = st.shapeCount > 0 ? (float)st.depthWeighted / (float)st.shapeCount : 0.0f;
+    out->sah_cost = st.sah;
     return SR_OK;
 }
 
@@ -2256,6 +2658,23 @@ int sr_bvh_export(sr_ctx* ctx, int32_t mesh, float* nodes, int32_t* items, uint3
 }
 
 namespace {
+// The device filter counters of a counting context (allocated and zeroed on first use; null
+// otherwise). Kernels that bump them leave the book to be folded at the next sync point.
+hipError_t filter_counters(sr_ctx* ctx, unsigned long long*& fc) {
+    fc = nullptr;
+    if (!ctx->countersOn) return hipSuccess;
+    if (!ctx->dFilterCtr.ptr) {
+        hipError_t e = hipMalloc((void**)&ctx->dFilterCtr.ptr, kFilterWords * sizeof(unsigned long long));
+        if (e != hipSuccess) return e;
+        ctx->dFilterCtr.n = kFilterWords;
+        e = hipMemset(ctx->dFilterCtr.ptr, 0, kFilterWords * sizeof(unsigned long long));
+        if (e != hipSuccess) return e;
+    }
+    fc = ctx->dFilterCtr.ptr;
+    ctx->filterCtrPending = true;
+    return hipSuccess;
+}
+
 int launch_filter(sr_ctx* ctx, const sr_denoise_params* p, const float* rad, const float* nrm, const float* dep,
                   const float* var, const float* alb, float albedoLimit, uint32_t W, uint32_t H, float* out) {
     if (!ctx || !p || !rad || !nrm || !dep || !var || !out || W == 0 || H == 0)
@@ -2276,12 +2695,12 @@ int launch_filter(sr_ctx* ctx, const sr_denoise_params* p, const float* rad, con
     fp.kr = (int32_t)ceilf(radiusPixels * 3.0f);
     fp.albedoLimitInv = alb ? 1.0f / (albedoLimit * albedoLimit * 2.0f) : 0.0f;
     const dim3 grid((W + 15) / 16, (H + 15) / 16);
-    if (alb)
-        hipLaunchKernelGGL(k_filter<true>, grid, dim3(256), 0, ctx->stream, fp, rad, nrm, dep, var, alb, (int32_t)W,
-                           (int32_t)H, out);
-    else
-        hipLaunchKernelGGL(k_filter<false>, grid, dim3(256), 0, ctx->stream, fp, rad, nrm, dep, var, alb, (int32_t)W,
-                           (int32_t)H, out);
+    unsigned long long* fc = nullptr;
+    HIP_TRY(filter_counters(ctx, fc));
+    auto kern = alb ? (fc ? k_filter<true, true> : k_filter<true, false>)
+                    : (fc ? k_filter<false, true> : k_filter<false, false>);
+    hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, fp, rad, nrm, dep, var, alb, (int32_t)W, (int32_t)H, out,
+                       fc);
     HIP_TRY(hipGetLastError());
     return SR_OK;
 }
@@ -2343,8 +2762,10 @@ int launch_filter_fog(sr_ctx* ctx, float radius, float strength, const float* fo
     const float krf = ceilf(radiusPixels * 3.0f);
     if (krf > 4096.0f) return fail(SR_ERR_LIMIT, "fog filter: kernel radius above 4096 pixels");
     const dim3 grid((W + 15) / 16, (H + 15) / 16);
+    unsigned long long* fc = nullptr;
+    HIP_TRY(filter_counters(ctx, fc));
     hipLaunchKernelGGL(k_filter_fog, grid, dim3(256), 0, ctx->stream, radiusInv, strength, (int32_t)krf, fog,
-                       (int32_t)W, (int32_t)H, out);
+                       (int32_t)W, (int32_t)H, out, fc);
     HIP_TRY(hipGetLastError());
     return SR_OK;
 }
@@ -2403,6 +2824,7 @@ int sr_compose(sr_ctx* ctx, const sr_compose_params* p, uint8_t* out_rgb) {
     if (!ctx || !p || !out_rgb) return fail(SR_ERR_INVALID, "compose: null argument");
     int rc = compose_ready(ctx);
     if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();  // TimeCompose (program.cs:279-283)
     const uint32_t W = ctx->params.width, H = ctx->params.height;
     float* dTmp = nullptr;
     HIP_TRY(hipMalloc((void**)&dTmp, (size_t)W * H * 24));
@@ -2414,6 +2836,7 @@ int sr_compose(sr_ctx* ctx, const sr_compose_params* p, uint8_t* out_rgb) {
     if (rc == SR_OK) rc = tonemap_to_host(ctx, dMain, dFog, p->tonemapper, p->exposure, out_rgb);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(dTmp);
+    if (rc == SR_OK && ctx->countersOn) book_bump_locked(ctx, SR_CT_TIME_COMPOSE, micros_since(t0));
     return rc;
 }
 

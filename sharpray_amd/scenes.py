@@ -65,9 +65,12 @@ class SceneSpec:
         return sum(int(m.shape[0]) for m in self.meshes)
 
 
-def instantiate(spec, device=0):
-    """Builds a GPU Scene (and its View) from a spec through the public API."""
+def instantiate(spec, device=0, counters=False):
+    """Builds a GPU Scene (and its View) from a spec through the public API; counters: collect the
+    Counters.Type book from the first mesh on."""
     scene = Scene(device)
+    if counters:
+        scene.counters_enable(True)
     textures = [Texture(t) for t in spec.textures]
     meshes = [Mesh(m, split_bins=b) for m, b in zip(spec.meshes, spec.mesh_bins)]
     for m in meshes:  # registration order = spec index = device mesh id

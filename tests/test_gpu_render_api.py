@@ -1,0 +1,153 @@
+"""GPU tests of the Renderer's host-facing behaviour: Tick progress in the reference's blocks,
+adaptive sampling at program.cs's settings against the oracle, the Counters.Type book against the
+oracle's counters, and a failed render start leaving no render behind."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.helpers import bits, rel_err
+
+pytestmark = pytest.mark.gpu
+
+RAD_TOL = 1e-3
+
+
+def _aovs(r):
+    return (r.radiance.copy(), r.radiance_fog.copy(), r.normals.copy(), r.uv.copy(), r.depth.copy(),
+            r.samples.copy(), r.variance.copy())
+
+
+def test_tick_reports_blocks_and_bands_are_bit_identical():
+    """Renderer.Tick (renderer.cs:98-105) in block_size blocks: a 5-band render reports monotone
+    progress, each intermediate value the number of 32 x 32 blocks whose rows all lie in finished
+    bands, then (total, total), and again (total, total) without blocking; its AOVs are
+    bit-identical to the one-band render's."""
+    from sharpray_amd import RNG_COUNTER, Renderer, scenes
+    spec = scenes.config_plain_cpu(scale=0.75, spp=2)  # 192 x 192: 6 x 6 blocks
+    scene, view = scenes.instantiate(spec)
+    scene.build()
+    out = {}
+    for bands in (1, 5):
+        r = Renderer(scene, view, spec.width, spec.height, 32, 2, 2, 0.0, spec.bounces, spec.indirect_clamp,
+                     rng_policy=RNG_COUNTER, rng_seed=3, tick_bands=bands)
+        steps = []
+        while True:
+            s, t = r.tick()
+            steps.append(s)
+            if s == t:
+                break
+        assert t == 36
+        assert r.tick() == (36, 36) and r.tick() == (36, 36)
+        print(f"bands {bands}: ticks {steps}", flush=True)
+        assert steps == sorted(steps) and steps[-1] == 36
+        # Band ends (rows, multiples of 8): ceil(192 / 5) -> 40 rows per band, ends 40, 80, 120, 160, 192.
+        ends = list(range(40, 192, 40)) + [192]
+        allowed = {6 * (e // 32) for e in ends} | {36}
+        assert set(steps) <= allowed, (steps, allowed)
+        if bands == 5:
+            assert len(set(steps)) >= 3  # progress arrives before the end
+        out[bands] = _aovs(r)
+    for a, b in zip(out[1], out[5]):
+        assert np.array_equal(bits(a), bits(b)) if a.dtype == np.float32 else np.array_equal(a, b)
+
+
+def test_adaptive_sampling_matches_oracle(oracle_lib):
+    """Adaptive sampling at program.cs:25-27's settings (64..2048 samples, relative standard error
+    threshold 0.075) on the 64 x 64 icosphere scene, reference xorwow policy: per-pixel sample
+    counts equal the oracle's on every pixel whose radiance agrees, some pixels converge early,
+    the reference's counters (SamplePixelConverged / Maxed, SampleCountMin / Max) agree, and the
+    driver stops enqueueing samples once every pixel is final."""
+    from sharpray_amd import Renderer, scenes
+    spec = scenes.config_plain_cpu(scale=0.25, spp=64)
+    spec.min_samples, spec.max_samples, spec.variance_threshold = 64, 2048, 0.075
+    scene, view = scenes.instantiate(spec, counters=True)
+    r = Renderer(scene, view, spec.width, spec.height, 32, 64, 2048, 0.075, spec.bounces, spec.indirect_clamp,
+                 counters=True, kernel_timing=True)
+    r.wait()
+    ref = oracle_lib.OracleScene(spec).render(counters=True)
+    err = rel_err(r.radiance, ref["radiance"])
+    ok = err <= RAD_TOL
+    frac = 1.0 - float(ok.mean())
+    g = r.counters()
+    print(f"adaptive: divergent {frac:.4f}; samples gpu min/max {r.samples.min()}/{r.samples.max()}, oracle "
+          f"{ref['samples'].min()}/{ref['samples'].max()}; converged gpu {g['SamplePixelConverged']} oracle "
+          f"{ref['counters']['SamplePixelConverged']}", flush=True)
+    assert frac <= 0.02
+    assert np.array_equal(r.samples[ok], ref["samples"][ok])
+    assert g["SamplePixelConverged"] > 0
+    oc = ref["counters"]
+    assert abs(g["SamplePixelConverged"] - oc["SamplePixelConverged"]) <= max(2, 0.02 * oc["SamplePixelConverged"])
+    book = scene.counters_all()
+    assert book["SampleCountMin"] == oc["SampleCountMin"] == int(ref["samples"].min())
+    assert book["SampleCountMax"] == oc["SampleCountMax"]
+    assert abs(book["SampleVarianceMax"] - oc["SampleVarianceMax"]) <= max(2, 1e-3 * abs(oc["SampleVarianceMax"]))
+    assert book["Block"] == 4 and book["Worker"] >= 1 and book["TimeRender"] > 0
+    # Samples enqueued: one accumulate launch per sample ("other" also holds the start launch). A
+    # pixel that stops at index i ran i + 1 samples; one that ran out ran 2048 (its count is 2049).
+    run = min(int(r.samples.max()), 2048)
+    enqueued = r.kernel_times()["other"][1] - 1
+    print(f"adaptive: {enqueued} samples enqueued, the longest pixel ran {run}", flush=True)
+    assert run <= enqueued <= run + 5
+    # A threshold every pixel meets at the minimum: 64 samples run, far fewer than 2048 enqueued.
+    r = Renderer(scene, view, spec.width, spec.height, 32, 64, 2048, 10.0, spec.bounces, spec.indirect_clamp,
+                 kernel_timing=True)
+    r.wait()
+    assert np.all(r.samples == 64)
+    enqueued = r.kernel_times()["other"][1] - 1
+    print(f"adaptive, all converge at 64: {enqueued} samples enqueued", flush=True)
+    assert 64 <= enqueued <= 69
+
+
+def test_counter_book_matches_oracle_counters(oracle_lib):
+    """The Counters.Type book after a counting render and a bilateral filter: render-path entries
+    as in sr_counters_read, the filter's FilterSample / DenoiseEarlyOut / DenoiseReject* /
+    DenoiseWeightMax / DenoiseMaxLum / DenoiseMaxLumBoost as the oracle's Filter counts them on the
+    same AOVs (compose.cs:101-204)."""
+    from sharpray_amd import Compositor, RNG_COUNTER, Renderer, scenes
+    spec = scenes.config_plain_cpu(scale=0.25, spp=8)
+    scene, view = scenes.instantiate(spec, counters=True)
+    r = Renderer(scene, view, spec.width, spec.height, 32, 8, 8, 0.0, spec.bounces, spec.indirect_clamp,
+                 rng_policy=RNG_COUNTER, rng_seed=17, counters=True)
+    r.wait()
+    g = r.counters()
+    comp = Compositor(0.02, 0.1, 1.75, 0.25, 2.0, 0.125, 0.2)
+    comp.filter(r)
+    book = scene.counters_all()
+    for k, v in g.items():
+        assert book[k] == v, k
+    assert book["SampleCountMin"] == book["SampleCountMax"] == 9  # M + 1 (renderer.cs:173,215)
+    assert book["MeshTriangle"] == spec.triangle_count and book["SceneObject"] == len(spec.objects)
+    _, fc = oracle_lib.bilateral(comp.as_array(), r.radiance, r.normals, r.depth, r.variance, spec.width,
+                                 spec.height)
+    print("filter counters gpu", {k: book.get(k) for k in fc}, "\noracle", fc, flush=True)
+    for k in ("FilterSample", "DenoiseEarlyOut", "DenoiseRejectMissing"):
+        assert book.get(k, 0) == fc[k], k
+    assert fc["FilterSample"] > 0
+    for k in ("DenoiseRejectNormal", "DenoiseRejectDepth", "DenoiseRejectLum"):
+        assert abs(book.get(k, 0) - fc[k]) <= max(3, 1e-3 * fc[k]), k
+    for k in ("DenoiseWeightMax", "DenoiseMaxLum", "DenoiseMaxLumBoost"):
+        assert abs(book[k] - fc[k]) <= 1 + 1e-5 * abs(fc[k]), k
+    scene.counters_reset()
+    assert scene.counters_all() == {}
+
+
+def test_failed_render_start_leaves_no_render():
+    """A render start that fails after the previous render finished (here: frame buffers beyond the
+    GPU's memory) leaves no render: Tick and the AOV read report SR_ERR_STATE instead of using the
+    previous render's buffers with the new sizes; a later render works."""
+    from sharpray_amd import SharpRayError, Renderer, _native as N, scenes
+    spec = scenes.config_plain_cpu(scale=0.125, spp=1)
+    scene, view = scenes.instantiate(spec)
+    r = Renderer(scene, view, spec.width, spec.height, 32, 1, 1, 0.0, spec.bounces, spec.indirect_clamp)
+    r.wait()
+    first = r.radiance.copy()
+    with pytest.raises(SharpRayError):
+        Renderer(scene, view, 32768, 32768, 32, 1, 1, 0.0, spec.bounces, spec.indirect_clamp)
+    s, t = ctypes.c_uint32(), ctypes.c_uint32()
+    assert N.lib().sr_render_tick(scene.ctx, ctypes.byref(s), ctypes.byref(t)) == -2
+    out = np.empty(spec.width * spec.height * 3, np.float32)
+    assert N.lib().sr_render_read(scene.ctx, N.fptr(out), None, None, None, None, None, None) == -2
+    r2 = Renderer(scene, view, spec.width, spec.height, 32, 1, 1, 0.0, spec.bounces, spec.indirect_clamp)
+    r2.wait()
+    assert np.array_equal(bits(r2.radiance), bits(first))
