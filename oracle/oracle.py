@@ -128,10 +128,12 @@ class OracleScene:
         self.spec = spec
         self.h = self.L.oc_scene_new()
         self._refs = []
-        for t in spec.textures:
+        modes = getattr(spec, "texture_modes", {})
+        for i, t in enumerate(spec.textures):
             rgb = _keep(t, np.float32)
             self._refs.append(rgb)
-            self.L.oc_scene_add_texture(self.h, f(rgb), rgb.shape[1], rgb.shape[0], 0, 1.0, 1.0)
+            flt, tx, ty = modes.get(i, (0, 1.0, 1.0))
+            self.L.oc_scene_add_texture(self.h, f(rgb), rgb.shape[1], rgb.shape[0], int(flt), float(tx), float(ty))
         self.mesh_ids = []
         for m, bins in zip(spec.meshes, spec.mesh_bins):
             m = _keep(m, np.float32)

@@ -44,6 +44,7 @@ class SceneSpec:
     mesh_bins: list = field(default_factory=list)
     objects: list = field(default_factory=list)
     textures: list = field(default_factory=list)     # (h, w, 3) float32
+    texture_modes: dict = field(default_factory=dict)  # texture index -> (filter, tiling x, tiling y)
     sky_texture: int = -1
     sky_multiplier: float = 1.0
     sky_yaw: float = 0.0
@@ -71,7 +72,7 @@ def instantiate(spec, device=0, counters=False):
     scene = Scene(device)
     if counters:
         scene.counters_enable(True)
-    textures = [Texture(t) for t in spec.textures]
+    textures = [Texture(t, *_texture_mode(spec, i)) for i, t in enumerate(spec.textures)]
     meshes = [Mesh(m, split_bins=b) for m, b in zip(spec.meshes, spec.mesh_bins)]
     for m in meshes:  # registration order = spec index = device mesh id
         scene.mesh_id(m)
@@ -101,6 +102,11 @@ def instantiate(spec, device=0, counters=False):
     v = spec.view
     view = View(Transform(v[0:3], Quat(*v[3:7])), float(v[7]), float(v[8]))
     return scene, view
+
+
+def _texture_mode(spec, i):
+    f, tx, ty = getattr(spec, "texture_modes", {}).get(i, (0, 1.0, 1.0))
+    return f, (tx, ty)
 
 
 def render(spec, device=0, **kw):
@@ -473,4 +479,72 @@ def config_analytic(scale=1.0, spp=4):
         s.objects.append(ObjectSpec(f"sphere_{i}", _identity(), _mat((1.0, 0.35, 0.05), rough, 1.0), 1, -1,
                                     np.array(list(c) + [0.75, 0, 0], dtype=np.float32)))
     s.view = _view((1.0, 5.0, -2.0), 35.0, 75.0)
+    return s
+
+
+def _normal_bumps(w, h, seed, strength=0.6):
+    """A tangent-space normal map (texels in [0, 1]) of random smooth bumps: texel = (n + 1) / 2 with
+    the G channel flipped as Texture.SampleNormal (texture.cs:71-75) reads it back."""
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    hgt = np.zeros((h, w))
+    for _ in range(12):
+        cx, cy, r = rng.uniform(0, w), rng.uniform(0, h), rng.uniform(2, w / 4)
+        dx = (xx - cx + w / 2) % w - w / 2
+        dy = (yy - cy + h / 2) % h - h / 2
+        hgt += np.exp(-(dx * dx + dy * dy) / (2 * r * r)) * rng.uniform(-1, 1)
+    gx = (np.roll(hgt, -1, 1) - np.roll(hgt, 1, 1)) * strength
+    gy = (np.roll(hgt, -1, 0) - np.roll(hgt, 1, 0)) * strength
+    n = np.stack([-gx, -gy, np.ones_like(hgt)], -1)
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    return np.stack([(n[..., 0] + 1) / 2, (1 - n[..., 1]) / 2, (n[..., 2] + 1) / 2], -1).astype(np.float32)
+
+
+def _unit_axis(v):
+    v = np.asarray(v, np.float64)
+    return tuple((v / np.linalg.norm(v)).tolist())
+
+
+def config_textured_transformed(scale=1.0, spp=4, seed=SEED_BASE + 5):
+    """Object transforms and textured materials (program.cs:95-130 uses both): a box floor with
+    tiled color, roughness and normal-map textures (Material.Sample*, scene.cs:173-187); an
+    icosphere mesh rotated, translated and non-uniformly scaled, with a metallic texture; an
+    ellipsoid (scaled, rotated sphere); a rotated box; a glass ellipsoid; under a log-normal sky."""
+    s = SceneSpec(name="textured_transformed")
+    s.width = max(8, int(round(160 * scale)))
+    s.height = max(8, int(round(96 * scale)))
+    s.min_samples = s.max_samples = spp
+    s.bounces = 6
+    s.indirect_clamp = 10.0
+    rng = np.random.default_rng(seed)
+    s.textures.append(lognormal_env(64, 32, 1.0, seed + 100))
+    s.sky_texture = 0
+    yy, xx = np.mgrid[0:32, 0:32]
+    checker = ((xx // 8 + yy // 8) % 2)[..., None]
+    s.textures.append((0.25 + 0.6 * checker * np.array([1.0, 0.8, 0.6]) + rng.uniform(0, 0.1, (32, 32, 3))
+                       ).astype(np.float32))                                   # 1 floor color
+    s.textures.append(np.repeat(rng.uniform(0.2, 1.0, (16, 16, 1)), 3, 2).astype(np.float32))  # 2 roughness
+    s.textures.append(_normal_bumps(32, 32, seed + 7))                          # 3 normal map
+    s.textures.append(np.repeat(rng.uniform(0.0, 1.0, (8, 8, 1)), 3, 2).astype(np.float32))   # 4 metallic
+    s.texture_modes = {1: (0, 6.0, 1.7), 2: (0, 6.0, 1.7), 3: (0, 6.0, 1.7), 4: (1, 2.0, 2.0)}
+    s.objects.append(ObjectSpec("floor", _identity(), _mat(1.0, 1.0), 2, -1,
+                                np.array([-30, -0.2, -2, 30, 0, 15], np.float32), textures=(1, 2, -1, 3)))
+    ico = icosphere(3, seed=seed)
+    s.meshes.append(triangles_build(ico, None, None, smooth=True))
+    s.mesh_bins.append(8)
+    q = Quat.angle_axis(math.radians(-40.0), _unit_axis((0.3, 1.0, 0.1))).xyzw
+    s.objects.append(ObjectSpec("ico", np.array([-1.2, 0.9, 4.5] + list(q) + [0.8, 1.1, 0.6], np.float32),
+                                _mat((0.8, 0.5, 0.3), 0.35, 0.9), 0, 0, textures=(-1, -1, 4, -1)))
+    q = Quat.angle_axis(math.radians(25.0), (0.0, 0.0, 1.0)).xyzw
+    s.objects.append(ObjectSpec("ellipsoid", np.array([1.4, 0.7, 4.0] + list(q) + [1.6, 0.7, 0.9], np.float32),
+                                _mat((0.2, 0.5, 0.9), 0.5, 0.0), 1, -1, np.array([0, 0, 0, 0.8, 0, 0], np.float32)))
+    q = Quat.angle_axis(math.radians(35.0), _unit_axis((1.0, 1.0, 0.0))).xyzw
+    s.objects.append(ObjectSpec("box", np.array([0.2, 0.5, 6.0] + list(q) + [1.0, 1.0, 1.0], np.float32),
+                                _mat((0.9, 0.9, 0.9), 0.15, 1.0), 2, -1,
+                                np.array([-0.5, -0.5, -0.5, 0.5, 0.5, 0.5], np.float32)))
+    q = Quat.angle_axis(math.radians(60.0), (0.0, 1.0, 0.0)).xyzw
+    s.objects.append(ObjectSpec("glass", np.array([0.0, 0.45, 3.0] + list(q) + [1.2, 0.9, 0.8], np.float32),
+                                _mat((0.95, 0.97, 1.0), 0.05, 0.0, 1.0, 1.5), 1, -1,
+                                np.array([0, 0, 0, 0.5, 0, 0], np.float32)))
+    s.view = _view((0.0, 1.6, -1.5), 15.0, 70.0)
     return s

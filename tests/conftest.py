@@ -16,3 +16,18 @@ def pytest_configure(config):
 def oracle_lib():
     from oracle import oracle as O
     return O
+
+
+def pytest_sessionfinish(session, exitstatus):
+    """Writes the divergent-path fractions the GPU parity tests measured (name, fraction, bar,
+    pixels) to gpurun_out/parity_divergence.txt."""
+    from tests.helpers import DIVERGENCE
+    if not DIVERGENCE:
+        return
+    out = os.path.join(ROOT, "gpurun_out")
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, "parity_divergence.txt"), "w") as f:
+        f.write("# case  divergent_fraction  bar  pixels  (pixels whose radiance differs from the oracle's by "
+                "more than 1e-3 relative)\n")
+        for name, frac, bar, px in DIVERGENCE:
+            f.write(f"{name:40s} {frac:.6f} {bar:.6f} {px}\n")

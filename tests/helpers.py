@@ -18,3 +18,12 @@ def rel_mse(x, ref, eps=1e-2):
     x = np.asarray(x, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(np.mean((x - ref) ** 2 / (ref ** 2 + eps)))
+
+
+# Divergent-path fractions measured by the GPU parity tests, written out at the end of the session
+# (tests/conftest.py) so every run leaves a summary of the measured fractions next to their bars.
+DIVERGENCE = []
+
+
+def record_divergence(name, frac, bar, pixels):
+    DIVERGENCE.append((name, float(frac), float(bar), int(pixels)))

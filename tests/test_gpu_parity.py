@@ -13,12 +13,26 @@ Bars (per the tracer's parity contract):
 import numpy as np
 import pytest
 
-from tests.helpers import bits, rel_err, rel_mse
+from tests.helpers import bits, rel_err, rel_mse, record_divergence
 
 pytestmark = pytest.mark.gpu
 
 RAD_TOL = 1e-3
 MAX_DIVERGENT = 0.02
+# Per-case bars on the fraction of divergent pixels (radiance off by more than RAD_TOL relative):
+# about 5-10x the fraction measured on MI355X, and never below 8 pixels of the image.
+# Measured (MI355X, this suite): plain_cpu xorwow 1.5e-5, counter 4.6e-5, analytic 1.6e-3, height fog
+# 6.1e-5 (fog radiance 0), transparent 1.7e-3, alias 1.5e-5, homogeneous medium 6.1e-5, grid 0.
+BARS = {"plain_cpu_xorwow": 2e-4, "plain_cpu_counter": 4e-4, "analytic_xorwow": 1e-2, "height_fog": 5e-4,
+        "height_fog_fog_radiance": 0.0, "transparent": 1e-2, "alias_sampler": 2e-4, "medium_1": 5e-4,
+        "medium_1_fog_radiance": 0.0, "medium_2": 0.0, "medium_2_fog_radiance": 0.0, "adjoint_small_forward": 0.0}
+
+
+def check_divergence(name, frac, pixels):
+    """Records the measured fraction for the session summary and applies the case's bar."""
+    bar = max(BARS.get(name, MAX_DIVERGENT), 8.0 / pixels)
+    record_divergence(name, frac, bar, pixels)
+    assert frac <= bar, (name, frac, bar)
 
 
 @pytest.fixture(scope="module")
@@ -74,7 +88,7 @@ def test_radiance_parity_plain_cpu(plain_cpu, oracle_lib):
     div = err > RAD_TOL
     frac = float(div.mean())
     print(f"plain_cpu radiance: divergent fraction {frac:.5f}, median rel err {np.median(err):.2e}", flush=True)
-    assert frac <= MAX_DIVERGENT
+    check_divergence("plain_cpu_xorwow", frac, err.size)
     ok = ~div
     assert np.array_equal(r.samples, ref["samples"])
     assert np.all(r.samples == spec.max_samples + 1)  # the reference's M/(M+1) quirk (renderer.cs:173,215)
@@ -146,7 +160,7 @@ def test_radiance_parity_counter_rng(plain_cpu, oracle_lib):
     err = rel_err(r.radiance, ref["radiance"])
     frac = float((err > RAD_TOL).mean())
     print(f"plain_cpu counter-rng radiance: divergent fraction {frac:.5f}", flush=True)
-    assert frac <= MAX_DIVERGENT
+    check_divergence("plain_cpu_counter", frac, err.size)
 
 
 def test_radiance_parity_analytic_shapes(oracle_lib):
@@ -156,7 +170,7 @@ def test_radiance_parity_analytic_shapes(oracle_lib):
     err = rel_err(r.radiance, ref["radiance"])
     frac = float((err > RAD_TOL).mean())
     print(f"analytic radiance: divergent fraction {frac:.5f}", flush=True)
-    assert frac <= MAX_DIVERGENT
+    check_divergence("analytic_xorwow", frac, err.size)
 
 
 def test_bilateral_filter_parity(plain_cpu, oracle_lib):
@@ -188,7 +202,8 @@ def test_radiance_parity_height_fog(oracle_lib):
     frac = float((err > RAD_TOL).mean())
     fracf = float((errf > RAD_TOL).mean())
     print(f"height fog: divergent fraction radiance {frac:.5f}, fog radiance {fracf:.5f}", flush=True)
-    assert frac <= MAX_DIVERGENT and fracf <= MAX_DIVERGENT
+    check_divergence("height_fog", frac, err.size)
+    check_divergence("height_fog_fog_radiance", fracf, err.size)
 
 
 def test_radiance_parity_transparent_objects(oracle_lib):
@@ -207,7 +222,7 @@ def test_radiance_parity_transparent_objects(oracle_lib):
     err = rel_err(r.radiance, ref["radiance"])
     frac = float((err > RAD_TOL).mean())
     print(f"transparent: divergent fraction {frac:.5f}", flush=True)
-    assert frac <= MAX_DIVERGENT
+    check_divergence("transparent", frac, err.size)
 
 
 def test_converged_image_matches_float64_oracle(oracle_lib):
@@ -240,7 +255,7 @@ def test_alias_light_sampler_parity(plain_cpu, oracle_lib):
     err = rel_err(r.radiance, ref["radiance"])
     frac = float((err > RAD_TOL).mean())
     print(f"alias sampler radiance: divergent fraction {frac:.5f}", flush=True)
-    assert frac <= MAX_DIVERGENT
+    check_divergence("alias_sampler", frac, err.size)
 
 
 def test_albedo_aov(plain_cpu, oracle_lib):
@@ -290,7 +305,8 @@ def test_radiance_parity_bounded_media(plain_cpu, oracle_lib, kind):
     frac = float((err > RAD_TOL).mean())
     fracf = float((errf > RAD_TOL).mean())
     print(f"medium kind {kind}: divergent fraction radiance {frac:.5f}, fog radiance {fracf:.5f}", flush=True)
-    assert frac <= MAX_DIVERGENT and fracf <= MAX_DIVERGENT
+    check_divergence(f"medium_{kind}", frac, err.size)
+    check_divergence(f"medium_{kind}_fog_radiance", fracf, err.size)
 
 
 def test_adjoint_matches_oracle(oracle_lib):
@@ -303,7 +319,7 @@ def test_adjoint_matches_oracle(oracle_lib):
     img = adj.forward(view, spec.width, spec.height, spec.max_samples, spec.bounces, spec.indirect_clamp, seed=7)
     osc = oracle_lib.OracleScene(spec)
     ref = osc.render(rng_policy=1, rng_seed=7)["radiance"]
-    assert float((rel_err(img, ref) > RAD_TOL).mean()) <= MAX_DIVERGENT
+    check_divergence("adjoint_small_forward", float((rel_err(img, ref) > RAD_TOL).mean()), img.shape[0])
     rng = np.random.default_rng(3)
     delta = rng.normal(size=(spec.width * spec.height, 3)).astype(np.float32)
     g = adj.backward(delta)
@@ -497,8 +513,9 @@ def test_kpn_training_reduces_loss():
 def test_kpn_shipped_weights_denoise_held_out_scene(scene_index):
     """The shipped weights on held-out procedural scenes (index 0: sun-and-gradient sky, 1:
     log-normal sky) at 128^2 and 1 spp, against a 1024 spp reference: the learned denoiser must
-    cut relMSE = mean (x - r)^2 / (r^2 + 0.01) well below the noisy input's and below the best
-    bilateral filter (strength grid) at the same sample count."""
+    cut relMSE = mean (x - r)^2 / (r^2 + 0.01) well below the noisy input's, below the best
+    bilateral filter (strength grid) at the same sample count, and below the best bilateral filter
+    at 4 spp (the learned denoiser reaching the bilateral filter's error at 1/4 of the samples)."""
     from sharpray_amd import RNG_COUNTER, Compositor, KpnDenoiser, train_denoiser as T
     seed = 0x5EED0100 + 50000 + scene_index
     spec = T.random_scene(seed, 128, 1)
@@ -513,9 +530,17 @@ def test_kpn_shipped_weights_denoise_held_out_scene(scene_index):
     kpn = T.rel_mse(den.denoise(r), ref)
     bil = min(T.rel_mse(Compositor(1.0 / 128, st, 1.75, 0.25, 2.0, 0.125, 0.2).filter(r), ref)
               for st in (0.05, 0.2, 1.0, 5.0, 25.0))
-    print(f"kpn shipped weights, scene {scene_index}: noisy {noisy:.4f}, kpn {kpn:.4f}, bilateral {bil:.4f}",
-          flush=True)
+    # The claim: the learned denoiser at 1 spp reaches below the bilateral filter's error at 4x the
+    # samples (its best strength on the 4-spp render, on Radiance and on the sample mean).
+    r4 = Renderer(scene, view, 128, 128, 32, 4, 4, 0.0, spec.bounces, spec.indirect_clamp,
+                  rng_policy=RNG_COUNTER, rng_seed=seed + 7)
+    r4.wait()
+    bil4 = min(T.rel_mse(Compositor(1.0 / 128, st, 1.75, 0.25, 2.0, 0.125, 0.2).filter(r4, sample_mean=sm), ref)
+               for st in (0.05, 0.2, 1.0, 5.0, 25.0) for sm in (False, True))
+    print(f"kpn shipped weights, scene {scene_index}: noisy {noisy:.4f}, kpn@1spp {kpn:.4f}, bilateral@1spp "
+          f"{bil:.4f}, bilateral@4spp {bil4:.4f}", flush=True)
     assert kpn < 0.5 * noisy and kpn < bil
+    assert kpn < bil4
 
 
 def test_compose_tail_parity(oracle_lib):
@@ -557,9 +582,9 @@ def _fbm_sphere_tris(rings, seed):
 
 
 @pytest.mark.parametrize("case", ["ico3_8", "ico5_16", "clusters_32", "fbm_64", "fbm1m_8", "tiny_2"])
-def test_device_bvh_builder_bit_exact(case):
+def test_device_bvh_builder_bit_exact(case, oracle_lib):
     """The device builder (level-wise large nodes, per-wave subtrees, depth-first renumbering)
-    against the host builder, which matches the reference's bvh.cs order: node arrays (bounds,
+    against the oracle's restatement of bvh.cs and against the host builder: node arrays (bounds,
     child, count) and item permutations bit-identical."""
     import time
     from sharpray_amd import Mesh, Scene, scenes, triangles_build
@@ -594,6 +619,14 @@ def test_device_bvh_builder_bit_exact(case):
     assert hn.shape == dn.shape
     assert np.array_equal(hn.view(np.uint32), dn.view(np.uint32))
     assert np.array_equal(hi, di)
+    spec = scenes.SceneSpec(name=f"bvh_{case}")
+    spec.textures.append(np.ones((4, 8, 3), np.float32))
+    spec.sky_texture = 0
+    spec.meshes.append(recs.reshape(-1, 36))
+    spec.mesh_bins.append(bins)
+    on, oi = oracle_lib.OracleScene(spec).bvh(0)
+    assert on.shape == dn.shape, (on.shape, dn.shape)
+    assert np.array_equal(bits(on), bits(dn)) and np.array_equal(oi, di)
 
 
 @pytest.mark.parametrize("tail", [0, 1, 2, 5])

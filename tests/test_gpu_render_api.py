@@ -115,7 +115,7 @@ def test_counter_book_matches_oracle_counters(oracle_lib):
     comp.filter(r)
     book = scene.counters_all()
     for k, v in g.items():
-        assert book[k] == v, k
+        assert book.get(k, 0) == v, k
     assert book["SampleCountMin"] == book["SampleCountMax"] == 9  # M + 1 (renderer.cs:173,215)
     assert book["MeshTriangle"] == spec.triangle_count and book["SceneObject"] == len(spec.objects)
     _, fc = oracle_lib.bilateral(comp.as_array(), r.radiance, r.normals, r.depth, r.variance, spec.width,
