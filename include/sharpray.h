@@ -244,6 +244,17 @@ typedef struct sr_compose_params {
 /* Compositor.Compose (compose.cs:74-99) of the last render without the overlay: Filter of the
  * radiance + FilterFog of the fog radiance, Tonemap, Color.ToPixel. Writes w*h*3 bytes (R, G, B). */
 int sr_compose(sr_ctx* ctx, const sr_compose_params* params, uint8_t* out_rgb);
+/* No counterpart in the reference: sr_compose for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same filters,
+ * tonemap and bytes as sr_compose, on caller device arrays (radiance, fog radiance, normals: w*h*3;
+ * depth, variance: w*h) into caller device memory (d_out_rgb: w*h*3 bytes), asynchronous on the
+ * context's stream. It calls no allocating routine: its two intermediate images live in a static
+ * device array of the library (48 MiB per GPU, held until the process ends), so w*h is limited to
+ * 2,097,152 pixels (SR_ERR_LIMIT above). Calls that overlap on one GPU share that array and race;
+ * the filter counters (sr_counters_read_all) are not collected by this call. */
+int sr_compose_device_noalloc(sr_ctx* ctx, const sr_compose_params* p, const float* d_radiance,
+                              const float* d_radiance_fog, const float* d_normals, const float* d_depth,
+                              const float* d_variance, uint32_t width, uint32_t height, uint8_t* d_out_rgb);
 /* Compositor.Preview (compose.cs:57-65) without the overlay: Tonemap(radiance + fog radiance). */
 int sr_preview(sr_ctx* ctx, int32_t tonemapper, float exposure, uint8_t* out_rgb);
 /* Compositor.FilterFog (compose.cs:206-245) of the last render's fog radiance: w*h*3 floats. */

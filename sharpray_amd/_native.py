@@ -146,6 +146,8 @@ SIGNATURES = {
                                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
                                                    ctypes.c_uint32, ctypes.c_void_p]),
     "sr_compose": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ComposeParams_), c_uint8_p]),
+    "sr_compose_device_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ComposeParams_)] + [ctypes.c_void_p] * 5
+                                  + [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]),
     "sr_preview": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, c_uint8_p]),
     "sr_filter_fog": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, c_float_p]),
     "sr_image_encode_bmp": (ctypes.c_int, [c_uint8_p, ctypes.c_uint32, ctypes.c_uint32, c_uint8_p,

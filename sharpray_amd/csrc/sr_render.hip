@@ -1106,12 +1106,23 @@ __global__ void __launch_bounds__(256) k_tonemap(const float* mainRad, const flo
 
 }  // namespace
 
+// Scratch of sr_compose_device_noalloc: the filtered radiance and the filtered fog radiance (two
+// float RGB images) of up to kNoAllocMaxPixels pixels, in a device array of its own so the entry
+// point allocates nothing. Host code uses the address hipGetSymbolAddress returns per device (the
+// array's name is only a host-side handle); every call on one GPU shares that GPU's array.
+constexpr uint32_t kNoAllocMaxPixels = 2u << 20;  // 2,097,152 pixels (1920 x 1080 and 2048 x 1024 fit)
+static __device__ __align__(256) float scratch[2 * 3 * (size_t)kNoAllocMaxPixels];
+
 // =================================================================================================
 // Host side
 // =================================================================================================
 namespace {
 
 thread_local std::string g_err;
+
+// Per device: the address of `scratch` (hipGetSymbolAddress), resolved once with that device current.
+std::mutex g_scratchMu;
+std::vector<void*> g_scratchPtr;
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
@@ -1604,6 +1615,17 @@ int sr_create(int32_t device, sr_ctx** out) {
     HIP_TRY(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(SR_ERR_INVALID, "no such HIP device");
     HIP_TRY(hipSetDevice(device));
+    {
+        // The allocation-free compose's scratch: resolve this device's copy of the array once.
+        std::lock_guard<std::mutex> lk(g_scratchMu);
+        if (g_scratchPtr.size() < (size_t)n) g_scratchPtr.resize(n, nullptr);
+        if (!g_scratchPtr[device]) {
+            void* ptr = nullptr;
+            HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            if (!ptr) return fail(SR_ERR_HIP, "scratch: no device address");
+            g_scratchPtr[device] = ptr;
+        }
+    }
     sr_ctx* c = new sr_ctx();
     c->device = device;
     book_reset(c);
@@ -2676,7 +2698,8 @@ hipError_t filter_counters(sr_ctx* ctx, unsigned long long*& fc) {
 }
 
 int launch_filter(sr_ctx* ctx, const sr_denoise_params* p, const float* rad, const float* nrm, const float* dep,
-                  const float* var, const float* alb, float albedoLimit, uint32_t W, uint32_t H, float* out) {
+                  const float* var, const float* alb, float albedoLimit, uint32_t W, uint32_t H, float* out,
+                  bool countable = true) {
     if (!ctx || !p || !rad || !nrm || !dep || !var || !out || W == 0 || H == 0)
         return fail(SR_ERR_INVALID, "denoise: bad arguments");
     if (alb && !(albedoLimit > 0.0f)) return fail(SR_ERR_INVALID, "denoise: albedo limit must be > 0");
@@ -2696,7 +2719,7 @@ int launch_filter(sr_ctx* ctx, const sr_denoise_params* p, const float* rad, con
     fp.albedoLimitInv = alb ? 1.0f / (albedoLimit * albedoLimit * 2.0f) : 0.0f;
     const dim3 grid((W + 15) / 16, (H + 15) / 16);
     unsigned long long* fc = nullptr;
-    HIP_TRY(filter_counters(ctx, fc));
+    if (countable) HIP_TRY(filter_counters(ctx, fc));
     auto kern = alb ? (fc ? k_filter<true, true> : k_filter<true, false>)
                     : (fc ? k_filter<false, true> : k_filter<false, false>);
     hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, fp, rad, nrm, dep, var, alb, (int32_t)W, (int32_t)H, out,
@@ -2754,7 +2777,7 @@ int sr_denoise_bilateral_mean(sr_ctx* ctx, const sr_denoise_params* p, float* ou
 // ---- Compose tail ----------------------------------------------------------------------------------
 namespace {
 int launch_filter_fog(sr_ctx* ctx, float radius, float strength, const float* fog, uint32_t W, uint32_t H,
-                      float* out) {
+                      float* out, bool countable = true) {
     if (!(radius >= 0.0f)) return fail(SR_ERR_INVALID, "fog filter: radius must be >= 0");
     // FilterFog's per-image constants (compose.cs:211-213).
     const float radiusPixels = radius * sqrtf((float)(W * H));
@@ -2763,7 +2786,7 @@ int launch_filter_fog(sr_ctx* ctx, float radius, float strength, const float* fo
     if (krf > 4096.0f) return fail(SR_ERR_LIMIT, "fog filter: kernel radius above 4096 pixels");
     const dim3 grid((W + 15) / 16, (H + 15) / 16);
     unsigned long long* fc = nullptr;
-    HIP_TRY(filter_counters(ctx, fc));
+    if (countable) HIP_TRY(filter_counters(ctx, fc));
     hipLaunchKernelGGL(k_filter_fog, grid, dim3(256), 0, ctx->stream, radiusInv, strength, (int32_t)krf, fog,
                        (int32_t)W, (int32_t)H, out, fc);
     HIP_TRY(hipGetLastError());
@@ -2838,6 +2861,39 @@ int sr_compose(sr_ctx* ctx, const sr_compose_params* p, uint8_t* out_rgb) {
     (void)hipFree(dTmp);
     if (rc == SR_OK && ctx->countersOn) book_bump_locked(ctx, SR_CT_TIME_COMPOSE, micros_since(t0));
     return rc;
+}
+
+int sr_compose_device_noalloc(sr_ctx* ctx, const sr_compose_params* p, const float* d_radiance,
+                              const float* d_radiance_fog, const float* d_normals, const float* d_depth,
+                              const float* d_variance, uint32_t width, uint32_t height, uint8_t* d_out_rgb) {
+    if (!ctx || !p || !d_radiance || !d_radiance_fog || !d_normals || !d_depth || !d_variance || !d_out_rgb ||
+        width == 0 || height == 0)
+        return fail(SR_ERR_INVALID, "compose_device_noalloc: bad arguments");
+    if (ctx->device < 0) return fail(SR_ERR_STATE, "host-only context (device -1) cannot render");
+    if ((uint64_t)width * height > kNoAllocMaxPixels)
+        return fail(SR_ERR_LIMIT, "compose_device_noalloc: images above 2,097,152 pixels exceed the scratch");
+    if (p->tonemapper != SR_TONEMAP_REINHARD && p->tonemapper != SR_TONEMAP_LINEAR_SMOOTH)
+        return fail(SR_ERR_INVALID, "unknown tonemapper");
+    HIP_TRY(hipSetDevice(ctx->device));
+    float* sc;
+    {
+        std::lock_guard<std::mutex> lk(g_scratchMu);
+        sc = (size_t)ctx->device < g_scratchPtr.size() ? (float*)g_scratchPtr[ctx->device] : nullptr;
+    }
+    if (!sc) return fail(SR_ERR_STATE, "compose_device_noalloc: scratch not resolved for this device");
+    float* dMain = sc;                              // filtered radiance, w * h * 3
+    float* dFog = sc + 3 * (size_t)kNoAllocMaxPixels;  // filtered fog radiance, w * h * 3
+    // Counting stays off here: the filter counters' buffer would be an allocation.
+    int rc = launch_filter(ctx, &p->denoise, d_radiance, d_normals, d_depth, d_variance, nullptr, 0.0f, width, height,
+                           dMain, false);
+    if (rc) return rc;
+    rc = launch_filter_fog(ctx, p->fog_radius, p->fog_strength, d_radiance_fog, width, height, dFog, false);
+    if (rc) return rc;
+    const uint32_t n = width * height;
+    hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, dMain, dFog, (int32_t)n,
+                       p->tonemapper, p->exposure, d_out_rgb);
+    HIP_TRY(hipGetLastError());
+    return SR_OK;
 }
 
 int sr_image_encode_bmp(const uint8_t* rgb, uint32_t width, uint32_t height, uint8_t* out, uint64_t* size) {
