@@ -9,7 +9,11 @@ timed region).
 Prints one JSON line: Msamples/s, the dominant kernel's roofline (the trace kernel: closest hit of
 a segment's rays plus the previous segment's shadow rays; algorithmic bytes per launch from the
 reference's traversal counters over its average launch time, measured with HIP events on the render
-stream) and the CPU oracle's throughput on a bounded row band of the same frame.
+stream), the CPU oracle's throughput on a bounded row band of the same frame, and, after the timed
+frames, the metric's second half: "denoise" (relMSE against a 4096-spp render of the learned
+denoiser at 1 spp and the bilateral filter at 4 spp, and the plain tracer's time to the same
+relMSE), "fog" (the 128^3 grid-fog frame at 4 spp: throughput and the 7x7 bilateral filter's
+relMSE) and "adjoint" (forward and backward time at 512 x 512, 16 spp, 100k triangles).
 """
 import argparse
 import json
@@ -33,30 +37,90 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def processor_count():
+    """The CPUs this process may run on, as .NET's Environment.ProcessorCount counts them (the
+    reference's worker count, renderer.cs:88): the affinity mask, capped by a cgroup CPU quota."""
+    n = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = int(q) / int(period)
+    except (OSError, ValueError):
+        try:
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_quota_us") as f:
+                q = int(f.read())
+            with open("/sys/fs/cgroup/cpu/cpu.cfs_period_us") as f:
+                period = int(f.read())
+            if q > 0:
+                quota = q / period
+        except (OSError, ValueError):
+            pass
+    if quota is not None:
+        n = max(1, min(n, int(-(-quota // 1))))
+    return n, quota
+
+
+def host_info():
+    """nproc, os.cpu_count(), the cgroup quota and the CPU model name (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    n, quota = processor_count()
+    return {"nproc": len(os.sched_getaffinity(0)), "os_cpu_count": os.cpu_count(), "cgroup_cpu_quota": quota,
+            "processor_count": n, "cpu_model": model}
+
+
 def cpu_baseline(spec, seconds_target=15.0):
-    """Times the CPU oracle (the C++ restatement with the reference's 32x32-block scheduler) on a
-    centred row band of the same frame, sized to about `seconds_target` seconds."""
+    """Times the CPU oracle (the C++ restatement with the reference's 32x32-block scheduler, built
+    here with -O3 -march=native -ffp-contract=off) on a centred row band of the same frame, sized
+    to about `seconds_target` seconds, on as many threads as .NET's ProcessorCount would give the
+    reference's renderer on this host."""
+    import tempfile
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
-    t0 = time.perf_counter()
-    osc = O.OracleScene(spec)
-    setup = time.perf_counter() - t0
-    mid = spec.height // 2
-    probe_rows = 2
-    t0 = time.perf_counter()
-    osc.render(mid, mid + probe_rows, threads=threads)
-    dt = max(time.perf_counter() - t0, 1e-3)
-    rows = int(max(probe_rows, min(spec.height, probe_rows * seconds_target / dt)))
-    y0 = max(0, mid - rows // 2)
-    y1 = min(spec.height, y0 + rows)
-    t0 = time.perf_counter()
-    out = osc.render(y0, y1, threads=threads)
-    dt = time.perf_counter() - t0
+    host = host_info()
+    threads = host["processor_count"]
+    with tempfile.TemporaryDirectory() as tmp:
+        O.build_native(tmp)
+        t0 = time.perf_counter()
+        osc = O.OracleScene(spec, precision="f32_native")
+        setup = time.perf_counter() - t0
+        mid = spec.height // 2
+        probe_rows = 2
+        t0 = time.perf_counter()
+        osc.render(mid, mid + probe_rows, threads=threads)
+        dt = max(time.perf_counter() - t0, 1e-3)
+        rows = int(max(probe_rows, min(spec.height, probe_rows * seconds_target / dt)))
+        y0 = max(0, mid - rows // 2)
+        y1 = min(spec.height, y0 + rows)
+        t0 = time.perf_counter()
+        out = osc.render(y0, y1, threads=threads)
+        dt = time.perf_counter() - t0
+        del osc
     samples = float(out["samples"].astype("int64").sum() - (y1 - y0) * spec.width)  # count paths traced
     msps = samples / dt / 1e6
     return {"value": round(msps, 4), "unit": "Msamples/s", "cores": threads, "kind": "port",
             "sample": f"rows {y0}-{y1} of the {spec.width}x{spec.height} frame at {spec.max_samples} spp "
-                      f"({int(samples)} samples, {dt:.1f} s, scene build {setup:.1f} s)"}
+                      f"({int(samples)} samples, {dt:.1f} s, scene build {setup:.1f} s; oracle built -O3 "
+                      f"-march=native -ffp-contract=off; {threads} threads)",
+            "host": host}
 
 
 def bench_adjoint(args, spec, scene, view):
@@ -149,14 +213,22 @@ def bench_cnn(args, spec, scene, view):
     print(json.dumps(line), flush=True)
 
 
-def bench_denoise(args, spec, scene, view):
-    """Equal-quality / equal-time comparison at 1080p on the 1M-triangle scene: the plain tracer's
-    relMSE = mean (x - r)^2 / (r^2 + 0.01) against a high-spp render (counter RNG, other seed) as a
-    function of its samples, the learned denoiser (shipped weights) and the bilateral filter (7x7
-    window, strength chosen by grid search against the reference, which favours it; on the per-pixel
-    sample mean and on the Radiance AOV) on 1 and 4 spp renders, and the time the plain tracer needs to reach each denoiser's relMSE (log-log
-    interpolation of its relMSE-time curve). value = that time for the learned denoiser at 1 spp
-    divided by the denoiser's own time (render + denoise)."""
+def _relmse_fn(ref, H, W):
+    import numpy as np
+
+    def rel(x):
+        x = np.asarray(x, np.float64).reshape(H, W, 3)
+        return float(np.mean((x - ref) ** 2 / (ref ** 2 + 0.01)))
+    return rel
+
+
+def denoise_summary(spec, scene, view, ref_spp, bilateral_radius=0.000694):
+    """Equal-quality / equal-time comparison at the spec's size: the plain tracer's relMSE =
+    mean (x - r)^2 / (r^2 + 0.01) against a ref_spp render (counter RNG, other seed) as a function
+    of its samples, the learned denoiser (shipped weights) and the bilateral filter (7x7 window at
+    1080p, strength chosen by grid search against the reference, which favours it; on the per-pixel
+    sample mean and on the Radiance AOV) on 1 and 4 spp renders, and the time the plain tracer needs
+    to reach each denoiser's relMSE (log-log interpolation of its relMSE-time curve)."""
     import math
     import numpy as np
     from sharpray_amd import RNG_COUNTER, Compositor, KpnDenoiser, Renderer
@@ -171,13 +243,9 @@ def bench_denoise(args, spec, scene, view):
 
     render(1, 3)  # warm-up
     t0 = time.perf_counter()
-    ref = render(args.ref_spp, 999)[0].sample_mean.reshape(H, W, 3).astype(np.float64)
-    log(f"[bench] reference {args.ref_spp} spp in {time.perf_counter() - t0:.1f} s")
-
-    def rel(x):
-        x = np.asarray(x, np.float64).reshape(H, W, 3)
-        return float(np.mean((x - ref) ** 2 / (ref ** 2 + 0.01)))
-
+    ref = render(ref_spp, 999)[0].sample_mean.reshape(H, W, 3).astype(np.float64)
+    log(f"[bench] reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+    rel = _relmse_fn(ref, H, W)
     # The plain tracer's per-pixel sample mean (its Radiance AOV divides pixels at the sample cap by
     # M + 1, like the reference; relmse_radiance_aov is that image's error).
     plain = []
@@ -209,7 +277,7 @@ def bench_denoise(args, spec, scene, view):
         for tag, mean in (("", True), ("_radiance_aov", False)):
             best = None
             for strength in (0.05, 0.2, 1.0, 5.0, 25.0):
-                comp = Compositor(0.000694, strength, 1.75, 0.25, 2.0, 0.125, 0.2)
+                comp = Compositor(bilateral_radius, strength, 1.75, 0.25, 2.0, 0.125, 0.2)
                 tf = time.perf_counter()
                 out = comp.filter(r, sample_mean=mean)
                 tf = time.perf_counter() - tf
@@ -222,6 +290,14 @@ def bench_denoise(args, spec, scene, view):
         v["plain_ms_at_same_relmse"] = round(matched_ms(v["relmse"]), 2)
         v["speedup_vs_plain"] = round(v["plain_ms_at_same_relmse"] / v["ms"], 3)
         log(f"[bench] {k}: {v}")
+    return plain, rows
+
+
+def bench_denoise(args, spec, scene, view):
+    """The learned denoiser's relMSE-matched speedup over the plain tracer (denoise_summary).
+    value = the plain tracer's time to the relMSE of the learned denoiser at 1 spp divided by the
+    denoiser's own time (render + denoise)."""
+    plain, rows = denoise_summary(spec, scene, view, args.ref_spp)
     line = {
         "metric": "relMSE-matched time: learned denoiser vs plain @1080p/1M tris",
         "value": rows["kpn_1spp"]["speedup_vs_plain"],
@@ -235,12 +311,88 @@ def bench_denoise(args, spec, scene, view):
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic (seeded procedural scene, shipped trained weights)",
-        "config": {"workload": spec.name, "width": W, "height": H, "bounces": spec.bounces,
+        "config": {"workload": spec.name, "width": spec.width, "height": spec.height, "bounces": spec.bounces,
                    "triangles": spec.triangle_count, "reference_spp": args.ref_spp, "relmse_eps": 0.01},
         "plain": plain,
         "denoisers": rows,
     }
     print(json.dumps(line), flush=True)
+
+
+def fog_summary(ref_spp, frames=3):
+    """The fog configuration at 1080p: the 1M-triangle scene in a 128^3 fp16 density grid, 4 spp,
+    depth 8. Throughput over `frames` frames, then relMSE against a ref_spp render of the noisy
+    4 spp sample mean, the 7x7 bilateral filter (radius 3, 29 taps) with and without the albedo
+    guide, and the learned denoiser (trained on fog-free scenes)."""
+    import numpy as np
+    from sharpray_amd import RNG_COUNTER, Compositor, KpnDenoiser, Renderer, scenes
+    spec = scenes.config_fog(scale=1.0, spp=4)
+    scene, view = scenes.instantiate(spec)
+    scene.build()
+    W, H = spec.width, spec.height
+
+    def render(spp, seed):
+        r = Renderer(scene, view, W, H, 32, spp, spp, 0.0, spec.bounces, spec.indirect_clamp,
+                     rng_policy=RNG_COUNTER, rng_seed=seed)
+        r.wait()
+        return r
+
+    render(4, 1)
+    t0 = time.perf_counter()
+    for i in range(frames):
+        r = render(4, 7)
+    ms = 1e3 * (time.perf_counter() - t0) / frames
+    t0 = time.perf_counter()
+    ref = render(ref_spp, 999).sample_mean.reshape(H, W, 3).astype(np.float64)
+    log(f"[bench] fog reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+    rel = _relmse_fn(ref, H, W)
+    out = {"workload": spec.name, "ms_per_frame": round(ms, 3), "msamples_per_s": round(W * H * 4 / ms / 1e3, 3),
+           "reference_spp": ref_spp, "relmse_noisy_4spp": round(rel(r.sample_mean), 6)}
+    radius = 0.000694  # ceil(3 * 0.000694 * sqrt(1920 * 1080)) = 3: a 7x7 window
+    for tag, kw in (("bilateral_4spp", {"sample_mean": True}), ("bilateral_4spp_albedo_guided", {"albedo_limit": 0.15})):
+        best = min((rel(Compositor(radius, st, 1.75, 0.25, 2.0, 0.125, 0.2).filter(r, **kw)), st)
+                   for st in (0.05, 0.2, 1.0, 5.0, 25.0))
+        out[tag] = {"relmse": round(best[0], 6), "strength": best[1]}
+    den = KpnDenoiser(scene)
+    den.load(KpnDenoiser.pretrained_weights())
+    out["kpn_4spp_relmse"] = round(rel(den.denoise(r)), 6)
+    log(f"[bench] fog: {out}")
+    return out
+
+
+def adjoint_summary(steps=5):
+    """Forward and backward time of the adjoint configuration (512 x 512, 16 spp, 100k triangles,
+    16 materials + the 256 x 128 environment) at parameters perturbed +-30 % from the target's."""
+    import numpy as np
+    from sharpray_amd import Adjoint, scenes
+    spec = scenes.config_adjoint(scale=1.0, spp=16)
+    scene, view = scenes.instantiate(spec)
+    adj = Adjoint(scene)
+    fw = (view, spec.width, spec.height, spec.max_samples, spec.bounces, spec.indirect_clamp)
+    target = adj.forward(*fw, seed=1)
+    p = adj.params()
+    p0 = (p * np.random.default_rng(0).uniform(0.7, 1.3, size=p.shape)).astype(np.float32)
+    m = p0[:len(spec.objects) * 4].reshape(-1, 4)
+    m[:, :3] = np.clip(m[:, :3], 0.0, 1.0)
+    m[:, 3] = np.clip(m[:, 3], 0.05, 1.0)
+    adj.set_params(p0)
+    img = adj.forward(*fw, seed=2)
+    adj.backward(img - target)
+    tf, tb = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        img = adj.forward(*fw, seed=2)
+        t1 = time.perf_counter()
+        adj.backward(img - target)
+        tf.append(t1 - t0)
+        tb.append(time.perf_counter() - t1)
+    mf, mb = 1e3 * sum(tf) / steps, 1e3 * sum(tb) / steps
+    paths = spec.width * spec.height * spec.max_samples
+    out = {"workload": spec.name, "forward_ms": round(mf, 3), "backward_ms": round(mb, 3),
+           "backward_over_forward": round(mb / mf, 3), "backward_mpaths_per_s": round(paths / mb / 1e3, 3),
+           "params": adj.param_count}
+    log(f"[bench] adjoint: {out}")
+    return out
 
 
 def main():
@@ -257,6 +409,10 @@ def main():
                          "cnn: the U-Net denoiser on a 1080p 4 spp render; denoise: relMSE-matched time of the learned "
                          "denoiser and the bilateral filter against the plain tracer at 1080p")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="plain workload: skip the denoiser, fog and adjoint summaries after the timed frames")
+    ap.add_argument("--extras-ref-spp", type=int, default=4096,
+                    help="reference samples of the denoiser and fog summaries' relMSE")
     ap.add_argument("--light-sampler", choices=["cdf", "alias"], default="cdf",
                     help="environment light sampling: the reference's row/column CDFs or the alias table")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -380,6 +536,26 @@ def main():
         cpu = cpu_baseline(spec, args.cpu_seconds)
         log(f"[bench] CPU: {cpu}")
 
+    # The metric's second half (relMSE-matched time of the denoisers and the adjoint against the
+    # plain tracer), measured after the timed frames on the same GPU.
+    extras = {}
+    if args.workload == "plain" and not args.no_extras:
+        _, rows = denoise_summary(spec, scene, view, args.extras_ref_spp)
+        k1, b4 = rows["kpn_1spp"], rows["bilateral_4spp"]
+        extras["denoise"] = {
+            "reference_spp": args.extras_ref_spp, "relmse_eps": 0.01,
+            "kpn_1spp": {k: k1[k] for k in ("relmse", "ms", "denoise_ms", "plain_ms_at_same_relmse",
+                                            "speedup_vs_plain")},
+            "bilateral_4spp": {k: b4[k] for k in ("relmse", "ms", "strength", "plain_ms_at_same_relmse",
+                                                  "speedup_vs_plain")},
+            "bilateral_4spp_radiance_aov_relmse": rows["bilateral_4spp_radiance_aov"]["relmse"],
+            "kpn_4spp_relmse": rows["kpn_4spp"]["relmse"],
+            "kpn_1spp_below_bilateral_4spp": k1["relmse"] < b4["relmse"],
+        }
+        del scene
+        extras["fog"] = fog_summary(args.extras_ref_spp)
+        extras["adjoint"] = adjoint_summary()
+
     line = {
         "metric": "Msamples/s @1080p/1M tris" + (" + 128^3 grid fog" if args.workload == "fog" else ""),
         "value": round(msps, 3),
@@ -407,6 +583,9 @@ def main():
         "fused_tail_ms_per_step": round(tail_ms / args.steps, 3),
         "cpu_baseline": cpu,
     }
+    if cpu:
+        line["gpu_over_cpu"] = round(msps / cpu["value"], 2)
+    line.update(extras)
     print(json.dumps(line), flush=True)
 
 

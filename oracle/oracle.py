@@ -98,18 +98,31 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def _load(key, path):
+    h = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(h, name)
+        fn.restype = res
+        fn.argtypes = args
+    _libs[key] = h
+    return h
+
+
 def lib(precision="f32"):
     if precision not in _libs:
         path = os.path.join(HERE, f"liboracle_{precision}.so")
         if not os.path.exists(path):
             build()
-        h = ctypes.CDLL(path)
-        for name, (res, args) in _SIGS.items():
-            fn = getattr(h, name)
-            fn.restype = res
-            fn.argtypes = args
-        _libs[precision] = h
+        _load(precision, path)
     return _libs[precision]
+
+
+def build_native(out_dir):
+    """Builds the float oracle with -O3 -march=native -ffp-contract=off for the host it runs on into
+    out_dir and makes it available as precision "f32_native" (used to time the CPU baseline)."""
+    path = os.path.join(out_dir, "liboracle_f32_native.so")
+    subprocess.run(["make", "-s", "-C", HERE, "native", f"OUT={path}"], check=True)
+    return _load("f32_native", path)
 
 
 def f(a):

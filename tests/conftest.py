@@ -31,3 +31,11 @@ def pytest_sessionfinish(session, exitstatus):
                 "more than 1e-3 relative)\n")
         for name, frac, bar, px in DIVERGENCE:
             f.write(f"{name:40s} {frac:.6f} {bar:.6f} {px}\n")
+
+
+def pytest_terminal_summary(terminalreporter, exitstatus, config):
+    """The measured divergent-path fractions, printed with the session's result."""
+    from tests.helpers import DIVERGENCE
+    if DIVERGENCE:
+        terminalreporter.write_line("divergent-path fractions (case: measured / bar):")
+        terminalreporter.write_line("  " + ", ".join(f"{n} {f:.2e}/{b:.1e}" for n, f, b, _ in DIVERGENCE))
