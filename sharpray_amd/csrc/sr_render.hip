@@ -753,6 +753,7 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     }
     bool again = false, converged = false, maxed = false;
     unsigned long long countMinKey = 0, countMaxKey = 0, varKey = 0;
+    uint32_t doneBlock = ~0u;  // the block of a pixel that finished in this pass
     // Fields are read and written one by one: a sample touches the accumulator's first 48 B and the
     // path's radiance and random state; the rest is read once, when the pixel finishes.
     PixAcc* A = F.acc + q;
@@ -818,10 +819,24 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
             countMinKey = ~(unsigned long long)count;
             countMaxKey = count;
             varKey = decimal_key(var);
-            // The pixel's block (renderer.cs:125-134) is complete once all its pixels are final.
-            const uint32_t bxi = x / F.bs, byi = y / F.bs;
+            doneBlock = (y / F.bs) * F.bx + x / F.bs;
+        }
+    }
+    // Tick's progress: the pixel's block (renderer.cs:125-134) is complete once all its pixels are
+    // final. A wave's 8x8 tile lies in one block when the block size is a multiple of 8: then one
+    // atomic per wave (per-pixel atomics on a few thousand addresses at a frame's last sample
+    // serialise); otherwise one per pixel.
+    const unsigned long long doneMask = __ballot(doneBlock != ~0u);
+    if (doneMask) {
+        const int leader = __ffsll(doneMask) - 1;
+        const uint32_t lb = __shfl(doneBlock, leader);
+        const bool oneBlock = __ballot(doneBlock != ~0u && doneBlock != lb) == 0;
+        const uint32_t mine = oneBlock ? ((int)lane_id() == leader ? lb : ~0u) : doneBlock;
+        if (mine != ~0u) {
+            const uint32_t add = oneBlock ? (uint32_t)__popcll(doneMask) : 1u;
+            const uint32_t bxi = mine % F.bx, byi = mine / F.bx;
             const uint32_t bw = min(F.bs, F.W - bxi * F.bs), bh = min(F.bs, F.H - byi * F.bs);
-            if (atomicAdd(&F.blockPixels[byi * F.bx + bxi], 1u) + 1u == bw * bh) atomicAdd(F.blocksDone, 1u);
+            if (atomicAdd(&F.blockPixels[mine], add) + add == bw * bh) atomicAdd(F.blocksDone, 1u);
         }
     }
     bump_wave(F, 1, again);
