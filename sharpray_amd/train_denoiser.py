@@ -3,8 +3,8 @@ the bilateral filter against high-spp references.
 
     python -m sharpray_amd.train_denoiser --scenes 96 --views 4 --size 128 --steps 4000 --out weights.npy
 
-Training scenes are seeded procedural scenes (fBm-displaced spheres of 10k-100k triangles at random
-positions over a ground quad, random GGX materials, log-normal or sun-and-gradient environment
+Training scenes are seeded procedural scenes (fBm-displaced spheres, `--tris` triangles in all, at
+random positions over a ground quad, random GGX materials, log-normal or sun-and-gradient environment
 maps). Each scene is seen from `--views` random cameras, and each view contributes one example:
 the AOVs of a low-spp render (radiance, albedo, normals, depth; the sample count cycles through
 `--noisy-spp`) and the radiance of a high-spp render of the same view, both as per-pixel sample
@@ -24,7 +24,9 @@ import numpy as np
 from . import (RNG_COUNTER, Compositor, KpnDenoiser, Quat, Renderer, Scene, Transform, View, scenes)
 
 
-def random_scene(seed, size, spp):
+def random_scene(seed, size, spp, tris=(10000, 100000)):
+    """A seeded training / evaluation scene; `tris` bounds its noise meshes' total triangle count
+    (drawn log-uniformly, split over 1-3 meshes)."""
     rng = np.random.default_rng(seed)
     s = scenes.SceneSpec(name=f"train_{seed}")
     s.width = s.height = size
@@ -46,8 +48,15 @@ def random_scene(seed, size, spp):
     s.mesh_bins.append(8)
     s.objects.append(scenes.ObjectSpec("ground", scenes._identity(),
                                        scenes._mat(rng.uniform(0.2, 0.8, 3), float(rng.uniform(0.3, 1.0))), 0, 0))
-    for k in range(int(rng.integers(1, 4))):
+    blobs = int(rng.integers(1, 4))
+    # The default range keeps the original draw (40-159 rings per mesh: 10k-100k triangles in all);
+    # another range draws the total from a stream of its own, so the rest of the scene is unchanged.
+    default = tuple(tris) == (10000, 100000)
+    total = math.exp(np.random.default_rng(seed ^ 0x3A7).uniform(math.log(tris[0]), math.log(tris[1])))
+    for k in range(blobs):
         rings = int(rng.integers(40, 160))
+        if not default:
+            rings = max(8, int(round(math.sqrt(total / blobs / 4.0))))  # 4 rings^2 triangles a mesh
         pos, uv, _ = scenes.fbm_sphere(rings, 2 * rings, float(rng.uniform(0.02, 0.2)), 4, seed * 7 + k,
                                        radius=float(rng.uniform(0.3, 0.8)))
         pos = pos.reshape(-1, 3, 3) + np.array([rng.uniform(-1.5, 1.5), 0.0, rng.uniform(1.5, 4.0)])
@@ -105,6 +114,8 @@ def main(argv=None):
     ap.add_argument("--noisy-spp", default="1", help="comma-separated sample counts of the noisy inputs")
     ap.add_argument("--target-spp", type=int, default=1024)
     ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--tris", default="10000,100000",
+                    help="min,max total triangles of a training scene's noise meshes (log-uniform)")
     ap.add_argument("--data-seconds", type=float, default=0.0,
                     help="stop rendering training scenes after this many seconds (0: no limit)")
     ap.add_argument("--lr", type=float, default=5e-4)
@@ -126,6 +137,7 @@ def main(argv=None):
     den.train_begin()
     t0 = time.perf_counter()
     noisy_spp = [int(v) for v in str(args.noisy_spp).split(",")]
+    tris = tuple(int(v) for v in str(args.tris).split(","))
     vrng = np.random.default_rng(args.seed ^ 0x7E3)
     n_ex = 0
     t_build = t_render = 0.0
@@ -135,7 +147,7 @@ def main(argv=None):
             log(f"[train] data time limit reached after {i} scenes")
             break
         ts = time.perf_counter()
-        spec = random_scene(args.seed + i, args.size, noisy_spp[0])
+        spec = random_scene(args.seed + i, args.size, noisy_spp[0], tris)
         scene, view = scenes.instantiate(spec)
         t_build += time.perf_counter() - ts
         n_scenes += 1
