@@ -78,3 +78,22 @@ def test_build_counters_in_counter_book():
     assert scene.counters_all()["SceneObject"] == len(spec.objects)
     scene.counters_reset()
     assert scene.counters_all() == {}
+
+
+def test_cpu_configuration_leaf_sizes_under_reference_sah():
+    """The CPU configuration asks for at most 4 triangles per leaf, a rule the reference's builder
+    does not have (its SAH leaf test decides, bvh.cs:332,472-476). The build keeps the reference's
+    rule, so node order stays bit-exact, and this is the leaf-size report that takes the rule's
+    place: 16 bins over the level-5 icosphere give leaves of 1-6 triangles, 2.88 on average, and
+    all but 14 of 7120 leaves hold at most 4."""
+    from sharpray_amd import scenes
+    spec = scenes.config_plain_cpu()
+    scene, _ = scenes.instantiate(spec, device=-1)
+    st = scene.bvh_stats(0)
+    nodes, _ = scene.bvh(0)
+    counts = nodes[:, 7].view(np.int32)
+    leaves = counts[counts > 0]
+    assert st["LeafCount"] == leaves.size == 7120
+    assert (st["LeafSizeMin"], st["LeafSizeMax"]) == (1, 6)
+    assert abs(st["LeafSizeAvg"] - 2.8764) < 1e-3
+    assert int((leaves > 4).sum()) == 14
