@@ -33,6 +33,7 @@ constexpr int kTaps = 25;                      // 5 x 5 predicted kernel
 constexpr int kStages = kKpnStages;            // kernel cascade, taps 1, 2, 4 and 8 pixels apart
 constexpr int kLogits = kTaps * kStages;       // 100 logits per pixel
 constexpr int kLogitStride = 128;              // their padded channel stride
+constexpr int kStageStride = 32;               // stage s's logits sit in channels [32 s, 32 s + 25)
 
 int pad32(int c) { return (c + 31) & ~31; }
 
@@ -95,8 +96,16 @@ SR_CNN_DEV uint4 fetch8(const ConvArgs& A, int c, int y, int x) {
 // in LDS; per tap each wave loads its NT weight fragments once and reuses them for its MT rows.
 // The result is transposed through LDS so that every lane stores whole 16-byte groups of a pixel's
 // channels.
+// Waves per SIMD the register allocation of k_conv3x3<NT, MT> must allow (1: the compiler's choice).
+#ifndef SR_W2_4
+#define SR_W2_4 1
+#endif
 template <int NT, int MT>
-__global__ void __launch_bounds__(256) k_conv3x3(ConvArgs A) {
+constexpr int conv_waves() { return NT == 2 && MT == 4 ? SR_W2_4 : 1; }
+
+template <int NT, int MT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(conv_waves<NT, MT>())))
+k_conv3x3(ConvArgs A) {
     constexpr int kRows = 4 * MT;
     constexpr int kHaloPx = kHW * (kRows + 2);
     __shared__ uint4 tile[kHaloPx * 4];  // halo pixels x 32 channels of bf16
@@ -232,6 +241,170 @@ __global__ void __launch_bounds__(NW * 64) k_conv3x3_split(ConvArgs A) {
     }
 }
 
+// Persistent 3x3 convolution. Wave w owns output channels [16 NTW cg, 16 NTW (cg + 1)) with
+// cg = w % NG and tile rows [MT rg, MT rg + MT) with rg = w / NG, so a workgroup covers the
+// kCo = NG 16 NTW channels [0, kCo) of a 16 x (4 / NG) MT pixel tile, and walks tiles
+// t = blockIdx.x, + gridDim.x, ..., each as cinPad / 32 steps of one 32-channel K chunk.
+// - Everything a step reads from memory is loaded into registers during the step before: the
+//   halo of its K chunk and, with more than one chunk, the chunk's weights (all 9 taps, the
+//   block's kCo channels). Both are then written to LDS, shared by the 4 waves. With one chunk each
+//   wave keeps its weight fragments in registers for the whole kernel.
+// - LDS rows of four 16-byte groups (a halo pixel's 32 channels, or a weight row's 32 input
+//   channels) have the groups of row p rotated by (p >> 1) & 3, so that the 8 lanes of a 128-byte
+//   LDS cycle read 8 different bank groups.
+// - Each pixel fragment is read once per halo row and column offset and used for the (up to)
+//   three output rows it feeds; every output still sums its taps in order 0..8 within a chunk
+//   and its chunks in order, as k_conv3x3 does.
+// - The MFMA takes the weights as A (rows: output channels) and the pixels as B, so each lane ends
+//   with 4 consecutive channels of one pixel and stores them as one 8-byte word: no LDS transpose.
+// MODE 0 reads a same-resolution source, MODE 2 the concatenation of the nearest-upsampled source
+// a (chunks below ca) and the skip source b.
+#ifndef SR_PIPE_WAVES
+#define SR_PIPE_WAVES 2  // waves per SIMD the register allocation must allow
+#endif
+SR_CNN_DEV int lds_slot(int p, int part) { return p * 4 + (part ^ ((p >> 1) & 3)); }
+
+template <int MODE>
+SR_CNN_DEV const uint4* chunk_src(const ConvArgs& A, int c0, int y, int x) {
+    if (MODE == 0) return (const uint4*)(A.a + ((size_t)y * A.W + x) * A.ca + c0);
+    if (c0 < A.ca) return (const uint4*)(A.a + ((size_t)(y >> 1) * A.wa + (x >> 1)) * A.ca + c0);
+    return (const uint4*)(A.b + ((size_t)y * A.W + x) * A.cb + (c0 - A.ca));
+}
+
+template <int NG, int NTW, int MT, int MODE, bool ONE_CHUNK>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SR_PIPE_WAVES)))
+k_conv3x3_pipe(ConvArgs A, int tilesX, int tiles) {
+    static_assert(MODE == 0 || MODE == 2, "same-resolution or upsample + concatenation sources");
+    constexpr int kRG = 4 / NG;
+    static_assert(NG * kRG == 4, "4 waves split into channel groups x row groups");
+    constexpr int kCo = NG * 16 * NTW;
+    constexpr int kRows = kRG * MT;
+    constexpr int kHaloPx = kHW * (kRows + 2);
+    constexpr int kHaloWords = kHaloPx * 4;
+    constexpr int kIter = (kHaloWords + 255) / 256;
+    constexpr int kWWords = ONE_CHUNK ? 0 : 9 * kCo * 4;  // weight rows [tap][channel], 4 words each
+    constexpr int kWIter = (kWWords + 255) / 256;
+    __shared__ uint4 smem[kHaloWords + kWWords];
+    uint4* const halo = smem;
+    uint4* const wlds = smem + kHaloWords;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int cg = wave % NG, rg = wave / NG;
+    const int col = lane & 15, grp = lane >> 4;
+    const int chunks = ONE_CHUNK ? 1 : A.cinPad >> 5;
+    bf16x8 wf[9][NTW];
+    if (ONE_CHUNK) {
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+            for (int h = 0; h < NTW; ++h)
+                wf[tap][h] = __builtin_bit_cast(
+                    bf16x8, *(const uint4*)(A.w + ((size_t)tap * A.coutPad + (cg * NTW + h) * 16 + col) * A.cinPad +
+                                            grp * 8));
+    }
+    float bias[NTW][4];
+#pragma unroll
+    for (int h = 0; h < NTW; ++h)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = (cg * NTW + h) * 16 + grp * 4 + r;
+            bias[h][r] = co < A.cout ? A.bias[co] : 0.0f;
+        }
+    uint4 st[kIter];
+    uint4 sw[kWIter > 0 ? kWIter : 1];
+    // Step (t, c) into registers. Halo word idx = threadIdx.x + 256 i is channel group idx & 3 of
+    // halo pixel idx >> 2, zero outside the image; weight word e is group e & 3 of row e >> 2 =
+    // tap * kCo + channel.
+    auto load_step = [&](int t, int c) {
+        const int tx0 = (t % tilesX) * kTW - 1, ty0 = (t / tilesX) * kRows - 1;
+#pragma unroll
+        for (int i = 0; i < kIter; ++i) {
+            const int idx = threadIdx.x + 256 * i;
+            const int p = idx >> 2, part = idx & 3;
+            const int y = ty0 + p / kHW, x = tx0 + p % kHW;
+            const bool ok = idx < kHaloWords && x >= 0 && y >= 0 && x < A.W && y < A.H;
+            const uint4 v = chunk_src<MODE>(A, c * 32 + part * 8, ok ? y : 0, ok ? x : 0)[0];
+            st[i] = ok ? v : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < kWIter; ++i) {
+            const int e = min((int)threadIdx.x + 256 * i, kWWords - 1);
+            const int row = e >> 2, tap = row / kCo, co = row % kCo;
+            sw[i] = *(const uint4*)(A.w + ((size_t)tap * A.coutPad + co) * A.cinPad + c * 32 + (e & 3) * 8);
+        }
+    };
+    int t = blockIdx.x, c = 0;
+    if (t < tiles) load_step(t, 0);
+    f32x4 acc[MT][NTW];
+    while (t < tiles) {
+        __syncthreads();  // the previous step's fragments are read
+#pragma unroll
+        for (int i = 0; i < kIter; ++i) {
+            const int idx = threadIdx.x + 256 * i;
+            if (idx < kHaloWords) halo[lds_slot(idx >> 2, idx & 3)] = st[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kWIter; ++i) {
+            const int e = threadIdx.x + 256 * i;
+            if (e < kWWords) wlds[lds_slot(e >> 2, e & 3)] = sw[i];
+        }
+        __syncthreads();
+        const int tn = c + 1 < chunks ? t : t + (int)gridDim.x, cn = c + 1 < chunks ? c + 1 : 0;
+        if (tn < tiles) load_step(tn, cn);
+        if (!ONE_CHUNK) {
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+                for (int h = 0; h < NTW; ++h)
+                    wf[tap][h] = __builtin_bit_cast(bf16x8, wlds[lds_slot(tap * kCo + (cg * NTW + h) * 16 + col, grp)]);
+        }
+        if (c == 0) {
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+                for (int h = 0; h < NTW; ++h) acc[mt][h] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int R = 0; R < MT + 2; ++R)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int p = (rg * MT + R) * kHW + col + dx;
+                const bf16x8 px = __builtin_bit_cast(bf16x8, halo[lds_slot(p, grp)]);
+#pragma unroll
+                for (int dy = 0; dy < 3; ++dy) {
+                    const int mt = R - dy;
+                    if (mt < 0 || mt >= MT) continue;
+#pragma unroll
+                    for (int h = 0; h < NTW; ++h)
+                        acc[mt][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[dy * 3 + dx][h], px, acc[mt][h], 0, 0, 0);
+                }
+            }
+        if (c + 1 == chunks) {
+            // acc[mt][h][r]: channel (cg NTW + h) 16 + 4 grp + r of pixel (tx0 + col, ty0 + rg MT + mt).
+            const int x = (t % tilesX) * kTW + col, ty0 = (t / tilesX) * kRows;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int y = ty0 + rg * MT + mt;
+                if (x >= A.W || y >= A.H) continue;
+                uint16_t* o = A.out + ((size_t)y * A.W + x) * A.coutPad + grp * 4;
+#pragma unroll
+                for (int h = 0; h < NTW; ++h) {
+                    float v[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int co = (cg * NTW + h) * 16 + grp * 4 + r;
+                        v[r] = acc[mt][h][r] + bias[h][r];
+                        if (A.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
+                        if (co >= A.cout) v[r] = 0.0f;
+                    }
+                    *(uint2*)(o + (cg * NTW + h) * 16) = make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3]));
+                }
+            }
+        }
+        t = tn;
+        c = cn;
+    }
+}
+
 // Input features: log(1 + radiance), albedo, normal, 1 / (1 + depth) (0 without a surface).
 __global__ void k_features(const float* rad, const float* alb, const float* nrm, const float* dep, int W, int H,
                            uint16_t* out) {
@@ -253,44 +426,89 @@ __global__ void k_features(const float* rad, const float* alb, const float* nrm,
 
 // One stage of the kernel cascade: out(p) = sum_k w_k(p) in(clamp(p + d (dx_k, dy_k))), w = softmax of
 // the stage's 25 logits, 5 x 5 taps spaced d = 2^stage pixels, edges clamped.
-// The stage's 25 logits sit at halves [25 STAGE, 25 STAGE + 25) of the pixel's 256-byte row; they
-// are read as the four 16-byte words from half 25 STAGE rounded down to a multiple of 8.
-template <int STAGE>
-__global__ void k_kpn_stage(const uint16_t* logits, const float* in, int W, int H, float* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= W * H) return;
-    constexpr int kA = (STAGE * kTaps) & ~7, kO = STAGE * kTaps - kA;
-    static_assert(kO + kTaps <= 32 && kA + 32 <= kLogitStride, "four words hold the stage's logits");
-    const int x = i % W, y = i / W, d = 1 << STAGE;
-    const uint4* src = (const uint4*)(logits + (size_t)i * kLogitStride + kA);
-    uint32_t wds[16];
+// The stage's 25 logits sit at halves [32 STAGE, 32 STAGE + 25) of the pixel's 256-byte row (one
+// aligned 64-byte run) and are read as four 16-byte words.
+// A 256-thread workgroup owns a TS x TS pixel tile (TS * TS / 256 pixels a thread) and first
+// stages the input over the tile and 2d pixels around it in LDS, one plane per colour channel,
+// with the image-edge clamp applied as it loads; each pixel then reads its 25 taps from LDS.
+template <int STAGE, int TS>
+__global__ void __launch_bounds__(256) k_kpn_stage(const uint16_t* logits, const float* in, int W, int H,
+                                                   float* out) {
+    constexpr int kD = 1 << STAGE;
+    constexpr int kSide = TS + 4 * kD;  // tile and 2d halo pixels either side
+    constexpr int kPx = kSide * kSide;
+    constexpr int kPPT = TS * TS / 256;
+    static_assert(kPPT * 256 == TS * TS, "whole pixels per thread");
+    __shared__ float plane[3][kPx];
+    const int tx0 = blockIdx.x * TS, ty0 = blockIdx.y * TS;
+    constexpr int kA = STAGE * kStageStride, kO = 0;
+    static_assert(kTaps <= kStageStride && kA + kStageStride <= kLogitStride, "four words hold the stage's logits");
+    // The thread's logits first (they do not wait for the staged input), then the staging loads in
+    // batches of 8 ahead of their LDS stores.
+    uint32_t wds[kPPT][16];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const uint4 v = src[j];
-        wds[4 * j] = v.x; wds[4 * j + 1] = v.y; wds[4 * j + 2] = v.z; wds[4 * j + 3] = v.w;
-    }
-    float l[kTaps];
-    float m = -INFINITY;
+    for (int q = 0; q < kPPT; ++q) {
+        const int lp = threadIdx.x + 256 * q;
+        const int x = min(tx0 + lp % TS, W - 1), y = min(ty0 + lp / TS, H - 1);
+        const uint4* src = (const uint4*)(logits + ((size_t)y * W + x) * kLogitStride + kA);
 #pragma unroll
-    for (int k = 0; k < kTaps; ++k) {
-        const uint32_t wv = wds[(kO + k) >> 1];
-        l[k] = bf2f((uint16_t)(((kO + k) & 1) ? (wv >> 16) : (wv & 0xffffu)));
-        m = fmaxf(m, l[k]);
+        for (int j = 0; j < 4; ++j) {
+            const uint4 v = src[j];
+            wds[q][4 * j] = v.x; wds[q][4 * j + 1] = v.y; wds[q][4 * j + 2] = v.z; wds[q][4 * j + 3] = v.w;
+        }
     }
-    float s = 0.0f, r = 0.0f, g = 0.0f, b = 0.0f;
+    // Row r of the staged area is image row clamp(ty0 - 2d + r); its 3 kSide floats are read as one
+    // run of columns clamp(tx0 - 2d + c).
+    constexpr int kF = kPx * 3, kFIt = (kF + 255) / 256, kB = 8;
 #pragma unroll
-    for (int k = 0; k < kTaps; ++k) {
-        const float e = expf(l[k] - m);
-        const int yy = min(max(y + d * (k / 5 - 2), 0), H - 1), xx = min(max(x + d * (k % 5 - 2), 0), W - 1);
-        const float* p = in + ((size_t)yy * W + xx) * 3;
-        s += e;
-        r += e * p[0];
-        g += e * p[1];
-        b += e * p[2];
+    for (int i0 = 0; i0 < kFIt; i0 += kB) {
+        float v[kB];
+#pragma unroll
+        for (int j = 0; j < kB; ++j) {
+            if (i0 + j >= kFIt) break;
+            const int f = min((int)threadIdx.x + 256 * (i0 + j), kF - 1);
+            const int r = f / (kSide * 3), rem = f - r * (kSide * 3), cl = rem / 3, ch = rem - cl * 3;
+            const int y = min(max(ty0 - 2 * kD + r, 0), H - 1), x = min(max(tx0 - 2 * kD + cl, 0), W - 1);
+            v[j] = in[((size_t)y * W + x) * 3 + ch];
+        }
+#pragma unroll
+        for (int j = 0; j < kB; ++j) {
+            if (i0 + j >= kFIt) break;
+            const int f = (int)threadIdx.x + 256 * (i0 + j);
+            const int r = f / (kSide * 3), rem = f - r * (kSide * 3), cl = rem / 3, ch = rem - cl * 3;
+            if (f < kF) plane[ch][r * kSide + cl] = v[j];
+        }
     }
-    out[i * 3] = r / s;
-    out[i * 3 + 1] = g / s;
-    out[i * 3 + 2] = b / s;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPPT; ++q) {
+        const int lp = threadIdx.x + 256 * q;
+        const int lx = lp % TS, ly = lp / TS;
+        const int x = tx0 + lx, y = ty0 + ly;
+        if (x >= W || y >= H) continue;
+        const int i = y * W + x;
+        float l[kTaps];
+        float m = -INFINITY;
+#pragma unroll
+        for (int k = 0; k < kTaps; ++k) {
+            const uint32_t wv = wds[q][(kO + k) >> 1];
+            l[k] = bf2f((uint16_t)(((kO + k) & 1) ? (wv >> 16) : (wv & 0xffffu)));
+            m = fmaxf(m, l[k]);
+        }
+        float s = 0.0f, r = 0.0f, g = 0.0f, b = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kTaps; ++k) {
+            const float e = expf(l[k] - m);
+            const int o = (ly + 2 * kD + kD * (k / 5 - 2)) * kSide + lx + 2 * kD + kD * (k % 5 - 2);
+            s += e;
+            r += e * plane[0][o];
+            g += e * plane[1][o];
+            b += e * plane[2][o];
+        }
+        out[i * 3] = r / s;
+        out[i * 3 + 1] = g / s;
+        out[i * 3 + 2] = b / s;
+    }
 }
 
 // ---- Training kernels --------------------------------------------------------------------------------
@@ -323,7 +541,7 @@ __global__ void k_kpn_stage_back(const uint16_t* logits, int stage, const float*
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= W * H) return;
     const int x = i % W, y = i / W, d = 1 << stage;
-    const uint16_t* lg = logits + (size_t)i * kLogitStride + stage * kTaps;
+    const uint16_t* lg = logits + (size_t)i * kLogitStride + stage * kStageStride;
     float w[kTaps];
     int q[kTaps];
     float m = -INFINITY;
@@ -346,7 +564,7 @@ __global__ void k_kpn_stage_back(const uint16_t* logits, int stage, const float*
         dw[k] = g0 * p[0] + g1 * p[1] + g2 * p[2];
         dot += w[k] * dw[k];
     }
-    uint16_t* dl = dlog + (size_t)i * kLogitStride + stage * kTaps;
+    uint16_t* dl = dlog + (size_t)i * kLogitStride + stage * kStageStride;
     for (int k = 0; k < kTaps; ++k) dl[k] = f2bf(w[k] * (dw[k] - dot));
     if (gin)
         for (int k = 0; k < kTaps; ++k) {
@@ -551,6 +769,14 @@ const std::vector<CnnLayerSpec>& cnn_layers() {
     return L;
 }
 
+// Device channel of logical output channel co of layer li: the head's logit (stage s, tap k),
+// logical channel 25 s + k, is channel 32 s + k; every other layer keeps its order.
+static int packed_channel(int li, int co) {
+    const std::vector<CnnLayerSpec>& Ls = cnn_layers();
+    if (li != (int)Ls.size() - 1) return co;
+    return kStageStride * (co / kTaps) + co % kTaps;
+}
+
 uint64_t cnn_weight_count() {
     uint64_t n = 0;
     for (const CnnLayerSpec& L : cnn_layers()) n += (uint64_t)L.cout * (L.cinA + L.cinB) * 9 + L.cout;
@@ -607,12 +833,13 @@ bool CnnNet::load(const float* weights, hipStream_t s) {
         for (int co = 0; co < L.cout; ++co)
             for (int ci = 0; ci < cin; ++ci) {
                 const int cp = ci < L.cinA ? ci : padA + (ci - L.cinA);
+                const int pc = packed_channel((int)li, co);
                 for (int tap = 0; tap < 9; ++tap)
-                    w[((size_t)tap * coutP + co) * cinP + cp] = host_f2bf(src[((size_t)co * cin + ci) * 9 + tap]);
+                    w[((size_t)tap * coutP + pc) * cinP + cp] = host_f2bf(src[((size_t)co * cin + ci) * 9 + tap]);
             }
         src += (size_t)L.cout * cin * 9;
         std::vector<float> b(coutP, 0.0f);
-        for (int co = 0; co < L.cout; ++co) b[co] = src[co];
+        for (int co = 0; co < L.cout; ++co) b[packed_channel((int)li, co)] = src[co];
         src += L.cout;
         if (hipMalloc((void**)&w_[li], w.size() * 2) != hipSuccess) return false;
         if (hipMalloc((void**)&b_[li], b.size() * 4) != hipSuccess) return false;
@@ -669,7 +896,9 @@ ConvArgs layer_args(int li, uint32_t W, uint32_t H, const uint16_t* feat, const 
     A.H = (int)h;
     A.cinPad = cin_pad(L, A.ca, A.cb);
     A.coutPad = pad32(L.cout);
-    A.cout = L.cout;
+    // The head's logit slots are spread over all 128 channels; its padding channels have zero
+    // weights and biases and come out zero.
+    A.cout = li == (int)Ls.size() - 1 ? A.coutPad : L.cout;
     A.w = w.empty() ? nullptr : w[li];
     A.bias = b.empty() ? nullptr : b[li];
     A.out = act[li];
@@ -739,8 +968,55 @@ void launch_conv_t(const ConvArgs& A, hipStream_t s) {
     hipLaunchKernelGGL((k_conv3x3<NT, MT>), grid, dim3(256), 0, s, A);
 }
 
+#ifndef SR_RES
+#define SR_RES 1
+#endif
+#ifndef SR_RES32_MT
+#define SR_RES32_MT 4
+#endif
+#ifndef SR_RES_HEAD_NG
+#define SR_RES_HEAD_NG 4
+#endif
+#ifndef SR_RES_HEAD_MT
+#define SR_RES_HEAD_MT 4
+#endif
+#ifndef SR_PIPE_UP
+#define SR_PIPE_UP 0
+#endif
+#ifndef SR_PIPE_UP_MT
+#define SR_PIPE_UP_MT 4
+#endif
+
+// One resident workgroup per slot the occupancy calculator reports on every CU, at most one per tile.
+template <int NG, int NTW, int MT, int MODE, bool ONE_CHUNK>
+void launch_pipe(const ConvArgs& A, hipStream_t s) {
+    constexpr int kRows = (4 / NG) * MT;
+    const int tilesX = (A.W + kTW - 1) / kTW, tiles = tilesX * ((A.H + kRows - 1) / kRows);
+    int dev = 0, cus = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_conv3x3_pipe<NG, NTW, MT, MODE, ONE_CHUNK>, 256,
+                                                       0);
+    const int grid = std::max(1, std::min(tiles, std::max(1, cus) * std::max(1, per)));
+    hipLaunchKernelGGL((k_conv3x3_pipe<NG, NTW, MT, MODE, ONE_CHUNK>), dim3(grid), dim3(256), 0, s, A, tilesX, tiles);
+}
+
 // Few output channels: 8 rows per wave (weights reused 8 times, 16-byte stores); more: 2 rows.
 bool launch_conv(const ConvArgs& A, hipStream_t s) {
+    if (SR_RES && A.mode == 0 && A.cinPad == 32 && A.ca == 32) {
+        if (A.coutPad == 32) {
+            launch_pipe<1, 2, SR_RES32_MT, 0, true>(A, s);
+            return true;
+        }
+        if (A.coutPad == 128) {
+            launch_pipe<SR_RES_HEAD_NG, 128 / 16 / SR_RES_HEAD_NG, SR_RES_HEAD_MT, 0, true>(A, s);
+            return true;
+        }
+    }
+    if (SR_PIPE_UP && A.mode == 2 && A.coutPad == 32 && A.cout == 32) {
+        launch_pipe<1, 2, SR_PIPE_UP_MT, 2, false>(A, s);
+        return true;
+    }
     switch ((A.cout + 15) / 16) {
         case 2: launch_conv_t<2, kSmallMT>(A, s); break;
         case 3: launch_conv_t<3, SR_MT3>(A, s); break;
@@ -778,13 +1054,15 @@ bool CnnNet::run(const float* rad, const float* alb, const float* nrm, const flo
     for (size_t li = 0; li < Ls.size(); ++li)
         if (!launch_conv(layer_args((int)li, W, H, feat_, act_, w_, b_), s)) return false;
     if (out) {
-        const dim3 g((n + 255) / 256);
         static_assert(kStages == 4, "one k_kpn_stage instance per stage");
         void (*const stage[kStages])(const uint16_t*, const float*, int, int, float*) = {
-            k_kpn_stage<0>, k_kpn_stage<1>, k_kpn_stage<2>, k_kpn_stage<3>};
-        for (int st = 0; st < kStages; ++st)
+            k_kpn_stage<0, 16>, k_kpn_stage<1, 16>, k_kpn_stage<2, 32>, k_kpn_stage<3, 32>};
+        const int side[kStages] = {16, 16, 32, 32};
+        for (int st = 0; st < kStages; ++st) {
+            const dim3 g((W + side[st] - 1) / side[st], (H + side[st] - 1) / side[st]);
             hipLaunchKernelGGL(stage[st], g, dim3(256), 0, s, act_.back(), st ? casc_[st - 1] : rad, (int)W, (int)H,
                                st == kStages - 1 ? out : casc_[st]);
+        }
     }
     return hipGetLastError() == hipSuccess;
 }
@@ -801,7 +1079,7 @@ bool CnnNet::read_layer(int layer, std::vector<float>& out, uint32_t& c, uint32_
         if (hipMemcpy(raw.data(), dyp_[li], raw.size() * 2, hipMemcpyDeviceToHost) != hipSuccess) return false;
         out.resize((size_t)w * h * c);
         for (size_t p = 0; p < (size_t)w * h; ++p)
-            for (uint32_t k = 0; k < c; ++k) out[p * c + k] = host_bf2f(raw[p * stride + k]);
+            for (uint32_t k = 0; k < c; ++k) out[p * c + k] = host_bf2f(raw[p * stride + packed_channel(li, (int)k)]);
         return true;
     }
     if (!feat_ || layer < -1 || layer >= (int)Ls.size()) return false;
@@ -814,7 +1092,8 @@ bool CnnNet::read_layer(int layer, std::vector<float>& out, uint32_t& c, uint32_
         return false;
     out.resize((size_t)w * h * c);
     for (size_t p = 0; p < (size_t)w * h; ++p)
-        for (uint32_t k = 0; k < c; ++k) out[p * c + k] = host_bf2f(raw[p * stride + k]);
+        for (uint32_t k = 0; k < c; ++k)
+            out[p * c + k] = host_bf2f(raw[p * stride + (layer < 0 ? (int)k : packed_channel(layer, (int)k))]);
     return true;
 }
 
@@ -1020,9 +1299,10 @@ bool CnnNet::export_blob(const std::vector<float*>& mw, const std::vector<float*
         for (int co = 0; co < L.cout; ++co)
             for (int ci = 0; ci < cin; ++ci) {
                 const int cp = ci < L.cinA ? ci : padA + (ci - L.cinA);
-                for (int tap = 0; tap < 9; ++tap) blob.push_back(w[((size_t)tap * coutP + co) * cinP + cp]);
+                const int pc = packed_channel((int)li, co);
+                for (int tap = 0; tap < 9; ++tap) blob.push_back(w[((size_t)tap * coutP + pc) * cinP + cp]);
             }
-        for (int co = 0; co < L.cout; ++co) blob.push_back(b[co]);
+        for (int co = 0; co < L.cout; ++co) blob.push_back(b[packed_channel((int)li, co)]);
     }
     return true;
 }

@@ -18,6 +18,8 @@ import json
 import math
 import sys
 import time
+from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -114,6 +116,10 @@ def main(argv=None):
     ap.add_argument("--noisy-spp", default="1", help="comma-separated sample counts of the noisy inputs")
     ap.add_argument("--target-spp", type=int, default=1024)
     ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--workers", type=int, default=1,
+                    help="training scenes rendered at once, each on its own device context (results unchanged)")
+    ap.add_argument("--slices", type=int, default=1,
+                    help="render training images as this many concurrent row slices (results unchanged)")
     ap.add_argument("--tris", default="10000,100000",
                     help="min,max total triangles of a training scene's noise meshes (log-uniform)")
     ap.add_argument("--data-seconds", type=float, default=0.0,
@@ -139,24 +145,49 @@ def main(argv=None):
     noisy_spp = [int(v) for v in str(args.noisy_spp).split(",")]
     tris = tuple(int(v) for v in str(args.tris).split(","))
     vrng = np.random.default_rng(args.seed ^ 0x7E3)
-    n_ex = 0
-    t_build = t_render = 0.0
-    n_scenes = 0
-    for i in range(args.scenes):
-        if args.data_seconds and time.perf_counter() - t0 > args.data_seconds:
-            log(f"[train] data time limit reached after {i} scenes")
-            break
+    # Extra cameras in the order a one-by-one pass draws them (scene-major), so that the examples do
+    # not depend on --workers.
+    extra_views = [[random_view(vrng) for _ in range(args.views - 1)] for _ in range(args.scenes)]
+
+    def scene_examples(i):
+        """Builds training scene i and renders its views: (triangles, build s, render s, examples)."""
         ts = time.perf_counter()
         spec = random_scene(args.seed + i, args.size, noisy_spp[0], tris)
         scene, view = scenes.instantiate(spec)
-        t_build += time.perf_counter() - ts
-        n_scenes += 1
-        for v in range(args.views):
+        if args.slices > 1:
+            scene.set_slices(args.slices)  # concurrent row slices: same images, launches overlap
+        tb = time.perf_counter() - ts
+        ex = []
+        for v, cam in enumerate([view] + extra_views[i]):
             spp = noisy_spp[(i * args.views + v) % len(noisy_spp)]
-            ts = time.perf_counter()
-            _, _, a, target = render_pair(spec, spp, args.target_spp, args.seed + 100000 + 97 * i + v,
-                                          scene, view if v == 0 else random_view(vrng))
-            t_render += time.perf_counter() - ts
+            _, _, a, target = render_pair(spec, spp, args.target_spp, args.seed + 100000 + 97 * i + v, scene, cam)
+            ex.append((a, target))
+        return spec.triangle_count, tb, time.perf_counter() - ts - tb, ex
+
+    n_ex = 0
+    checksum = 0.0  # sum of every example's noisy and target radiance: equal for any --workers
+    t_build = t_render = 0.0
+    n_scenes = 0
+    # Each worker renders a scene of its own on its own device context and stream: a 256 x 256
+    # render fills a fraction of the GPU, so several at once raise the sample rate.
+    pool = ThreadPoolExecutor(max_workers=max(1, args.workers))
+    pending = deque()
+    submitted = 0
+    while submitted < args.scenes or pending:
+        while submitted < args.scenes and len(pending) < 2 * max(1, args.workers):
+            if args.data_seconds and time.perf_counter() - t0 > args.data_seconds:
+                log(f"[train] data time limit reached: no scenes after {submitted}")
+                args.scenes = submitted
+                break
+            pending.append(pool.submit(scene_examples, submitted))
+            submitted += 1
+        if not pending:
+            break
+        ntri, tb, tr, ex = pending.popleft().result()
+        t_build += tb
+        t_render += tr
+        for a, target in ex:
+            checksum += float(np.sum(target, dtype=np.float64)) + float(np.sum(a["rad"], dtype=np.float64))
             c = args.crop or args.size
             for y0 in range(0, args.size - c + 1, c):
                 for x0 in range(0, args.size - c + 1, c):
@@ -165,9 +196,11 @@ def main(argv=None):
                                   np.ascontiguousarray(a["nrm"][sl]), np.ascontiguousarray(a["dep"][sl]),
                                   np.ascontiguousarray(target[sl]), c, c)
                     n_ex += 1
-        if (i + 1) % 16 == 0:
-            log(f"[train] {i + 1}/{args.scenes} scenes, {n_ex} examples ({time.perf_counter() - t0:.1f} s: "
-                f"scene build {t_build:.1f} s, rendering {t_render:.1f} s)")
+        n_scenes += 1
+        log(f"[train] {n_scenes}/{args.scenes} scenes ({ntri} triangles), {n_ex} examples "
+            f"({time.perf_counter() - t0:.1f} s: scene build {t_build:.1f} s, rendering {t_render:.1f} s, "
+            f"summed over workers)")
+    pool.shutdown()
     t_data = time.perf_counter() - t0
     t0 = time.perf_counter()
     chunk = 250
@@ -188,7 +221,8 @@ def main(argv=None):
 
     # Held-out evaluation.
     report = {"train_scenes": n_scenes, "train_examples": n_ex, "train_size": args.size, "noisy_spp": args.noisy_spp,
-              "target_spp": args.target_spp, "steps": args.steps, "data_seconds": round(t_data, 1),
+              "target_spp": args.target_spp, "tris": args.tris, "data_checksum": checksum, "steps": args.steps,
+              "data_seconds": round(t_data, 1),
               "train_seconds": round(t_train, 1), "final_loss": losses[-1] if losses else None,
               "eval_size": args.eval_size, "ref_spp": args.ref_spp, "eval": {}}
     for espp in [int(v) for v in str(args.eval_spp).split(",")]:
@@ -220,7 +254,8 @@ def main(argv=None):
                     row[f"bilateral_{mult}x{tag}"] = best
             res.append(row)
             log(f"[eval {espp} spp] scene {j}: " + ", ".join(f"{k} {v:.4f}" for k, v in row.items()))
-        report["eval"][f"{espp}spp"] = {k: round(float(np.mean([r[k] for r in res])), 5) for k in res[0]}
+        if res:
+            report["eval"][f"{espp}spp"] = {k: round(float(np.mean([r[k] for r in res])), 5) for k in res[0]}
     print(json.dumps(report), flush=True)
 
 

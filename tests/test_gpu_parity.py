@@ -401,6 +401,34 @@ def test_kpn_denoiser_layers_match_cpu_restatement():
     assert rel.max() <= 2e-2
 
 
+def test_kpn_full_resolution_layers_match_cpu_restatement_on_many_tiles():
+    """The full-resolution layers (0, 1, 16 with its upsampled 48-channel source and 32-channel skip,
+    17) and the logit head (18) at 648 x 372, where each resident workgroup of their persistent
+    kernel walks several tiles (984 tiles of 16 x 16 pixels, 3813 of 16 x 4 for the head, ragged at
+    the right and bottom edges), against the float32 restatement from the GPU's own input
+    activations."""
+    from oracle import cnn_ref as C
+    from sharpray_amd import KpnDenoiser, RNG_COUNTER, scenes
+    spec = scenes.config_plain_cpu(scale=0.25, spp=1)
+    spec.width, spec.height = 648, 372
+    scene, view, r = scenes.render(spec, rng_policy=RNG_COUNTER, rng_seed=5)
+    r.wait()
+    den = KpnDenoiser(scene)
+    blob = KpnDenoiser.random_weights(23)
+    den.load(blob)
+    den.denoise(r)
+    wb = C.split_weights(blob)
+    gpu = {i: den.layer(i) for i in (-1, 0, 1, 15, 16, 17, 18)}
+    for i in (0, 1, 16, 17, 18):
+        ref = C.layer_forward(i, wb, gpu)
+        assert ref.shape == gpu[i].shape == (372, 648, C.LAYERS[i][2]), (i, ref.shape, gpu[i].shape)
+        scale = max(float(np.abs(ref).max()), 1e-6)
+        err = np.abs(gpu[i] - ref)
+        bad = err > np.abs(ref) * 2.0 ** -7 + 1e-4 * scale
+        print(f"layer {i}: max err {float(err.max()):.3e} of scale {scale:.3e}, {int(bad.sum())} over", flush=True)
+        assert not bad.any(), (i, np.argwhere(bad)[:5])
+
+
 def _kpn_pair(spec_fn, seed):
     from sharpray_amd import RNG_COUNTER, scenes
     spec = spec_fn()
