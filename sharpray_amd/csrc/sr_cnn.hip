@@ -91,6 +91,43 @@ SR_CNN_DEV uint4 fetch8(const ConvArgs& A, int c, int y, int x) {
     return make_uint4(r[0], r[1], r[2], r[3]);
 }
 
+// LDS slot of 16-byte group `part` of row p (a halo pixel's 32 channels, or a weight row's 32
+// input channels): the groups of row p are rotated by (p >> 1) & 3, so that the 8 lanes of a
+// 128-byte LDS cycle, reading one group of 8 consecutive rows, hit 8 different bank groups.
+SR_CNN_DEV int lds_slot(int p, int part) { return p * 4 + (part ^ ((p >> 1) & 3)); }
+
+// Epilogue of a wave's MT rows x NT channel tiles from MFMAs with the weights as A: acc[mt][nt][r]
+// is channel c0 + 16 nt + 4 grp + r of pixel (x, y0 + mt). Bias, ReLU and bf16 rounding, then one
+// 8-byte store per 4 channels; channels at or past cout are written as 0.
+template <int NT, int MT>
+SR_CNN_DEV void store_tile(const ConvArgs& A, const f32x4 (&acc)[MT][NT], int x, int y0, int c0, int grp) {
+    float bias[NT][4];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int co = c0 + nt * 16 + grp * 4 + r;
+            bias[nt][r] = co < A.cout ? A.bias[co] : 0.0f;
+        }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int y = y0 + mt;
+        if (x >= A.W || y >= A.H) continue;
+        uint16_t* o = A.out + ((size_t)y * A.W + x) * A.coutPad + c0 + grp * 4;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            float v[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                v[r] = acc[mt][nt][r] + bias[nt][r];
+                if (A.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
+                if (c0 + nt * 16 + grp * 4 + r >= A.cout) v[r] = 0.0f;
+            }
+            *(uint2*)(o + nt * 16) = make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3]));
+        }
+    }
+}
+
 // A 4-wave workgroup computes a 16-wide, 4*MT-tall output tile: wave w owns rows w*MT .. w*MT+MT-1,
 // one 16-pixel MFMA M tile per row. Per 32-channel K chunk the (4*MT+2) x 18 input halo is staged
 // in LDS; per tap each wave loads its NT weight fragments once and reuses them for its MT rows.
@@ -98,17 +135,49 @@ SR_CNN_DEV uint4 fetch8(const ConvArgs& A, int c, int y, int x) {
 // channels.
 // Waves per SIMD the register allocation of k_conv3x3<NT, MT> must allow (1: the compiler's choice).
 #ifndef SR_W2_4
-#define SR_W2_4 1
+#define SR_W2_4 3
 #endif
 template <int NT, int MT>
 constexpr int conv_waves() { return NT == 2 && MT == 4 ? SR_W2_4 : 1; }
+// k_conv3x3<NT, MT> stages its halo with all of a thread's loads in flight at once (else one
+// load-and-store round trip at a time, which takes fewer registers).
+#ifndef SR_BATCH2_4
+#define SR_BATCH2_4 1
+#endif
+template <int NT, int MT>
+constexpr bool conv_batched() { return NT == 2 && MT == 4 && SR_BATCH2_4; }
+
+// Halo words [0, N) of K chunk c0 of a MODE source into LDS slots with NT threads, all of a
+// thread's loads issued before its first LDS store.
+template <int MODE, int N, int NT>
+SR_CNN_DEV void stage_halo(const ConvArgs& A, int c0, int tx0, int ty0, uint4* tile) {
+    constexpr int kIt = (N + NT - 1) / NT;
+    uint4 v[kIt];
+#pragma unroll
+    for (int i = 0; i < kIt; ++i) {
+        const int idx = min((int)threadIdx.x + NT * i, N - 1);
+        const int p = idx >> 2, part = idx & 3, y = ty0 - 1 + p / kHW, x = tx0 - 1 + p % kHW;
+        const bool ok = x >= 0 && y >= 0 && x < A.W && y < A.H;
+        const int yc = ok ? y : 0, xc = ok ? x : 0, c = c0 + part * 8;
+        uint4 w;
+        if (MODE == 0) w = *(const uint4*)(A.a + ((size_t)yc * A.W + xc) * A.ca + c);
+        else w = c < A.ca ? *(const uint4*)(A.a + ((size_t)(yc >> 1) * A.wa + (xc >> 1)) * A.ca + c)
+                          : *(const uint4*)(A.b + ((size_t)yc * A.W + xc) * A.cb + (c - A.ca));
+        v[i] = ok ? w : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < kIt; ++i) {
+        const int idx = (int)threadIdx.x + NT * i;
+        if (idx < N) tile[lds_slot(idx >> 2, idx & 3)] = v[i];
+    }
+}
 
 template <int NT, int MT>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(conv_waves<NT, MT>())))
 k_conv3x3(ConvArgs A) {
     constexpr int kRows = 4 * MT;
     constexpr int kHaloPx = kHW * (kRows + 2);
-    __shared__ uint4 tile[kHaloPx * 4];  // halo pixels x 32 channels of bf16
+    __shared__ uint4 tile[kHaloPx * 4];  // halo pixels x 32 channels of bf16, rows as lds_slot
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx0 = blockIdx.x * kTW, ty0 = blockIdx.y * kRows;
     f32x4 acc[MT][NT];
@@ -118,60 +187,36 @@ k_conv3x3(ConvArgs A) {
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int c0 = 0; c0 < A.cinPad; c0 += 32) {
         __syncthreads();
-        for (int idx = threadIdx.x; idx < kHaloPx * 4; idx += 256) {
-            const int p = idx >> 2, part = idx & 3;
-            tile[idx] = fetch8(A, c0 + part * 8, ty0 - 1 + p / kHW, tx0 - 1 + p % kHW);
-        }
+        if (conv_batched<NT, MT>() && A.mode == 0) stage_halo<0, kHaloPx * 4, 256>(A, c0, tx0, ty0, tile);
+        else if (conv_batched<NT, MT>() && A.mode == 2) stage_halo<2, kHaloPx * 4, 256>(A, c0, tx0, ty0, tile);
+        else
+            for (int idx = threadIdx.x; idx < kHaloPx * 4; idx += 256) {
+                const int p = idx >> 2, part = idx & 3;
+                tile[lds_slot(p, part)] = fetch8(A, c0 + part * 8, ty0 - 1 + p / kHW, tx0 - 1 + p % kHW);
+            }
         __syncthreads();
         const uint16_t* wc = A.w + c0 + (lane >> 4) * 8;
         #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             const int dy = tap / 3, dx = tap % 3;
             const uint16_t* wt = wc + (size_t)tap * A.coutPad * A.cinPad;
-            bf16x8 bfr[NT];
+            bf16x8 wfr[NT];
             #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
-                bfr[nt] = __builtin_bit_cast(bf16x8, *(const uint4*)(wt + (size_t)(nt * 16 + (lane & 15)) * A.cinPad));
+                wfr[nt] = __builtin_bit_cast(bf16x8, *(const uint4*)(wt + (size_t)(nt * 16 + (lane & 15)) * A.cinPad));
             #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const int p = (wave * MT + mt + dy) * kHW + (lane & 15) + dx;
-                const bf16x8 af = __builtin_bit_cast(bf16x8, tile[p * 4 + (lane >> 4)]);
+                const bf16x8 px = __builtin_bit_cast(bf16x8, tile[lds_slot(p, lane >> 4)]);
                 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[nt], acc[mt][nt], 0, 0, 0);
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfr[nt], px, acc[mt][nt], 0, 0, 0);
             }
         }
     }
-    // Epilogue, one 32-channel group at a time: bias, ReLU and bf16 rounding into LDS as
-    // [pixel][32], then 16-byte stores of each pixel's group (channels past cout are written as 0).
-    uint16_t* ob = (uint16_t*)tile;  // [kRows * 16 pixels][32 channels]
-#pragma unroll
-    for (int g = 0; g < (NT + 1) / 2; ++g) {
-        __syncthreads();  // the halo (or the previous group) is no longer read
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int nt = 2 * g + h;
-                const int co = nt * 16 + (lane & 15);
-                const bool live = nt < NT && co < A.cout;
-                const float bias = live ? A.bias[co] : 0.0f;
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    float v = nt < NT ? acc[mt][nt < NT ? nt : 0][r] + bias : 0.0f;
-                    if (A.relu) v = v > 0.0f ? v : 0.0f;
-                    ob[((wave * MT + mt) * 16 + 4 * (lane >> 4) + r) * 32 + h * 16 + (lane & 15)] =
-                        live ? f2bf(v) : (uint16_t)0;
-                }
-            }
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < kRows * 16 * 4; idx += 256) {
-            const int pl = idx >> 2, part = idx & 3;
-            const int y = ty0 + pl / 16, x = tx0 + pl % 16;
-            if (y < A.H && x < A.W)
-                ((uint4*)(A.out + ((size_t)y * A.W + x) * A.coutPad + 32 * g))[part] = tile[idx];
-        }
-    }
+    // acc[mt][nt][r]: channel 16 nt + 4 (lane >> 4) + r of pixel (tx0 + (lane & 15), row wave MT + mt):
+    // bias, ReLU and bf16 rounding, stored as one 8-byte word per 4 channels (channels past cout as 0).
+    store_tile<NT>(A, acc, tx0 + (lane & 15), ty0 + wave * MT, 0, lane >> 4);
 }
 
 // Layers split by output channels: wave w of the NW-wave workgroup (NW = coutPad / 32) computes
@@ -181,8 +226,7 @@ template <int MT, int NW>
 __global__ void __launch_bounds__(NW * 64) k_conv3x3_split(ConvArgs A) {
     constexpr int kHaloPx = kHW * (MT + 2);
     constexpr int kHaloWords = kHaloPx * 4;
-    constexpr int kOutWords = NW * MT * 16 * 4;  // NW waves x MT*16 pixels x 32 channels of bf16
-    __shared__ uint4 smem[kHaloWords > kOutWords ? kHaloWords : kOutWords];
+    __shared__ uint4 smem[kHaloWords];  // rows as lds_slot
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int tx0 = blockIdx.x * kTW, ty0 = blockIdx.y * MT;
     f32x4 acc[MT][2];
@@ -192,7 +236,7 @@ __global__ void __launch_bounds__(NW * 64) k_conv3x3_split(ConvArgs A) {
         __syncthreads();
         for (int idx = threadIdx.x; idx < kHaloWords; idx += NW * 64) {
             const int p = idx >> 2, part = idx & 3;
-            smem[idx] = fetch8(A, c0 + part * 8, ty0 - 1 + p / kHW, tx0 - 1 + p % kHW);
+            smem[lds_slot(p, part)] = fetch8(A, c0 + part * 8, ty0 - 1 + p / kHW, tx0 - 1 + p % kHW);
         }
         __syncthreads();
         const uint16_t* wc = A.w + c0 + (lane >> 4) * 8;
@@ -200,45 +244,21 @@ __global__ void __launch_bounds__(NW * 64) k_conv3x3_split(ConvArgs A) {
         for (int tap = 0; tap < 9; ++tap) {
             const int dy = tap / 3, dx = tap % 3;
             const uint16_t* wt = wc + (size_t)tap * A.coutPad * A.cinPad;
-            bf16x8 bfr[2];
+            bf16x8 wfr[2];
 #pragma unroll
             for (int h = 0; h < 2; ++h)
-                bfr[h] = __builtin_bit_cast(
+                wfr[h] = __builtin_bit_cast(
                     bf16x8, *(const uint4*)(wt + (size_t)((2 * wave + h) * 16 + (lane & 15)) * A.cinPad));
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 const int p = (mt + dy) * kHW + (lane & 15) + dx;
-                const bf16x8 af = __builtin_bit_cast(bf16x8, smem[p * 4 + (lane >> 4)]);
-                acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[0], acc[mt][0], 0, 0, 0);
-                acc[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[1], acc[mt][1], 0, 0, 0);
+                const bf16x8 px = __builtin_bit_cast(bf16x8, smem[lds_slot(p, lane >> 4)]);
+                acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfr[0], px, acc[mt][0], 0, 0, 0);
+                acc[mt][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wfr[1], px, acc[mt][1], 0, 0, 0);
             }
         }
     }
-    // Epilogue: bias, ReLU and bf16 rounding into the wave's LDS region as [pixel][32 channels],
-    // then 16-byte stores of each pixel's 32-channel group (channels past cout are written as 0).
-    __syncthreads();  // the halo is no longer read
-    uint16_t* ob = (uint16_t*)smem + (size_t)wave * MT * 16 * 32;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int co = 32 * wave + 16 * h + (lane & 15);
-        const bool live = co < A.cout;
-        const float bias = live ? A.bias[co] : 0.0f;
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                float v = acc[mt][h][r] + bias;
-                if (A.relu) v = v > 0.0f ? v : 0.0f;
-                ob[(mt * 16 + 4 * (lane >> 4) + r) * 32 + h * 16 + (lane & 15)] = live ? f2bf(v) : (uint16_t)0;
-            }
-    }
-    __syncthreads();
-    const uint4* ow = smem + (size_t)wave * MT * 16 * 4;
-    for (int idx = lane; idx < MT * 16 * 4; idx += 64) {
-        const int pl = idx >> 2, part = idx & 3;
-        const int y = ty0 + pl / 16, x = tx0 + pl % 16;
-        if (y < A.H && x < A.W) ((uint4*)(A.out + ((size_t)y * A.W + x) * A.coutPad + 32 * wave))[part] = ow[idx];
-    }
+    store_tile<2>(A, acc, tx0 + (lane & 15), ty0, 32 * wave, lane >> 4);
 }
 
 // Persistent 3x3 convolution. Wave w owns output channels [16 NTW cg, 16 NTW (cg + 1)) with
@@ -262,7 +282,6 @@ __global__ void __launch_bounds__(NW * 64) k_conv3x3_split(ConvArgs A) {
 #ifndef SR_PIPE_WAVES
 #define SR_PIPE_WAVES 2  // waves per SIMD the register allocation must allow
 #endif
-SR_CNN_DEV int lds_slot(int p, int part) { return p * 4 + (part ^ ((p >> 1) & 3)); }
 
 template <int MODE>
 SR_CNN_DEV const uint4* chunk_src(const ConvArgs& A, int c0, int y, int x) {
