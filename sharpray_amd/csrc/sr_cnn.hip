@@ -55,9 +55,16 @@ struct ConvArgs {
     int cinPad, coutPad, cout;
     const uint16_t* w;  // [9][coutPad][cinPad]
     const float* bias;  // [coutPad]
-    uint16_t* out;
+    uint16_t* out;      // [pixel][coutPad], or with plane > 0 [coutPad / 32][pixel][32]
+    size_t plane;       // elements between those 32-channel planes (0: interleaved output)
     int relu;
 };
+
+// Address of output channels [c, c + 4) of pixel `pixel` (c a multiple of 4).
+SR_CNN_DEV uint16_t* out_at(const ConvArgs& A, size_t pixel, int c) {
+    if (A.plane) return A.out + (size_t)(c >> 5) * A.plane + pixel * 32 + (c & 31);
+    return A.out + pixel * A.coutPad + c;
+}
 
 // 8 consecutive input channels (16 bytes) of K chunk c0 at output-resolution pixel (y, x), zero
 // outside the image.
@@ -113,7 +120,6 @@ SR_CNN_DEV void store_tile(const ConvArgs& A, const f32x4 (&acc)[MT][NT], int x,
     for (int mt = 0; mt < MT; ++mt) {
         const int y = y0 + mt;
         if (x >= A.W || y >= A.H) continue;
-        uint16_t* o = A.out + ((size_t)y * A.W + x) * A.coutPad + c0 + grp * 4;
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             float v[4];
@@ -123,7 +129,8 @@ SR_CNN_DEV void store_tile(const ConvArgs& A, const f32x4 (&acc)[MT][NT], int x,
                 if (A.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
                 if (c0 + nt * 16 + grp * 4 + r >= A.cout) v[r] = 0.0f;
             }
-            *(uint2*)(o + nt * 16) = make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3]));
+            *(uint2*)out_at(A, (size_t)y * A.W + x, c0 + nt * 16 + grp * 4) =
+                make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3]));
         }
     }
 }
@@ -404,7 +411,6 @@ k_conv3x3_pipe(ConvArgs A, int tilesX, int tiles) {
             for (int mt = 0; mt < MT; ++mt) {
                 const int y = ty0 + rg * MT + mt;
                 if (x >= A.W || y >= A.H) continue;
-                uint16_t* o = A.out + ((size_t)y * A.W + x) * A.coutPad + grp * 4;
 #pragma unroll
                 for (int h = 0; h < NTW; ++h) {
                     float v[4];
@@ -415,7 +421,8 @@ k_conv3x3_pipe(ConvArgs A, int tilesX, int tiles) {
                         if (A.relu) v[r] = v[r] > 0.0f ? v[r] : 0.0f;
                         if (co >= A.cout) v[r] = 0.0f;
                     }
-                    *(uint2*)(o + (cg * NTW + h) * 16) = make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3]));
+                    *(uint2*)out_at(A, (size_t)y * A.W + x, (cg * NTW + h) * 16 + grp * 4) =
+                        make_uint2(pack2(v[0], v[1]), pack2(v[2], v[3]));
                 }
             }
         }
@@ -445,8 +452,8 @@ __global__ void k_features(const float* rad, const float* alb, const float* nrm,
 
 // One stage of the kernel cascade: out(p) = sum_k w_k(p) in(clamp(p + d (dx_k, dy_k))), w = softmax of
 // the stage's 25 logits, 5 x 5 taps spaced d = 2^stage pixels, edges clamped.
-// The stage's 25 logits sit at halves [32 STAGE, 32 STAGE + 25) of the pixel's 256-byte row (one
-// aligned 64-byte run) and are read as four 16-byte words.
+// The logits are stage-planar ([stage][pixel][32 channels]); the stage's 25 sit at the start of the
+// pixel's 64-byte row in plane STAGE and are read as four 16-byte words.
 // A 256-thread workgroup owns a TS x TS pixel tile (TS * TS / 256 pixels a thread) and first
 // stages the input over the tile and 2d pixels around it in LDS, one plane per colour channel,
 // with the image-edge clamp applied as it loads; each pixel then reads its 25 taps from LDS.
@@ -460,8 +467,9 @@ __global__ void __launch_bounds__(256) k_kpn_stage(const uint16_t* logits, const
     static_assert(kPPT * 256 == TS * TS, "whole pixels per thread");
     __shared__ float plane[3][kPx];
     const int tx0 = blockIdx.x * TS, ty0 = blockIdx.y * TS;
-    constexpr int kA = STAGE * kStageStride, kO = 0;
-    static_assert(kTaps <= kStageStride && kA + kStageStride <= kLogitStride, "four words hold the stage's logits");
+    constexpr int kO = 0;
+    static_assert(kTaps <= kStageStride && kStages * kStageStride == kLogitStride, "one 64-byte row a stage");
+    const uint16_t* lplane = logits + (size_t)STAGE * W * H * kStageStride;
     // The thread's logits first (they do not wait for the staged input), then the staging loads in
     // batches of 8 ahead of their LDS stores.
     uint32_t wds[kPPT][16];
@@ -469,7 +477,7 @@ __global__ void __launch_bounds__(256) k_kpn_stage(const uint16_t* logits, const
     for (int q = 0; q < kPPT; ++q) {
         const int lp = threadIdx.x + 256 * q;
         const int x = min(tx0 + lp % TS, W - 1), y = min(ty0 + lp / TS, H - 1);
-        const uint4* src = (const uint4*)(logits + ((size_t)y * W + x) * kLogitStride + kA);
+        const uint4* src = (const uint4*)(lplane + ((size_t)y * W + x) * kStageStride);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const uint4 v = src[j];
@@ -560,7 +568,7 @@ __global__ void k_kpn_stage_back(const uint16_t* logits, int stage, const float*
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= W * H) return;
     const int x = i % W, y = i / W, d = 1 << stage;
-    const uint16_t* lg = logits + (size_t)i * kLogitStride + stage * kStageStride;
+    const uint16_t* lg = logits + ((size_t)stage * W * H + i) * kStageStride;  // stage-planar logits
     float w[kTaps];
     int q[kTaps];
     float m = -INFINITY;
@@ -918,6 +926,9 @@ ConvArgs layer_args(int li, uint32_t W, uint32_t H, const uint16_t* feat, const 
     // The head's logit slots are spread over all 128 channels; its padding channels have zero
     // weights and biases and come out zero.
     A.cout = li == (int)Ls.size() - 1 ? A.coutPad : L.cout;
+    // The head writes its logits stage-planar, [stage][pixel][32], so that each cascade stage reads
+    // whole 64-byte rows of one plane.
+    A.plane = li == (int)Ls.size() - 1 ? (size_t)wd * h * kStageStride : 0;
     A.w = w.empty() ? nullptr : w[li];
     A.bias = b.empty() ? nullptr : b[li];
     A.out = act[li];
@@ -1110,9 +1121,13 @@ bool CnnNet::read_layer(int layer, std::vector<float>& out, uint32_t& c, uint32_
     if (hipMemcpy(raw.data(), layer < 0 ? feat_ : act_[layer], raw.size() * 2, hipMemcpyDeviceToHost) != hipSuccess)
         return false;
     out.resize((size_t)w * h * c);
+    const bool planar = layer == (int)Ls.size() - 1;  // the head's stage-planar logits
     for (size_t p = 0; p < (size_t)w * h; ++p)
-        for (uint32_t k = 0; k < c; ++k)
-            out[p * c + k] = host_bf2f(raw[p * stride + (layer < 0 ? (int)k : packed_channel(layer, (int)k))]);
+        for (uint32_t k = 0; k < c; ++k) {
+            const int pc = layer < 0 ? (int)k : packed_channel(layer, (int)k);
+            const size_t at = planar ? (size_t)(pc >> 5) * w * h * 32 + p * 32 + (pc & 31) : p * stride + pc;
+            out[p * c + k] = host_bf2f(raw[at]);
+        }
     return true;
 }
 
