@@ -445,9 +445,7 @@ __global__ void k_features(const float* rad, const float* alb, const float* nrm,
     for (int c = kFeat; c < 16; ++c) f[c] = 0.0f;
     uint4* o = (uint4*)(out + (size_t)i * 32);
     o[0] = make_uint4(pack2(f[0], f[1]), pack2(f[2], f[3]), pack2(f[4], f[5]), pack2(f[6], f[7]));
-    o[1] = make_uint4(pack2(f[8], f[9]), 0u, 0u, 0u);
-    o[2] = make_uint4(0u, 0u, 0u, 0u);
-    o[3] = make_uint4(0u, 0u, 0u, 0u);
+    o[1] = make_uint4(pack2(f[8], f[9]), 0u, 0u, 0u);  // channels 16..31 stay zero from allocation
 }
 
 // One stage of the kernel cascade: out(p) = sum_k w_k(p) in(clamp(p + d (dx_k, dy_k))), w = softmax of
@@ -883,6 +881,7 @@ bool CnnNet::ensure(uint32_t W, uint32_t H) {
     release_acts();
     const std::vector<CnnLayerSpec>& Ls = cnn_layers();
     if (hipMalloc((void**)&feat_, (size_t)W * H * 32 * 2) != hipSuccess) return false;
+    if (hipMemset(feat_, 0, (size_t)W * H * 32 * 2) != hipSuccess) return false;
     for (float*& p : casc_)
         if (hipMalloc((void**)&p, (size_t)W * H * 12) != hipSuccess) return false;
     act_.assign(Ls.size(), nullptr);
@@ -998,6 +997,11 @@ void launch_conv_t(const ConvArgs& A, hipStream_t s) {
     hipLaunchKernelGGL((k_conv3x3<NT, MT>), grid, dim3(256), 0, s, A);
 }
 
+// Same-resolution layers of one 32-channel K chunk (the full-resolution 32-channel layers and the
+// head) run as k_conv3x3_pipe, measured at 1080p against k_conv3x3 / k_conv3x3_split: 32 channels
+// 55-60 us vs 95-108 us, the head 197 us vs 300 us. Rows per wave and head channel groups measured:
+// 2 rows 59-65 us on the 32-channel layers; head with 2 rows 254 us, with 2 channel groups of 64
+// (4 or 2 rows) 250 / 286 us, against 238 us for 4 groups and 4 rows (before stage-planar logits).
 #ifndef SR_RES
 #define SR_RES 1
 #endif
@@ -1010,9 +1014,13 @@ void launch_conv_t(const ConvArgs& A, hipStream_t s) {
 #ifndef SR_RES_HEAD_MT
 #define SR_RES_HEAD_MT 4
 #endif
+// The upsampling 32-channel layer (3 K chunks) through k_conv3x3_pipe, measured at 1080p: 272 us
+// with the chunk weights staged in LDS, 223 us with them loaded into registers per step, against
+// 209 us on k_conv3x3<2, 4> with batched halo loads (3 waves per SIMD).
 #ifndef SR_PIPE_UP
 #define SR_PIPE_UP 0
 #endif
+
 #ifndef SR_PIPE_UP_MT
 #define SR_PIPE_UP_MT 4
 #endif
