@@ -118,6 +118,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=4000)
     ap.add_argument("--workers", type=int, default=1,
                     help="training scenes rendered at once, each on its own device context (results unchanged)")
+    ap.add_argument("--augment", type=int, default=1, choices=[1, 2, 3, 4],
+                    help="examples per crop: the crop, then its left-right, top-bottom and double mirror images")
     ap.add_argument("--slices", type=int, default=1,
                     help="render training images as this many concurrent row slices (results unchanged)")
     ap.add_argument("--tris", default="10000,100000",
@@ -192,10 +194,13 @@ def main(argv=None):
             for y0 in range(0, args.size - c + 1, c):
                 for x0 in range(0, args.size - c + 1, c):
                     sl = (slice(y0, y0 + c), slice(x0, x0 + c))
-                    den.train_add(np.ascontiguousarray(a["rad"][sl]), np.ascontiguousarray(a["alb"][sl]),
-                                  np.ascontiguousarray(a["nrm"][sl]), np.ascontiguousarray(a["dep"][sl]),
-                                  np.ascontiguousarray(target[sl]), c, c)
-                    n_ex += 1
+                    # The crop, then its mirror images (left-right, top-bottom, both) with --augment.
+                    for fl in ((), (1,), (0,), (0, 1))[:max(1, args.augment)]:
+                        def crop(x):
+                            return np.ascontiguousarray(np.flip(x[sl], fl) if fl else x[sl])
+                        den.train_add(crop(a["rad"]), crop(a["alb"]), crop(a["nrm"]), crop(a["dep"]), crop(target),
+                                      c, c)
+                        n_ex += 1
         n_scenes += 1
         log(f"[train] {n_scenes}/{args.scenes} scenes ({ntri} triangles), {n_ex} examples "
             f"({time.perf_counter() - t0:.1f} s: scene build {t_build:.1f} s, rendering {t_render:.1f} s, "
@@ -221,7 +226,8 @@ def main(argv=None):
 
     # Held-out evaluation.
     report = {"train_scenes": n_scenes, "train_examples": n_ex, "train_size": args.size, "noisy_spp": args.noisy_spp,
-              "target_spp": args.target_spp, "tris": args.tris, "data_checksum": checksum, "steps": args.steps,
+              "target_spp": args.target_spp, "tris": args.tris, "augment": args.augment, "data_checksum": checksum,
+              "steps": args.steps,
               "data_seconds": round(t_data, 1),
               "train_seconds": round(t_train, 1), "final_loss": losses[-1] if losses else None,
               "eval_size": args.eval_size, "ref_spp": args.ref_spp, "eval": {}}
