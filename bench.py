@@ -417,6 +417,12 @@ def main():
     ap.add_argument("--light-sampler", choices=["cdf", "alias"], default="cdf",
                     help="environment light sampling: the reference's row/column CDFs or the alias table")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--in-flight", type=int, default=16,
+                    help="samples of every pixel in flight at once with the counter policy (1 to 16)")
+    ap.add_argument("--rng", choices=["counter", "xorwow"], default="counter",
+                    help="random numbers of the plain workload: the counter-based policy (Philox keyed by "
+                         "pixel and sample; samples of a pixel independent) or the reference's per-pixel xorwow "
+                         "stream (samples in order)")
     ap.add_argument("--tail-segment", type=int, default=None,
                     help="segment from which paths finish in one fused launch (default: the library's)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -454,6 +460,7 @@ def main():
     scene, view = scenes.instantiate(spec)
     if args.tail_segment is not None:
         scene.set_tail_segment(args.tail_segment)
+    scene.set_samples_in_flight(args.in_flight)
     scene.build()
     log(f"[bench] BVH build + upload {time.perf_counter() - t0:.1f} s")
     if args.workload == "adjoint":
@@ -466,12 +473,14 @@ def main():
         bench_denoise(args, spec, scene, view)
         return
 
-    from sharpray_amd import LIGHT_ALIAS, LIGHT_CDF, Renderer
+    from sharpray_amd import LIGHT_ALIAS, LIGHT_CDF, RNG_COUNTER, RNG_REFERENCE, Renderer
 
     light = LIGHT_ALIAS if args.light_sampler == "alias" else LIGHT_CDF
+    policy = RNG_COUNTER if args.rng == "counter" else RNG_REFERENCE
 
     def frame(**kw):
         kw.setdefault("light_sampler", light)
+        kw.setdefault("rng_policy", policy)
         return Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
                         spec.variance_threshold, spec.bounces, spec.indirect_clamp, **kw)
 
@@ -501,6 +510,18 @@ def main():
     samples = spec.width * spec.height * spec.max_samples
     ms_per_step = 1e3 * sum(walls) / len(walls)
     msps = samples / (sum(walls) / len(walls)) / 1e6
+    # The same frame with the reference's per-pixel xorwow stream, whose samples run one at a time.
+    xorwow = None
+    if args.rng == "counter":
+        xw = []
+        for i in range(2):
+            t0 = time.perf_counter()
+            frame(rng_policy=RNG_REFERENCE).wait()
+            xw.append(time.perf_counter() - t0)
+        xorwow = {"ms_per_step": round(1e3 * sum(xw) / len(xw), 3),
+                  "msamples_per_s": round(samples / (sum(xw) / len(xw)) / 1e6, 3),
+                  "rng": "reference xorwow per pixel (a pixel's samples in order, one in flight)"}
+        log(f"[bench] xorwow policy: {xorwow}")
 
     # Traversal counters of one (untimed) frame give the algorithmic bytes of the trace kernel
     # (its node and triangle tests include the shadow rays'): all traversal minus the fused tail
@@ -572,7 +593,9 @@ def main():
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": spec.name, "width": spec.width, "height": spec.height, "spp": spec.max_samples,
                    "bounces": spec.bounces, "triangles": spec.triangle_count, "env_map": "2048x1024 f32",
-                   "rng": "reference xorwow per pixel",
+                   "rng": ("counter-based (Philox4x32-10 keyed by pixel, counter by sample)" if args.rng == "counter"
+                           else "reference xorwow per pixel"),
+                   "samples_in_flight": args.in_flight if args.rng == "counter" else 1,
                    "light_sampler": args.light_sampler},
         "gpu_ms_per_step": round(1e3 * sum(gpus) / len(gpus), 3),
         "roofline": {"kernel": "k_trace (closest hit + deferred shadow rays)", "bound": "hbm", "achieved": round(achieved, 2),
@@ -582,6 +605,7 @@ def main():
                      "trace_busy_ms_per_frame": round(busy_ms / args.steps, 3),
                      "algorithmic_bytes_per_launch": round(bytes_per_launch)},
         "fused_tail_ms_per_step": round(tail_ms / args.steps, 3),
+        "xorwow": xorwow,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -354,6 +354,11 @@ class Scene:
         unchanged)."""
         check(N.lib().sr_set_slices(self._ctx, int(slices)), "slices")
 
+    def set_samples_in_flight(self, samples):
+        """Renders with independent samples (counter RNG, fixed sample count, counters off) keep
+        this many samples of every pixel in flight at once (results unchanged)."""
+        check(N.lib().sr_set_samples_in_flight(self._ctx, int(samples)), "samples in flight")
+
     def set_bvh_builder(self, device):
         """Mesh BVHs of later meshes built on the GPU (True) or the host (False); both give the
         reference's tree bit for bit."""

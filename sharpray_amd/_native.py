@@ -165,6 +165,7 @@ SIGNATURES = {
     "sr_profile_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), c_uint32_p]),
     "sr_profile_busy": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "sr_set_slices": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "sr_set_samples_in_flight": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "sr_render_gpu_seconds": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "sr_adjoint_param_count": (ctypes.c_int, [ctypes.c_void_p, c_uint32_p]),
     "sr_adjoint_params_get": (ctypes.c_int, [ctypes.c_void_p, c_float_p]),

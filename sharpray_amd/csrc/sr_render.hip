@@ -55,7 +55,7 @@ struct alignas(16) PathRec {  // 96 B per path: everything shade keeps between s
     vec3 energy;
     float scatterPdf;
     vec3 rad;
-    uint32_t flags;  // bit0: scatterPdf valid, bit1: inside a medium
+    uint32_t flags;  // bit0: scatterPdf valid, bit1: inside a medium, bit2: kPrimaryHit
     vec3 radFog;
     uint32_t seg;    // segment index of the queued ray (0 = camera ray)
     vec3 medium;
@@ -108,7 +108,18 @@ enum : int { kTailNode = SR_COUNTER_COUNT, kTailShape, kTailTrace, kTailOcclude,
 // such rays are traced for their counts only.
 constexpr uint32_t kToFog = 0x80000000u, kCountOnly = 0x40000000u, kPathMask = 0x3fffffffu;
 constexpr int kCtrSpread = 16;  // copies of the counter words (a power of two)
+// PathRec::flags bit: the sample's camera ray hit a surface, whose primary-hit record (Frame::prim)
+// the next-sample pass folds into the pixel's sums.
+constexpr uint32_t kPrimaryHit = 4u;
 constexpr int kProgRing = 4;    // samples an adaptive render's driver runs ahead of its progress reads
+constexpr uint32_t kMaxInFlight = 16;  // samples of a pixel in flight at most
+// Device bytes per path slot (rays, hit, path record, two shadow records, two queue entries,
+// primary-hit record) and the most the path slots of one render may take: samples in flight are
+// capped to fit (a 1080p frame holds 16 samples per pixel in 12 GB).
+constexpr size_t kPathSlotBytes = 3 * sizeof(float4) + sizeof(int32_t) + sizeof(PathRec) + 2 * sizeof(ShadowRec) +
+                                  2 * sizeof(uint32_t) + 3 * sizeof(float4);
+static_assert(sizeof(PathRec) == 96 && sizeof(ShadowRec) == 80, "record sizes");
+constexpr size_t kMaxPathBytes = (size_t)16 << 30;
 
 // Queue counters (uint32 each), per slice. kOverflow collects the queues an append found full (bit
 // 0: ray queues, bit 1: shadow queues); the append is then dropped and the host fails the render.
@@ -124,6 +135,9 @@ struct Frame {
     PathRec* paths;
     ShadowRec* shadows[2];  // shadow rays of even / odd segments
     PixAcc* acc;
+    // Primary-hit record per path slot (normal + depth, albedo + hit flag, uv), written by the
+    // segment-0 shade and folded into the pixel's sums by the next-sample pass in sample order.
+    float4* prim;
     uint32_t* queue[2];
     uint32_t* counts;   // kCounterWords words
     unsigned long long* ctr;  // kCtrWords counters or null
@@ -141,6 +155,9 @@ struct Frame {
     uint32_t* outSamples;
     float* outVar;
     uint32_t W, H, y0, rows, tilesX, nSlots;
+    // Samples in flight per pixel: path slot j = b * nSlots + q runs sample (first of the batch + b)
+    // of pixel slot q; nPaths = batch * nSlots is the capacity of every ray and shadow queue.
+    uint32_t batch, nPaths;
     uint32_t minS, maxS, bounces, seed;
     float threshold, clamp;
     ViewD view;
@@ -156,17 +173,17 @@ SR_DEV bool slot_pixel(const Frame& F, uint32_t q, uint32_t& x, uint32_t& y) {
 
 SR_DEV uint32_t lane_id() { return __lane_id(); }
 
-// Queue appends are bounded by the queue's capacity (nSlots entries per queue and per shadow
+// Queue appends are bounded by the queue's capacity (nPaths entries per queue and per shadow
 // buffer): an append that finds the queue full is dropped and flagged in kOverflow, which the host
 // turns into an error after the band instead of a write past the buffer.
 SR_DEV bool queue_room(const Frame& F, uint32_t slot, uint32_t bit) {
-    if (slot < F.nSlots) return true;
+    if (slot < F.nPaths) return true;
     atomicOr(&F.counts[kOverflow], bit);
     return false;
 }
 // A queue length read back for a launch, clamped to the capacity (only an overflowed queue's count
 // exceeds it, and that render fails).
-SR_DEV uint32_t queue_len(const Frame& F, int word) { return min(F.counts[word], F.nSlots); }
+SR_DEV uint32_t queue_len(const Frame& F, int word) { return min(F.counts[word], F.nPaths); }
 
 
 // Appends `pred` lanes of the whole block (NW waves) to a queue with one atomic per block; slots
@@ -302,12 +319,14 @@ __global__ void k_reset(Frame F) {
     if (threadIdx.x < kCounterWords) F.counts[threadIdx.x] = 0;
 }
 
-// new Rng(x, y) per pixel (renderer.cs:156), zeroed sums, and sample 0's camera ray.
+// new Rng(x, y) per pixel (renderer.cs:156), zeroed sums, and the camera rays of the first batch
+// (samples 0 .. batch - 1; one sample unless the samples are independent, see Frame::batch).
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_start(Frame F) {
     __shared__ uint32_t sh[kBlock / 64 + 1];
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     bool live = false;
+    const uint32_t nb = min(F.batch, F.maxS);
     if (q < F.nSlots) {
         uint32_t x, y;
         PixAcc a;
@@ -316,16 +335,20 @@ __global__ void __launch_bounds__(kBlock) k_start(Frame F) {
         a.active = live ? 1u : 0u;
         F.acc[q] = a;
         if (live) {
-            PathRec p;
-            rng_store(p, xorwow_seed(x, y));
-            begin_sample<R>(F, q, x, y, 0, p);
-            F.paths[q] = p;
+            for (uint32_t b = 0; b < nb; ++b) {
+                PathRec p;
+                rng_store(p, xorwow_seed(x, y));
+                begin_sample<R>(F, q + b * F.nSlots, x, y, b, p);
+                F.paths[q + b * F.nSlots] = p;
+            }
         }
     }
     bump_wave(F, 0, live);
-    bump_wave(F, 1, live);
-    const uint32_t slot = block_append<kBlock / 64>(&F.counts[kRayCount0], live, sh);
-    if (live && queue_room(F, slot, 1u)) F.queue[0][slot] = q;
+    for (uint32_t b = 0; b < F.batch; ++b) {  // queue 0 holds batch 0's tiles, then batch 1's, ...
+        bump_wave(F, 1, live && b < nb);
+        const uint32_t slot = block_append<kBlock / 64>(&F.counts[kRayCount0], live && b < nb, sh);
+        if (live && b < nb && queue_room(F, slot, 1u)) F.queue[0][slot] = q + b * F.nSlots;
+    }
 }
 
 // Counts of one shade pass, flushed into the render counters once per thread.
@@ -390,15 +413,12 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
         const vec3 hitPos = at(ray, dist - bias);
         if (p.flags & 2u) p.energy = p.energy * beer(p.medium, dist);
         p.rad = p.rad + s.radiance * p.energy;
-        if (primary) {
-            PixAcc* a = &F.acc[q];
-            a->normalSum = a->normalSum + s.normal;
-            a->normalFallback = s.normal;
-            a->normalCount += 1;
-            a->depthSum += dist;
-            a->depthCount += 1;
-            a->albedoSum = a->albedoSum + s.albedo;
-            if (!a->hasUv) { a->hasUv = 1; a->uv = s.uv; }
+        if (primary) {  // the pixel's normal, depth, albedo and uv sums take it in sample order
+            float4* pr = F.prim + (size_t)q * 3;
+            pr[0] = make_float4(s.normal.x, s.normal.y, s.normal.z, dist);
+            pr[1] = make_float4(s.albedo.x, s.albedo.y, s.albedo.z, 0.0f);
+            pr[2] = make_float4(s.uv.x, s.uv.y, 0.0f, 0.0f);
+            p.flags |= kPrimaryHit;
         }
         // SampleSkyDirect (scene.cs:774-794): the sides are tested before the shadow ray,
         // which only skips rays whose contribution is black either way.
@@ -739,10 +759,10 @@ SR_DEV float rel_std_err(float sum, float sumSqr, uint32_t n) {
     return sqrtf(variance / (float)n) / net_max(mean, 1e-4f);
 }
 
-// After a sample's last segment, per active pixel: accumulation and the convergence test
-// (renderer.cs:180-212); then either the pixel's next camera ray, appended to queue 0 in slot
-// order so the next sample's camera rays stay tile-coherent, or its final AOVs
-// (renderer.cs:214-229, 142-148).
+// After a batch's last segment, per active pixel and in sample order: accumulation, the primary
+// hit's normal / depth / albedo / uv and the convergence test (renderer.cs:180-212); then either
+// the pixel's next batch of camera rays, appended to queue 0 in slot order (batch by batch) so the
+// next camera rays stay tile-coherent, or its final AOVs (renderer.cs:214-229, 142-148).
 template <class R>
 __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     __shared__ uint32_t sh[kNextBlock / 64 + 1];
@@ -751,50 +771,73 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
         F.counts[kShadowCount0] = 0;
         F.counts[kShadowCount1] = 0;
     }
-    bool again = false, converged = false, maxed = false;
+    bool converged = false, maxed = false;
+    uint32_t again = 0;  // camera rays started for the next batch
     unsigned long long countMinKey = 0, countMaxKey = 0, varKey = 0;
     uint32_t doneBlock = ~0u;  // the block of a pixel that finished in this pass
     // Fields are read and written one by one: a sample touches the accumulator's first 48 B and the
     // path's radiance and random state; the rest is read once, when the pixel finishes.
     PixAcc* A = F.acc + q;
     if (q < F.nSlots && A->active) {
-        const PathRec* P = F.paths + q;
-        const vec3 rad = P->rad, radFog = P->radFog;
-        const vec3 radSum = A->radSum + rad;
-        const vec3 radFogSum = A->radFogSum + radFog;
-        const float lum = luminance(rad) + luminance(radFog);
-        const float lumSum = A->lumSum + lum;
-        const float lumSumSqr = A->lumSumSqr + lum * lum;
-        const float lumSurf = luminance(rad);
-        const float lumSurfAcc = A->lumSurf + lumSurf;
-        const float lumSurfSqr = A->lumSurfSqr + lumSurf * lumSurf;
+        const uint32_t base = A->sampleIndex;
+        const uint32_t nb = min(F.batch, F.maxS - base);
+        vec3 radSum = A->radSum, radFogSum = A->radFogSum;
+        float lumSum = A->lumSum, lumSumSqr = A->lumSumSqr, lumSurfAcc = A->lumSurf, lumSurfSqr = A->lumSurfSqr;
+        bool done = false;
+        uint32_t count = 0;
+        for (uint32_t b = 0; b < nb && !done; ++b) {
+            const uint32_t si = base + b;
+            const size_t j = (size_t)q + (size_t)b * F.nSlots;
+            const PathRec* P = F.paths + j;
+            if (P->flags & kPrimaryHit) {  // the sample's camera ray hit a surface (renderer.cs:187-203)
+                const float4 pr0 = F.prim[j * 3], pr1 = F.prim[j * 3 + 1], pr2 = F.prim[j * 3 + 2];
+                const vec3 nrm = mk3(pr0.x, pr0.y, pr0.z);
+                A->normalSum = A->normalSum + nrm;
+                A->normalFallback = nrm;
+                A->normalCount += 1;
+                A->depthSum += pr0.w;
+                A->depthCount += 1;
+                A->albedoSum = A->albedoSum + mk3(pr1.x, pr1.y, pr1.z);
+                if (!A->hasUv) { A->hasUv = 1; A->uv = mk2(pr2.x, pr2.y); }
+            }
+            const vec3 rad = P->rad, radFog = P->radFog;
+            radSum = radSum + rad;
+            radFogSum = radFogSum + radFog;
+            const float lum = luminance(rad) + luminance(radFog);
+            lumSum = lumSum + lum;
+            lumSumSqr = lumSumSqr + lum * lum;
+            const float lumSurf = luminance(rad);
+            lumSurfAcc = lumSurfAcc + lumSurf;
+            lumSurfSqr = lumSurfSqr + lumSurf * lumSurf;
+            if (si >= F.minS - 1u && rel_std_err(lumSum, lumSumSqr, si + 1) < F.threshold) {
+                done = true;
+                count = si + 1;
+            } else if (si + 1 == F.maxS) {
+                done = true;
+                count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
+            }
+        }
         A->radSum = radSum;
         A->radFogSum = radFogSum;
         A->lumSum = lumSum;
         A->lumSumSqr = lumSumSqr;
         A->lumSurf = lumSurfAcc;
         A->lumSurfSqr = lumSurfSqr;
-        const uint32_t si = A->sampleIndex;
-        bool done = false;
-        uint32_t count = 0;
-        if (si >= F.minS - 1u && rel_std_err(lumSum, lumSumSqr, si + 1) < F.threshold) {
-            done = true;
-            count = si + 1;
-        } else if (si + 1 == F.maxS) {
-            done = true;
-            count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
-        }
         uint32_t x, y;
         slot_pixel(F, q, x, y);
         if (!done) {
-            A->sampleIndex = si + 1;
-            PathRec p;
-            for (int k = 0; k < 5; ++k) p.rng[k] = P->rng[k];
-            p.pad0 = 0;
-            p.pad1[0] = p.pad1[1] = p.pad1[2] = 0;
-            begin_sample<R>(F, q, x, y, si + 1, p);
-            F.paths[q] = p;
-            again = true;
+            const uint32_t next = base + nb;
+            A->sampleIndex = next;
+            again = min(F.batch, F.maxS - next);
+            for (uint32_t b = 0; b < again; ++b) {
+                const size_t j = (size_t)q + (size_t)b * F.nSlots;
+                PathRec p;
+                for (int k = 0; k < 5; ++k) p.rng[k] = F.paths[j].rng[k];
+                p.pad0 = 0;
+                p.pad1[0] = p.pad1[1] = p.pad1[2] = 0;
+                begin_sample<R>(F, (uint32_t)j, x, y, next + b, p);
+                F.paths[j] = p;
+            }
         } else {
             A->active = 0;
             const PixAcc a = *A;
@@ -839,16 +882,18 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
             if (atomicAdd(&F.blockPixels[mine], add) + add == bw * bh) atomicAdd(F.blocksDone, 1u);
         }
     }
-    bump_wave(F, 1, again);
     bump_wave(F, 8, converged);
     bump_wave(F, 9, maxed);
     bump_max(F, kCountMinKey, countMinKey);
     bump_max(F, kCountMaxKey, countMaxKey);
     bump_max(F, kVarMaxKey, varKey);
-    // One atomic per block: one per wave on the single queue counter serialised the launch
-    // (26.1 -> 14.0 ms per 1080p frame).
-    const uint32_t slot = block_append<kNextBlock / 64>(&F.counts[kRayCount0], again, sh);
-    if (again && queue_room(F, slot, 1u)) F.queue[0][slot] = q;
+    // One atomic per block and batch index: one per wave on the single queue counter serialised the
+    // launch (26.1 -> 14.0 ms per 1080p frame).
+    for (uint32_t b = 0; b < F.batch; ++b) {
+        bump_wave(F, 1, b < again);
+        const uint32_t slot = block_append<kNextBlock / 64>(&F.counts[kRayCount0], b < again, sh);
+        if (b < again && queue_room(F, slot, 1u)) F.queue[0][slot] = q + b * F.nSlots;
+    }
 }
 
 // Primary-hit diagnostic: sample 0 of every pixel of a band.
@@ -1236,6 +1281,7 @@ struct sr_ctx {
     DevBuf<float4> dRayO, dRayD, dHitA;
     DevBuf<int32_t> dHitSlot;
     DevBuf<PathRec> dPaths;
+    DevBuf<float4> dPrim;  // 3 per path slot
     DevBuf<ShadowRec> dShadows;
     DevBuf<PixAcc> dAcc;
     DevBuf<uint32_t> dQueue0, dQueue1, dCounts;
@@ -1276,6 +1322,13 @@ struct sr_ctx {
     // every kernel launch has the GPU to itself and its timing measures that kernel alone.
     // SHARPRAY_SLICES or sr_set_slices override it.
     uint32_t slices = 1;
+    // Samples of a pixel in flight at once when samples are independent (counter RNG policy, a
+    // fixed sample count, counters off): every launch then holds that many samples of every pixel,
+    // so the fixed cost of each launch's slowest waves is paid once per batch instead of once per
+    // sample. Results are identical for any value. SHARPRAY_IN_FLIGHT or
+    // sr_set_samples_in_flight override it.
+    uint32_t inFlight = kMaxInFlight;
+    uint32_t batch = 1;                     // samples in flight of the current render
     uint32_t sliceCount = 1;                // slices of the current render
     uint32_t sliceRows = 0, sliceSlots = 0;  // rows (multiple of 8) and slots per slice
     std::vector<hipStream_t> sliceStreams;   // slices 1.. (slice 0 runs on `stream`)
@@ -1650,6 +1703,8 @@ int sr_create(int32_t device, sr_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->gridCUs = prop.multiProcessorCount;
     if (const char* t = getenv("SHARPRAY_TAIL_SEGMENT")) c->tailSegment = (uint32_t)strtoul(t, nullptr, 10);
     if (const char* t = getenv("SHARPRAY_SLICES")) c->slices = std::max(1u, (uint32_t)strtoul(t, nullptr, 10));
+    if (const char* t = getenv("SHARPRAY_IN_FLIGHT"))
+        c->inFlight = std::min(kMaxInFlight, std::max(1u, (uint32_t)strtoul(t, nullptr, 10)));
     *out = c;
     return SR_OK;
 }
@@ -1951,17 +2006,20 @@ ViewD make_view(const sr_view* v, uint32_t W, uint32_t H) {
 Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     Frame F;
     memset(&F, 0, sizeof(F));
-    const size_t o = (size_t)slice * c->sliceSlots;
-    F.rayO = c->dRayO.ptr + o;
-    F.rayD = c->dRayD.ptr + o;
-    F.hitA = c->dHitA.ptr + o;
-    F.hitSlot = c->dHitSlot.ptr + o;
-    F.paths = c->dPaths.ptr + o;
-    F.shadows[0] = c->dShadows.ptr + 2 * o;
-    F.shadows[1] = c->dShadows.ptr + 2 * o + c->sliceSlots;
+    const size_t o = (size_t)slice * c->sliceSlots;             // pixel slots
+    const size_t pathSlots = (size_t)c->sliceSlots * c->batch;  // path slots per slice
+    const size_t op = (size_t)slice * pathSlots;
+    F.rayO = c->dRayO.ptr + op;
+    F.rayD = c->dRayD.ptr + op;
+    F.hitA = c->dHitA.ptr + op;
+    F.hitSlot = c->dHitSlot.ptr + op;
+    F.paths = c->dPaths.ptr + op;
+    F.prim = c->dPrim.ptr + 3 * op;
+    F.shadows[0] = c->dShadows.ptr + 2 * op;
+    F.shadows[1] = c->dShadows.ptr + 2 * op + pathSlots;
     F.acc = c->dAcc.ptr + o;
-    F.queue[0] = c->dQueue0.ptr + o;
-    F.queue[1] = c->dQueue1.ptr + o;
+    F.queue[0] = c->dQueue0.ptr + op;
+    F.queue[1] = c->dQueue1.ptr + op;
     F.counts = c->dCounts.ptr + (size_t)slice * kCounterWords;
     F.ctr = c->countersOn ? c->dCtr.ptr : nullptr;
     F.outRad = c->dRad.ptr;
@@ -1978,6 +2036,8 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     F.rows = rows;
     F.tilesX = (F.W + 7) / 8;
     F.nSlots = F.tilesX * ((rows + 7) / 8) * 64;
+    F.batch = c->batch;
+    F.nPaths = F.nSlots * F.batch;
     F.minS = c->params.min_samples;
     F.maxS = c->params.max_samples;
     F.bounces = c->params.bounces;
@@ -1992,15 +2052,18 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     return F;
 }
 
-// Frame buffers for `slices` slices of `sliceSlots` slots each.
+// Frame buffers for `slices` slices of `sliceSlots` pixel slots each, with c->batch path slots per
+// pixel slot.
 int alloc_frame(sr_ctx* c, uint32_t sliceSlots, uint32_t slices) {
     const size_t px = (size_t)c->params.width * c->params.height;
     const size_t maxSlots = (size_t)sliceSlots * slices;
+    const size_t maxPaths = maxSlots * c->batch;
     c->sliceSlots = sliceSlots;
     int rc;
-    if ((rc = ensure(c->dRayO, maxSlots)) || (rc = ensure(c->dRayD, maxSlots)) || (rc = ensure(c->dHitA, maxSlots)) ||
-        (rc = ensure(c->dHitSlot, maxSlots)) || (rc = ensure(c->dPaths, maxSlots)) || (rc = ensure(c->dShadows, maxSlots * 2)) ||
-        (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxSlots)) || (rc = ensure(c->dQueue1, maxSlots)) ||
+    if ((rc = ensure(c->dRayO, maxPaths)) || (rc = ensure(c->dRayD, maxPaths)) || (rc = ensure(c->dHitA, maxPaths)) ||
+        (rc = ensure(c->dHitSlot, maxPaths)) || (rc = ensure(c->dPaths, maxPaths)) || (rc = ensure(c->dShadows, maxPaths * 2)) ||
+        (rc = ensure(c->dPrim, maxPaths * 3)) ||
+        (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxPaths)) || (rc = ensure(c->dQueue1, maxPaths)) ||
         (rc = ensure(c->dCounts, (size_t)kCounterWords * slices)) ||
         (rc = ensure(c->dCtr, (size_t)kCtrWords * kCtrSpread)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
         (rc = ensure(c->dNormal, px * 3)) || (rc = ensure(c->dUv, px * 2)) || (rc = ensure(c->dDepth, px)) || (rc = ensure(c->dAlbedo, px * 3)) ||
@@ -2045,8 +2108,8 @@ void mark(sr_ctx* c, uint32_t slice, int kind) {
 // once): the dispatcher then hands chunks to whichever CU frees up, where a grid-stride loop over
 // a resident-sized grid fixes each wave's share in advance and the launch waits for the unluckiest
 // wave (trace 207 -> 188 ms, shade 85.5 -> 82.8 ms per 1080p frame; one-wave trace blocks,
-// 188 -> 179 ms, free a CU slot per finished wave). A trace launch holds at most nSlots rays
-// (padded to whole waves) plus nSlots shadow rays; a shade launch at most nSlots rays.
+// 188 -> 179 ms, free a CU slot per finished wave). A trace launch holds at most nPaths rays
+// (padded to whole waves) plus nPaths shadow rays; a shade launch at most nPaths rays.
 struct SliceRun {
     Frame F;
     uint32_t slice;
@@ -2060,8 +2123,8 @@ SliceRun slice_run(sr_ctx* c, const Frame& F, uint32_t slice) {
     r.slice = slice;
     r.s = slice_stream(c, slice);
     r.slotBlocks = (int)((F.nSlots + kBlock - 1) / kBlock);
-    r.gTrace = (int)((2u * F.nSlots + 64u + kTraceBlock - 1) / kTraceBlock);
-    r.gShade = (int)((F.nSlots + kShadeBlock - 1) / kShadeBlock);
+    r.gTrace = (int)((2u * F.nPaths + 64u + kTraceBlock - 1) / kTraceBlock);
+    r.gShade = (int)((F.nPaths + kShadeBlock - 1) / kShadeBlock);
     r.gTail = c->gridCUs * 8;
     return r;
 }
@@ -2156,7 +2219,7 @@ int run_frame(sr_ctx* c) {
             if (live == 0) bandIdle = true;
             return SR_OK;
         };
-        for (uint32_t smp = 0; smp < c->params.max_samples && !bandIdle; ++smp) {
+        for (uint32_t smp = 0; smp < c->params.max_samples && !bandIdle; smp += c->batch) {
             for (const SliceRun& r : runs) enqueue_sample<R, COUNT>(c, r);
             c->iterations++;
             if (!adaptive) continue;
@@ -2272,6 +2335,14 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
         ctx->sliceStreams.push_back(st);
     }
     ctx->blocksTotal = ((p->width + p->block_size - 1) / p->block_size) * ((p->height + p->block_size - 1) / p->block_size);
+    // Several samples of a pixel in flight need independent samples: the counter RNG (the xorwow
+    // stream of a pixel runs through its samples in order), no adaptive stop (it decides after every
+    // sample whether the next one runs) and no counters (they follow the reference's work).
+    const bool adaptive = p->min_samples < p->max_samples && p->variance_threshold > 0.0f;
+    const size_t pixelSlots = (size_t)tilesX * (ctx->sliceRows / 8) * 64 * ctx->sliceCount;
+    const uint32_t fit = (uint32_t)std::max<size_t>(1, kMaxPathBytes / (pixelSlots * kPathSlotBytes));
+    ctx->batch = (p->rng_policy == SR_RNG_COUNTER && !adaptive && !ctx->countersOn)
+                     ? std::min(std::min(ctx->inFlight, p->max_samples), fit) : 1u;
     int rc = alloc_frame(ctx, tilesX * (ctx->sliceRows / 8) * 64, ctx->sliceCount);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(ctx->dBlockPixels.ptr, 0, (size_t)ctx->blocksTotal * 4, ctx->stream));
@@ -2435,6 +2506,13 @@ int sr_profile_busy(sr_ctx* ctx, double* ms) {
 int sr_set_slices(sr_ctx* ctx, uint32_t slices) {
     if (!ctx || slices == 0) return fail(SR_ERR_INVALID, "set_slices: need a context and at least one slice");
     ctx->slices = slices;
+    return SR_OK;
+}
+
+int sr_set_samples_in_flight(sr_ctx* ctx, uint32_t samples) {
+    if (!ctx || samples == 0 || samples > kMaxInFlight)
+        return fail(SR_ERR_INVALID, "set_samples_in_flight: need a context and 1 to " + std::to_string(kMaxInFlight) + " samples");
+    ctx->inFlight = samples;
     return SR_OK;
 }
 

@@ -151,3 +151,38 @@ def test_failed_render_start_leaves_no_render():
     r2 = Renderer(scene, view, spec.width, spec.height, 32, 1, 1, 0.0, spec.bounces, spec.indirect_clamp)
     r2.wait()
     assert np.array_equal(bits(r2.radiance), bits(first))
+
+
+@pytest.mark.parametrize("case", ["plain-2", "plain-3-tail0", "plain-5-slices2", "plain-16", "fog-4"])
+def test_samples_in_flight_match_one_sample_at_a_time(case):
+    """Several samples of every pixel in flight (counter RNG, fixed sample count) give AOVs
+    bit-identical to running the samples one at a time: each sample draws from its own (pixel,
+    sample) counter stream and each pixel folds its samples into its sums in sample order. Covers a
+    sample count that the batch does not divide (7 spp), every path in the fused tail launch from
+    its camera ray on (tail segment 0), concurrent row slices, more samples in flight than samples,
+    and grid fog (camera rays that scatter in the medium record no primary surface)."""
+    from sharpray_amd import RNG_COUNTER, Renderer, scenes
+    kind, k = case.split("-")[:2]
+    k = int(k)
+    if kind == "fog":
+        spec = scenes.config_fog(scale=0.1, spp=7)
+    else:
+        spec = scenes.config_plain_gpu(scale=0.25, spp=7)
+        spec.height = 268
+    scene, view = scenes.instantiate(spec)
+    if "tail0" in case:
+        scene.set_tail_segment(0)
+    if "slices2" in case:
+        scene.set_slices(2)
+    scene.build()
+    out = {}
+    for n in (1, k):
+        scene.set_samples_in_flight(n)
+        r = Renderer(scene, view, spec.width, spec.height, 32, spec.max_samples, spec.max_samples, 0.0,
+                     spec.bounces, spec.indirect_clamp, rng_policy=RNG_COUNTER, rng_seed=11)
+        r.wait()
+        out[n] = _aovs(r) + (r.albedo.copy(),)
+    print(f"{case}: {spec.width}x{spec.height}, {spec.max_samples} spp, {k} samples in flight", flush=True)
+    for a, b in zip(out[1], out[k]):
+        assert np.array_equal(bits(a), bits(b)) if a.dtype == np.float32 else np.array_equal(a, b)
+    assert (out[1][5] == spec.max_samples + 1).all()
