@@ -70,11 +70,13 @@ struct alignas(16) ShadowRec {  // 80 B
     vec3 energy;
     uint32_t path;  // bit31: add to radianceFog
 };
-struct alignas(16) PixAcc {  // 112 B; the first 48 B change every sample, the rest at primary hits
+struct alignas(16) PixSums {  // 48 B, changed by every sample
     vec3 radSum, radFogSum;
     float lumSum, lumSumSqr, lumSurf, lumSurfSqr;
-    uint32_t sampleIndex;
+    uint32_t sampleIndex;  // first sample of the batch in flight
     uint32_t active;
+};
+struct alignas(16) PixPrimary {  // 64 B, changed by samples whose camera ray hits a surface
     vec3 normalSum, normalFallback;
     uint32_t normalCount;
     vec2 uv;
@@ -83,6 +85,10 @@ struct alignas(16) PixAcc {  // 112 B; the first 48 B change every sample, the r
     uint32_t depthCount;
     vec3 albedoSum;  // primary-hit Surface.Albedo, averaged over depthCount
     uint32_t pad;
+};
+struct alignas(16) PixAcc {  // 112 B per pixel: the sums of renderer.cs:160-203
+    PixSums s;
+    PixPrimary h;
 };
 
 struct ViewD {
@@ -229,18 +235,30 @@ SR_DEV void rng_store(PathRec& p, const CounterRng& r) {
 }
 // The RNG that draws sample `sample` of pixel (x, y): the pixel's persistent xorwow stream, or a
 // fresh Philox counter for that (pixel, sample).
-template <class R> SR_DEV R rng_for_sample(const Frame& F, const PathRec& p, uint32_t x, uint32_t y, uint32_t sample);
-template <> SR_DEV XorwowRng rng_for_sample<XorwowRng>(const Frame&, const PathRec& p, uint32_t, uint32_t, uint32_t) {
-    return rng_load<XorwowRng>(p);
+// The RNG that draws sample `sample` of pixel (x, y): the pixel's persistent xorwow stream (new
+// Rng(x, y) for sample 0, renderer.cs:156; the state the previous sample left in path slot j
+// otherwise), or a fresh Philox counter for that (pixel, sample).
+template <class R> SR_DEV R rng_for_sample(const Frame& F, size_t j, uint32_t x, uint32_t y, uint32_t sample);
+template <> SR_DEV XorwowRng rng_for_sample<XorwowRng>(const Frame& F, size_t j, uint32_t x, uint32_t y, uint32_t sample) {
+    if (sample == 0) return xorwow_seed(x, y);
+    const uint32_t* w = F.paths[j].rng;
+    XorwowRng r;
+    r.s0 = w[0]; r.s1 = w[1]; r.s2 = w[2]; r.s3 = w[3]; r.s4 = w[4];
+    return r;
 }
-template <> SR_DEV CounterRng rng_for_sample<CounterRng>(const Frame& F, const PathRec&, uint32_t x, uint32_t y,
-                                                         uint32_t sample) {
+template <> SR_DEV CounterRng rng_for_sample<CounterRng>(const Frame& F, size_t, uint32_t x, uint32_t y, uint32_t sample) {
     CounterRng r;
     r.key0 = F.seed;
     r.key1 = y * F.W + x;
     r.sample = sample;
     r.draw = 0;
     return r;
+}
+SR_DEV void rng_store_words(PathRec* p, const XorwowRng& r) {
+    p->rng[0] = r.s0; p->rng[1] = r.s1; p->rng[2] = r.s2; p->rng[3] = r.s3; p->rng[4] = r.s4;
+}
+SR_DEV void rng_store_words(PathRec* p, const CounterRng& r) {
+    *(uint4*)p->rng = make_uint4(r.key0, r.key1, r.sample, r.draw);
 }
 
 // Camera ray of one sample: jitter X then Y (renderer.cs:175), View.Ray (math.cs:1613-1636).
@@ -258,14 +276,23 @@ SR_DEV void camera_ray(const ViewD& v, uint32_t W, uint32_t H, uint32_t x, uint3
     d = rotate(v.rot, normalized(mk3(ndcX * th, ndcY * tv, 1.0f)));
 }
 
-// Starts sample `sample` of slot q: camera ray plus a fresh path record carrying the RNG.
+// Starts sample `sample` of path slot j: its camera ray and the random state after the ray's two
+// jitter draws. The shade of segment 0 starts the path record's other fields (start_path).
 template <class R>
-SR_DEV void begin_sample(const Frame& F, uint32_t q, uint32_t x, uint32_t y, uint32_t sample, PathRec& p) {
-    R rng = rng_for_sample<R>(F, p, x, y, sample);
+SR_DEV void begin_sample(const Frame& F, size_t j, uint32_t x, uint32_t y, uint32_t sample) {
+    R rng = rng_for_sample<R>(F, j, x, y, sample);
     vec3 o, d;
     camera_ray(F.view, F.W, F.H, x, y, rng, o, d);
-    F.rayO[q] = make_float4(o.x, o.y, o.z, 0.0f);
-    F.rayD[q] = make_float4(d.x, d.y, d.z, 0.0f);
+    F.rayO[j] = make_float4(o.x, o.y, o.z, 0.0f);
+    F.rayD[j] = make_float4(d.x, d.y, d.z, 0.0f);
+    rng_store_words(F.paths + j, rng);
+}
+
+// The record of a path at its camera ray (Scene.Sample's initial state, scene.cs:632-640) with the
+// random state begin_sample stored.
+template <class R>
+SR_DEV PathRec start_path(const Frame& F, uint32_t j) {
+    PathRec p;
     p.energy = splat3(1.0f);
     p.scatterPdf = 0.0f;
     p.rad = splat3(0.0f);
@@ -273,7 +300,11 @@ SR_DEV void begin_sample(const Frame& F, uint32_t q, uint32_t x, uint32_t y, uin
     p.radFog = splat3(0.0f);
     p.seg = 0;
     p.medium = splat3(0.0f);
-    rng_store(p, rng);
+    p.pad0 = 0;
+    const uint32_t* w = F.paths[j].rng;
+    for (int k = 0; k < 5; ++k) p.rng[k] = w[k];
+    p.pad1[0] = p.pad1[1] = p.pad1[2] = 0;
+    return p;
 }
 
 // The render counters are kept in kCtrSpread copies, picked by block index, so the many short
@@ -332,16 +363,10 @@ __global__ void __launch_bounds__(kBlock) k_start(Frame F) {
         PixAcc a;
         memset(&a, 0, sizeof(a));
         live = slot_pixel(F, q, x, y);
-        a.active = live ? 1u : 0u;
+        a.s.active = live ? 1u : 0u;
         F.acc[q] = a;
-        if (live) {
-            for (uint32_t b = 0; b < nb; ++b) {
-                PathRec p;
-                rng_store(p, xorwow_seed(x, y));
-                begin_sample<R>(F, q + b * F.nSlots, x, y, b, p);
-                F.paths[q + b * F.nSlots] = p;
-            }
-        }
+        if (live)
+            for (uint32_t b = 0; b < nb; ++b) begin_sample<R>(F, (size_t)q + (size_t)b * F.nSlots, x, y, b);
     }
     bump_wave(F, 0, live);
     for (uint32_t b = 0; b < F.batch; ++b) {  // queue 0 holds batch 0's tiles, then batch 1's, ...
@@ -486,7 +511,10 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
 #endif
 // FOG = false compiles the participating-medium code out for scenes without fog or media (fewer
 // registers: 3 waves per SIMD instead of 2).
-template <class R, bool FOG>
+// FIRST: the launch's rays are camera rays, whose path records start here (start_path); its own
+// instance keeps the other segments' register count (a runtime flag cost the fog-free instance
+// its third wave per SIMD).
+template <class R, bool FOG, bool FIRST>
 __global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, DevScene S, int cur) {
     const uint32_t n = queue_len(F, cur);
     const uint32_t* Q = F.queue[cur];
@@ -530,7 +558,7 @@ __global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame
         uint32_t q = 0;
         if (i < n) {
             q = sQ[threadIdx.x];
-            PathRec p = F.paths[q];
+            PathRec p = FIRST ? start_path<R>(F, q) : F.paths[q];
             const float4 ho = F.hitA[q];
             const float4 ro = F.rayO[q], rd = F.rayD[q];
             HitRec h;
@@ -625,7 +653,7 @@ SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, 
 #define SR_TAIL_MIN_WAVES 1
 #endif
 template <class R, bool COUNT, bool TRANSP>
-__global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, DevScene S, int cur) {
+__global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, DevScene S, int cur, int first) {
     const uint32_t n = queue_len(F, cur);
     const uint32_t* Q = F.queue[cur];
     ShadeCount c;
@@ -633,7 +661,7 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
     uint32_t extra = 0, traces = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t q = Q[i];
-        PathRec p = F.paths[q];
+        PathRec p = first ? start_path<R>(F, q) : F.paths[q];
         R rng = rng_load<R>(p);
         const float4 ho = F.hitA[q];
         const float4 ro = F.rayO[q], rd = F.rayD[q];
@@ -777,39 +805,46 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     uint32_t doneBlock = ~0u;  // the block of a pixel that finished in this pass
     // Fields are read and written one by one: a sample touches the accumulator's first 48 B and the
     // path's radiance and random state; the rest is read once, when the pixel finishes.
+    // The pixel's sums stay in registers over the batch: its 48-byte per-sample part is read and
+    // written once, the 64-byte primary-hit part only when one of its samples hit a surface or the
+    // pixel finishes; of each path the radiance, fog radiance and flags are read.
     PixAcc* A = F.acc + q;
-    if (q < F.nSlots && A->active) {
-        const uint32_t base = A->sampleIndex;
+    PixSums a = q < F.nSlots ? A->s : PixSums{};
+    if (q < F.nSlots && a.active) {
+        const uint32_t base = a.sampleIndex;
         const uint32_t nb = min(F.batch, F.maxS - base);
-        vec3 radSum = A->radSum, radFogSum = A->radFogSum;
-        float lumSum = A->lumSum, lumSumSqr = A->lumSumSqr, lumSurfAcc = A->lumSurf, lumSurfSqr = A->lumSurfSqr;
+        PixPrimary h;
+        bool hLoaded = false;
         bool done = false;
         uint32_t count = 0;
         for (uint32_t b = 0; b < nb && !done; ++b) {
             const uint32_t si = base + b;
             const size_t j = (size_t)q + (size_t)b * F.nSlots;
             const PathRec* P = F.paths + j;
-            if (P->flags & kPrimaryHit) {  // the sample's camera ray hit a surface (renderer.cs:187-203)
+            const float4 r0 = *(const float4*)&P->rad;     // rad, flags
+            const float4 r1 = *(const float4*)&P->radFog;  // radFog, seg
+            if (__float_as_uint(r0.w) & kPrimaryHit) {  // the camera ray hit a surface (renderer.cs:187-203)
+                if (!hLoaded) { h = A->h; hLoaded = true; }
                 const float4 pr0 = F.prim[j * 3], pr1 = F.prim[j * 3 + 1], pr2 = F.prim[j * 3 + 2];
                 const vec3 nrm = mk3(pr0.x, pr0.y, pr0.z);
-                A->normalSum = A->normalSum + nrm;
-                A->normalFallback = nrm;
-                A->normalCount += 1;
-                A->depthSum += pr0.w;
-                A->depthCount += 1;
-                A->albedoSum = A->albedoSum + mk3(pr1.x, pr1.y, pr1.z);
-                if (!A->hasUv) { A->hasUv = 1; A->uv = mk2(pr2.x, pr2.y); }
+                h.normalSum = h.normalSum + nrm;
+                h.normalFallback = nrm;
+                h.normalCount += 1;
+                h.depthSum += pr0.w;
+                h.depthCount += 1;
+                h.albedoSum = h.albedoSum + mk3(pr1.x, pr1.y, pr1.z);
+                if (!h.hasUv) { h.hasUv = 1; h.uv = mk2(pr2.x, pr2.y); }
             }
-            const vec3 rad = P->rad, radFog = P->radFog;
-            radSum = radSum + rad;
-            radFogSum = radFogSum + radFog;
+            const vec3 rad = mk3(r0.x, r0.y, r0.z), radFog = mk3(r1.x, r1.y, r1.z);
+            a.radSum = a.radSum + rad;
+            a.radFogSum = a.radFogSum + radFog;
             const float lum = luminance(rad) + luminance(radFog);
-            lumSum = lumSum + lum;
-            lumSumSqr = lumSumSqr + lum * lum;
+            a.lumSum = a.lumSum + lum;
+            a.lumSumSqr = a.lumSumSqr + lum * lum;
             const float lumSurf = luminance(rad);
-            lumSurfAcc = lumSurfAcc + lumSurf;
-            lumSurfSqr = lumSurfSqr + lumSurf * lumSurf;
-            if (si >= F.minS - 1u && rel_std_err(lumSum, lumSumSqr, si + 1) < F.threshold) {
+            a.lumSurf = a.lumSurf + lumSurf;
+            a.lumSurfSqr = a.lumSurfSqr + lumSurf * lumSurf;
+            if (si >= F.minS - 1u && rel_std_err(a.lumSum, a.lumSumSqr, si + 1) < F.threshold) {
                 done = true;
                 count = si + 1;
             } else if (si + 1 == F.maxS) {
@@ -817,43 +852,32 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
                 count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
             }
         }
-        A->radSum = radSum;
-        A->radFogSum = radFogSum;
-        A->lumSum = lumSum;
-        A->lumSumSqr = lumSumSqr;
-        A->lumSurf = lumSurfAcc;
-        A->lumSurfSqr = lumSurfSqr;
         uint32_t x, y;
         slot_pixel(F, q, x, y);
         if (!done) {
             const uint32_t next = base + nb;
-            A->sampleIndex = next;
+            a.sampleIndex = next;
+            A->s = a;
+            if (hLoaded) A->h = h;
             again = min(F.batch, F.maxS - next);
-            for (uint32_t b = 0; b < again; ++b) {
-                const size_t j = (size_t)q + (size_t)b * F.nSlots;
-                PathRec p;
-                for (int k = 0; k < 5; ++k) p.rng[k] = F.paths[j].rng[k];
-                p.pad0 = 0;
-                p.pad1[0] = p.pad1[1] = p.pad1[2] = 0;
-                begin_sample<R>(F, (uint32_t)j, x, y, next + b, p);
-                F.paths[j] = p;
-            }
+            for (uint32_t b = 0; b < again; ++b) begin_sample<R>(F, (size_t)q + (size_t)b * F.nSlots, x, y, next + b);
         } else {
-            A->active = 0;
-            const PixAcc a = *A;
+            if (!hLoaded) h = A->h;
+            a.active = 0;
+            A->s.active = 0;
             const size_t o = (size_t)y * F.W + x;
-            const float var = rel_std_err(lumSurfAcc, lumSurfSqr, count);
-            const vec3 r = radSum / (float)count;
-            const vec3 rf = radFogSum / (float)count;
-            const vec3 nrm = a.normalCount > 0 ? normalized_or(a.normalSum, a.normalFallback) : splat3(0.0f);
-            const float depth = a.depthCount > 0 ? a.depthSum / (float)a.depthCount : kInf;
+            const float var = rel_std_err(a.lumSurf, a.lumSurfSqr, count);
+            const vec3 r = a.radSum / (float)count;
+            const vec3 rf = a.radFogSum / (float)count;
+            const vec3 nrm = h.normalCount > 0 ? normalized_or(h.normalSum, h.normalFallback) : splat3(0.0f);
+            const float depth = h.depthCount > 0 ? h.depthSum / (float)h.depthCount : kInf;
             F.outRad[o * 3] = r.x; F.outRad[o * 3 + 1] = r.y; F.outRad[o * 3 + 2] = r.z;
             F.outRadFog[o * 3] = rf.x; F.outRadFog[o * 3 + 1] = rf.y; F.outRadFog[o * 3 + 2] = rf.z;
             F.outNormal[o * 3] = nrm.x; F.outNormal[o * 3 + 1] = nrm.y; F.outNormal[o * 3 + 2] = nrm.z;
-            F.outUv[o * 2] = a.hasUv ? a.uv.x : 0.0f;
-            F.outUv[o * 2 + 1] = a.hasUv ? a.uv.y : 0.0f;
+            F.outUv[o * 2] = h.hasUv ? h.uv.x : 0.0f;
+            F.outUv[o * 2 + 1] = h.hasUv ? h.uv.y : 0.0f;
             F.outDepth[o] = depth;
-            const vec3 alb = a.depthCount > 0 ? a.albedoSum / (float)a.depthCount : splat3(0.0f);
+            const vec3 alb = h.depthCount > 0 ? h.albedoSum / (float)h.depthCount : splat3(0.0f);
             F.outAlbedo[o * 3] = alb.x; F.outAlbedo[o * 3 + 1] = alb.y; F.outAlbedo[o * 3 + 2] = alb.z;
             F.outSamples[o] = count;
             F.outVar[o] = var;
@@ -2147,20 +2171,21 @@ void enqueue_sample(sr_ctx* c, const SliceRun& r) {
     const hipStream_t s = r.s;
     auto trace = c->dev.anyTransparent ? k_trace<COUNT, true> : k_trace<COUNT, false>;
     auto tail = c->dev.anyTransparent ? k_tail<R, COUNT, true> : k_tail<R, COUNT, false>;
-    auto shade = c->dev.fog.enabled ? k_shade<R, true> : k_shade<R, false>;
+    auto shade = c->dev.fog.enabled ? k_shade<R, true, false> : k_shade<R, false, false>;
+    auto shadeFirst = c->dev.fog.enabled ? k_shade<R, true, true> : k_shade<R, false, true>;
     int cur = 0;
     bool fused = false;
     for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
         hipLaunchKernelGGL(trace, dim3(r.gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
         mark(c, r.slice, 0);
         if (seg == c->tailSegment) {  // the rest of every path in one launch
-            hipLaunchKernelGGL(tail, dim3(r.gTail), dim3(kBlock), trav_lds_bytes(kBlock), s, F, c->dev, cur);
+            hipLaunchKernelGGL(tail, dim3(r.gTail), dim3(kBlock), trav_lds_bytes(kBlock), s, F, c->dev, cur, (int)(seg == 0));
             mark(c, r.slice, 2);
             hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, F);
             fused = true;
             break;
         }
-        hipLaunchKernelGGL(shade, dim3(r.gShade), dim3(kShadeBlock), 0, s, F, c->dev, cur);
+        hipLaunchKernelGGL(seg == 0 ? shadeFirst : shade, dim3(r.gShade), dim3(kShadeBlock), 0, s, F, c->dev, cur);
         mark(c, r.slice, 1);
         cur ^= 1;
     }
