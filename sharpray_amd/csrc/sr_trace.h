@@ -30,24 +30,30 @@ constexpr int kLdsDepth = SR_LDS_DEPTH;  // stack entries per lane kept in LDS
 
 // Mesh traversal stack: the first LD entries of each lane live in LDS (each wave owns LD rows of
 // 64 entries, one column per lane, so a wave's accesses are bank-conflict free); deeper entries
-// spill to private memory.
+// spill to private memory. The column pointer is a value of the traversal, not a member: stored
+// beside the private arrays it lived in scratch and every LDS access became a generic (flat) load
+// through a pointer reloaded from scratch.
 template <int LD>
 struct TravStack {
-    uint2* col;  // this lane's LDS column (stride 64 entries)
     uint32_t node[kStack - LD];
     float t[kStack - LD];
-    SR_DEV void put(int i, uint32_t n, float tt) {
+    SR_DEV void put(uint2* col, int i, uint32_t n, float tt) {
         if (LD > 0 && i < LD) col[i * 64] = make_uint2(n, __float_as_uint(tt));
         else { node[i - LD] = n; t[i - LD] = tt; }
     }
-    SR_DEV void get(int i, uint32_t& n, float& tt) const {
+    SR_DEV void get(const uint2* col, int i, uint32_t& n, float& tt) const {
         if (LD > 0 && i < LD) {
             const uint2 e = col[i * 64];
             n = e.x;
             tt = __uint_as_float(e.y);
         } else {
+            // The empty asm statements keep the compiler from merging this branch's loads with the
+            // LDS load above into one generic (flat) load from a selected address, which would turn
+            // every pop of an LDS entry into a flat load waiting on all outstanding memory operations.
+            asm volatile("" ::: "memory");
             n = node[i - LD];
             tt = t[i - LD];
+            asm volatile("" ::: "memory");
         }
     }
 };
@@ -69,7 +75,6 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
                          float& bu, float& bv, int32_t& bslot, Tally& tl, uint2* lds) {
     const float4* N = S.nodes;
     TravStack<LD> st;
-    st.col = lds;
     int sp = 0;
     bestT = kInf;
     bool found = false;
@@ -90,7 +95,7 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
                 const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
                 if (hA && hB) {
                     const bool aNear = tA < tB;
-                    st.put(sp, aNear ? pB : pA, aNear ? tB : tA);
+                    st.put(lds, sp, aNear ? pB : pA, aNear ? tB : tA);
                     ++sp;
                     cur = aNear ? pA : pB;
                     curT = aNear ? tA : tB;
@@ -104,7 +109,7 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
             if (!next) {
                 if (sp == 0) { live = false; break; }
                 --sp;
-                st.get(sp, cur, curT);
+                st.get(lds, sp, cur, curT);
             }
         }
         if (!live) break;
@@ -123,7 +128,7 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
         }
         if (sp == 0) break;
         --sp;
-        st.get(sp, cur, curT);
+        st.get(lds, sp, cur, curT);
     }
     return found;
 }
@@ -134,7 +139,6 @@ template <bool COUNT, int LD>
 SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, Tally& tl, uint2* lds) {
     const float4* N = S.nodes;
     TravStack<LD> st;
-    st.col = lds;
     int sp = 0;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
     uint32_t cur = pack_node(root.z, root.w);
@@ -150,7 +154,7 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
             const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
             if (hA && hB) {
                 const bool aNear = tA < tB;
-                st.put(sp++, aNear ? pB : pA, 0.0f);
+                st.put(lds, sp++, aNear ? pB : pA, 0.0f);
                 cur = aNear ? pA : pB;
             } else if (hA) {
                 cur = pA;
@@ -158,7 +162,7 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
                 cur = pB;
             } else {
                 if (sp == 0) return false;
-                st.get(--sp, cur, unused);
+                st.get(lds, --sp, cur, unused);
             }
         }
         const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
@@ -169,7 +173,7 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
             if (tri_hit(T, r, t, u, v)) return true;
         }
         if (sp == 0) return false;
-        st.get(--sp, cur, unused);
+        st.get(lds, --sp, cur, unused);
     }
 }
 
@@ -185,7 +189,6 @@ SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBa
                               float& bu, float& bv, int32_t& bslot, uint2* lds) {
     const float4* N = S.nodes;
     TravStack<LD> st;
-    st.col = lds;
     int sp = 0;
     bestT = kInf;
     bool found = false;
@@ -203,7 +206,7 @@ SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBa
                 leafT = curT;
                 if (sp == 0) { live = false; break; }
                 --sp;
-                st.get(sp, cur, curT);
+                st.get(lds, sp, cur, curT);
             } else {
                 bool next = false;
                 if (curT < bestT) {
@@ -215,7 +218,7 @@ SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBa
                     const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
                     if (hA && hB) {
                         const bool aNear = tA < tB;
-                        st.put(sp, aNear ? pB : pA, aNear ? tB : tA);
+                        st.put(lds, sp, aNear ? pB : pA, aNear ? tB : tA);
                         ++sp;
                         cur = aNear ? pA : pB;
                         curT = aNear ? tA : tB;
@@ -229,7 +232,7 @@ SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBa
                 if (!next) {
                     if (sp == 0) { live = false; break; }
                     --sp;
-                    st.get(sp, cur, curT);
+                    st.get(lds, sp, cur, curT);
                 }
             }
             // Every lane still descending holds a leaf: intersect them together.
@@ -263,7 +266,6 @@ template <int LD>
 SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, uint2* lds) {
     const float4* N = S.nodes;
     TravStack<LD> st;
-    st.col = lds;
     int sp = 0;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
     uint32_t cur = pack_node(root.z, root.w);
@@ -276,7 +278,7 @@ SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, 
                 if (leaf != 0) break;
                 leaf = cur;
                 if (sp == 0) { live = false; break; }
-                st.get(--sp, cur, unused);
+                st.get(lds, --sp, cur, unused);
             } else {
                 const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
                 const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
@@ -286,7 +288,7 @@ SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, 
                 const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
                 if (hA && hB) {
                     const bool aNear = tA < tB;
-                    st.put(sp++, aNear ? pB : pA, 0.0f);
+                    st.put(lds, sp++, aNear ? pB : pA, 0.0f);
                     cur = aNear ? pA : pB;
                 } else if (hA) {
                     cur = pA;
@@ -294,7 +296,7 @@ SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, 
                     cur = pB;
                 } else {
                     if (sp == 0) { live = false; break; }
-                    st.get(--sp, cur, unused);
+                    st.get(lds, --sp, cur, unused);
                 }
             }
             if (__ballot(leaf != 0) == __ballot(1)) break;
