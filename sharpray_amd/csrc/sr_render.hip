@@ -51,17 +51,22 @@ extern __shared__ uint2 sr_lds[];
 SR_DEV uint2* lds_column() { return sr_lds + (threadIdx.x >> 6) * (kLdsDepth * 64) + (threadIdx.x & 63); }
 
 // ---- Frame state --------------------------------------------------------------------------------
-struct alignas(16) PathRec {  // 96 B per path: everything shade keeps between segments
+struct alignas(16) PathRec {  // 64 B per path: what every segment of a path reads and writes
     vec3 energy;
     float scatterPdf;
     vec3 rad;
     uint32_t flags;  // bit0: scatterPdf valid, bit1: inside a medium, bit2: kPrimaryHit
-    vec3 radFog;
-    uint32_t seg;    // segment index of the queued ray (0 = camera ray)
-    vec3 medium;
-    uint32_t pad0;
     uint32_t rng[5];
-    uint32_t pad1[3];
+    uint32_t seg;    // segment index of the queued ray (0 = camera ray)
+    uint32_t pad[2];
+};
+// The rest of a path's state, touched only where it is used: the fog radiance (scenes with fog or
+// a medium) and the interior medium's albedo (paths inside a transparent object).
+struct alignas(16) PathCold {  // 32 B
+    vec3 radFog;
+    uint32_t pad0;
+    vec3 medium;
+    uint32_t pad1;
 };
 struct alignas(16) ShadowRec {  // 80 B
     vec3 o, d;
@@ -122,9 +127,9 @@ constexpr uint32_t kMaxInFlight = 16;  // samples of a pixel in flight at most
 // Device bytes per path slot (rays, hit, path record, two shadow records, two queue entries,
 // primary-hit record) and the most the path slots of one render may take: samples in flight are
 // capped to fit (a 1080p frame holds 16 samples per pixel in 12 GB).
-constexpr size_t kPathSlotBytes = 3 * sizeof(float4) + sizeof(int32_t) + sizeof(PathRec) + 2 * sizeof(ShadowRec) +
-                                  2 * sizeof(uint32_t) + 3 * sizeof(float4);
-static_assert(sizeof(PathRec) == 96 && sizeof(ShadowRec) == 80, "record sizes");
+constexpr size_t kPathSlotBytes = 3 * sizeof(float4) + sizeof(int32_t) + sizeof(PathRec) + sizeof(PathCold) +
+                                  2 * sizeof(ShadowRec) + 2 * sizeof(uint32_t) + 3 * sizeof(float4);
+static_assert(sizeof(PathRec) == 64 && sizeof(PathCold) == 32 && sizeof(ShadowRec) == 80, "record sizes");
 constexpr size_t kMaxPathBytes = (size_t)16 << 30;
 
 // Queue counters (uint32 each), per slice. kOverflow collects the queues an append found full (bit
@@ -139,6 +144,7 @@ struct Frame {
     float4* hitA;       // t, u, v, object (int bits)
     int32_t* hitSlot;   // mesh item slot
     PathRec* paths;
+    PathCold* cold;
     ShadowRec* shadows[2];  // shadow rays of even / odd segments
     PixAcc* acc;
     // Primary-hit record per path slot (normal + depth, albedo + hit flag, uv), written by the
@@ -164,6 +170,7 @@ struct Frame {
     // Samples in flight per pixel: path slot j = b * nSlots + q runs sample (first of the batch + b)
     // of pixel slot q; nPaths = batch * nSlots is the capacity of every ray and shadow queue.
     uint32_t batch, nPaths;
+    uint32_t fog;  // the scene has fog or a medium: paths carry fog radiance
     uint32_t minS, maxS, bounces, seed;
     float threshold, clamp;
     ViewD view;
@@ -297,13 +304,11 @@ SR_DEV PathRec start_path(const Frame& F, uint32_t j) {
     p.scatterPdf = 0.0f;
     p.rad = splat3(0.0f);
     p.flags = 0;
-    p.radFog = splat3(0.0f);
-    p.seg = 0;
-    p.medium = splat3(0.0f);
-    p.pad0 = 0;
     const uint32_t* w = F.paths[j].rng;
     for (int k = 0; k < 5; ++k) p.rng[k] = w[k];
-    p.pad1[0] = p.pad1[1] = p.pad1[2] = 0;
+    p.seg = 0;
+    p.pad[0] = p.pad[1] = 0;
+    if (F.fog) F.cold[j].radFog = splat3(0.0f);  // the medium is written when a path enters one
     return p;
 }
 
@@ -436,7 +441,7 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
         const vec3 view = -ray.d;
         const float bias = net_max(1e-4f, dist * 1e-4f);
         const vec3 hitPos = at(ray, dist - bias);
-        if (p.flags & 2u) p.energy = p.energy * beer(p.medium, dist);
+        if (p.flags & 2u) p.energy = p.energy * beer(F.cold[q].medium, dist);
         p.rad = p.rad + s.radiance * p.energy;
         if (primary) {  // the pixel's normal, depth, albedo and uv sums take it in sample order
             float4* pr = F.prim + (size_t)q * 3;
@@ -479,7 +484,7 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
             p.energy = clamp_lum(p.energy, F.clamp);
             const bool enter = dot3(sc.dir, s.normalGeo) < 0.0f;
             if (sc.refracted) {
-                if (enter) { p.flags |= 2u; p.medium = s.albedo; }
+                if (enter) { p.flags |= 2u; F.cold[q].medium = s.albedo; }
                 else p.flags &= ~2u;
             } else if (!enter) {
                 p.flags &= ~2u;
@@ -637,11 +642,16 @@ template <bool COUNT, bool TRANSP>
 SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, Tally& tl, uint32_t& extra) {
     const vec3 tr = shadow_transmittance<COUNT, TRANSP>(S, sr, tl, extra);
     if ((sr.path & kCountOnly) || !(max_comp(tr) > 0.0f)) return;
-    PathRec* p = &F.paths[sr.path & kPathMask];
-    vec3 rad = p->rad, radFog = p->radFog;
-    add_direct(S, sr, tr, rad, radFog);
-    if (sr.path & kToFog) p->radFog = radFog;
-    else p->rad = rad;
+    const uint32_t j = sr.path & kPathMask;
+    if (sr.path & kToFog) {
+        vec3 rad, radFog = F.cold[j].radFog;
+        add_direct(S, sr, tr, rad, radFog);
+        F.cold[j].radFog = radFog;
+    } else {
+        vec3 rad = F.paths[j].rad, radFog;
+        add_direct(S, sr, tr, rad, radFog);
+        F.paths[j].rad = rad;
+    }
 }
 
 // The remaining segments of every queued path in one launch: each lane runs its path's
@@ -662,6 +672,7 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t q = Q[i];
         PathRec p = first ? start_path<R>(F, q) : F.paths[q];
+        vec3 radFog = F.fog ? F.cold[q].radFog : splat3(0.0f);
         R rng = rng_load<R>(p);
         const float4 ho = F.hitA[q];
         const float4 ro = F.rayO[q], rd = F.rayD[q];
@@ -676,7 +687,7 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
             if (shadow) {  // side-culled rays (counters only) count with this launch's traversal
                 ++c.occ;
                 const vec3 tr = shadow_transmittance<COUNT, TRANSP>(S, sr, tl, extra);
-                if (!(sr.path & kCountOnly)) add_direct(S, sr, tr, p.rad, p.radFog);
+                if (!(sr.path & kCountOnly)) add_direct(S, sr, tr, p.rad, radFog);
             }
             if (!cont) break;
             p.seg = p.seg + 1;
@@ -687,6 +698,7 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
         }
         rng_store(p, rng);
         F.paths[q] = p;
+        if (F.fog) F.cold[q].radFog = radFog;
     }
     if (F.ctr) {
         bump_sum(F, 3, c.hit); bump_sum(F, 4, c.miss); bump_sum(F, 5, c.term); bump_sum(F, 6, c.fog);
@@ -821,8 +833,8 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
             const uint32_t si = base + b;
             const size_t j = (size_t)q + (size_t)b * F.nSlots;
             const PathRec* P = F.paths + j;
-            const float4 r0 = *(const float4*)&P->rad;     // rad, flags
-            const float4 r1 = *(const float4*)&P->radFog;  // radFog, seg
+            const float4 r0 = *(const float4*)&P->rad;  // rad, flags
+            const vec3 radFog = F.fog ? F.cold[j].radFog : splat3(0.0f);
             if (__float_as_uint(r0.w) & kPrimaryHit) {  // the camera ray hit a surface (renderer.cs:187-203)
                 if (!hLoaded) { h = A->h; hLoaded = true; }
                 const float4 pr0 = F.prim[j * 3], pr1 = F.prim[j * 3 + 1], pr2 = F.prim[j * 3 + 2];
@@ -835,7 +847,7 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
                 h.albedoSum = h.albedoSum + mk3(pr1.x, pr1.y, pr1.z);
                 if (!h.hasUv) { h.hasUv = 1; h.uv = mk2(pr2.x, pr2.y); }
             }
-            const vec3 rad = mk3(r0.x, r0.y, r0.z), radFog = mk3(r1.x, r1.y, r1.z);
+            const vec3 rad = mk3(r0.x, r0.y, r0.z);
             a.radSum = a.radSum + rad;
             a.radFogSum = a.radFogSum + radFog;
             const float lum = luminance(rad) + luminance(radFog);
@@ -1305,6 +1317,7 @@ struct sr_ctx {
     DevBuf<float4> dRayO, dRayD, dHitA;
     DevBuf<int32_t> dHitSlot;
     DevBuf<PathRec> dPaths;
+    DevBuf<PathCold> dCold;
     DevBuf<float4> dPrim;  // 3 per path slot
     DevBuf<ShadowRec> dShadows;
     DevBuf<PixAcc> dAcc;
@@ -2038,6 +2051,7 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     F.hitA = c->dHitA.ptr + op;
     F.hitSlot = c->dHitSlot.ptr + op;
     F.paths = c->dPaths.ptr + op;
+    F.cold = c->dCold.ptr + op;
     F.prim = c->dPrim.ptr + 3 * op;
     F.shadows[0] = c->dShadows.ptr + 2 * op;
     F.shadows[1] = c->dShadows.ptr + 2 * op + pathSlots;
@@ -2062,6 +2076,7 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     F.nSlots = F.tilesX * ((rows + 7) / 8) * 64;
     F.batch = c->batch;
     F.nPaths = F.nSlots * F.batch;
+    F.fog = c->dev.fog.enabled ? 1u : 0u;
     F.minS = c->params.min_samples;
     F.maxS = c->params.max_samples;
     F.bounces = c->params.bounces;
@@ -2086,7 +2101,7 @@ int alloc_frame(sr_ctx* c, uint32_t sliceSlots, uint32_t slices) {
     int rc;
     if ((rc = ensure(c->dRayO, maxPaths)) || (rc = ensure(c->dRayD, maxPaths)) || (rc = ensure(c->dHitA, maxPaths)) ||
         (rc = ensure(c->dHitSlot, maxPaths)) || (rc = ensure(c->dPaths, maxPaths)) || (rc = ensure(c->dShadows, maxPaths * 2)) ||
-        (rc = ensure(c->dPrim, maxPaths * 3)) ||
+        (rc = ensure(c->dPrim, maxPaths * 3)) || (rc = ensure(c->dCold, maxPaths)) ||
         (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxPaths)) || (rc = ensure(c->dQueue1, maxPaths)) ||
         (rc = ensure(c->dCounts, (size_t)kCounterWords * slices)) ||
         (rc = ensure(c->dCtr, (size_t)kCtrWords * kCtrSpread)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
