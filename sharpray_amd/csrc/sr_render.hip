@@ -43,7 +43,10 @@ constexpr int kBlock = kTravBlock;
 #define SR_TRACE_BLOCK 64
 #endif
 constexpr int kTraceBlock = SR_TRACE_BLOCK;  // threads per block of k_trace (one wave)
-constexpr int kShadeBlock = 256;  // threads per block of k_shade (the hit / miss partition's span)
+#ifndef SR_SHADE_BLOCK
+#define SR_SHADE_BLOCK 256
+#endif
+constexpr int kShadeBlock = SR_SHADE_BLOCK;  // threads per block of k_shade (the hit / miss partition's span)
 constexpr int kNextBlock = 256;
 // Per-lane LDS columns of the mesh traversal stack (trav_lds_bytes(threads) per block).
 extern __shared__ uint2 sr_lds[];
@@ -222,6 +225,38 @@ SR_DEV uint32_t block_append(uint32_t* count, bool pred, uint32_t* sh) {
     __syncthreads();  // sh serves the next call
     return slot;
 }
+
+// Appends the lanes whose key is in [0, NK) to a queue with one atomic per block, grouped by key:
+// all of the block's key-0 items first, then key 1, ...; within a key in thread order. Every
+// thread of the block calls it; `sh` is NW * NK + 1 words of LDS.
+template <int NW, int NK>
+SR_DEV uint32_t block_append_keyed(uint32_t* count, int key, uint32_t* sh) {
+    const uint32_t wave = threadIdx.x >> 6;
+    const unsigned long long below = (1ull << lane_id()) - 1ull;
+    uint32_t rank = 0;
+    for (int k = 0; k < NK; ++k) {
+        const unsigned long long m = __ballot(key == k);
+        if (lane_id() == 0) sh[k * NW + wave] = (uint32_t)__popcll(m);
+        if (key == k) rank = (uint32_t)__popcll(m & below);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive prefix over (key, wave)
+        uint32_t total = 0;
+        for (int i = 0; i < NW * NK; ++i) {
+            const uint32_t c = sh[i];
+            sh[i] = total;
+            total += c;
+        }
+        sh[NW * NK] = total ? atomicAdd(count, total) : 0u;
+    }
+    __syncthreads();
+    const uint32_t slot = (key >= 0 && key < NK) ? sh[NW * NK] + sh[key * NW + wave] + rank : 0u;
+    __syncthreads();  // sh serves the next call
+    return slot;
+}
+
+// Octant of a direction (sign bits of x, y, z): the key that groups a block's appended rays.
+SR_DEV int dir_octant(vec3 d) { return (d.x < 0.0f ? 1 : 0) | (d.y < 0.0f ? 2 : 0) | (d.z < 0.0f ? 4 : 0); }
 
 template <class R> SR_DEV R rng_load(const PathRec& p);
 template <> SR_DEV XorwowRng rng_load<XorwowRng>(const PathRec& p) {
@@ -527,7 +562,7 @@ __global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame
     const uint32_t stride = gridDim.x * blockDim.x;
     __shared__ uint32_t sQ[kShadeBlock];
     __shared__ uint32_t sHits[kShadeBlock / 64];
-    __shared__ uint32_t shApp[kShadeBlock / 64 + 1];
+    __shared__ uint32_t shApp[(kShadeBlock / 64) * 8 + 1];
     const uint32_t wave = threadIdx.x >> 6;
     // Grid-stride over the queue with a block-uniform base, so every lane of a wave runs the same
     // number of iterations (the wave-level appends need the whole wave).
@@ -561,6 +596,7 @@ __global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame
         bool cont = false, shadow = false;
         ShadowRec sr;
         uint32_t q = 0;
+        vec3 nd = splat3(0.0f);
         if (i < n) {
             q = sQ[threadIdx.x];
             PathRec p = FIRST ? start_path<R>(F, q) : F.paths[q];
@@ -570,7 +606,7 @@ __global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame
             h.t = ho.x; h.u = ho.y; h.v = ho.z; h.obj = __float_as_int(ho.w); h.slot = F.hitSlot[q];
             R rng = rng_load<R>(p);
             const ray3 ray = mk_ray(mk3(ro.x, ro.y, ro.z), mk3(rd.x, rd.y, rd.z));
-            vec3 no, nd;
+            vec3 no;
             shade_path<R, FOG>(F, S, q, p, rng, h, ray, cont, shadow, sr, no, nd, c);
             if (cont) {
                 p.seg = p.seg + 1;
@@ -585,10 +621,13 @@ __global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame
             else ++c.occ;
         }
         // Block-aggregated appends: one atomic per block and queue (per-wave atomics on the two
-        // counters cost 66 vs 54 ms of shade per 1080p frame).
-        const uint32_t ns = block_append<kShadeBlock / 64>(&F.counts[kShadowCount0 + cur], shadow, shApp);
+        // counters cost 66 vs 54 ms of shade per 1080p frame). A block's rays come from a few
+        // neighbouring tiles; grouping them by direction octant gives each traversal wave rays
+        // that start close together and mostly share their child orderings.
+        const uint32_t ns = block_append_keyed<kShadeBlock / 64, 8>(&F.counts[kShadowCount0 + cur],
+                                                                    shadow ? dir_octant(sr.d) : -1, shApp);
         if (shadow && queue_room(F, ns, 2u)) F.shadows[cur][ns] = sr;
-        const uint32_t nc = block_append<kShadeBlock / 64>(&F.counts[cur ^ 1], cont, shApp);
+        const uint32_t nc = block_append_keyed<kShadeBlock / 64, 8>(&F.counts[cur ^ 1], cont ? dir_octant(nd) : -1, shApp);
         if (cont && queue_room(F, nc, 1u)) F.queue[cur ^ 1][nc] = q;
         __syncthreads();  // sQ and sHits serve the block's next chunk
     }
