@@ -127,6 +127,7 @@ constexpr int kCtrSpread = 16;  // copies of the counter words (a power of two)
 constexpr uint32_t kPrimaryHit = 4u;
 constexpr int kProgRing = 4;    // samples an adaptive render's driver runs ahead of its progress reads
 constexpr uint32_t kMaxInFlight = 16;  // samples of a pixel in flight at most
+constexpr uint32_t kTailAuto = 0xffffffffu;
 // Device bytes per path slot (rays, hit, path record, two shadow records, two queue entries,
 // primary-hit record) and the most the path slots of one render may take: samples in flight are
 // capped to fit (a 1080p frame holds 16 samples per pixel in 12 GB).
@@ -1314,7 +1315,10 @@ struct sr_ctx {
     int bvhBuilder = SR_BVH_BUILDER_HOST;
     // Segment from which a render finishes every path in one fused launch (k_tail); beyond the
     // bounce count it never does. SHARPRAY_TAIL_SEGMENT overrides the default.
-    uint32_t tailSegment = 3;
+    // Segment from which the fused tail launch finishes every path; kTailAuto picks 3 with one or a
+    // few samples in flight and 4 with 8 or more (1080p frame: 3 / 4 / 5 / 6 with one sample
+    // 240 / 261 / 278 / - ms; with 16 in flight 124.6 / 119.0 / 120.7 / 121.3 ms).
+    uint32_t tailSegment = kTailAuto;
     hipStream_t stream = nullptr;
     std::vector<HostTexture> textures;
     std::vector<HostMesh> meshes;
@@ -2229,10 +2233,11 @@ void enqueue_sample(sr_ctx* c, const SliceRun& r) {
     auto shadeFirst = c->dev.fog.enabled ? k_shade<R, true, true> : k_shade<R, false, true>;
     int cur = 0;
     bool fused = false;
+    const uint32_t tailSeg = c->tailSegment != kTailAuto ? c->tailSegment : (F.batch >= 8 ? 4u : 3u);
     for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
         hipLaunchKernelGGL(trace, dim3(r.gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
         mark(c, r.slice, 0);
-        if (seg == c->tailSegment) {  // the rest of every path in one launch
+        if (seg == tailSeg) {  // the rest of every path in one launch
             hipLaunchKernelGGL(tail, dim3(r.gTail), dim3(kBlock), trav_lds_bytes(kBlock), s, F, c->dev, cur, (int)(seg == 0));
             mark(c, r.slice, 2);
             hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, F);
