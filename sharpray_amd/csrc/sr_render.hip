@@ -985,7 +985,13 @@ struct AdjointArgs {
     float* gEnv;          // sky texels * 3
     float* gTotal;        // one float: the through-total part of the sky-pdf derivatives
 };
-__global__ void __launch_bounds__(kBlock) k_adjoint(AdjointArgs A, DevScene S) {
+// Register budget of the path-replay kernel: 4 waves per SIMD (128 VGPRs, some spilled) against
+// 2 unconstrained (175 VGPRs): backward 25.8 -> 21.8-23.3 ms at 512 x 512 x 16 spp (3, 5, 6, 8 waves:
+// 23.3, 22.9, 23.6, 22.0 ms).
+#ifndef SR_ADJOINT_MIN_WAVES
+#define SR_ADJOINT_MIN_WAVES 4
+#endif
+__global__ void __launch_bounds__(kBlock, SR_ADJOINT_MIN_WAVES) k_adjoint(AdjointArgs A, DevScene S) {
     __shared__ float gl[kAdjLdsObjects * 4];
     __shared__ float glTotal;
     const uint32_t nl = (A.nObj < (uint32_t)kAdjLdsObjects ? A.nObj : (uint32_t)kAdjLdsObjects) * 4;
