@@ -982,9 +982,25 @@ struct AdjointArgs {
     float clampLum;
     const float* dImage;  // W * H * 3
     float* gMat;          // nObj * 4
-    float* gEnv;          // sky texels * 3
+    float* gEnv;          // envCopies copies of sky texels * 3, envStride floats apart
+    size_t envStride;
+    uint32_t envCopies;
     float* gTotal;        // one float: the through-total part of the sky-pdf derivatives
 };
+// Copies of the sky-texel gradient: block b adds into copy b % envCopies, so lanes that hit the
+// same bright texels contend on envCopies addresses instead of one (512 x 512 x 16 spp backward
+// against a 256 x 128 sky: 22.2 ms with one copy, 14.6 / 12.2 / 10.9 / 10.9 ms with 8 / 64 / 128 /
+// 256, 9.4 ms with the sky gradient left out); k_env_reduce sums the copies in copy order. Up to
+// kMaxEnvCopies copies, at least kMinEnvCopies, within kEnvCopyBytes of device memory.
+constexpr uint32_t kMinEnvCopies = 8, kMaxEnvCopies = 128;
+constexpr size_t kEnvCopyBytes = (size_t)1 << 30;
+__global__ void k_env_reduce(const float* copies, size_t stride, size_t n, uint32_t ncopies, float* out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = 0.0f;
+    for (uint32_t c = 0; c < ncopies; ++c) v += copies[(size_t)c * stride + i];
+    out[i] = v;
+}
 // Register budget of the path-replay kernel: 4 waves per SIMD (128 VGPRs, some spilled) against
 // 2 unconstrained (175 VGPRs): backward 25.8 -> 21.8-23.3 ms at 512 x 512 x 16 spp (3, 5, 6, 8 waves:
 // 23.3, 22.9, 23.6, 22.0 ms).
@@ -1014,7 +1030,8 @@ __global__ void __launch_bounds__(kBlock, SR_ADJOINT_MIN_WAVES) k_adjoint(Adjoin
             rng.draw = 0;
             vec3 o, d;
             camera_ray(A.view, A.W, A.H, x, y, rng, o, d);
-            adjoint_path(S, mk_ray(o, d), rng, A.bounces, A.clampLum, delta, gm, A.gEnv, lds_column());
+            adjoint_path(S, mk_ray(o, d), rng, A.bounces, A.clampLum, delta, gm,
+                         A.gEnv + (size_t)(blockIdx.x % A.envCopies) * A.envStride, lds_column());
         }
     }
     if (gm.gTotal != 0.0f) atomicAdd(&glTotal, gm.gTotal);
@@ -1351,7 +1368,7 @@ struct sr_ctx {
     bool adjReady = false;
     sr_view adjView{};
     sr_render_params adjParams{};
-    DevBuf<float> dAdjImage, dAdjGrad;
+    DevBuf<float> dAdjImage, dAdjGrad, dAdjEnv;
     sr::CnnNet* cnn = nullptr;  // kernel-predicting denoiser (weights loaded by sr_cnn_load)
     struct CnnExample {  // training pair kept in device memory
         DevBuf<float> rad, alb, nrm, dep, target;
@@ -1451,7 +1468,7 @@ struct sr_ctx {
         if (evEnd) (void)hipEventDestroy(evEnd);
         dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release(); dAlias.release(); dFogGrid.release();
         dSlotTri.release(); dSceneItems.release(); dObjects.release(); dTextures.release();
-        dAdjImage.release(); dAdjGrad.release();
+        dAdjImage.release(); dAdjGrad.release(); dAdjEnv.release();
         delete cnn;
         cnn = nullptr;
         for (CnnExample* e : cnnExamples) {
@@ -3233,10 +3250,17 @@ int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad) {
     int rc = sr_adjoint_param_count(ctx, &n);
     if (rc) return rc;
     const size_t px = (size_t)p.width * p.height;
-    if ((rc = ensure(ctx->dAdjImage, px * 3)) || (rc = ensure(ctx->dAdjGrad, (size_t)n + 1))) return rc;
+    const size_t envN = (size_t)n - ctx->objects.size() * 4;  // sky texels * 3 (0 without a texture sky)
+    const size_t envStride = (envN + 63) / 64 * 64;
+    const uint32_t envCopies = (uint32_t)std::min<size_t>(
+        kMaxEnvCopies, std::max<size_t>(kMinEnvCopies, kEnvCopyBytes / std::max<size_t>(1, envStride * sizeof(float))));
+    if ((rc = ensure(ctx->dAdjImage, px * 3)) || (rc = ensure(ctx->dAdjGrad, (size_t)n + 1)) ||
+        (rc = ensure(ctx->dAdjEnv, std::max<size_t>(1, envStride * envCopies))))
+        return rc;
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemcpyAsync(ctx->dAdjImage.ptr, d_image, px * 12, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(ctx->dAdjGrad.ptr, 0, ((size_t)n + 1) * sizeof(float), s));
+    if (envN) HIP_TRY(hipMemsetAsync(ctx->dAdjEnv.ptr, 0, envStride * envCopies * sizeof(float), s));
     AdjointArgs A;
     A.view = make_view(&ctx->adjView, p.width, p.height);
     A.W = p.width; A.H = p.height; A.spp = p.max_samples; A.bounces = p.bounces; A.seed = p.rng_seed;
@@ -3244,7 +3268,9 @@ int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad) {
     A.clampLum = p.indirect_clamp;
     A.dImage = ctx->dAdjImage.ptr;
     A.gMat = ctx->dAdjGrad.ptr;
-    A.gEnv = ctx->dAdjGrad.ptr + ctx->objects.size() * 4;
+    A.gEnv = ctx->dAdjEnv.ptr;
+    A.envStride = envStride;
+    A.envCopies = envCopies;
     A.gTotal = ctx->dAdjGrad.ptr + n;
     const uint64_t total = (uint64_t)px * p.max_samples;
     const uint64_t blocks = (total + kBlock - 1) / kBlock;
@@ -3252,6 +3278,9 @@ int sr_adjoint_backward(sr_ctx* ctx, const float* d_image, float* grad) {
     // The replay keeps its whole traversal stack in private memory (LDS rows measured 2.7 ms slower
     // on the 512^2 / 16 spp backward pass).
     hipLaunchKernelGGL(k_adjoint, dim3((uint32_t)blocks), dim3(kBlock), 0, s, A, ctx->dev);
+    if (envN)
+        hipLaunchKernelGGL(k_env_reduce, dim3((uint32_t)((envN + 255) / 256)), dim3(256), 0, s, ctx->dAdjEnv.ptr, envStride,
+                           envN, envCopies, ctx->dAdjGrad.ptr + ctx->objects.size() * 4);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(grad, ctx->dAdjGrad.ptr, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
     float gTotal = 0.0f;
