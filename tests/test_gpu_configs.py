@@ -3,7 +3,8 @@ configurations leave out (object transforms, textured materials and normal maps)
 
   - the 1080p / 1M-triangle forward scene at 1/10 scale (192 x 108, all 1,048,578 triangles, eight
     materials down to roughness 0.02, the log-normal 2048 x 1024 environment): radiance against the
-    oracle on the reference's per-pixel xorwow stream;
+    oracle on the reference's per-pixel xorwow stream, and on the counter-based stream with 16
+    samples of every pixel in flight (the benchmark's render path);
   - the fog configuration at 1/10 scale (the 1M-triangle scene inside the 128^3 fp16 density grid,
     delta tracking, HG g = 0.3): radiance and fog radiance against the oracle, then the 7 x 7
     (radius 3, 29 taps) albedo-guided bilateral filter on the GPU's AOVs against the oracle's;
@@ -62,6 +63,26 @@ def test_plain_gpu_config_radiance_matches_oracle(oracle_lib):
     assert np.array_equal(r.samples, ref["samples"])
     fin = ok & np.isfinite(ref["depth"])
     assert np.allclose(r.depth[fin], ref["depth"][fin], rtol=1e-5)
+
+
+def test_plain_gpu_config_counter_policy_in_flight_matches_oracle(oracle_lib):
+    """The benchmark's own render path: the counter RNG policy with 16 samples of every pixel in
+    flight (the bench default), on the forward scene at 1/10 scale, against the oracle driven by
+    the same counter-based random numbers."""
+    from sharpray_amd import RNG_COUNTER, Renderer, scenes
+    spec = scenes.config_plain_gpu(scale=0.1, spp=16)
+    scene, view = scenes.instantiate(spec)
+    scene.set_samples_in_flight(16)
+    scene.build()
+    r = Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
+                 spec.variance_threshold, spec.bounces, spec.indirect_clamp, rng_policy=RNG_COUNTER, rng_seed=5)
+    r.wait()
+    ref = oracle_lib.OracleScene(spec).render(rng_policy=1, rng_seed=5)
+    ok = _divergent("plain_gpu_0.1_counter_16_in_flight", r.radiance, ref["radiance"], 1e-2)  # measured 1.7e-3
+    assert np.array_equal(r.samples, ref["samples"])
+    fin = ok & np.isfinite(ref["depth"])
+    assert np.allclose(r.depth[fin], ref["depth"][fin], rtol=1e-5)
+    assert np.allclose(r.normals[ok], ref["normals"][ok], atol=2e-5)
 
 
 def test_fog_config_radiance_and_7x7_guided_filter_match_oracle(oracle_lib):
