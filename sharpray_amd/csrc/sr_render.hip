@@ -171,8 +171,9 @@ struct Frame {
     uint32_t* outSamples;
     float* outVar;
     uint32_t W, H, y0, rows, tilesX, nSlots;
-    // Samples in flight per pixel: path slot j = b * nSlots + q runs sample (first of the batch + b)
-    // of pixel slot q; nPaths = batch * nSlots is the capacity of every ray and shadow queue.
+    // Samples in flight per pixel (a power of two): path slot j = q * batch + b runs sample (first
+    // of the batch + b) of pixel slot q; nPaths = batch * nSlots is the capacity of every ray and
+    // shadow queue.
     uint32_t batch, nPaths;
     uint32_t fog;  // the scene has fog or a medium: paths carry fog radiance
     uint32_t minS, maxS, bounces, seed;
@@ -392,29 +393,32 @@ __global__ void k_reset(Frame F) {
 }
 
 // new Rng(x, y) per pixel (renderer.cs:156), zeroed sums, and the camera rays of the first batch
-// (samples 0 .. batch - 1; one sample unless the samples are independent, see Frame::batch).
+// (samples 0 .. batch - 1; one sample unless the samples are independent, see Frame::batch). One
+// thread per path slot: slot j = q * batch + b is sample b of pixel slot q, so a pixel's samples sit
+// next to each other in every buffer and in queue 0.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_start(Frame F) {
     __shared__ uint32_t sh[kBlock / 64 + 1];
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    bool live = false;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t q = j / F.batch, b = j - q * F.batch;
     const uint32_t nb = min(F.batch, F.maxS);
+    bool live = false;
     if (q < F.nSlots) {
         uint32_t x, y;
-        PixAcc a;
-        memset(&a, 0, sizeof(a));
         live = slot_pixel(F, q, x, y);
-        a.s.active = live ? 1u : 0u;
-        F.acc[q] = a;
-        if (live)
-            for (uint32_t b = 0; b < nb; ++b) begin_sample<R>(F, (size_t)q + (size_t)b * F.nSlots, x, y, b);
+        if (b == 0) {
+            PixAcc a;
+            memset(&a, 0, sizeof(a));
+            a.s.active = live ? 1u : 0u;
+            F.acc[q] = a;
+        }
+        live = live && b < nb;
+        if (live) begin_sample<R>(F, j, x, y, b);
     }
-    bump_wave(F, 0, live);
-    for (uint32_t b = 0; b < F.batch; ++b) {  // queue 0 holds batch 0's tiles, then batch 1's, ...
-        bump_wave(F, 1, live && b < nb);
-        const uint32_t slot = block_append<kBlock / 64>(&F.counts[kRayCount0], live && b < nb, sh);
-        if (live && b < nb && queue_room(F, slot, 1u)) F.queue[0][slot] = q + b * F.nSlots;
-    }
+    bump_wave(F, 0, live && b == 0);
+    bump_wave(F, 1, live);
+    const uint32_t slot = block_append<kBlock / 64>(&F.counts[kRayCount0], live, sh);
+    if (live && queue_room(F, slot, 1u)) F.queue[0][slot] = j;
 }
 
 // Counts of one shade pass, flushed into the render counters once per thread.
@@ -841,81 +845,102 @@ SR_DEV float rel_std_err(float sum, float sumSqr, uint32_t n) {
 
 // After a batch's last segment, per active pixel and in sample order: accumulation, the primary
 // hit's normal / depth / albedo / uv and the convergence test (renderer.cs:180-212); then either
-// the pixel's next batch of camera rays, appended to queue 0 in slot order (batch by batch) so the
-// next camera rays stay tile-coherent, or its final AOVs (renderer.cs:214-229, 142-148).
+// the pixel's next batch of camera rays, appended to queue 0 in slot order so the next camera rays
+// stay tile-coherent, or its final AOVs (renderer.cs:214-229, 142-148).
+// One thread per path slot (j = q * batch + b; batch a power of two, so a pixel's slots are a
+// group of batch lanes of one wave): every thread loads its own path's radiance, flags and primary
+// hit record (coalesced), the group's first lane folds them into the pixel's sums in sample order
+// through lane shuffles, and every thread starts its own sample of the next batch.
 template <class R>
 __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     __shared__ uint32_t sh[kNextBlock / 64 + 1];
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q == 0) {  // both shadow queues are spent; the next sample's first trace reads one of them
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t B = F.batch;
+    const uint32_t q = j / B, b = j - q * B;
+    const int lead = (int)(lane_id() - b);  // the pixel's first lane
+    if (j == 0) {  // both shadow queues are spent; the next sample's first trace reads one of them
         F.counts[kShadowCount0] = 0;
         F.counts[kShadowCount1] = 0;
     }
     bool converged = false, maxed = false;
-    uint32_t again = 0;  // camera rays started for the next batch
-    unsigned long long countMinKey = 0, countMaxKey = 0, varKey = 0;
     uint32_t doneBlock = ~0u;  // the block of a pixel that finished in this pass
-    // Fields are read and written one by one: a sample touches the accumulator's first 48 B and the
-    // path's radiance and random state; the rest is read once, when the pixel finishes.
-    // The pixel's sums stay in registers over the batch: its 48-byte per-sample part is read and
-    // written once, the 64-byte primary-hit part only when one of its samples hit a surface or the
-    // pixel finishes; of each path the radiance, fog radiance and flags are read.
+    unsigned long long countMinKey = 0, countMaxKey = 0, varKey = 0;
     PixAcc* A = F.acc + q;
-    PixSums a = q < F.nSlots ? A->s : PixSums{};
-    if (q < F.nSlots && a.active) {
-        const uint32_t base = a.sampleIndex;
-        const uint32_t nb = min(F.batch, F.maxS - base);
-        PixPrimary h;
-        bool hLoaded = false;
-        bool done = false;
-        uint32_t count = 0;
-        for (uint32_t b = 0; b < nb && !done; ++b) {
-            const uint32_t si = base + b;
-            const size_t j = (size_t)q + (size_t)b * F.nSlots;
-            const PathRec* P = F.paths + j;
-            const float4 r0 = *(const float4*)&P->rad;  // rad, flags
-            const vec3 radFog = F.fog ? F.cold[j].radFog : splat3(0.0f);
-            if (__float_as_uint(r0.w) & kPrimaryHit) {  // the camera ray hit a surface (renderer.cs:187-203)
-                if (!hLoaded) { h = A->h; hLoaded = true; }
-                const float4 pr0 = F.prim[j * 3], pr1 = F.prim[j * 3 + 1], pr2 = F.prim[j * 3 + 2];
-                const vec3 nrm = mk3(pr0.x, pr0.y, pr0.z);
-                h.normalSum = h.normalSum + nrm;
-                h.normalFallback = nrm;
-                h.normalCount += 1;
-                h.depthSum += pr0.w;
-                h.depthCount += 1;
-                h.albedoSum = h.albedoSum + mk3(pr1.x, pr1.y, pr1.z);
-                if (!h.hasUv) { h.hasUv = 1; h.uv = mk2(pr2.x, pr2.y); }
-            }
-            const vec3 rad = mk3(r0.x, r0.y, r0.z);
-            a.radSum = a.radSum + rad;
-            a.radFogSum = a.radFogSum + radFog;
-            const float lum = luminance(rad) + luminance(radFog);
-            a.lumSum = a.lumSum + lum;
-            a.lumSumSqr = a.lumSumSqr + lum * lum;
-            const float lumSurf = luminance(rad);
-            a.lumSurf = a.lumSurf + lumSurf;
-            a.lumSurfSqr = a.lumSurfSqr + lumSurf * lumSurf;
-            if (si >= F.minS - 1u && rel_std_err(a.lumSum, a.lumSumSqr, si + 1) < F.threshold) {
-                done = true;
-                count = si + 1;
-            } else if (si + 1 == F.maxS) {
-                done = true;
-                count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
-            }
+    const bool pixel = q < F.nSlots;
+    PixSums a = pixel ? A->s : PixSums{};  // the group's lanes read the same words
+    const bool active = pixel && a.active;
+    const uint32_t base = a.sampleIndex;
+    const uint32_t nb = active ? min(B, F.maxS - base) : 0u;
+    // This lane's path of the finished batch.
+    float4 r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pr0 = r0, pr1 = r0, pr2 = r0;
+    vec3 radFog = splat3(0.0f);
+    if (b < nb) {
+        r0 = *(const float4*)&F.paths[j].rad;  // rad, flags
+        if (F.fog) radFog = F.cold[j].radFog;
+        if (__float_as_uint(r0.w) & kPrimaryHit) {
+            pr0 = F.prim[(size_t)j * 3];
+            pr1 = F.prim[(size_t)j * 3 + 1];
+            pr2 = F.prim[(size_t)j * 3 + 2];
         }
-        uint32_t x, y;
-        slot_pixel(F, q, x, y);
+    }
+    // The first lane folds the group's samples in order (the others follow the shuffles).
+    PixPrimary h;
+    bool hLoaded = false, done = false;
+    uint32_t count = 0;
+    for (uint32_t k = 0; k < B; ++k) {
+        const int src = lead + (int)k;
+        const float4 s0 = make_float4(__shfl(r0.x, src), __shfl(r0.y, src), __shfl(r0.z, src), __shfl(r0.w, src));
+        const vec3 sFog = mk3(__shfl(radFog.x, src), __shfl(radFog.y, src), __shfl(radFog.z, src));
+        const bool hitK = (__float_as_uint(s0.w) & kPrimaryHit) != 0;
+        float4 p0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p1 = p0;
+        float2 p2 = make_float2(0.0f, 0.0f);
+        if (__ballot(hitK && b == 0 && k < nb) != 0) {
+            p0 = make_float4(__shfl(pr0.x, src), __shfl(pr0.y, src), __shfl(pr0.z, src), __shfl(pr0.w, src));
+            p1 = make_float4(__shfl(pr1.x, src), __shfl(pr1.y, src), __shfl(pr1.z, src), 0.0f);
+            p2 = make_float2(__shfl(pr2.x, src), __shfl(pr2.y, src));
+        }
+        if (b != 0 || k >= nb || done) continue;
+        const uint32_t si = base + k;
+        if (hitK) {
+            if (!hLoaded) { h = A->h; hLoaded = true; }
+            const vec3 nrm = mk3(p0.x, p0.y, p0.z);
+            h.normalSum = h.normalSum + nrm;
+            h.normalFallback = nrm;
+            h.normalCount += 1;
+            h.depthSum += p0.w;
+            h.depthCount += 1;
+            h.albedoSum = h.albedoSum + mk3(p1.x, p1.y, p1.z);
+            if (!h.hasUv) { h.hasUv = 1; h.uv = mk2(p2.x, p2.y); }
+        }
+        const vec3 rad = mk3(s0.x, s0.y, s0.z);
+        a.radSum = a.radSum + rad;
+        a.radFogSum = a.radFogSum + sFog;
+        const float lum = luminance(rad) + luminance(sFog);
+        a.lumSum = a.lumSum + lum;
+        a.lumSumSqr = a.lumSumSqr + lum * lum;
+        const float lumSurf = luminance(rad);
+        a.lumSurf = a.lumSurf + lumSurf;
+        a.lumSurfSqr = a.lumSurfSqr + lumSurf * lumSurf;
+        if (si >= F.minS - 1u && rel_std_err(a.lumSum, a.lumSumSqr, si + 1) < F.threshold) {
+            done = true;
+            count = si + 1;
+        } else if (si + 1 == F.maxS) {
+            done = true;
+            count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
+        }
+    }
+    uint32_t x = 0, y = 0;
+    if (pixel) slot_pixel(F, q, x, y);
+    uint32_t again = 0;  // samples of the next batch this pixel starts
+    if (active && b == 0) {
         if (!done) {
             const uint32_t next = base + nb;
             a.sampleIndex = next;
             A->s = a;
             if (hLoaded) A->h = h;
-            again = min(F.batch, F.maxS - next);
-            for (uint32_t b = 0; b < again; ++b) begin_sample<R>(F, (size_t)q + (size_t)b * F.nSlots, x, y, next + b);
+            again = min(B, F.maxS - next);
         } else {
             if (!hLoaded) h = A->h;
-            a.active = 0;
             A->s.active = 0;
             const size_t o = (size_t)y * F.W + x;
             const float var = rel_std_err(a.lumSurf, a.lumSurfSqr, count);
@@ -941,6 +966,11 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
             doneBlock = (y / F.bs) * F.bx + x / F.bs;
         }
     }
+    // The first lane's decision reaches the group: every lane starts its own sample of the batch.
+    again = (uint32_t)__shfl((int)again, lead);
+    const uint32_t next = (uint32_t)__shfl((int)(base + nb), lead);
+    const bool start = b < again;
+    if (start) begin_sample<R>(F, j, x, y, next + b);
     // Tick's progress: the pixel's block (renderer.cs:125-134) is complete once all its pixels are
     // final. A wave's 8x8 tile lies in one block when the block size is a multiple of 8: then one
     // atomic per wave (per-pixel atomics on a few thousand addresses at a frame's last sample
@@ -963,13 +993,11 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     bump_max(F, kCountMinKey, countMinKey);
     bump_max(F, kCountMaxKey, countMaxKey);
     bump_max(F, kVarMaxKey, varKey);
-    // One atomic per block and batch index: one per wave on the single queue counter serialised the
-    // launch (26.1 -> 14.0 ms per 1080p frame).
-    for (uint32_t b = 0; b < F.batch; ++b) {
-        bump_wave(F, 1, b < again);
-        const uint32_t slot = block_append<kNextBlock / 64>(&F.counts[kRayCount0], b < again, sh);
-        if (b < again && queue_room(F, slot, 1u)) F.queue[0][slot] = q + b * F.nSlots;
-    }
+    // One atomic per block: one per wave on the single queue counter serialised the launch
+    // (26.1 -> 14.0 ms per 1080p frame). A pixel's samples of the batch sit together.
+    bump_wave(F, 1, start);
+    const uint32_t slot = block_append<kNextBlock / 64>(&F.counts[kRayCount0], start, sh);
+    if (start && queue_room(F, slot, 1u)) F.queue[0][slot] = j;
 }
 
 // Primary-hit diagnostic: sample 0 of every pixel of a band.
@@ -2227,7 +2255,7 @@ SliceRun slice_run(sr_ctx* c, const Frame& F, uint32_t slice) {
     r.F = F;
     r.slice = slice;
     r.s = slice_stream(c, slice);
-    r.slotBlocks = (int)((F.nSlots + kBlock - 1) / kBlock);
+    r.slotBlocks = (int)((F.nPaths + kBlock - 1) / kBlock);
     r.gTrace = (int)((2u * F.nPaths + 64u + kTraceBlock - 1) / kTraceBlock);
     r.gShade = (int)((F.nPaths + kShadeBlock - 1) / kShadeBlock);
     r.gTail = c->gridCUs * 8;
@@ -2275,7 +2303,7 @@ void enqueue_sample(sr_ctx* c, const SliceRun& r) {
         hipLaunchKernelGGL(trace, dim3(r.gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
         mark(c, r.slice, 2);
     }
-    hipLaunchKernelGGL(k_next<R>, dim3((F.nSlots + kNextBlock - 1) / kNextBlock), dim3(kNextBlock), 0, s, F);
+    hipLaunchKernelGGL(k_next<R>, dim3((F.nPaths + kNextBlock - 1) / kNextBlock), dim3(kNextBlock), 0, s, F);
     mark(c, r.slice, 3);
 }
 
@@ -2448,8 +2476,10 @@ int sr_render_begin(sr_ctx* ctx, const sr_view* view, const sr_render_params* p)
     const bool adaptive = p->min_samples < p->max_samples && p->variance_threshold > 0.0f;
     const size_t pixelSlots = (size_t)tilesX * (ctx->sliceRows / 8) * 64 * ctx->sliceCount;
     const uint32_t fit = (uint32_t)std::max<size_t>(1, kMaxPathBytes / (pixelSlots * kPathSlotBytes));
-    ctx->batch = (p->rng_policy == SR_RNG_COUNTER && !adaptive && !ctx->countersOn)
-                     ? std::min(std::min(ctx->inFlight, p->max_samples), fit) : 1u;
+    uint32_t batch = (p->rng_policy == SR_RNG_COUNTER && !adaptive && !ctx->countersOn)
+                         ? std::min(std::min(ctx->inFlight, p->max_samples), fit) : 1u;
+    while (batch & (batch - 1)) batch &= batch - 1;  // a power of two: a pixel's slots share a wave
+    ctx->batch = batch;
     int rc = alloc_frame(ctx, tilesX * (ctx->sliceRows / 8) * 64, ctx->sliceCount);
     if (rc) return rc;
     HIP_TRY(hipMemsetAsync(ctx->dBlockPixels.ptr, 0, (size_t)ctx->blocksTotal * 4, ctx->stream));
