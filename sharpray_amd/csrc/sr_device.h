@@ -83,6 +83,9 @@ struct DevScene {
     const DevTexture* textures;
     const float* cdf;
     const float4* alias;  // per texel: acceptance threshold, alias texel (int bits), pdf, alias pdf
+    // Texture sky, per texel (or null): the direction toward its centre and its light pdf, then the
+    // sky radiance seen in that direction (k_sky_table), two float4 per texel.
+    const float4* skyTab;
     int32_t sceneBase;   // pad node of the scene BVH
     int32_t sceneNodes;  // node count of the scene BVH (0 = empty scene)
     int32_t anyTransparent;
@@ -435,8 +438,28 @@ SR_DEV uint32_t cdf_sample_guided(const float* cdf, uint32_t n, float x, const u
     }
     return lo < n - 1 ? lo : n - 1;
 }
+// SkyTexture.LightDirRand's direction toward the centre of texel (x, y) (scene.cs:413) and the pdf
+// of sampling that texel (scene.cs:414). k_sky_table tabulates both with these same functions.
+SR_DEV vec3 sky_texel_dir(const DevSky& k, uint32_t x, uint32_t y) {
+    const vec2 uv = mk2(((float)(int32_t)x + 0.5f) / (float)k.w, ((float)(int32_t)y + 0.5f) / (float)k.h);
+    return rotate(k.rot, from_equirect_uv(uv));
+}
+SR_DEV float sky_texel_pdf(const DevScene& S, uint32_t texel) {
+    const DevTexture t = S.textures[S.sky.tex];
+    return luminance(tex_get(S, t, texel)) * S.sky.pdfScale;
+}
+// Sky radiance toward a light sample: the table's value when sky_light_rand read one (ok), else
+// evaluated on demand.
+struct SkyRad {
+    vec3 v;
+    bool ok;
+};
+SR_DEV vec3 sky_light_radiance(const DevScene& S, const SkyRad& r, vec3 dir) { return r.ok ? r.v : sky_radiance(S, dir); }
+
 template <class R>
-SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf, uint32_t* texelOut = nullptr) {
+SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf, uint32_t* texelOut = nullptr,
+                           SkyRad* radOut = nullptr) {
+    if (radOut) radOut->ok = false;
     const DevSky& k = S.sky;
     if (k.kind == 1) {
         if (k.total <= 0.0f) return false;
@@ -466,12 +489,23 @@ SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf, uin
                 const float* cond = marg + (k.h + 1) + (size_t)y * (k.w + 1);
                 x = cdf_sample(cond, (uint32_t)k.w, c.x);
             }
-            const DevTexture t = S.textures[k.tex];
-            pdf = luminance(tex_get(S, t, y * (uint32_t)k.w + x)) * k.pdfScale;
         }
-        const vec2 uv = mk2(((float)(int32_t)x + 0.5f) / (float)k.w, ((float)(int32_t)y + 0.5f) / (float)k.h);
-        dir = rotate(k.rot, from_equirect_uv(uv));
-        if (texelOut) *texelOut = y * (uint32_t)k.w + x;
+        const uint32_t texel = y * (uint32_t)k.w + x;
+        if (S.skyTab) {
+            // One 32-byte entry instead of the texel's pdf, four trigonometric functions for the
+            // direction and the bilinear sky lookup along it (bit-identical: tabulated by the same code).
+            const float4 e0 = S.skyTab[2 * (size_t)texel], e1 = S.skyTab[2 * (size_t)texel + 1];
+            dir = mk3(e0.x, e0.y, e0.z);
+            if (k.sampler != 1) pdf = e0.w;
+            if (radOut) {
+                radOut->v = mk3(e1.x, e1.y, e1.z);
+                radOut->ok = true;
+            }
+        } else {
+            if (k.sampler != 1) pdf = sky_texel_pdf(S, texel);
+            dir = sky_texel_dir(k, x, y);
+        }
+        if (texelOut) *texelOut = texel;
         return pdf > 0.0f;
     }
     if (k.kind == 2) {

@@ -447,10 +447,11 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
         const vec3 pos = at(ray, sd);
         vec3 ld;
         float lpdf;
-        if (sky_light_rand(S, rng, ld, lpdf) && lpdf > 0.0f) {
+        SkyRad lrad;
+        if (sky_light_rand(S, rng, ld, lpdf, nullptr, &lrad) && lpdf > 0.0f) {
             const float fogPdf = hg_pdf(dot3(ray.d, ld), S.fog.anisotropy);
             sr.o = pos; sr.d = ld;
-            sr.skyRad = sky_radiance(S, ld);
+            sr.skyRad = sky_light_radiance(S, lrad, ld);
             sr.f = splat3(fogPdf);
             sr.w = power_heuristic(lpdf, fogPdf);
             sr.pdf = lpdf;
@@ -494,12 +495,13 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
         // which only skips rays whose contribution is black either way.
         vec3 ld;
         float lpdf;
-        if (sky_light_rand(S, rng, ld, lpdf) && lpdf > 0.0f) {
+        SkyRad lrad;
+        if (sky_light_rand(S, rng, ld, lpdf, nullptr, &lrad) && lpdf > 0.0f) {
             const vec3 nn = dot3(s.normal, view) >= 0.0f ? s.normal : -s.normal;
             const vec3 ng = dot3(s.normalGeo, view) >= 0.0f ? s.normalGeo : -s.normalGeo;
             if (dot3(nn, ld) > 0.0f && dot3(ng, ld) > 0.0f) {
                 sr.o = hitPos; sr.d = ld;
-                sr.skyRad = sky_radiance(S, ld);
+                sr.skyRad = sky_light_radiance(S, lrad, ld);
                 sr.f = opaque_eval(s, view, ld);
                 sr.w = power_heuristic(lpdf, opaque_pdf(s, view, ld, spec_prob(s, view)));
                 sr.pdf = lpdf;
@@ -829,6 +831,22 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
 
 // After the fused tail launch: both ray queues are spent. k_next appends the next sample's camera
 // rays to queue 0, which the tail launch read but did not empty.
+// Per texel of a texture sky: the direction toward its centre and its light pdf, then the sky
+// radiance seen in that direction, so a light sample reads one 32-byte entry (sky_light_rand). S has
+// no table while this runs.
+__global__ void k_sky_table(DevScene S, float4* out) {
+    const DevSky& k = S.sky;
+    const uint32_t n = (uint32_t)k.w * (uint32_t)k.h;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = i % (uint32_t)k.w, y = i / (uint32_t)k.w;
+    const vec3 d = sky_texel_dir(k, x, y);
+    const float pdf = sky_texel_pdf(S, i);
+    const vec3 r = sky_radiance(S, d);
+    out[2 * (size_t)i] = make_float4(d.x, d.y, d.z, pdf);
+    out[2 * (size_t)i + 1] = make_float4(r.x, r.y, r.z, 0.0f);
+}
+
 __global__ void k_queue_reset(Frame F) {
     if (threadIdx.x == 0) {
         F.counts[kRayCount0] = 0;
@@ -1391,6 +1409,7 @@ struct sr_ctx {
     DevBuf<float4> dNodes;
     DevBuf<float> dTris, dAttrs, dTexels, dCdf;
     DevBuf<float4> dAlias;
+    DevBuf<float4> dSkyTab;  // texture sky: two float4 per texel (k_sky_table)
     DevBuf<uint16_t> dFogGrid;
     // Adjoint: the last forward render's view and parameters, and the backward pass's buffers.
     bool adjReady = false;
@@ -1494,7 +1513,7 @@ struct sr_ctx {
         dBlockPixels.release(); dProgress.release(); dFilterCtr.release();
         if (evStart) (void)hipEventDestroy(evStart);
         if (evEnd) (void)hipEventDestroy(evEnd);
-        dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release(); dAlias.release(); dFogGrid.release();
+        dNodes.release(); dTris.release(); dAttrs.release(); dTexels.release(); dCdf.release(); dAlias.release(); dSkyTab.release(); dFogGrid.release();
         dSlotTri.release(); dSceneItems.release(); dObjects.release(); dTextures.release();
         dAdjImage.release(); dAdjGrad.release(); dAdjEnv.release();
         delete cnn;
@@ -1532,6 +1551,27 @@ int ensure(DevBuf<T>& b, size_t n) {
     b.release();
     HIP_TRY(hipMalloc((void**)&b.ptr, std::max<size_t>(n, 1) * sizeof(T)));
     b.n = n;
+    return SR_OK;
+}
+
+// Tabulates a texture sky's light samples (k_sky_table) once its texels are on the device and S
+// points at them. SHARPRAY_SKY_TABLE=0, or no memory for it, leaves the table out: sky_light_rand
+// then evaluates the same values per sample.
+int build_sky_table(sr_ctx* ctx, DevScene& S, hipStream_t s) {
+    S.skyTab = nullptr;
+    const char* env = getenv("SHARPRAY_SKY_TABLE");
+    if (S.sky.kind != 1 || S.sky.total <= 0.0f || (env && strcmp(env, "0") == 0)) {
+        ctx->dSkyTab.release();
+        return SR_OK;
+    }
+    const size_t n = (size_t)S.sky.w * (size_t)S.sky.h;
+    if (ensure(ctx->dSkyTab, 2 * n) != SR_OK) {
+        ctx->dSkyTab.release();
+        return SR_OK;
+    }
+    k_sky_table<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(S, ctx->dSkyTab.ptr);
+    HIP_TRY(hipGetLastError());
+    S.skyTab = ctx->dSkyTab.ptr;
     return SR_OK;
 }
 
@@ -2115,6 +2155,8 @@ int sr_scene_build(sr_ctx* ctx) {
     S.cdf = ctx->dCdf.ptr;
     S.alias = ctx->dAlias.ptr;
     S.fog.grid = ctx->dFogGrid.ptr;
+    if ((rc = build_sky_table(ctx, S, s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
     ctx->built = true;
     return SR_OK;
 }
@@ -3246,6 +3288,8 @@ int sr_adjoint_params_set(sr_ctx* ctx, const float* params) {
     ctx->dev.objects = ctx->dObjects.ptr;
     ctx->dev.cdf = ctx->dCdf.ptr;
     ctx->dev.alias = ctx->dAlias.ptr;
+    if ((rc = build_sky_table(ctx, ctx->dev, s))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
     return SR_OK;
 }
 

@@ -743,6 +743,36 @@ def test_cdf_guide_tables_bit_identical(monkeypatch):
     assert out[0][5] == out[1][5]
 
 
+@pytest.mark.parametrize("case", ["cdf_counting", "alias_counting", "cdf_in_flight", "fog_grid"])
+def test_sky_light_table_bit_identical(case, monkeypatch):
+    """Light samples of a texture sky read their direction, pdf and radiance from a per-texel table
+    (tabulated by the functions that evaluate them per sample): renders with and without it
+    (SHARPRAY_SKY_TABLE=0 when the scene is built) give bit-identical AOVs and counters, for the CDF
+    and alias samplers, the counter policy with samples in flight, and fog NEE."""
+    from sharpray_amd import LIGHT_ALIAS, LIGHT_CDF, RNG_COUNTER, RNG_REFERENCE, Renderer, scenes
+    spec = scenes.config_fog(scale=0.1, spp=4) if case == "fog_grid" else scenes.config_plain_gpu(scale=0.2, spp=8)
+    assert spec.sky_texture >= 0
+    counting = case.endswith("counting")
+    sampler = LIGHT_ALIAS if case.startswith("alias") else LIGHT_CDF
+    policy = RNG_REFERENCE if counting else RNG_COUNTER
+    out = []
+    for table in ("1", "0"):
+        monkeypatch.setenv("SHARPRAY_SKY_TABLE", table)
+        scene, view = scenes.instantiate(spec)
+        scene.build()
+        r = Renderer(scene, view, spec.width, spec.height, 32, spec.min_samples, spec.max_samples,
+                     spec.variance_threshold, spec.bounces, spec.indirect_clamp, rng_policy=policy, rng_seed=5,
+                     counters=counting, light_sampler=sampler)
+        r.wait()
+        out.append((r.radiance.copy(), r.radiance_fog.copy(), r.depth.copy(), r.normals.copy(), r.albedo.copy(),
+                    r.samples.copy(), r.counters() if counting else None))
+    print(f"{case}: {spec.width}x{spec.height}, {spec.max_samples} spp, sky table on / off", flush=True)
+    assert np.abs(out[0][0]).sum() > 0
+    for a, b in zip(out[0][:6], out[1][:6]):
+        assert np.array_equal(bits(a), bits(b)) if a.dtype == np.float32 else np.array_equal(a, b)
+    assert out[0][6] == out[1][6]
+
+
 @pytest.mark.parametrize("size", [(1, 1), (7, 3), (13, 9), (9, 17)])
 def test_odd_image_sizes_match_oracle(size, oracle_lib):
     """Images smaller than one 8x8 tile or with partial tiles on both edges: every pixel is rendered
