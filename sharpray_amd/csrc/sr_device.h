@@ -11,7 +11,8 @@
 //    closest hit.
 //  - objects: transform, material and shape per Object (scene.cs:204-223).
 //  - texels / textures: linear RGB texels (12 B) and per-texture descriptors.
-//  - cdf: the sky's marginal and conditional CDF tables (cdf.cs:62-98).
+//  - cdf: the sky's marginal and conditional CDF tables (cdf.cs:62-98), then their bucket records.
+//  - skyTab: per sky texel, the light sample's direction, pdf and radiance (k_sky_table).
 #pragma once
 
 #include "sr_math.h"
@@ -50,7 +51,7 @@ struct DevSky {
     quat rot, rotInv;
     int32_t w, h;
     int32_t cdfOff;  // marginal CDF (h+1 floats) followed by h conditional CDFs (w+1 floats each)
-    int32_t guideOff;  // -1, or the CDFs' bucket bounds (uint32 bits in the same pool, see cdf_sample_guided)
+    int32_t guideOff;  // -1, or the CDFs' bucket records (float4 each, in the same pool; cdf_sample_guided)
     int32_t sampler;  // 0: row/column CDFs, 1: alias table (DevScene::alias)
     float total, pdfScale;
     vec3 top, middle, bottom, sunDir, sunRad;
@@ -416,25 +417,36 @@ SR_DEV uint32_t cdf_sample(const float* cdf, uint32_t n, float x) {
     }
     return lo < n - 1 ? lo : n - 1;
 }
-// Bucket bounds of a CDF table: with m buckets (a power of two), entry j < m is the search's
-// unclamped answer for x = j / m and entry m is n. The marginal CDF has kGuideMarg buckets, each
-// conditional CDF kGuideCond.
+// Bucket records of a CDF table with n + 1 entries: with m buckets (a power of two), bucket j covers
+// x in [j / m, (j + 1) / m) and its 16-byte record holds the search's unclamped answers lo for
+// x = j / m and hi for x = (j + 1) / m (n for the last bucket), then copies of cdf[lo + 1] and
+// cdf[lo + 2] (0 past the table). The marginal CDF has kGuideMarg buckets, each conditional CDF
+// kGuideCond: as many buckets as the 2048-texel rows of a 2048 x 1024 map have texels, so a bucket
+// mostly holds at most two texel boundaries and one record read settles the search.
 #ifndef SR_GUIDE_COND
-#define SR_GUIDE_COND 512
+#define SR_GUIDE_COND 2048
 #endif
 constexpr uint32_t kGuideMarg = 1024, kGuideCond = SR_GUIDE_COND;
-// Cdf1.Sample with the search confined to the bounds of x's bucket j = floor(x m): x lies in
-// [j / m, (j + 1) / m), the predicate cdf[i + 1] <= x falls from true to false once along i (the
-// table does not decrease), so its first false index lies between the bucket's bounds and the
-// confined search returns the index of the full one in about log2(n / m) steps instead of log2(n).
-SR_DEV uint32_t cdf_sample_guided(const float* cdf, uint32_t n, float x, const uint32_t* g, uint32_t m) {
+// Cdf1.Sample with the search confined to x's bucket j = floor(x m): x lies in [j / m, (j + 1) / m)
+// and the predicate cdf[i + 1] <= x falls from true to false once along i (the table does not
+// decrease), so its first false index lies in [lo, hi]. The record's copies decide the first two
+// candidates lo and lo + 1; a bucket holding more texel boundaries continues with a binary search
+// of the table over the rest of [lo, hi]. Same index as the full search.
+SR_DEV uint32_t cdf_sample_guided(const float* cdf, uint32_t n, float x, const float4* g, uint32_t m) {
     const float s = x * (float)m;
     const uint32_t j = s >= 0.0f ? (s < (float)m ? (uint32_t)s : m - 1) : 0u;
-    uint32_t lo = g[j], hi = g[j + 1];
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (cdf[mid + 1] <= x) lo = mid + 1;
-        else hi = mid;
+    const float4 r = g[j];
+    uint32_t lo = __float_as_uint(r.x), hi = __float_as_uint(r.y);
+    if (lo < hi && r.z <= x) {
+        ++lo;
+        if (lo < hi && r.w <= x) {
+            ++lo;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (cdf[mid + 1] <= x) lo = mid + 1;
+                else hi = mid;
+            }
+        }
     }
     return lo < n - 1 ? lo : n - 1;
 }
@@ -479,11 +491,10 @@ SR_DEV bool sky_light_rand(const DevScene& S, R& rng, vec3& dir, float& pdf, uin
         } else {
             const float* marg = S.cdf + k.cdfOff;
             if (k.guideOff >= 0) {
-                const uint32_t* G = (const uint32_t*)(S.cdf + k.guideOff);
+                const float4* G = (const float4*)(S.cdf + k.guideOff);
                 y = cdf_sample_guided(marg, (uint32_t)k.h, c.y, G, kGuideMarg);
                 const float* cond = marg + (k.h + 1) + (size_t)y * (k.w + 1);
-                x = cdf_sample_guided(cond, (uint32_t)k.w, c.x, G + (kGuideMarg + 1) + (size_t)y * (kGuideCond + 1),
-                                      kGuideCond);
+                x = cdf_sample_guided(cond, (uint32_t)k.w, c.x, G + kGuideMarg + (size_t)y * kGuideCond, kGuideCond);
             } else {
                 y = cdf_sample(marg, (uint32_t)k.h, c.y);
                 const float* cond = marg + (k.h + 1) + (size_t)y * (k.w + 1);

@@ -1760,18 +1760,23 @@ void build_sky(sr_ctx* ctx, DevSky& K, std::vector<float>& cdf, std::vector<floa
                 }
                 return lo;
             };
-            auto put_bounds = [&](const float* c, uint32_t n, uint32_t m, std::vector<float>& out) {
+            auto put_records = [&](const float* c, uint32_t n, uint32_t m, std::vector<float>& out) {
+                uint32_t lo = first_false(c, n, 0.0f);
                 for (uint32_t j = 0; j < m; ++j) {
-                    const uint32_t b = first_false(c, n, (float)j / (float)m);
-                    out.push_back(__builtin_bit_cast(float, b));
+                    const uint32_t hi = j + 1 < m ? first_false(c, n, (float)(j + 1) / (float)m) : n;
+                    out.push_back(__builtin_bit_cast(float, lo));
+                    out.push_back(__builtin_bit_cast(float, hi));
+                    out.push_back(lo + 1 <= n ? c[lo + 1] : 0.0f);
+                    out.push_back(lo + 2 <= n ? c[lo + 2] : 0.0f);
+                    lo = hi;
                 }
-                out.push_back(__builtin_bit_cast(float, n));
             };
+            while (cdf.size() % 4) cdf.push_back(0.0f);  // float4 records
             std::vector<float> guide;
-            guide.reserve((size_t)(kGuideMarg + 1) + (size_t)H * (kGuideCond + 1));
-            put_bounds(cdf.data(), (uint32_t)H, kGuideMarg, guide);
+            guide.reserve(4 * ((size_t)kGuideMarg + (size_t)H * kGuideCond));
+            put_records(cdf.data(), (uint32_t)H, kGuideMarg, guide);
             for (int y = 0; y < H; ++y)
-                put_bounds(cdf.data() + (H + 1) + (size_t)y * (W + 1), (uint32_t)W, kGuideCond, guide);
+                put_records(cdf.data() + (H + 1) + (size_t)y * (W + 1), (uint32_t)W, kGuideCond, guide);
             K.guideOff = (int32_t)cdf.size();
             cdf.insert(cdf.end(), guide.begin(), guide.end());
         }
