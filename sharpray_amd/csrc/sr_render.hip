@@ -277,8 +277,6 @@ SR_DEV void rng_store(PathRec& p, const XorwowRng& r) {
 SR_DEV void rng_store(PathRec& p, const CounterRng& r) {
     p.rng[0] = r.key0; p.rng[1] = r.key1; p.rng[2] = r.sample; p.rng[3] = r.draw;
 }
-// The RNG that draws sample `sample` of pixel (x, y): the pixel's persistent xorwow stream, or a
-// fresh Philox counter for that (pixel, sample).
 // The RNG that draws sample `sample` of pixel (x, y): the pixel's persistent xorwow stream (new
 // Rng(x, y) for sample 0, renderer.cs:156; the state the previous sample left in path slot j
 // otherwise), or a fresh Philox counter for that (pixel, sample).
@@ -829,8 +827,6 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
     bump_sum(F, 11, extra);
 }
 
-// After the fused tail launch: both ray queues are spent. k_next appends the next sample's camera
-// rays to queue 0, which the tail launch read but did not empty.
 // Per texel of a texture sky: the direction toward its centre and its light pdf, then the sky
 // radiance seen in that direction, so a light sample reads one 32-byte entry (sky_light_rand). S has
 // no table while this runs.
@@ -847,6 +843,8 @@ __global__ void k_sky_table(DevScene S, float4* out) {
     out[2 * (size_t)i + 1] = make_float4(r.x, r.y, r.z, 0.0f);
 }
 
+// After the fused tail launch: both ray queues are spent. k_next appends the next sample's camera
+// rays to queue 0, which the tail launch read but did not empty.
 __global__ void k_queue_reset(Frame F) {
     if (threadIdx.x == 0) {
         F.counts[kRayCount0] = 0;
