@@ -210,7 +210,7 @@ struct MatGrad {
 
 // One path's adjoint (the forward steps are those of the shade kernel's surface and sky branches,
 // with the shadow ray traced inline).
-SR_DEV void adjoint_path(const DevScene& S, ray3 ray, CounterRng& rng, uint32_t bounces, float clampLum,
+SR_DEV void adjoint_path(const DevScene& S, ray3 ray, CounterRngT<false>& rng, uint32_t bounces, float clampLum,
                          vec3 delta, MatGrad& gm, float* gEnv, uint2* lds) {
     vec3 energy = splat3(1.0f);
     bool hasScatterPdf = false;

@@ -149,16 +149,30 @@ SR_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
         k1 += 0xBB67AE85u;
     }
 }
-struct CounterRng {
+// CACHE keeps the last block drawn from in registers (blk, for block index blkIdx), so four
+// consecutive draws cost one Philox evaluation instead of four; the stream is the same. A path
+// record keeps only (key, sample, draw): a reloaded generator starts with no block. The adjoint
+// kernel, which runs at a tight register budget, draws without the cache.
+template <bool CACHE>
+struct CounterRngT {
     uint32_t key0, key1, sample, draw;
+    uint32_t blkIdx = ~0u;
+    uint32_t blk0 = 0, blk1 = 0, blk2 = 0, blk3 = 0;
     SR_HD uint32_t next_uint() {
-        uint32_t c[4] = {sample, draw >> 2, 0u, 0u};
-        philox4x32_10(c, key0, key1);
-        const uint32_t v = c[draw & 3u];
+        const uint32_t b = draw >> 2;
+        if (!CACHE || b != blkIdx) {
+            uint32_t c[4] = {sample, b, 0u, 0u};
+            philox4x32_10(c, key0, key1);
+            blk0 = c[0]; blk1 = c[1]; blk2 = c[2]; blk3 = c[3];
+            blkIdx = b;
+        }
+        const uint32_t w = draw & 3u;
+        const uint32_t v = w == 0u ? blk0 : w == 1u ? blk1 : w == 2u ? blk2 : blk3;
         ++draw;
         return v;
     }
 };
+using CounterRng = CounterRngT<true>;
 
 template <class R>
 SR_HD float next_float(R& r) {

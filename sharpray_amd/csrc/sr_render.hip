@@ -1067,7 +1067,7 @@ __global__ void __launch_bounds__(kBlock, SR_ADJOINT_MIN_WAVES) k_adjoint(Adjoin
         const vec3 dI = mk3(A.dImage[pix * 3], A.dImage[pix * 3 + 1], A.dImage[pix * 3 + 2]);
         if (dI.x != 0.0f || dI.y != 0.0f || dI.z != 0.0f) {
             const vec3 delta = dI / (float)(A.spp + 1);  // Radiance = sum / (spp + 1) (renderer.cs:173,215)
-            CounterRng rng;
+            CounterRngT<false> rng;
             rng.key0 = A.seed;
             rng.key1 = y * A.W + x;
             rng.sample = smp;
