@@ -234,9 +234,12 @@ SR_DEV vec3 perp3(vec3 v) {  // math.cs:383-388
     return normalized(cross3(up, v));
 }
 
-// Slab test of AABox.IntersectDist (math.cs:983-1004). Inputs are finite, so hardware min/max
-// agree with MathF's; a -0 entry distance only ever meets ordered comparisons.
-SR_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, const ray3& r, float& t) {
+// Slab test of AABox.IntersectDist (math.cs:983-1004) with the caller's bound: the box counts as hit
+// when t = max(0, tmin) <= min(tmax, limit), which is the reference's test (tmax >= 0 and
+// tmin <= tmax) followed by t <= limit in fewer instructions; hardware min / max return the other
+// operand of a NaN, so both forms agree for NaN slab distances too. Inputs are finite, so hardware
+// min/max agree with MathF's; a -0 entry distance only ever meets ordered comparisons.
+SR_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, const ray3& r, float limit, float& t) {
     const float t1 = (mnx - r.o.x) * r.inv.x;
     const float t2 = (mxx - r.o.x) * r.inv.x;
     const float t3 = (mny - r.o.y) * r.inv.y;
@@ -245,9 +248,8 @@ SR_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float mx
     const float t6 = (mxz - r.o.z) * r.inv.z;
     const float tmin = fmaxf(fmaxf(fminf(t1, t2), fminf(t3, t4)), fminf(t5, t6));
     const float tmax = fminf(fminf(fmaxf(t1, t2), fmaxf(t3, t4)), fmaxf(t5, t6));
-    if (tmax < 0.0f || tmin > tmax) return false;
     t = fmaxf(0.0f, tmin);
-    return true;
+    return t <= fminf(tmax, limit);
 }
 
 // TriangleLean.Intersect (Moller-Trumbore, math.cs:1407-1435).

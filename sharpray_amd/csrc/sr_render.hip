@@ -1695,9 +1695,11 @@ int append_bvh(std::vector<float4>& pool, const BvhResult& b, int32_t& base) {
         const RefNode& nd = b.nodes[i];
         if ((uint32_t)nd.child > kMaxChild || nd.count > 255 || nd.count < 0)
             return fail(SR_ERR_LIMIT, "BVH node exceeds the device layout (child < 2^24, leaf size < 256)");
-        int32_t ci = nd.child, cn = nd.count;
+        // The traversal's stack word (count << 24 | child) in place of the child index.
+        const uint32_t word = ((uint32_t)nd.count << 24) | (uint32_t)nd.child;
+        const int32_t cn = nd.count;
         pool.push_back(make_float4(nd.bounds.mn.x, nd.bounds.mn.y, nd.bounds.mn.z, nd.bounds.mx.x));
-        pool.push_back(make_float4(nd.bounds.mx.y, nd.bounds.mx.z, __builtin_bit_cast(float, ci), __builtin_bit_cast(float, cn)));
+        pool.push_back(make_float4(nd.bounds.mx.y, nd.bounds.mx.z, __builtin_bit_cast(float, word), __builtin_bit_cast(float, cn)));
     }
     return SR_OK;
 }

@@ -60,9 +60,8 @@ struct TravStack {
 // Dynamic LDS of a traversal launch with `threads` threads per block.
 constexpr size_t trav_lds_bytes(int threads) { return (size_t)kLdsDepth * (size_t)threads * sizeof(uint2); }
 
-SR_DEV uint32_t pack_node(float childBits, float countBits) {
-    return ((uint32_t)__float_as_int(countBits) << 24) | (uint32_t)__float_as_int(childBits);
-}
+// A node's stack word (count << 24 | child), stored in its child field by the node pool.
+SR_DEV uint32_t node_word(float wordBits) { return (uint32_t)__float_as_int(wordBits); }
 
 // Closest hit in one mesh BVH (bvh.cs:165-216 + mesh.cs:51-62). Nearer child first, ties to
 // child B; prune when the entry distance is >= the best hit; accept strictly closer hits.
@@ -79,7 +78,7 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
     bestT = kInf;
     bool found = false;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
+    uint32_t cur = node_word(root.z);
     float curT = 0.0f;
     bool live = true;
     while (live) {
@@ -90,9 +89,9 @@ SR_DEV bool mesh_closest(const DevScene& S, int32_t nodeBase, int32_t triBase, c
                 const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
                 const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
                 float tA = 0.0f, tB = 0.0f;
-                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= bestT;
-                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= bestT;
-                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, bestT, tA);
+                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, bestT, tB);
+                const uint32_t pA = node_word(a1.z), pB = node_word(b1.z);
                 if (hA && hB) {
                     const bool aNear = tA < tB;
                     st.put(lds, sp, aNear ? pB : pA, aNear ? tB : tA);
@@ -141,7 +140,7 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
     TravStack<LD> st;
     int sp = 0;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
+    uint32_t cur = node_word(root.z);
     float unused;
     for (;;) {
         while ((cur >> 24) == 0) {
@@ -149,9 +148,9 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
             const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
             const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
             float tA = 0.0f, tB = 0.0f;
-            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
-            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
-            const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, kInf, tA);
+            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, kInf, tB);
+            const uint32_t pA = node_word(a1.z), pB = node_word(b1.z);
             if (hA && hB) {
                 const bool aNear = tA < tB;
                 st.put(lds, sp++, aNear ? pB : pA, 0.0f);
@@ -193,7 +192,7 @@ SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBa
     bestT = kInf;
     bool found = false;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
+    uint32_t cur = node_word(root.z);
     float curT = 0.0f;
     bool live = true;   // cur is a node still to visit
     uint32_t leaf = 0;  // postponed leaf (count << 24 | first item; 0 = none)
@@ -213,21 +212,19 @@ SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBa
                     const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
                     const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
                     float tA = 0.0f, tB = 0.0f;
-                    const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= bestT;
-                    const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= bestT;
-                    const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+                    const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, bestT, tA);
+                    const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, bestT, tB);
+                    const uint32_t pA = node_word(a1.z), pB = node_word(b1.z);
+                    // Nearer child first, ties to B, as selects: one push of the farther child
+                    // when both are hit.
+                    const bool goA = hA && (!hB || tA < tB);
                     if (hA && hB) {
-                        const bool aNear = tA < tB;
-                        st.put(lds, sp, aNear ? pB : pA, aNear ? tB : tA);
+                        st.put(lds, sp, goA ? pB : pA, goA ? tB : tA);
                         ++sp;
-                        cur = aNear ? pA : pB;
-                        curT = aNear ? tA : tB;
-                        next = true;
-                    } else if (hA) {
-                        cur = pA; curT = tA; next = true;
-                    } else if (hB) {
-                        cur = pB; curT = tB; next = true;
                     }
+                    cur = goA ? pA : pB;
+                    curT = goA ? tA : tB;
+                    next = hA || hB;
                 }
                 if (!next) {
                     if (sp == 0) { live = false; break; }
@@ -268,7 +265,7 @@ SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, 
     TravStack<LD> st;
     int sp = 0;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
+    uint32_t cur = node_word(root.z);
     float unused;
     bool live = true;
     uint32_t leaf = 0;
@@ -283,17 +280,13 @@ SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, 
                 const float4* P = N + (size_t)(nodeBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
                 const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
                 float tA = 0.0f, tB = 0.0f;
-                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
-                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
-                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
-                if (hA && hB) {
-                    const bool aNear = tA < tB;
-                    st.put(lds, sp++, aNear ? pB : pA, 0.0f);
-                    cur = aNear ? pA : pB;
-                } else if (hA) {
-                    cur = pA;
-                } else if (hB) {
-                    cur = pB;
+                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, kInf, tA);
+                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, kInf, tB);
+                const uint32_t pA = node_word(a1.z), pB = node_word(b1.z);
+                const bool goA = hA && (!hB || tA < tB);
+                if (hA && hB) st.put(lds, sp++, goA ? pB : pA, 0.0f);
+                if (hA || hB) {
+                    cur = goA ? pA : pB;
                 } else {
                     if (sp == 0) { live = false; break; }
                     st.get(lds, --sp, cur, unused);
@@ -367,7 +360,7 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
     float best = kInf;
     bool found = false;
     const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
+    uint32_t cur = node_word(root.z);
     float curT = 0.0f;
     bool live = true;
     while (live) {
@@ -378,9 +371,9 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
                 const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
                 const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
                 float tA = 0.0f, tB = 0.0f;
-                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA) && tA <= best;
-                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB) && tB <= best;
-                const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+                const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, best, tA);
+                const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, best, tB);
+                const uint32_t pA = node_word(a1.z), pB = node_word(b1.z);
                 if (hA && hB) {
                     const bool aNear = tA < tB;
                     stk[sp] = aNear ? pB : pA;
@@ -434,16 +427,16 @@ SR_DEV bool scene_any(const DevScene& S, const ray3& r, Tally& tl, uint2* lds) {
     uint32_t stk[kStack];
     int sp = 0;
     const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
-    uint32_t cur = pack_node(root.z, root.w);
+    uint32_t cur = node_word(root.z);
     for (;;) {
         while ((cur >> 24) == 0) {
             if (COUNT) tl.box += 2;
             const float4* P = N + (size_t)(S.sceneBase + 1 + (int32_t)(cur & kMaxChild)) * 2;
             const float4 a0 = P[0], a1 = P[1], b0 = P[2], b1 = P[3];
             float tA = 0.0f, tB = 0.0f;
-            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, tA);
-            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, tB);
-            const uint32_t pA = pack_node(a1.z, a1.w), pB = pack_node(b1.z, b1.w);
+            const bool hA = slab(a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, r, kInf, tA);
+            const bool hB = slab(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, r, kInf, tB);
+            const uint32_t pA = node_word(a1.z), pB = node_word(b1.z);
             if (hA && hB) {
                 const bool aNear = tA < tB;
                 stk[sp++] = aNear ? pB : pA;
