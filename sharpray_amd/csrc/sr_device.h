@@ -42,6 +42,7 @@ struct DevObject {
     vec3 radiance;
     float f0;           // MathF.Pow((ior - 1) / (ior + 1), 2f) (brdf.cs:14), evaluated on the host
     float shape[6];     // sphere centre + radius, or box min + max
+    int32_t xform;      // the first object with a bit-identical transform (rot, pos, scale)
 };
 
 struct DevSky {

@@ -1836,6 +1836,14 @@ void build_objects(sr_ctx* ctx, std::vector<DevObject>& dobj, int32_t& anyTransp
         d.f0 = libm_powf((o.mat.ior - 1.0f) / (o.mat.ior + 1.0f), 2.0f);
         memcpy(d.shape, o.shape, sizeof(d.shape));
         if (o.mat.transparency > 0.0f) anyTransparent = 1;
+        // Objects with bit-identical transforms share the object-space ray of a traversal.
+        d.xform = (int32_t)i;
+        for (size_t k = 0; k < i; ++k)
+            if (memcmp(&dobj[k].rot, &d.rot, sizeof(quat)) == 0 && memcmp(&dobj[k].pos, &d.pos, sizeof(vec3)) == 0 &&
+                memcmp(&dobj[k].scale, &d.scale, sizeof(vec3)) == 0) {
+                d.xform = dobj[k].xform;
+                break;
+            }
     }
 }
 

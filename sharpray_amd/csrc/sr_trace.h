@@ -183,13 +183,16 @@ SR_DEV bool mesh_any(const DevScene& S, int32_t nodeBase, int32_t triBase, const
 // leaf the plain traversal would have pruned holds no strictly closer hit); the extra box tests
 // come from descending before a postponed leaf has lowered the best distance, which is why
 // counting renders use mesh_closest itself.
+// `bound`: hits at or beyond it are not looked for (kInf for the whole mesh); the closest hit below
+// it is the one the whole-mesh search finds, since every node holding it or an equal hit found
+// before it enters below the bound.
 template <int LD>
 SR_DEV bool mesh_closest_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, const ray3& r, float& bestT,
-                              float& bu, float& bv, int32_t& bslot, uint2* lds) {
+                              float& bu, float& bv, int32_t& bslot, uint2* lds, float bound = kInf) {
     const float4* N = S.nodes;
     TravStack<LD> st;
     int sp = 0;
-    bestT = kInf;
+    bestT = bound;
     bool found = false;
     const float4 root = N[(size_t)(nodeBase + 1) * 2 + 1];
     uint32_t cur = node_word(root.z);
@@ -308,21 +311,46 @@ SR_DEV bool mesh_any_spec(const DevScene& S, int32_t nodeBase, int32_t triBase, 
     }
 }
 
-// Object.Intersect (scene.cs:227 -> math.cs:67-79): distance in world units only; the hit
-// record keeps (u, v, slot) so shading can rebuild the full ShapeHit.
-template <bool COUNT, bool SPEC, int LD>
-SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r, float& dist, float& u, float& v,
-                           int32_t& slot, Tally& tl, uint2* lds) {
+// The object-space ray of one object transform (Transform.TransformRayInv, math.cs:816-822) kept
+// while a traversal visits objects: objects with a bit-identical transform (DevObject::xform) get
+// the same ray, so it is computed once for all of them (the bench scene's nine objects share one).
+struct ObjRay {
+    int32_t xform = -1;
+    ray3 r;
     float scale;
-    const ray3 lr = to_object(ob, r, scale);
+    SR_DEV const ray3& get(const DevObject& ob, const ray3& world) {
+        if (ob.xform != xform) {
+            r = to_object(ob, world, scale);
+            xform = ob.xform;
+        }
+        return r;
+    }
+};
+
+// Object.Intersect (scene.cs:227 -> math.cs:67-79): distance in world units only; the hit
+// record keeps (u, v, slot) so shading can rebuild the full ShapeHit. `best` is the closest world
+// distance found so far: outside counting renders the mesh search stops looking at object-space
+// distances whose world distance would round to `best` or beyond (a bound 2^-18 above
+// best * scale), which loses no hit the caller would accept (it takes d < best only).
+template <bool COUNT, bool SPEC, int LD>
+SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r, ObjRay& orr, float best, float& dist,
+                           float& u, float& v, int32_t& slot, Tally& tl, uint2* lds) {
+    const ray3& lr = orr.get(ob, r);
+    const float scale = orr.scale;
     float t = 0.0f;
     slot = -1;
     u = v = 0.0f;
     bool hit;
     if (ob.kind == 0) {
         if (COUNT) tl.mesh += 1;
-        if (SPEC && !COUNT) hit = ob.nodeBase >= 0 && mesh_closest_spec<LD>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, lds);
-        else hit = ob.nodeBase >= 0 && mesh_closest<COUNT, LD>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, tl, lds);
+        if (SPEC && !COUNT) {
+            // best * scale in the normal range keeps its relative rounding error below 2^-24.
+            const float bs = best * scale;
+            const float bound = best < kInf && bs >= 1e-30f ? bs * (1.0f + 0x1p-18f) : kInf;
+            hit = ob.nodeBase >= 0 && mesh_closest_spec<LD>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, lds, bound);
+        } else {
+            hit = ob.nodeBase >= 0 && mesh_closest<COUNT, LD>(S, ob.nodeBase, ob.triBase, lr, t, u, v, slot, tl, lds);
+        }
     } else {
         ShapeHitD h;
         hit = ob.kind == 1 ? sphere_hit(ob.shape, lr, h, false) : box_hit(ob.shape, lr, h, false);
@@ -334,9 +362,8 @@ SR_DEV bool object_closest(const DevScene& S, const DevObject& ob, const ray3& r
 }
 
 template <bool COUNT, int LD>
-SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, Tally& tl, uint2* lds) {
-    float scale;
-    const ray3 lr = to_object(ob, r, scale);
+SR_DEV bool object_any(const DevScene& S, const DevObject& ob, const ray3& r, ObjRay& orr, Tally& tl, uint2* lds) {
+    const ray3& lr = orr.get(ob, r);
     if (ob.kind == 0) {
         if (COUNT) tl.mesh += 1;
         if (!COUNT) return ob.nodeBase >= 0 && mesh_any_spec<LD>(S, ob.nodeBase, ob.triBase, lr, lds);
@@ -363,6 +390,7 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
     uint32_t cur = node_word(root.z);
     float curT = 0.0f;
     bool live = true;
+    ObjRay orr;
     while (live) {
         while ((cur >> 24) == 0) {
             bool next = false;
@@ -403,7 +431,7 @@ SR_DEV bool scene_closest(const DevScene& S, const ray3& r, HitRec& out, Tally& 
                 const int32_t oi = S.sceneItems[child + i];
                 float d, u, v;
                 int32_t slot;
-                if (object_closest<COUNT, SPEC, LD>(S, S.objects[oi], r, d, u, v, slot, tl, lds) && d < best) {
+                if (object_closest<COUNT, SPEC, LD>(S, S.objects[oi], r, orr, best, d, u, v, slot, tl, lds) && d < best) {
                     best = d;
                     out.t = d; out.u = u; out.v = v; out.obj = oi; out.slot = slot;
                     found = true;
@@ -428,6 +456,7 @@ SR_DEV bool scene_any(const DevScene& S, const ray3& r, Tally& tl, uint2* lds) {
     int sp = 0;
     const float4 root = N[(size_t)(S.sceneBase + 1) * 2 + 1];
     uint32_t cur = node_word(root.z);
+    ObjRay orr;
     for (;;) {
         while ((cur >> 24) == 0) {
             if (COUNT) tl.box += 2;
@@ -453,7 +482,7 @@ SR_DEV bool scene_any(const DevScene& S, const ray3& r, Tally& tl, uint2* lds) {
         const uint32_t cnt = cur >> 24, child = cur & kMaxChild;
         for (uint32_t i = 0; i < cnt; ++i) {
             if (COUNT) tl.shape += 1;
-            if (object_any<COUNT, LD>(S, S.objects[S.sceneItems[child + i]], r, tl, lds)) return true;
+            if (object_any<COUNT, LD>(S, S.objects[S.sceneItems[child + i]], r, orr, tl, lds)) return true;
         }
         if (sp == 0) return false;
         cur = stk[--sp];

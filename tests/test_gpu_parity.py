@@ -677,11 +677,17 @@ def test_fused_tail_launch_matches_wavefront(tail):
     assert a[3] == b[3], (a[3], b[3])
 
 
-def test_speculative_traversal_matches_counting_render():
-    """Renders without counters traverse mesh BVHs speculatively (postponed leaves); counting renders
-    use the plain order. Both must give bit-identical AOVs on the 1M-triangle scene."""
+@pytest.mark.parametrize("case", ["plain_gpu", "textured_transformed"])
+def test_speculative_traversal_matches_counting_render(case):
+    """Renders without counters traverse mesh BVHs speculatively (postponed leaves) and bound each
+    object's search by the closest world distance found so far; counting renders use the plain
+    order over whole meshes. Both must give bit-identical AOVs, on the 1M-triangle scene (nine
+    objects sharing one transform) and on a scene of rotated, translated and scaled objects."""
     from sharpray_amd import Renderer, scenes
-    spec = scenes.config_plain_gpu(scale=0.25, spp=4)
+    if case == "plain_gpu":
+        spec = scenes.config_plain_gpu(scale=0.25, spp=4)
+    else:
+        spec = scenes.config_textured_transformed(scale=1.0, spp=16)
     scene, view = scenes.instantiate(spec)
     scene.build()
     out = {}
@@ -690,7 +696,8 @@ def test_speculative_traversal_matches_counting_render():
                      spec.variance_threshold, spec.bounces, spec.indirect_clamp, counters=counting)
         r.wait()
         out[counting] = (r.radiance.copy(), r.radiance_fog.copy(), r.depth.copy(), r.normals.copy())
-    print(f"plain_gpu 1/4 scale {spec.width}x{spec.height}, {spec.max_samples} spp", flush=True)
+    print(f"{case} {spec.width}x{spec.height}, {spec.max_samples} spp", flush=True)
+    assert np.abs(out[True][0]).sum() > 0
     for a, b in zip(out[False], out[True]):
         assert np.array_equal(bits(a), bits(b))
 
