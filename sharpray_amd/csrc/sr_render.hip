@@ -760,10 +760,11 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
 // summation order), and the short tail launches of late segments share the GPU with the shadow
 // rays instead of running alone. Each wave takes only one kind of item.
 template <bool COUNT, bool TRANSP>
-// Register budget of the trace kernel: at least 7 waves per SIMD (72 VGPRs, a few spilled) beat
-// the unconstrained 85 VGPRs / 5 waves by 4 % of trace time on the 1080p / 1M-triangle frame.
+// Register budget of the trace kernel: 8 waves per SIMD, the most a SIMD holds (64 VGPRs, 24 spilled),
+// on the 1080p / 1M-triangle frame 65.05 ms of trace against 65.75 with 7 waves (72 VGPRs) and
+// 69.0 with 6.
 #ifndef SR_TRACE_MIN_WAVES
-#define SR_TRACE_MIN_WAVES 7
+#define SR_TRACE_MIN_WAVES 8
 #endif
 __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame F, DevScene S, int cur) {
     // The two queues this segment's shade fills start empty; nothing else touches them meanwhile.
