@@ -853,6 +853,33 @@ __global__ void k_queue_reset(Frame F) {
     }
 }
 
+// The batch fold of k_next<R, true>: PixAcc viewed as 28 words, each updated by one lane per batch
+// with the operation the sequential fold applies to it, from a column of the samples' LDS rows.
+// Row columns: 0-2 radiance, 3-5 fog radiance, 6 luminance, 7 its square, 8 surface luminance,
+// 9 its square, 10-12 primary normal, 13 depth, 14-16 albedo, 17-18 uv, 19 the constant 1.
+constexpr uint32_t kFoldRow = 21;  // words per row (odd: a group's lanes reading a row hit distinct banks)
+constexpr uint32_t kFoldAdd = 0, kFoldAddHit = 1, kFoldSetHit = 2, kFoldIncHit = 3, kFoldFirstHit = 4;
+constexpr uint32_t kFoldJobs = 25;
+#define SR_FOLD(word, src, kind) ((word) | ((src) << 8) | ((kind) << 16))
+__constant__ uint32_t kFoldJob[kFoldJobs] = {
+    SR_FOLD(0, 0, 0), SR_FOLD(1, 1, 0), SR_FOLD(2, 2, 0),          // radSum
+    SR_FOLD(3, 3, 0), SR_FOLD(4, 4, 0), SR_FOLD(5, 5, 0),          // radFogSum
+    SR_FOLD(6, 6, 0), SR_FOLD(7, 7, 0), SR_FOLD(8, 8, 0), SR_FOLD(9, 9, 0),  // lumSum, lumSumSqr, lumSurf, lumSurfSqr
+    SR_FOLD(12, 10, 1), SR_FOLD(13, 11, 1), SR_FOLD(14, 12, 1),    // normalSum
+    SR_FOLD(15, 10, 2), SR_FOLD(16, 11, 2), SR_FOLD(17, 12, 2),    // normalFallback (last hit)
+    SR_FOLD(18, 0, 3),                                             // normalCount
+    SR_FOLD(19, 17, 4), SR_FOLD(20, 18, 4), SR_FOLD(21, 19, 4),    // uv, hasUv (first hit)
+    SR_FOLD(22, 13, 1), SR_FOLD(23, 0, 3),                         // depthSum, depthCount
+    SR_FOLD(24, 14, 1), SR_FOLD(25, 15, 1), SR_FOLD(26, 16, 1),    // albedoSum
+};
+#undef SR_FOLD
+static_assert(sizeof(PixAcc) == 28 * 4 && offsetof(PixAcc, s) == 0 && offsetof(PixAcc, h) == 12 * 4, "PixAcc words");
+static_assert(offsetof(PixSums, lumSum) == 6 * 4 && offsetof(PixSums, lumSurfSqr) == 9 * 4, "PixSums words");
+static_assert(offsetof(PixPrimary, normalFallback) == 3 * 4 && offsetof(PixPrimary, normalCount) == 6 * 4 &&
+              offsetof(PixPrimary, uv) == 7 * 4 && offsetof(PixPrimary, hasUv) == 9 * 4 &&
+              offsetof(PixPrimary, depthSum) == 10 * 4 && offsetof(PixPrimary, depthCount) == 11 * 4 &&
+              offsetof(PixPrimary, albedoSum) == 12 * 4, "PixPrimary words");
+
 // Renderer.RelativeStdErr (renderer.cs:232-237).
 SR_DEV float rel_std_err(float sum, float sumSqr, uint32_t n) {
     const float mean = sum / (float)n;
@@ -868,7 +895,10 @@ SR_DEV float rel_std_err(float sum, float sumSqr, uint32_t n) {
 // group of batch lanes of one wave): every thread loads its own path's radiance, flags and primary
 // hit record (coalesced), the group's first lane folds them into the pixel's sums in sample order
 // through lane shuffles, and every thread starts its own sample of the next batch.
-template <class R>
+// MULTI (more than one sample in flight): the fold runs on the group's lanes, one PixAcc word per
+// lane (kFoldJobs), over the samples' values staged in LDS rows; the sequential fold below serves
+// one sample in flight.
+template <class R, bool MULTI>
 __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     __shared__ uint32_t sh[kNextBlock / 64 + 1];
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -900,50 +930,105 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
             pr2 = F.prim[(size_t)j * 3 + 2];
         }
     }
-    // The first lane folds the group's samples in order (the others follow the shuffles).
     PixPrimary h;
     bool hLoaded = false, done = false;
     uint32_t count = 0;
-    for (uint32_t k = 0; k < B; ++k) {
-        const int src = lead + (int)k;
-        const float4 s0 = make_float4(__shfl(r0.x, src), __shfl(r0.y, src), __shfl(r0.z, src), __shfl(r0.w, src));
-        const vec3 sFog = mk3(__shfl(radFog.x, src), __shfl(radFog.y, src), __shfl(radFog.z, src));
-        const bool hitK = (__float_as_uint(s0.w) & kPrimaryHit) != 0;
-        float4 p0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p1 = p0;
-        float2 p2 = make_float2(0.0f, 0.0f);
-        if (__ballot(hitK && b == 0 && k < nb) != 0) {
-            p0 = make_float4(__shfl(pr0.x, src), __shfl(pr0.y, src), __shfl(pr0.z, src), __shfl(pr0.w, src));
-            p1 = make_float4(__shfl(pr1.x, src), __shfl(pr1.y, src), __shfl(pr1.z, src), 0.0f);
-            p2 = make_float2(__shfl(pr2.x, src), __shfl(pr2.y, src));
+    if constexpr (MULTI) {
+        // Every lane stages its sample's values in its LDS row; lane b of the group then folds PixAcc
+        // words b, b + batch, ... over the group's rows in sample order, with the operation the
+        // sequential fold applies to that word. With more than one sample in flight no sample but a
+        // pixel's last can end it (no adaptive stop), so every sample of the batch is added.
+        __shared__ float rows[kNextBlock * kFoldRow];
+        const bool hitMine = b < nb && (__float_as_uint(r0.w) & kPrimaryHit) != 0;
+        if (b < nb) {
+            float* row = rows + threadIdx.x * kFoldRow;
+            const vec3 rad = mk3(r0.x, r0.y, r0.z);
+            const float lumSurf = luminance(rad);
+            const float lum = lumSurf + luminance(radFog);
+            row[0] = rad.x; row[1] = rad.y; row[2] = rad.z;
+            row[3] = radFog.x; row[4] = radFog.y; row[5] = radFog.z;
+            row[6] = lum; row[7] = lum * lum; row[8] = lumSurf; row[9] = lumSurf * lumSurf;
+            row[10] = pr0.x; row[11] = pr0.y; row[12] = pr0.z; row[13] = pr0.w;
+            row[14] = pr1.x; row[15] = pr1.y; row[16] = pr1.z;
+            row[17] = pr2.x; row[18] = pr2.y; row[19] = __uint_as_float(1u);
         }
-        if (b != 0 || k >= nb || done) continue;
-        const uint32_t si = base + k;
-        if (hitK) {
-            if (!hLoaded) { h = A->h; hLoaded = true; }
-            const vec3 nrm = mk3(p0.x, p0.y, p0.z);
-            h.normalSum = h.normalSum + nrm;
-            h.normalFallback = nrm;
-            h.normalCount += 1;
-            h.depthSum += p0.w;
-            h.depthCount += 1;
-            h.albedoSum = h.albedoSum + mk3(p1.x, p1.y, p1.z);
-            if (!h.hasUv) { h.hasUv = 1; h.uv = mk2(p2.x, p2.y); }
+        const uint32_t groupHits = (uint32_t)(__ballot(hitMine) >> lead) & ((1u << B) - 1u);
+        // hasUv before this batch (the word's own fold lane rewrites it).
+        const bool hasUv0 = active && groupHits != 0 && A->h.hasUv != 0;
+        __syncthreads();
+        uint32_t* W = (uint32_t*)A;
+        if (active) {
+            for (uint32_t job = b; job < kFoldJobs; job += B) {
+                const uint32_t d = kFoldJob[job];
+                const uint32_t word = d & 0xffu, src = (d >> 8) & 0xffu, kind = d >> 16;
+                if (kind != kFoldAdd && groupHits == 0) continue;  // primary-hit words change only at hits
+                uint32_t acc = W[word];
+                bool has = kind == kFoldFirstHit && hasUv0;
+                const float* rk = rows + (threadIdx.x - b) * kFoldRow + src;
+                for (uint32_t k = 0; k < nb; ++k, rk += kFoldRow) {
+                    const bool hitK = ((groupHits >> k) & 1u) != 0;
+                    const float v = *rk;
+                    const uint32_t added = __float_as_uint(__uint_as_float(acc) + v);
+                    const uint32_t nv = kind <= kFoldAddHit ? added : kind == kFoldIncHit ? acc + 1u : __float_as_uint(v);
+                    const bool go = kind == kFoldAdd || (hitK && !(kind == kFoldFirstHit && has));
+                    acc = go ? nv : acc;
+                    has = has || (kind == kFoldFirstHit && hitK);
+                }
+                W[word] = acc;
+            }
         }
-        const vec3 rad = mk3(s0.x, s0.y, s0.z);
-        a.radSum = a.radSum + rad;
-        a.radFogSum = a.radFogSum + sFog;
-        const float lum = luminance(rad) + luminance(sFog);
-        a.lumSum = a.lumSum + lum;
-        a.lumSumSqr = a.lumSumSqr + lum * lum;
-        const float lumSurf = luminance(rad);
-        a.lumSurf = a.lumSurf + lumSurf;
-        a.lumSurfSqr = a.lumSurfSqr + lumSurf * lumSurf;
-        if (si >= F.minS - 1u && rel_std_err(a.lumSum, a.lumSumSqr, si + 1) < F.threshold) {
-            done = true;
-            count = si + 1;
-        } else if (si + 1 == F.maxS) {
-            done = true;
-            count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
+        __syncthreads();  // the group's words are in the pixel's record
+        done = active && base + nb >= F.maxS;
+        if (active && b == 0 && done) {
+            a = A->s;
+            h = A->h;
+            hLoaded = true;
+            const uint32_t si = base + nb - 1u;
+            count = si >= F.minS - 1u && rel_std_err(a.lumSum, a.lumSumSqr, si + 1) < F.threshold ? si + 1 : F.maxS + 1;
+        }
+    } else {
+        // The first lane folds the group's samples in order (the others follow the shuffles).
+        for (uint32_t k = 0; k < B; ++k) {
+            const int src = lead + (int)k;
+            const float4 s0 = make_float4(__shfl(r0.x, src), __shfl(r0.y, src), __shfl(r0.z, src), __shfl(r0.w, src));
+            const vec3 sFog = mk3(__shfl(radFog.x, src), __shfl(radFog.y, src), __shfl(radFog.z, src));
+            const bool hitK = (__float_as_uint(s0.w) & kPrimaryHit) != 0;
+            float4 p0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), p1 = p0;
+            float2 p2 = make_float2(0.0f, 0.0f);
+            if (__ballot(hitK && b == 0 && k < nb) != 0) {
+                p0 = make_float4(__shfl(pr0.x, src), __shfl(pr0.y, src), __shfl(pr0.z, src), __shfl(pr0.w, src));
+                p1 = make_float4(__shfl(pr1.x, src), __shfl(pr1.y, src), __shfl(pr1.z, src), 0.0f);
+                p2 = make_float2(__shfl(pr2.x, src), __shfl(pr2.y, src));
+            }
+            if (b != 0 || k >= nb || done) continue;
+            const uint32_t si = base + k;
+            if (hitK) {
+                if (!hLoaded) { h = A->h; hLoaded = true; }
+                const vec3 nrm = mk3(p0.x, p0.y, p0.z);
+                h.normalSum = h.normalSum + nrm;
+                h.normalFallback = nrm;
+                h.normalCount += 1;
+                h.depthSum += p0.w;
+                h.depthCount += 1;
+                h.albedoSum = h.albedoSum + mk3(p1.x, p1.y, p1.z);
+                if (!h.hasUv) { h.hasUv = 1; h.uv = mk2(p2.x, p2.y); }
+            }
+            const vec3 rad = mk3(s0.x, s0.y, s0.z);
+            a.radSum = a.radSum + rad;
+            a.radFogSum = a.radFogSum + sFog;
+            const float lum = luminance(rad) + luminance(sFog);
+            a.lumSum = a.lumSum + lum;
+            a.lumSumSqr = a.lumSumSqr + lum * lum;
+            const float lumSurf = luminance(rad);
+            a.lumSurf = a.lumSurf + lumSurf;
+            a.lumSurfSqr = a.lumSurfSqr + lumSurf * lumSurf;
+            if (si >= F.minS - 1u && rel_std_err(a.lumSum, a.lumSumSqr, si + 1) < F.threshold) {
+                done = true;
+                count = si + 1;
+            } else if (si + 1 == F.maxS) {
+                done = true;
+                count = F.maxS + 1;  // the loop exits with sampleIndex == maxSamples (renderer.cs:173,215)
+            }
         }
     }
     uint32_t x = 0, y = 0;
@@ -952,9 +1037,13 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     if (active && b == 0) {
         if (!done) {
             const uint32_t next = base + nb;
-            a.sampleIndex = next;
-            A->s = a;
-            if (hLoaded) A->h = h;
+            if (MULTI) {
+                A->s.sampleIndex = next;  // the sums are in place
+            } else {
+                a.sampleIndex = next;
+                A->s = a;
+                if (hLoaded) A->h = h;
+            }
             again = min(B, F.maxS - next);
         } else {
             if (!hLoaded) h = A->h;
@@ -2359,7 +2448,8 @@ void enqueue_sample(sr_ctx* c, const SliceRun& r) {
         hipLaunchKernelGGL(trace, dim3(r.gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
         mark(c, r.slice, 2);
     }
-    hipLaunchKernelGGL(k_next<R>, dim3((F.nPaths + kNextBlock - 1) / kNextBlock), dim3(kNextBlock), 0, s, F);
+    auto nextPass = F.batch > 1 ? k_next<R, true> : k_next<R, false>;
+    hipLaunchKernelGGL(nextPass, dim3((F.nPaths + kNextBlock - 1) / kNextBlock), dim3(kNextBlock), 0, s, F);
     mark(c, r.slice, 3);
 }
 
