@@ -551,8 +551,10 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
 // One path segment for every queued path (scene.cs:641-737). Direct sky light becomes a deferred
 // shadow ray whose contribution the next trace launch adds before the path's next segment, which
 // keeps the reference's summation order.
+// Waves per SIMD the fog-free shade is held to (at most 168 VGPRs); the instance with the
+// participating-medium code runs at 2.
 #ifndef SR_SHADE_MIN_WAVES
-#define SR_SHADE_MIN_WAVES 1
+#define SR_SHADE_MIN_WAVES 3
 #endif
 // FOG = false compiles the participating-medium code out for scenes without fog or media (fewer
 // registers: 3 waves per SIMD instead of 2).
@@ -560,7 +562,7 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
 // instance keeps the other segments' register count (a runtime flag cost the fog-free instance
 // its third wave per SIMD).
 template <class R, bool FOG, bool FIRST>
-__global__ void __launch_bounds__(kShadeBlock, SR_SHADE_MIN_WAVES) k_shade(Frame F, DevScene S, int cur) {
+__global__ void __launch_bounds__(kShadeBlock, FOG ? 1 : SR_SHADE_MIN_WAVES) k_shade(Frame F, DevScene S, int cur) {
     const uint32_t n = queue_len(F, cur);
     const uint32_t* Q = F.queue[cur];
     ShadeCount c;
@@ -1613,7 +1615,7 @@ struct sr_ctx {
         cnnExamples.clear();
         dCnnLoss.release();
         dRayO.release(); dRayD.release(); dHitA.release(); dHitSlot.release();
-        dPaths.release(); dShadows.release(); dAcc.release();
+        dPaths.release(); dCold.release(); dPrim.release(); dShadows.release(); dAcc.release();
         dQueue0.release(); dQueue1.release(); dCounts.release(); dCtr.release();
         dRad.release(); dRadFog.release(); dNormal.release(); dUv.release(); dDepth.release(); dVar.release();
         dAlbedo.release();
