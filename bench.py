@@ -417,8 +417,8 @@ def main():
     ap.add_argument("--light-sampler", choices=["cdf", "alias"], default="cdf",
                     help="environment light sampling: the reference's row/column CDFs or the alias table")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--in-flight", type=int, default=16,
-                    help="samples of every pixel in flight at once with the counter policy (1 to 16)")
+    ap.add_argument("--in-flight", type=int, default=32,
+                    help="samples of every pixel in flight at once with the counter policy (1 to 32)")
     ap.add_argument("--rng", choices=["counter", "xorwow"], default="counter",
                     help="random numbers of the plain workload: the counter-based policy (Philox keyed by "
                          "pixel and sample; samples of a pixel independent) or the reference's per-pixel xorwow "

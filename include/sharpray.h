@@ -355,8 +355,8 @@ int sr_set_tail_segment(sr_ctx* ctx, uint32_t segment);
  * environment variable SHARPRAY_SLICES sets the default of new contexts. No counterpart in the
  * reference (its worker threads claim 32x32 blocks, renderer.cs:122-151). */
 int sr_set_slices(sr_ctx* ctx, uint32_t slices);
-/* Later renders keep up to `samples` samples of every pixel in flight at once (1 to 16, default 16;
- * the largest power of two not above it, the sample count and a 16 GiB budget is used) when the
+/* Later renders keep up to `samples` samples of every pixel in flight at once (1 to 32, default 32;
+ * the largest power of two not above it, the sample count and a 32 GiB budget is used) when the
  * samples are independent: counter RNG policy, a fixed sample count (min = max or threshold 0) and
  * counters off; other renders run one sample at a time. Every launch then holds that many samples
  * of every pixel, so the fixed cost of a launch's slowest waves is paid once per batch of samples. Each pixel still folds its samples into its sums in sample order, so results

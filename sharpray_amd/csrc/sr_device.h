@@ -7,8 +7,8 @@
 //    record that a wave fetches with four 16-byte loads.
 //  - tris: TriangleLean (PosA, AB, AC; 36 B) stored in BVH item order, so a leaf's triangles are
 //    contiguous; slot_tri maps an item slot back to the mesh's triangle index.
-//  - attrs: TriangleAttributes (mesh.cs:6-11; 27 floats) in triangle order, read once per
-//    closest hit.
+//  - attrs: TriangleAttributes (mesh.cs:6-11; 27 floats) in the same item order, read once per
+//    closest hit at the hit's item slot.
 //  - objects: transform, material and shape per Object (scene.cs:204-223).
 //  - texels / textures: linear RGB texels (12 B) and per-texture descriptors.
 //  - cdf: the sky's marginal and conditional CDF tables (cdf.cs:62-98), then their bucket records.

@@ -126,15 +126,15 @@ constexpr int kCtrSpread = 16;  // copies of the counter words (a power of two)
 // the next-sample pass folds into the pixel's sums.
 constexpr uint32_t kPrimaryHit = 4u;
 constexpr int kProgRing = 4;    // samples an adaptive render's driver runs ahead of its progress reads
-constexpr uint32_t kMaxInFlight = 16;  // samples of a pixel in flight at most
+constexpr uint32_t kMaxInFlight = 32;  // samples of a pixel in flight at most
 constexpr uint32_t kTailAuto = 0xffffffffu;
 // Device bytes per path slot (rays, hit, path record, two shadow records, two queue entries,
 // primary-hit record) and the most the path slots of one render may take: samples in flight are
-// capped to fit (a 1080p frame holds 16 samples per pixel in 12 GB).
+// capped to fit (a 1080p frame holds 32 samples per pixel in 24 GB).
 constexpr size_t kPathSlotBytes = 3 * sizeof(float4) + sizeof(int32_t) + sizeof(PathRec) + sizeof(PathCold) +
                                   2 * sizeof(ShadowRec) + 2 * sizeof(uint32_t) + 3 * sizeof(float4);
 static_assert(sizeof(PathRec) == 64 && sizeof(PathCold) == 32 && sizeof(ShadowRec) == 80, "record sizes");
-constexpr size_t kMaxPathBytes = (size_t)16 << 30;
+constexpr size_t kMaxPathBytes = (size_t)32 << 30;
 
 // Queue counters (uint32 each), per slice. kOverflow collects the queues an append found full (bit
 // 0: ray queues, bit 1: shadow queues); the append is then dropped and the host fails the render.
@@ -954,7 +954,7 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
             row[14] = pr1.x; row[15] = pr1.y; row[16] = pr1.z;
             row[17] = pr2.x; row[18] = pr2.y; row[19] = __uint_as_float(1u);
         }
-        const uint32_t groupHits = (uint32_t)(__ballot(hitMine) >> lead) & ((1u << B) - 1u);
+        const uint32_t groupHits = (uint32_t)((__ballot(hitMine) >> lead) & ((1ull << B) - 1ull));
         // hasUv before this batch (the word's own fold lane rewrites it).
         const bool hasUv0 = active && groupHits != 0 && A->h.hasUv != 0;
         __syncthreads();
@@ -2171,15 +2171,13 @@ int sr_scene_build(sr_ctx* ctx) {
         const int32_t base = (int32_t)slotTri.size();
         ctx->meshTriBase[m] = base;
         const size_t n = hm.bvh.items.size();
+        // Triangles and their attributes both in item order: a hit's slot addresses both.
         for (size_t s = 0; s < n; ++s) {
             const int32_t t = hm.bvh.items[s];
             slotTri.push_back(t);
             const float* src = hm.tris.data() + (size_t)t * 36;
             tris.insert(tris.end(), src, src + 9);
-        }
-        for (size_t t = 0; t < n; ++t) {
-            const float* src = hm.tris.data() + t * 36 + 9;
-            attrs.insert(attrs.end(), src, src + 27);
+            attrs.insert(attrs.end(), src + 9, src + 36);
         }
     }
     int32_t sceneBase = 0;

@@ -494,8 +494,7 @@ SR_DEV void make_surface(const DevScene& S, const ray3& r, const HitRec& h, Surf
     const DevObject& ob = S.objects[h.obj];
     ShapeHitD sh;
     if (ob.kind == 0) {
-        const int32_t tri = S.slotTri[ob.triBase + h.slot];
-        inflate(S.tris + (size_t)(ob.triBase + h.slot) * 9, S.attrs + (size_t)(ob.triBase + tri) * 27, h.u, h.v, sh);
+        inflate(S.tris + (size_t)(ob.triBase + h.slot) * 9, S.attrs + (size_t)(ob.triBase + h.slot) * 27, h.u, h.v, sh);
     } else {
         float scale;
         const ray3 lr = to_object(ob, r, scale);

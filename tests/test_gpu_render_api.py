@@ -153,21 +153,24 @@ def test_failed_render_start_leaves_no_render():
     assert np.array_equal(bits(r2.radiance), bits(first))
 
 
-@pytest.mark.parametrize("case", ["plain-2", "plain-3-tail0", "plain-5-slices2", "plain-16", "fog-4"])
+@pytest.mark.parametrize("case", ["plain-2", "plain-3-tail0", "plain-5-slices2", "plain-16", "fog-4",
+                                  "plain-32-spp40", "plain-16-spp24"])
 def test_samples_in_flight_match_one_sample_at_a_time(case):
     """Several samples of every pixel in flight (counter RNG, fixed sample count) give AOVs
     bit-identical to running the samples one at a time: each sample draws from its own (pixel,
     sample) counter stream and each pixel folds its samples into its sums in sample order. Covers a
     sample count that the batch does not divide (7 spp), every path in the fused tail launch from
     its camera ray on (tail segment 0), concurrent row slices, more samples in flight than samples,
-    and grid fog (camera rays that scatter in the medium record no primary surface)."""
+    grid fog (camera rays that scatter in the medium record no primary surface), and batches of 32
+    and 16 samples followed by a partial batch (40 and 24 spp)."""
     from sharpray_amd import RNG_COUNTER, Renderer, scenes
     kind, k = case.split("-")[:2]
     k = int(k)
+    spp = int(case.split("spp")[1]) if "spp" in case else 7
     if kind == "fog":
-        spec = scenes.config_fog(scale=0.1, spp=7)
+        spec = scenes.config_fog(scale=0.1, spp=spp)
     else:
-        spec = scenes.config_plain_gpu(scale=0.25, spp=7)
+        spec = scenes.config_plain_gpu(scale=0.25, spp=spp)
         spec.height = 268
     scene, view = scenes.instantiate(spec)
     if "tail0" in case:
