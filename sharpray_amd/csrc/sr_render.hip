@@ -967,6 +967,7 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
                 uint32_t acc = W[word];
                 bool has = kind == kFoldFirstHit && hasUv0;
                 const float* rk = rows + (threadIdx.x - b) * kFoldRow + src;
+#pragma unroll 8
                 for (uint32_t k = 0; k < nb; ++k, rk += kFoldRow) {
                     const bool hitK = ((groupHits >> k) & 1u) != 0;
                     const float v = *rk;
