@@ -175,6 +175,10 @@ struct Frame {
     // of the batch + b) of pixel slot q; nPaths = batch * nSlots is the capacity of every ray and
     // shadow queue.
     uint32_t batch, nPaths;
+    // camSlots: with a batch in flight (every pixel of the band runs the same samples), k_start and
+    // k_next write the camera-ray queue as every path slot in slot order, kNoPath where the slot
+    // starts no sample, without a queue atomic; the segment-0 trace and shade skip kNoPath.
+    uint32_t camSlots;
     uint32_t fog;  // the scene has fog or a medium: paths carry fog radiance
     uint32_t minS, maxS, bounces, seed;
     float threshold, clamp;
@@ -190,6 +194,9 @@ SR_DEV bool slot_pixel(const Frame& F, uint32_t q, uint32_t& x, uint32_t& y) {
 }
 
 SR_DEV uint32_t lane_id() { return __lane_id(); }
+
+// A camera-ray queue entry of a slot that starts no sample (Frame::camSlots).
+constexpr uint32_t kNoPath = 0xffffffffu;
 
 // Queue appends are bounded by the queue's capacity (nPaths entries per queue and per shadow
 // buffer): an append that finds the queue full is dropped and flagged in kOverflow, which the host
@@ -415,6 +422,11 @@ __global__ void __launch_bounds__(kBlock) k_start(Frame F) {
     }
     bump_wave(F, 0, live && b == 0);
     bump_wave(F, 1, live);
+    if (F.camSlots) {  // the queue in slot order, no atomic
+        if (j < F.nPaths) F.queue[0][j] = live ? j : kNoPath;
+        if (j == 0) F.counts[kRayCount0] = F.nPaths;
+        return;
+    }
     const uint32_t slot = block_append<kBlock / 64>(&F.counts[kRayCount0], live, sh);
     if (live && queue_room(F, slot, 1u)) F.queue[0][slot] = j;
 }
@@ -569,6 +581,7 @@ __global__ void __launch_bounds__(kShadeBlock, FOG ? 1 : SR_SHADE_MIN_WAVES) k_s
     const uint32_t stride = gridDim.x * blockDim.x;
     __shared__ uint32_t sQ[kShadeBlock];
     __shared__ uint32_t sHits[kShadeBlock / 64];
+    __shared__ uint32_t sItems[kShadeBlock / 64];
     __shared__ uint32_t shApp[(kShadeBlock / 64) * 8 + 1];
     const uint32_t wave = threadIdx.x >> 6;
     // Grid-stride over the queue with a block-uniform base, so every lane of a wave runs the same
@@ -578,25 +591,37 @@ __global__ void __launch_bounds__(kShadeBlock, FOG ? 1 : SR_SHADE_MIN_WAVES) k_s
         // Hits first, then misses, within the block's chunk of the queue (each class in queue
         // order): most waves then run only one of the two shading paths (83 -> 67 ms of shade per
         // 1080p frame). Every path is shaded exactly once whatever lane takes it.
+        // The chunk's paths: its queue entries, but for kNoPath entries (camera rays queued in slot
+        // order, Frame::camSlots), which only a segment-0 queue holds.
+        uint32_t items = 0;
         {
-            uint32_t qj = 0;
+            const uint32_t qj = i < n ? Q[i] : kNoPath;
+            const bool valid = qj != kNoPath;
             bool isHit = false;
-            if (i < n) {
-                qj = Q[i];
-                isHit = __float_as_int(F.hitA[qj].w) >= 0;
-            }
+            if (valid) isHit = __float_as_int(F.hitA[qj].w) >= 0;
             const unsigned long long hm = __ballot(isHit);
-            if (lane_id() == 0) sHits[wave] = (uint32_t)__popcll(hm);
+            const unsigned long long vm = FIRST ? __ballot(valid) : 0ull;
+            if (lane_id() == 0) {
+                sHits[wave] = (uint32_t)__popcll(hm);
+                if (FIRST) sItems[wave] = (uint32_t)__popcll(vm);
+            }
             __syncthreads();
-            uint32_t hitsBefore = 0, hitsTotal = 0;
+            uint32_t hitsBefore = 0, hitsTotal = 0, itemsBefore = 0;
             for (uint32_t w = 0; w < kShadeBlock / 64; ++w) {
                 const uint32_t hw = sHits[w];
                 hitsTotal += hw;
                 if (w < wave) hitsBefore += hw;
+                if (FIRST) {
+                    items += sItems[w];
+                    if (w < wave) itemsBefore += sItems[w];
+                }
             }
-            if (i < n) {
-                const uint32_t hitRank = hitsBefore + (uint32_t)__popcll(hm & ((1ull << lane_id()) - 1ull));
-                sQ[isHit ? hitRank : hitsTotal + (threadIdx.x - hitRank)] = qj;
+            if (!FIRST) items = min(n - base, (uint32_t)kShadeBlock);
+            if (valid) {
+                const unsigned long long below = (1ull << lane_id()) - 1ull;
+                const uint32_t hitRank = hitsBefore + (uint32_t)__popcll(hm & below);
+                const uint32_t itemRank = FIRST ? itemsBefore + (uint32_t)__popcll(vm & below) : threadIdx.x;
+                sQ[isHit ? hitRank : hitsTotal + (itemRank - hitRank)] = qj;
             }
             __syncthreads();
         }
@@ -604,7 +629,7 @@ __global__ void __launch_bounds__(kShadeBlock, FOG ? 1 : SR_SHADE_MIN_WAVES) k_s
         ShadowRec sr;
         uint32_t q = 0;
         vec3 nd = splat3(0.0f);
-        if (i < n) {
+        if (threadIdx.x < items) {
             q = sQ[threadIdx.x];
             PathRec p = FIRST ? start_path<R>(F, q) : F.paths[q];
             const float4 ho = F.hitA[q];
@@ -793,8 +818,8 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
     for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < total; base += stride) {
         const uint32_t i = base + lane_id();
         if (base < nE64) {
-            if (i < nE) {
-                const uint32_t q = Q[i];
+            const uint32_t q = i < nE ? Q[i] : kNoPath;
+            if (q != kNoPath) {
                 const float4 o = F.rayO[q], d = F.rayD[q];
                 const ray3 r = mk_ray(mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z));
                 HitRec h;
@@ -1105,6 +1130,11 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     // One atomic per block: one per wave on the single queue counter serialised the launch
     // (26.1 -> 14.0 ms per 1080p frame). A pixel's samples of the batch sit together.
     bump_wave(F, 1, start);
+    if (F.camSlots) {  // the queue in slot order, no atomic
+        if (j < F.nPaths) F.queue[0][j] = start ? j : kNoPath;
+        if (j == 0) F.counts[kRayCount0] = F.nPaths;
+        return;
+    }
     const uint32_t slot = block_append<kNextBlock / 64>(&F.counts[kRayCount0], start, sh);
     if (start && queue_room(F, slot, 1u)) F.queue[0][slot] = j;
 }
@@ -2280,6 +2310,12 @@ ViewD make_view(const sr_view* v, uint32_t W, uint32_t H) {
 }
 
 // The frame state of rows [y0, y0 + rows) run as slice `slice` (its own part of every buffer).
+// Segment from which the fused tail launch finishes every path (SHARPRAY_TAIL_SEGMENT /
+// sr_set_tail_segment, else 3 with one or a few samples in flight and 4 with 8 or more).
+uint32_t tail_segment(const sr_ctx* c, uint32_t batch) {
+    return c->tailSegment != kTailAuto ? c->tailSegment : (batch >= 8 ? 4u : 3u);
+}
+
 Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     Frame F;
     memset(&F, 0, sizeof(F));
@@ -2316,6 +2352,9 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     F.nSlots = F.tilesX * ((rows + 7) / 8) * 64;
     F.batch = c->batch;
     F.nPaths = F.nSlots * F.batch;
+    // A batch's camera rays can be queued in slot order when every pixel runs the batch's samples
+    // (more than one in flight) and a segment-0 trace and shade consume them (not the fused tail).
+    F.camSlots = c->batch > 1 && tail_segment(c, c->batch) > 0 ? 1u : 0u;
     F.fog = c->dev.fog.enabled ? 1u : 0u;
     F.minS = c->params.min_samples;
     F.maxS = c->params.max_samples;
@@ -2430,7 +2469,7 @@ void enqueue_sample(sr_ctx* c, const SliceRun& r) {
     auto shadeFirst = c->dev.fog.enabled ? k_shade<R, true, true> : k_shade<R, false, true>;
     int cur = 0;
     bool fused = false;
-    const uint32_t tailSeg = c->tailSegment != kTailAuto ? c->tailSegment : (F.batch >= 8 ? 4u : 3u);
+    const uint32_t tailSeg = tail_segment(c, F.batch);
     for (uint32_t seg = 0; seg <= F.bounces; ++seg) {
         hipLaunchKernelGGL(trace, dim3(r.gTrace), dim3(kTraceBlock), trav_lds_bytes(kTraceBlock), s, F, c->dev, cur);
         mark(c, r.slice, 0);
