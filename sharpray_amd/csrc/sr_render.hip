@@ -310,6 +310,17 @@ SR_DEV void rng_store_words(PathRec* p, const CounterRng& r) {
     *(uint4*)p->rng = make_uint4(r.key0, r.key1, r.sample, r.draw);
 }
 
+// The record of a path that ends in this segment: only what the next-sample pass reads (radiance
+// and flags, 16 of its 64 bytes) and, for the xorwow stream, the random state the pixel's next
+// sample continues from; a counter generator's next sample starts from its own counter.
+SR_DEV void store_finished(const Frame& F, uint32_t q, const PathRec& p, const CounterRng&) {
+    *(float4*)&F.paths[q].rad = make_float4(p.rad.x, p.rad.y, p.rad.z, __uint_as_float(p.flags));
+}
+SR_DEV void store_finished(const Frame& F, uint32_t q, const PathRec& p, const XorwowRng& r) {
+    *(float4*)&F.paths[q].rad = make_float4(p.rad.x, p.rad.y, p.rad.z, __uint_as_float(p.flags));
+    rng_store_words(F.paths + q, r);
+}
+
 // Camera ray of one sample: jitter X then Y (renderer.cs:175), View.Ray (math.cs:1613-1636).
 template <class R>
 SR_DEV void camera_ray(const ViewD& v, uint32_t W, uint32_t H, uint32_t x, uint32_t y, R& rng, vec3& o, vec3& d) {
@@ -645,8 +656,12 @@ __global__ void __launch_bounds__(kShadeBlock, FOG ? 1 : SR_SHADE_MIN_WAVES) k_s
                 F.rayO[q] = make_float4(no.x, no.y, no.z, 0.0f);
                 F.rayD[q] = make_float4(nd.x, nd.y, nd.z, 0.0f);
             }
-            rng_store(p, rng);
-            F.paths[q] = p;
+            if (cont) {
+                rng_store(p, rng);
+                F.paths[q] = p;
+            } else {
+                store_finished(F, q, p, rng);
+            }
         }
         if (shadow) {
             if (sr.path & kCountOnly) ++c.occCulled;
@@ -767,8 +782,7 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
             scene_closest<COUNT>(S, ray, h, tl, lds_column());
             ++traces;
         }
-        rng_store(p, rng);
-        F.paths[q] = p;
+        store_finished(F, q, p, rng);
         if (F.fog) F.cold[q].radFog = radFog;
     }
     if (F.ctr) {
