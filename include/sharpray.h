@@ -344,7 +344,8 @@ int sr_counters_read_fused(sr_ctx* ctx, int64_t* out);
  * of a render without counters is the totals minus this share. No counterpart in the reference. */
 int sr_counters_read_reference_only(sr_ctx* ctx, int64_t* out);
 /* Later renders trace segments 0..segment-1 as separate wavefront launches and finish every path
- * from `segment` on in one fused launch (by default 3, or 4 with 8 or more samples in flight; a
+ * from `segment` on in one fused launch (by default 3, or 4 with 8 or more samples in flight, and
+ * in scenes with fog or a medium 5, or 6 with 4 or more samples in flight; a
  * value above the bounce count disables it; results are identical either way). The environment variable SHARPRAY_TAIL_SEGMENT sets the
  * default of new contexts. No counterpart in the reference. */
 int sr_set_tail_segment(sr_ctx* ctx, uint32_t segment);

@@ -1518,10 +1518,10 @@ struct sr_ctx {
     int device = 0;
     int bvhBuilder = SR_BVH_BUILDER_HOST;
     // Segment from which a render finishes every path in one fused launch (k_tail); beyond the
-    // bounce count it never does. SHARPRAY_TAIL_SEGMENT overrides the default.
-    // Segment from which the fused tail launch finishes every path; kTailAuto picks 3 with one or a
-    // few samples in flight and 4 with 8 or more (1080p frame: 3 / 4 / 5 / 6 with one sample
-    // 240 / 261 / 278 / - ms; with 16 in flight 124.6 / 119.0 / 120.7 / 121.3 ms).
+    // bounce count it never does. SHARPRAY_TAIL_SEGMENT overrides the default, kTailAuto
+    // (tail_segment: 1080p frame with 3 / 4 / 5 / 6: one sample in flight 240 / 261 / 278 / - ms,
+    // 16 in flight 124.6 / 119.0 / 120.7 / 121.3 ms; with grid fog, whose paths scatter and live
+    // longer, 4 in flight 29.2 / 23.6 / 22.5 / 22.1 ms and one in flight 34.2 / 30.1 / 29.6 / 30.1 ms).
     uint32_t tailSegment = kTailAuto;
     hipStream_t stream = nullptr;
     std::vector<HostTexture> textures;
@@ -2325,9 +2325,12 @@ ViewD make_view(const sr_view* v, uint32_t W, uint32_t H) {
 
 // The frame state of rows [y0, y0 + rows) run as slice `slice` (its own part of every buffer).
 // Segment from which the fused tail launch finishes every path (SHARPRAY_TAIL_SEGMENT /
-// sr_set_tail_segment, else 3 with one or a few samples in flight and 4 with 8 or more).
+// sr_set_tail_segment, else 3 with one or a few samples in flight and 4 with 8 or more; in scenes
+// with fog or a medium 5 with one or two in flight and 6 with more).
 uint32_t tail_segment(const sr_ctx* c, uint32_t batch) {
-    return c->tailSegment != kTailAuto ? c->tailSegment : (batch >= 8 ? 4u : 3u);
+    if (c->tailSegment != kTailAuto) return c->tailSegment;
+    if (c->dev.fog.enabled) return batch >= 4 ? 6u : 5u;
+    return batch >= 8 ? 4u : 3u;
 }
 
 Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
