@@ -179,6 +179,10 @@ struct Frame {
     // k_next write the camera-ray queue as every path slot in slot order, kNoPath where the slot
     // starts no sample, without a queue atomic; the segment-0 trace and shade skip kNoPath.
     uint32_t camSlots;
+    // First sample of the batch (set for the segment-0 shade of a camSlots render): with the counter
+    // generator a camera ray's random state is then (seed, pixel, sample0 + b, 2 draws) and is not
+    // stored by k_start / k_next nor read back.
+    uint32_t sample0;
     uint32_t fog;  // the scene has fog or a medium: paths carry fog radiance
     uint32_t minS, maxS, bounces, seed;
     float threshold, clamp;
@@ -338,6 +342,12 @@ SR_DEV void camera_ray(const ViewD& v, uint32_t W, uint32_t H, uint32_t x, uint3
 
 // Starts sample `sample` of path slot j: its camera ray and the random state after the ray's two
 // jitter draws. The shade of segment 0 starts the path record's other fields (start_path).
+// Whether the random state after a camera ray is derived by the segment-0 shade (counter
+// generator in a camSlots render) rather than stored.
+template <class R> SR_DEV bool camera_rng_derived(const Frame& F);
+template <> SR_DEV bool camera_rng_derived<XorwowRng>(const Frame&) { return false; }
+template <> SR_DEV bool camera_rng_derived<CounterRng>(const Frame& F) { return F.camSlots != 0; }
+
 template <class R>
 SR_DEV void begin_sample(const Frame& F, size_t j, uint32_t x, uint32_t y, uint32_t sample) {
     R rng = rng_for_sample<R>(F, j, x, y, sample);
@@ -345,11 +355,12 @@ SR_DEV void begin_sample(const Frame& F, size_t j, uint32_t x, uint32_t y, uint3
     camera_ray(F.view, F.W, F.H, x, y, rng, o, d);
     F.rayO[j] = make_float4(o.x, o.y, o.z, 0.0f);
     F.rayD[j] = make_float4(d.x, d.y, d.z, 0.0f);
-    rng_store_words(F.paths + j, rng);
+    if (!camera_rng_derived<R>(F)) rng_store_words(F.paths + j, rng);
 }
 
 // The record of a path at its camera ray (Scene.Sample's initial state, scene.cs:632-640) with the
-// random state begin_sample stored.
+// random state begin_sample left: stored, or for a derived counter state the generator of
+// (pixel, sample0 + b) after camera_ray's two draws.
 template <class R>
 SR_DEV PathRec start_path(const Frame& F, uint32_t j) {
     PathRec p;
@@ -357,8 +368,19 @@ SR_DEV PathRec start_path(const Frame& F, uint32_t j) {
     p.scatterPdf = 0.0f;
     p.rad = splat3(0.0f);
     p.flags = 0;
-    const uint32_t* w = F.paths[j].rng;
-    for (int k = 0; k < 5; ++k) p.rng[k] = w[k];
+    if (camera_rng_derived<R>(F)) {
+        const uint32_t q = j / F.batch, b = j - q * F.batch;
+        uint32_t x, y;
+        slot_pixel(F, q, x, y);
+        p.rng[0] = F.seed;
+        p.rng[1] = y * F.W + x;
+        p.rng[2] = F.sample0 + b;
+        p.rng[3] = 2;
+        p.rng[4] = 0;
+    } else {
+        const uint32_t* w = F.paths[j].rng;
+        for (int k = 0; k < 5; ++k) p.rng[k] = w[k];
+    }
     p.seg = 0;
     p.pad[0] = p.pad[1] = 0;
     if (F.fog) F.cold[j].radFog = splat3(0.0f);  // the medium is written when a path enters one
@@ -2372,6 +2394,7 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     // A batch's camera rays can be queued in slot order when every pixel runs the batch's samples
     // (more than one in flight) and a segment-0 trace and shade consume them (not the fused tail).
     F.camSlots = c->batch > 1 && tail_segment(c, c->batch) > 0 ? 1u : 0u;
+    F.sample0 = 0;
     F.fog = c->dev.fog.enabled ? 1u : 0u;
     F.minS = c->params.min_samples;
     F.maxS = c->params.max_samples;
@@ -2477,8 +2500,10 @@ void enqueue_start(sr_ctx* c, const SliceRun& r) {
 // tail segment on in one fused launch), then the accumulate / next-camera-ray pass. Queues that run
 // empty (converged pixels, terminated paths) make the remaining launches exit at once.
 template <class R, bool COUNT>
-void enqueue_sample(sr_ctx* c, const SliceRun& r) {
+void enqueue_sample(sr_ctx* c, const SliceRun& r, uint32_t smp) {
     const Frame& F = r.F;
+    Frame F0 = F;  // the segment-0 shade's view: the batch's first sample
+    F0.sample0 = smp;
     const hipStream_t s = r.s;
     auto trace = c->dev.anyTransparent ? k_trace<COUNT, true> : k_trace<COUNT, false>;
     auto tail = c->dev.anyTransparent ? k_tail<R, COUNT, true> : k_tail<R, COUNT, false>;
@@ -2497,7 +2522,8 @@ void enqueue_sample(sr_ctx* c, const SliceRun& r) {
             fused = true;
             break;
         }
-        hipLaunchKernelGGL(seg == 0 ? shadeFirst : shade, dim3(r.gShade), dim3(kShadeBlock), 0, s, F, c->dev, cur);
+        hipLaunchKernelGGL(seg == 0 ? shadeFirst : shade, dim3(r.gShade), dim3(kShadeBlock), 0, s, seg == 0 ? F0 : F,
+                           c->dev, cur);
         mark(c, r.slice, 1);
         cur ^= 1;
     }
@@ -2558,7 +2584,7 @@ int run_frame(sr_ctx* c) {
             return SR_OK;
         };
         for (uint32_t smp = 0; smp < c->params.max_samples && !bandIdle; smp += c->batch) {
-            for (const SliceRun& r : runs) enqueue_sample<R, COUNT>(c, r);
+            for (const SliceRun& r : runs) enqueue_sample<R, COUNT>(c, r, smp);
             c->iterations++;
             if (!adaptive) continue;
             if (pending.size() == (size_t)kProgRing) {
