@@ -54,7 +54,10 @@ extern __shared__ uint2 sr_lds[];
 SR_DEV uint2* lds_column() { return sr_lds + (threadIdx.x >> 6) * (kLdsDepth * 64) + (threadIdx.x & 63); }
 
 // ---- Frame state --------------------------------------------------------------------------------
-struct alignas(16) PathRec {  // 64 B per path: what every segment of a path reads and writes
+// A path's state while a kernel works on it (registers, 64 B). In memory it is split into PathHot
+// (48 B: what only the segments of the path read and write) and PathRad (16 B, a dense array: the
+// radiance and flags, which the shadow rays update and the next-sample pass reads in slot order).
+struct alignas(16) PathRec {
     vec3 energy;
     float scatterPdf;
     vec3 rad;
@@ -62,6 +65,17 @@ struct alignas(16) PathRec {  // 64 B per path: what every segment of a path rea
     uint32_t rng[5];
     uint32_t seg;    // segment index of the queued ray (0 = camera ray)
     uint32_t pad[2];
+};
+struct alignas(16) PathHot {
+    vec3 energy;
+    float scatterPdf;
+    uint32_t rng[5];
+    uint32_t seg;
+    uint32_t pad[2];
+};
+struct alignas(16) PathRad {
+    vec3 rad;
+    uint32_t flags;
 };
 // The rest of a path's state, touched only where it is used: the fog radiance (scenes with fog or
 // a medium) and the interior medium's albedo (paths inside a transparent object).
@@ -131,9 +145,10 @@ constexpr uint32_t kTailAuto = 0xffffffffu;
 // Device bytes per path slot (rays, hit, path record, two shadow records, two queue entries,
 // primary-hit record) and the most the path slots of one render may take: samples in flight are
 // capped to fit (a 1080p frame holds 32 samples per pixel in 24 GB).
-constexpr size_t kPathSlotBytes = 3 * sizeof(float4) + sizeof(int32_t) + sizeof(PathRec) + sizeof(PathCold) +
-                                  2 * sizeof(ShadowRec) + 2 * sizeof(uint32_t) + 3 * sizeof(float4);
-static_assert(sizeof(PathRec) == 64 && sizeof(PathCold) == 32 && sizeof(ShadowRec) == 80, "record sizes");
+constexpr size_t kPathSlotBytes = 3 * sizeof(float4) + sizeof(int32_t) + sizeof(PathHot) + sizeof(PathRad) +
+                                  sizeof(PathCold) + 2 * sizeof(ShadowRec) + 2 * sizeof(uint32_t) + 3 * sizeof(float4);
+static_assert(sizeof(PathRec) == 64 && sizeof(PathHot) == 48 && sizeof(PathRad) == 16 && sizeof(PathCold) == 32 &&
+              sizeof(ShadowRec) == 80, "record sizes");
 constexpr size_t kMaxPathBytes = (size_t)32 << 30;
 
 // Queue counters (uint32 each), per slice. kOverflow collects the queues an append found full (bit
@@ -147,7 +162,8 @@ struct Frame {
     float4* rayD;       // direction per slot (xyz)
     float4* hitA;       // t, u, v, object (int bits)
     int32_t* hitSlot;   // mesh item slot
-    PathRec* paths;
+    PathHot* paths;
+    PathRad* prad;
     PathCold* cold;
     ShadowRec* shadows[2];  // shadow rays of even / odd segments
     PixAcc* acc;
@@ -307,10 +323,10 @@ template <> SR_DEV CounterRng rng_for_sample<CounterRng>(const Frame& F, size_t,
     r.draw = 0;
     return r;
 }
-SR_DEV void rng_store_words(PathRec* p, const XorwowRng& r) {
+SR_DEV void rng_store_words(PathHot* p, const XorwowRng& r) {
     p->rng[0] = r.s0; p->rng[1] = r.s1; p->rng[2] = r.s2; p->rng[3] = r.s3; p->rng[4] = r.s4;
 }
-SR_DEV void rng_store_words(PathRec* p, const CounterRng& r) {
+SR_DEV void rng_store_words(PathHot* p, const CounterRng& r) {
     *(uint4*)p->rng = make_uint4(r.key0, r.key1, r.sample, r.draw);
 }
 
@@ -318,11 +334,38 @@ SR_DEV void rng_store_words(PathRec* p, const CounterRng& r) {
 // and flags, 16 of its 64 bytes) and, for the xorwow stream, the random state the pixel's next
 // sample continues from; a counter generator's next sample starts from its own counter.
 SR_DEV void store_finished(const Frame& F, uint32_t q, const PathRec& p, const CounterRng&) {
-    *(float4*)&F.paths[q].rad = make_float4(p.rad.x, p.rad.y, p.rad.z, __uint_as_float(p.flags));
+    F.prad[q] = PathRad{p.rad, p.flags};
 }
 SR_DEV void store_finished(const Frame& F, uint32_t q, const PathRec& p, const XorwowRng& r) {
-    *(float4*)&F.paths[q].rad = make_float4(p.rad.x, p.rad.y, p.rad.z, __uint_as_float(p.flags));
+    F.prad[q] = PathRad{p.rad, p.flags};
     rng_store_words(F.paths + q, r);
+}
+
+// A path's state from its two records, and back.
+SR_DEV PathRec load_path(const Frame& F, uint32_t j) {
+    const PathHot h = F.paths[j];
+    const PathRad r = F.prad[j];
+    PathRec p;
+    p.energy = h.energy;
+    p.scatterPdf = h.scatterPdf;
+    p.rad = r.rad;
+    p.flags = r.flags;
+    for (int k = 0; k < 5; ++k) p.rng[k] = h.rng[k];
+    p.seg = h.seg;
+    p.pad[0] = h.pad[0];
+    p.pad[1] = h.pad[1];
+    return p;
+}
+SR_DEV void store_path(const Frame& F, uint32_t j, const PathRec& p) {
+    PathHot h;
+    h.energy = p.energy;
+    h.scatterPdf = p.scatterPdf;
+    for (int k = 0; k < 5; ++k) h.rng[k] = p.rng[k];
+    h.seg = p.seg;
+    h.pad[0] = p.pad[0];
+    h.pad[1] = p.pad[1];
+    F.paths[j] = h;
+    F.prad[j] = PathRad{p.rad, p.flags};
 }
 
 // Camera ray of one sample: jitter X then Y (renderer.cs:175), View.Ray (math.cs:1613-1636).
@@ -664,7 +707,7 @@ __global__ void __launch_bounds__(kShadeBlock, FOG ? 1 : SR_SHADE_MIN_WAVES) k_s
         vec3 nd = splat3(0.0f);
         if (threadIdx.x < items) {
             q = sQ[threadIdx.x];
-            PathRec p = FIRST ? start_path<R>(F, q) : F.paths[q];
+            PathRec p = FIRST ? start_path<R>(F, q) : load_path(F, q);
             const float4 ho = F.hitA[q];
             const float4 ro = F.rayO[q], rd = F.rayD[q];
             HitRec h;
@@ -680,7 +723,7 @@ __global__ void __launch_bounds__(kShadeBlock, FOG ? 1 : SR_SHADE_MIN_WAVES) k_s
             }
             if (cont) {
                 rng_store(p, rng);
-                F.paths[q] = p;
+                store_path(F, q, p);
             } else {
                 store_finished(F, q, p, rng);
             }
@@ -756,9 +799,9 @@ SR_DEV void shadow_item(const Frame& F, const DevScene& S, const ShadowRec& sr, 
         add_direct(S, sr, tr, rad, radFog);
         F.cold[j].radFog = radFog;
     } else {
-        vec3 rad = F.paths[j].rad, radFog;
+        vec3 rad = F.prad[j].rad, radFog;
         add_direct(S, sr, tr, rad, radFog);
-        F.paths[j].rad = rad;
+        F.prad[j].rad = rad;
     }
 }
 
@@ -779,7 +822,7 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
     uint32_t extra = 0, traces = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t q = Q[i];
-        PathRec p = first ? start_path<R>(F, q) : F.paths[q];
+        PathRec p = first ? start_path<R>(F, q) : load_path(F, q);
         vec3 radFog = F.fog ? F.cold[q].radFog : splat3(0.0f);
         R rng = rng_load<R>(p);
         const float4 ho = F.hitA[q];
@@ -985,7 +1028,7 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     float4 r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), pr0 = r0, pr1 = r0, pr2 = r0;
     vec3 radFog = splat3(0.0f);
     if (b < nb) {
-        r0 = *(const float4*)&F.paths[j].rad;  // rad, flags
+        r0 = *(const float4*)&F.prad[j];  // rad, flags
         if (F.fog) radFog = F.cold[j].radFog;
         if (__float_as_uint(r0.w) & kPrimaryHit) {
             pr0 = F.prim[(size_t)j * 3];
@@ -1586,7 +1629,8 @@ struct sr_ctx {
     // frame
     DevBuf<float4> dRayO, dRayD, dHitA;
     DevBuf<int32_t> dHitSlot;
-    DevBuf<PathRec> dPaths;
+    DevBuf<PathHot> dPaths;
+    DevBuf<PathRad> dPrad;
     DevBuf<PathCold> dCold;
     DevBuf<float4> dPrim;  // 3 per path slot
     DevBuf<ShadowRec> dShadows;
@@ -1682,7 +1726,7 @@ struct sr_ctx {
         cnnExamples.clear();
         dCnnLoss.release();
         dRayO.release(); dRayD.release(); dHitA.release(); dHitSlot.release();
-        dPaths.release(); dCold.release(); dPrim.release(); dShadows.release(); dAcc.release();
+        dPaths.release(); dPrad.release(); dCold.release(); dPrim.release(); dShadows.release(); dAcc.release();
         dQueue0.release(); dQueue1.release(); dCounts.release(); dCtr.release();
         dRad.release(); dRadFog.release(); dNormal.release(); dUv.release(); dDepth.release(); dVar.release();
         dAlbedo.release();
@@ -2366,6 +2410,7 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     F.hitA = c->dHitA.ptr + op;
     F.hitSlot = c->dHitSlot.ptr + op;
     F.paths = c->dPaths.ptr + op;
+    F.prad = c->dPrad.ptr + op;
     F.cold = c->dCold.ptr + op;
     F.prim = c->dPrim.ptr + 3 * op;
     F.shadows[0] = c->dShadows.ptr + 2 * op;
@@ -2419,7 +2464,7 @@ int alloc_frame(sr_ctx* c, uint32_t sliceSlots, uint32_t slices) {
     c->sliceSlots = sliceSlots;
     int rc;
     if ((rc = ensure(c->dRayO, maxPaths)) || (rc = ensure(c->dRayD, maxPaths)) || (rc = ensure(c->dHitA, maxPaths)) ||
-        (rc = ensure(c->dHitSlot, maxPaths)) || (rc = ensure(c->dPaths, maxPaths)) || (rc = ensure(c->dShadows, maxPaths * 2)) ||
+        (rc = ensure(c->dHitSlot, maxPaths)) || (rc = ensure(c->dPaths, maxPaths)) || (rc = ensure(c->dPrad, maxPaths)) || (rc = ensure(c->dShadows, maxPaths * 2)) ||
         (rc = ensure(c->dPrim, maxPaths * 3)) || (rc = ensure(c->dCold, maxPaths)) ||
         (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxPaths)) || (rc = ensure(c->dQueue1, maxPaths)) ||
         (rc = ensure(c->dCounts, (size_t)kCounterWords * slices)) ||
