@@ -91,7 +91,9 @@ struct DevScene {
     int32_t sceneBase;   // pad node of the scene BVH
     int32_t sceneNodes;  // node count of the scene BVH (0 = empty scene)
     int32_t anyTransparent;
-    int32_t pad;
+    // Hit records carry (object << 24 | item slot) in one word (fewer than 128 objects, mesh item
+    // slots below 2^24 - 1); otherwise the object alone, with the slot in its own array.
+    int32_t packHit;
     DevSky sky;
     DevFog fog;
 };

@@ -215,6 +215,21 @@ SR_DEV bool slot_pixel(const Frame& F, uint32_t q, uint32_t& x, uint32_t& y) {
 
 SR_DEV uint32_t lane_id() { return __lane_id(); }
 
+// A hit record's last word (DevScene::packHit): -1 for a miss, else (object << 24 | item slot)
+// (an analytic object's slot, unused, packs as 0).
+SR_DEV int32_t pack_hit(const HitRec& h) {
+    return h.obj < 0 ? -1 : (int32_t)(((uint32_t)h.obj << 24) | ((uint32_t)max(h.slot, 0) & 0xffffffu));
+}
+SR_DEV void unpack_hit(const Frame& F, const DevScene& S, uint32_t q, int32_t w, HitRec& h) {
+    if (S.packHit) {
+        h.obj = w < 0 ? -1 : (int32_t)((uint32_t)w >> 24);
+        h.slot = w < 0 ? -1 : (int32_t)((uint32_t)w & 0xffffffu);
+    } else {
+        h.obj = w;
+        h.slot = F.hitSlot[q];
+    }
+}
+
 // A camera-ray queue entry of a slot that starts no sample (Frame::camSlots).
 constexpr uint32_t kNoPath = 0xffffffffu;
 
@@ -711,7 +726,8 @@ __global__ void __launch_bounds__(kShadeBlock, FOG ? 1 : SR_SHADE_MIN_WAVES) k_s
             const float4 ho = F.hitA[q];
             const float4 ro = F.rayO[q], rd = F.rayD[q];
             HitRec h;
-            h.t = ho.x; h.u = ho.y; h.v = ho.z; h.obj = __float_as_int(ho.w); h.slot = F.hitSlot[q];
+            h.t = ho.x; h.u = ho.y; h.v = ho.z;
+            unpack_hit(F, S, q, __float_as_int(ho.w), h);
             R rng = rng_load<R>(p);
             const ray3 ray = mk_ray(mk3(ro.x, ro.y, ro.z), mk3(rd.x, rd.y, rd.z));
             vec3 no;
@@ -828,7 +844,8 @@ __global__ void __launch_bounds__(kBlock, SR_TAIL_MIN_WAVES) k_tail(Frame F, Dev
         const float4 ho = F.hitA[q];
         const float4 ro = F.rayO[q], rd = F.rayD[q];
         HitRec h;
-        h.t = ho.x; h.u = ho.y; h.v = ho.z; h.obj = __float_as_int(ho.w); h.slot = F.hitSlot[q];
+        h.t = ho.x; h.u = ho.y; h.v = ho.z;
+        unpack_hit(F, S, q, __float_as_int(ho.w), h);
         ray3 ray = mk_ray(mk3(ro.x, ro.y, ro.z), mk3(rd.x, rd.y, rd.z));
         for (;;) {
             bool cont, shadow;
@@ -905,8 +922,12 @@ __global__ void __launch_bounds__(kTraceBlock, SR_TRACE_MIN_WAVES) k_trace(Frame
                 h.t = kInf; h.u = 0.0f; h.v = 0.0f; h.obj = -1; h.slot = -1;
                 // Speculative mesh traversal outside counting renders: same hit, wider waves.
                 scene_closest<COUNT, !COUNT>(S, r, h, tl, lds_column());
-                F.hitA[q] = make_float4(h.t, h.u, h.v, __int_as_float(h.obj));
-                F.hitSlot[q] = h.slot;
+                if (S.packHit) {
+                    F.hitA[q] = make_float4(h.t, h.u, h.v, __int_as_float(pack_hit(h)));
+                } else {
+                    F.hitA[q] = make_float4(h.t, h.u, h.v, __int_as_float(h.obj));
+                    F.hitSlot[q] = h.slot;
+                }
             }
         } else if (i - nE64 < nS) {
             const ShadowRec& rec = SR[i - nE64];
@@ -2340,6 +2361,9 @@ int sr_scene_build(sr_ctx* ctx) {
 
     std::vector<DevObject> dobj;
     build_objects(ctx, dobj, S.anyTransparent);
+    S.packHit = ctx->objects.size() < 128 ? 1 : 0;
+    for (const HostMesh& hm : ctx->meshes)
+        if (hm.bvh.items.size() >= 0xffffffu) S.packHit = 0;
 
     if (ctx->device < 0) {  // host-only context
         ctx->built = true;
