@@ -1209,17 +1209,42 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
     // final. A wave's 8x8 tile lies in one block when the block size is a multiple of 8: then one
     // atomic per wave (per-pixel atomics on a few thousand addresses at a frame's last sample
     // serialise); otherwise one per pixel.
+    // The waves of a workgroup whose finished pixels lie in one block pass (block, count) through
+    // LDS and the workgroup's first thread adds each run of equal blocks with one atomic.
+    __shared__ uint32_t waveBlk[kNextBlock / 64], waveAdd[kNextBlock / 64];
     const unsigned long long doneMask = __ballot(doneBlock != ~0u);
+    const uint32_t wv = threadIdx.x / 64;
+    uint32_t wBlk = ~0u, wAdd = 0;
     if (doneMask) {
         const int leader = __ffsll(doneMask) - 1;
         const uint32_t lb = __shfl(doneBlock, leader);
         const bool oneBlock = __ballot(doneBlock != ~0u && doneBlock != lb) == 0;
-        const uint32_t mine = oneBlock ? ((int)lane_id() == leader ? lb : ~0u) : doneBlock;
-        if (mine != ~0u) {
-            const uint32_t add = oneBlock ? (uint32_t)__popcll(doneMask) : 1u;
-            const uint32_t bxi = mine % F.bx, byi = mine / F.bx;
+        if (oneBlock) {
+            wBlk = lb;
+            wAdd = (uint32_t)__popcll(doneMask);
+        } else if (doneBlock != ~0u) {
+            const uint32_t bxi = doneBlock % F.bx, byi = doneBlock / F.bx;
             const uint32_t bw = min(F.bs, F.W - bxi * F.bs), bh = min(F.bs, F.H - byi * F.bs);
-            if (atomicAdd(&F.blockPixels[mine], add) + add == bw * bh) atomicAdd(F.blocksDone, 1u);
+            if (atomicAdd(&F.blockPixels[doneBlock], 1u) + 1u == bw * bh) atomicAdd(F.blocksDone, 1u);
+        }
+    }
+    if (lane_id() == 0) {
+        waveBlk[wv] = wBlk;
+        waveAdd[wv] = wAdd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t runBlk = ~0u, runAdd = 0;
+        for (uint32_t w = 0; w <= blockDim.x / 64; ++w) {
+            const uint32_t blk = w < blockDim.x / 64 ? waveBlk[w] : ~0u;
+            if (blk == runBlk && blk != ~0u) { runAdd += waveAdd[w]; continue; }
+            if (runBlk != ~0u) {
+                const uint32_t bxi = runBlk % F.bx, byi = runBlk / F.bx;
+                const uint32_t bw = min(F.bs, F.W - bxi * F.bs), bh = min(F.bs, F.H - byi * F.bs);
+                if (atomicAdd(&F.blockPixels[runBlk], runAdd) + runAdd == bw * bh) atomicAdd(F.blocksDone, 1u);
+            }
+            runBlk = blk;
+            runAdd = blk != ~0u ? waveAdd[w] : 0u;
         }
     }
     bump_wave(F, 8, converged);
