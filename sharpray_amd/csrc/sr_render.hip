@@ -146,7 +146,7 @@ constexpr uint32_t kTailAuto = 0xffffffffu;
 // primary-hit record) and the most the path slots of one render may take: samples in flight are
 // capped to fit (a 1080p frame holds 32 samples per pixel in 24 GB).
 constexpr size_t kPathSlotBytes = 3 * sizeof(float4) + sizeof(int32_t) + sizeof(PathHot) + sizeof(PathRad) +
-                                  sizeof(PathCold) + 2 * sizeof(ShadowRec) + 2 * sizeof(uint32_t) + 3 * sizeof(float4);
+                                  sizeof(PathCold) + 2 * sizeof(ShadowRec) + 2 * sizeof(uint32_t) + 2 * sizeof(float4) + sizeof(float);
 static_assert(sizeof(PathRec) == 64 && sizeof(PathHot) == 48 && sizeof(PathRad) == 16 && sizeof(PathCold) == 32 &&
               sizeof(ShadowRec) == 80, "record sizes");
 constexpr size_t kMaxPathBytes = (size_t)32 << 30;
@@ -167,9 +167,11 @@ struct Frame {
     PathCold* cold;
     ShadowRec* shadows[2];  // shadow rays of even / odd segments
     PixAcc* acc;
-    // Primary-hit record per path slot (normal + depth, albedo + hit flag, uv), written by the
-    // segment-0 shade and folded into the pixel's sums by the next-sample pass in sample order.
+    // Primary-hit record per path slot in three arrays (normal + depth, albedo + u, v), written by
+    // the segment-0 shade and folded into the pixel's sums by the next-sample pass in sample order.
     float4* prim;
+    float4* primB;
+    float* primV;
     uint32_t* queue[2];
     uint32_t* counts;   // kCounterWords words
     unsigned long long* ctr;  // kCtrWords counters or null
@@ -586,10 +588,9 @@ SR_DEV void shade_path(const Frame& F, const DevScene& S, uint32_t q, PathRec& p
         if (p.flags & 2u) p.energy = p.energy * beer(F.cold[q].medium, dist);
         p.rad = p.rad + s.radiance * p.energy;
         if (primary) {  // the pixel's normal, depth, albedo and uv sums take it in sample order
-            float4* pr = F.prim + (size_t)q * 3;
-            pr[0] = make_float4(s.normal.x, s.normal.y, s.normal.z, dist);
-            pr[1] = make_float4(s.albedo.x, s.albedo.y, s.albedo.z, 0.0f);
-            pr[2] = make_float4(s.uv.x, s.uv.y, 0.0f, 0.0f);
+            F.prim[q] = make_float4(s.normal.x, s.normal.y, s.normal.z, dist);
+            F.primB[q] = make_float4(s.albedo.x, s.albedo.y, s.albedo.z, s.uv.x);
+            F.primV[q] = s.uv.y;
             p.flags |= kPrimaryHit;
         }
         // SampleSkyDirect (scene.cs:774-794): the sides are tested before the shadow ray,
@@ -1052,9 +1053,10 @@ __global__ void __launch_bounds__(kNextBlock) k_next(Frame F) {
         r0 = *(const float4*)&F.prad[j];  // rad, flags
         if (F.fog) radFog = F.cold[j].radFog;
         if (__float_as_uint(r0.w) & kPrimaryHit) {
-            pr0 = F.prim[(size_t)j * 3];
-            pr1 = F.prim[(size_t)j * 3 + 1];
-            pr2 = F.prim[(size_t)j * 3 + 2];
+            pr0 = F.prim[j];
+            const float4 b1 = F.primB[j];
+            pr1 = make_float4(b1.x, b1.y, b1.z, 0.0f);
+            pr2 = make_float4(b1.w, F.primV[j], 0.0f, 0.0f);
         }
     }
     PixPrimary h;
@@ -1678,7 +1680,8 @@ struct sr_ctx {
     DevBuf<PathHot> dPaths;
     DevBuf<PathRad> dPrad;
     DevBuf<PathCold> dCold;
-    DevBuf<float4> dPrim;  // 3 per path slot
+    DevBuf<float4> dPrim, dPrimB;  // one each per path slot
+    DevBuf<float> dPrimV;
     DevBuf<ShadowRec> dShadows;
     DevBuf<PixAcc> dAcc;
     DevBuf<uint32_t> dQueue0, dQueue1, dCounts;
@@ -1772,7 +1775,7 @@ struct sr_ctx {
         cnnExamples.clear();
         dCnnLoss.release();
         dRayO.release(); dRayD.release(); dHitA.release(); dHitSlot.release();
-        dPaths.release(); dPrad.release(); dCold.release(); dPrim.release(); dShadows.release(); dAcc.release();
+        dPaths.release(); dPrad.release(); dCold.release(); dPrim.release(); dPrimB.release(); dPrimV.release(); dShadows.release(); dAcc.release();
         dQueue0.release(); dQueue1.release(); dCounts.release(); dCtr.release();
         dRad.release(); dRadFog.release(); dNormal.release(); dUv.release(); dDepth.release(); dVar.release();
         dAlbedo.release();
@@ -2461,7 +2464,9 @@ Frame frame_for(sr_ctx* c, uint32_t y0, uint32_t rows, uint32_t slice = 0) {
     F.paths = c->dPaths.ptr + op;
     F.prad = c->dPrad.ptr + op;
     F.cold = c->dCold.ptr + op;
-    F.prim = c->dPrim.ptr + 3 * op;
+    F.prim = c->dPrim.ptr + op;
+    F.primB = c->dPrimB.ptr + op;
+    F.primV = c->dPrimV.ptr + op;
     F.shadows[0] = c->dShadows.ptr + 2 * op;
     F.shadows[1] = c->dShadows.ptr + 2 * op + pathSlots;
     F.acc = c->dAcc.ptr + o;
@@ -2514,7 +2519,7 @@ int alloc_frame(sr_ctx* c, uint32_t sliceSlots, uint32_t slices) {
     int rc;
     if ((rc = ensure(c->dRayO, maxPaths)) || (rc = ensure(c->dRayD, maxPaths)) || (rc = ensure(c->dHitA, maxPaths)) ||
         (rc = ensure(c->dHitSlot, maxPaths)) || (rc = ensure(c->dPaths, maxPaths)) || (rc = ensure(c->dPrad, maxPaths)) || (rc = ensure(c->dShadows, maxPaths * 2)) ||
-        (rc = ensure(c->dPrim, maxPaths * 3)) || (rc = ensure(c->dCold, maxPaths)) ||
+        (rc = ensure(c->dPrim, maxPaths)) || (rc = ensure(c->dPrimB, maxPaths)) || (rc = ensure(c->dPrimV, maxPaths)) || (rc = ensure(c->dCold, maxPaths)) ||
         (rc = ensure(c->dAcc, maxSlots)) || (rc = ensure(c->dQueue0, maxPaths)) || (rc = ensure(c->dQueue1, maxPaths)) ||
         (rc = ensure(c->dCounts, (size_t)kCounterWords * slices)) ||
         (rc = ensure(c->dCtr, (size_t)kCtrWords * kCtrSpread)) || (rc = ensure(c->dRad, px * 3)) || (rc = ensure(c->dRadFog, px * 3)) ||
